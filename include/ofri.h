@@ -1,0 +1,147 @@
+/*
+ * libofri: MI355X (gfx950) dense optical-flow kernels for PIV, C ABI.
+ *
+ * Every entry point returns an int status: OFRI_OK (0) or a negative error
+ * class; ofri_last_error() gives the message of the calling thread's last
+ * failure.  Images and flows are H x W row-major float32 arrays.  Pointers
+ * named d_* are device pointers (from ofri_malloc or any HIP allocation on
+ * the context's device); pointers named h_* are host pointers.  All work is
+ * enqueued on the context's stream; entry points returning values to the host
+ * (errors, iteration counts) synchronise that stream first.
+ *
+ * Each operation cites the reference code path it replaces (paths relative
+ * to the OpticalFlow-RI checkout).
+ */
+#ifndef OFRI_H
+#define OFRI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFRI_OK 0
+#define OFRI_E_ARG (-1)      /* invalid argument (shape, size, null pointer) */
+#define OFRI_E_HIP (-2)      /* HIP runtime failure */
+#define OFRI_E_NOMEM (-3)    /* device allocation failed */
+#define OFRI_E_STATE (-4)    /* invalid context / plan state */
+#define OFRI_E_NUMERIC (-5)  /* NaN / zero-division condition the reference raises on */
+
+typedef struct ofri_ctx ofri_ctx;
+
+/* ---- context, errors, device memory -------------------------------------- */
+const char* ofri_last_error(void);
+int ofri_version(void);                                   /* 10000*major + 100*minor + patch */
+int ofri_create(int device, ofri_ctx** out);              /* one HIP stream per context */
+int ofri_destroy(ofri_ctx* ctx);
+int ofri_sync(ofri_ctx* ctx);
+int ofri_malloc(ofri_ctx* ctx, size_t bytes, void** d_out);
+int ofri_free(ofri_ctx* ctx, void* d_ptr);
+int ofri_upload(ofri_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int ofri_download(ofri_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int ofri_copy(ofri_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+int ofri_memset(ofri_ctx* ctx, void* d_dst, int value, size_t bytes);
+/* Device-side timing of the context stream: start/stop an event pair, then read ms. */
+int ofri_timer_start(ofri_ctx* ctx);
+int ofri_timer_stop(ofri_ctx* ctx, float* ms_out);
+
+/* ---- pre-filter ---------------------------------------------------------- */
+/* Separable Gaussian with the reference's padding and float32 tap-order
+ * accumulation; weights h_w[n] (n odd, n <= 127) are the float32 kernel.
+ * d_src may equal d_dst.
+ * Replaces gaussian_filter.py:54-94 (convolveSeparableFilter / gaussian_filterPx). */
+int ofri_gauss_sep(ofri_ctx* ctx, const float* d_src, float* d_dst, int H, int W,
+                   const float* h_w, int n);
+
+/* ---- pyramid helpers ----------------------------------------------------- */
+/* Pillow-compatible antialiased resize of a float image, filter 3 = BICUBIC,
+ * 2 = BILINEAR.  Replaces GenericPyramidalOpticalFlow.py:66-67 (imresize) and
+ * the Farneback_PyCL.py:62 resize. */
+int ofri_resize(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, int H2, int W2,
+                int filter);
+/* Not-a-knot bicubic spline upsample of a flow component on arange(n)/n
+ * grids with clamped extrapolation, then multiplication by `scale` (pass 1.0f
+ * for none).  Requires Hc, Wc >= 4.
+ * Replaces GenericPyramidalOpticalFlow.py:151-171 (RectBivariateSpline + scaling). */
+int ofri_spline_upsample(ofri_ctx* ctx, const float* d_src, int Hc, int Wc, float* d_dst, int H,
+                         int W, float scale);
+/* Symmetric bilinear warp: out1 = im1 at (y - V/2, x - U/2), out2 = im2 at
+ * (y + V/2, x + U/2), round-half-even integer part, clamped neighbours.
+ * Replaces GenericPyramidalOpticalFlow.py:69-115 as called at :197-200. */
+int ofri_warp_symmetric(ofri_ctx* ctx, const float* d_im1, const float* d_im2, const float* d_U,
+                        const float* d_V, int H, int W, float* d_out1, float* d_out2);
+/* Bilinear sample of d_img at float32 coordinates (d_cy, d_cx) with the same
+ * rounding and clamping.  Replaces GenericPyramidalOpticalFlow.py:69-115
+ * (doBiLinearWarping) for arbitrary coordinates. */
+int ofri_warp_coords(ofri_ctx* ctx, const float* d_img, const float* d_cy, const float* d_cx, int H,
+                     int W, float* d_out);
+/* d_acc += d_x (float32), n elements.  Replaces GenericPyramidalOpticalFlow.py:412-413. */
+int ofri_accumulate(ofri_ctx* ctx, float* d_acc, const float* d_x, size_t n);
+
+/* ---- Horn-Schunck -------------------------------------------------------- */
+/* niter Jacobi sweeps from the flow in d_U, d_V (in/out).  d_im1 is the frame
+ * at t, d_im2 at t+1.  *h_err (may be NULL) receives (||U-U0||_F +
+ * ||V-V0||_F)/(H*W), the reference's returned error.
+ * Replaces HornSchunck.py:72-126 (HS, computeDerivatives, HS_helper,
+ * HS_helper2) behind HSOpticalFlowAlgoAdapter.compute (HornSchunck.py:34-36). */
+int ofri_hs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
+            int niter, float* d_U, float* d_V, float* h_err);
+
+/* ---- Liu-Shen ------------------------------------------------------------ */
+/* Liu-Shen physics-based refinement in the solver's own orientation: d_u is
+ * the row (y) component, d_v the column (x) component, both in/out.  Sweeps
+ * until the update norm (||du||_2 + ||dv||_2)/(H*W) <= tol or maxnum sweeps.
+ * *h_err / *h_iters (may be NULL) receive the last norm and the sweep count.
+ * As in the reference, an all-zero frame (maximum 0) yields NaN flow.
+ * Replaces PhysicsBasedOpticalFlowLiuShen.py:47-158 (generate_invmatrix,
+ * helper, physicsBasedOpticalFlowLiuShen); the adapter at :37-39 swaps U, V. */
+int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+            int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
+
+/* ---- whole LS(HS) pyramid ------------------------------------------------ */
+typedef struct ofri_lshs_params {
+    int levels;            /* pyramidalLevels */
+    const float* alphas;   /* HS alpha per level, in the order popped (coarsest first) */
+    int niter;             /* HS sweeps per level */
+    const float* filt_w;   /* FILTER kernel (float32), NULL = no pre-filter */
+    int filt_n;
+    const float* opt_w;    /* FILTER_OPT kernel, NULL = unfiltered */
+    int opt_n;
+    float ls_alpha;        /* Liu-Shen h; <= 0 = no Liu-Shen stage */
+    int ls_maxnum;         /* 60 in the reference */
+    double ls_tol;         /* 1e-8 in the reference */
+} ofri_lshs_params;
+
+/* Per-level statistics written by ofri_pyr_lshs (may be NULL). */
+typedef struct ofri_level_stats {
+    int H, W;
+    float hs_err;
+    float ls_err;
+    int ls_iters;
+} ofri_level_stats;
+
+/* Coarse-to-fine LS(HS): bicubic downsample, spline upsample + x2 scaling,
+ * symmetric bilinear warp, FILTER on the warped pair, HS, FILTER_OPT on the
+ * un-warped pair, Liu-Shen seeded with HS's flow, accumulation; kLevels = 1.
+ * Device in/out.  Replaces genericPyramidalOpticalFlow
+ * (GenericPyramidalOpticalFlow.py:237-415) driven with
+ * HSOpticalFlowAlgoAdapter as main and LiuShenOpticalFlowAlgoAdapter as the
+ * optional adapter (examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py:133-139). */
+int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                  const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
+                  ofri_level_stats* h_stats);
+
+/* Tuning knobs of the temporally blocked sweeps (0 = library default). */
+typedef struct ofri_tuning {
+    int hs_depth;   /* sweeps fused per launch, Horn-Schunck */
+    int ls_depth;   /* sweeps fused per launch, Liu-Shen */
+} ofri_tuning;
+int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t);
+int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OFRI_H */

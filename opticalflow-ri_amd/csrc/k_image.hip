@@ -1,0 +1,518 @@
+// Pre-filter, Pillow-compatible resize, not-a-knot spline upsample, symmetric
+// bilinear warp, accumulation and small reductions.  All are HBM-bound
+// per-pixel kernels: one thread per output pixel, rows mapped to x so every
+// wavefront reads and writes 64 consecutive floats.
+#include <cmath>
+#include <cstring>
+
+#include "ofri_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ofri {
+namespace {
+
+constexpr int kMaxTaps = 127;
+
+struct Taps {
+    float w[kMaxTaps];
+    int n;
+};
+
+// Padding of a line of length L for a filter of half width h, as the
+// reference pads (gaussian_filter.py:63-66): p < 0 -> -1-p, p >= L -> p-h.
+__device__ __forceinline__ int pf_index(int p, int L, int h) {
+    return p < 0 ? -1 - p : (p >= L ? p - h : p);
+}
+
+// Row pass: out[m] = sum_j w[j] * line(m + h - j), float32, tap order, from 0.
+__global__ void gauss_rows(const float* __restrict__ src, float* __restrict__ dst, int H, int W,
+                           Taps t) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    const float* row = src + (size_t)y * W;
+    int h = t.n / 2;
+    float acc = 0.0f;
+    for (int j = 0; j < t.n; ++j) {
+        float prod = t.w[j] * row[pf_index(x + h - j, W, h)];
+        acc = acc + prod;
+    }
+    dst[(size_t)y * W + x] = acc;
+}
+
+// Column pass on the row-filtered image.
+__global__ void gauss_cols(const float* __restrict__ src, float* __restrict__ dst, int H, int W,
+                           Taps t) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    int h = t.n / 2;
+    float acc = 0.0f;
+    for (int j = 0; j < t.n; ++j) {
+        float prod = t.w[j] * src[(size_t)pf_index(y + h - j, H, h) * W + x];
+        acc = acc + prod;
+    }
+    dst[(size_t)y * W + x] = acc;
+}
+
+// Pillow resample, horizontal pass over the rows [first, first+Ht).
+__global__ void resize_h(const float* __restrict__ src, int W, float* __restrict__ tmp, int W2,
+                         int Ht, int first, const double* __restrict__ k, const int* __restrict__ b,
+                         int ks) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (xx >= W2 || y >= Ht) return;
+    int xmin = b[2 * xx], xn = b[2 * xx + 1];
+    const double* kk = k + (size_t)xx * ks;
+    const float* row = src + (size_t)(y + first) * W + xmin;
+    double ss = 0.0;
+    for (int x = 0; x < xn; ++x) ss = ss + (double)row[x] * kk[x];
+    tmp[(size_t)y * W2 + xx] = (float)ss;
+}
+
+__global__ void resize_v(const float* __restrict__ tmp, int W2, float* __restrict__ dst, int H2,
+                         int first, const double* __restrict__ k, const int* __restrict__ b, int ks) {
+    int xx = blockIdx.x * blockDim.x + threadIdx.x;
+    int yy = blockIdx.y;
+    if (xx >= W2 || yy >= H2) return;
+    int ymin = b[2 * yy] - first, yn = b[2 * yy + 1];
+    const double* kk = k + (size_t)yy * ks;
+    double ss = 0.0;
+    for (int y = 0; y < yn; ++y) ss = ss + (double)tmp[(size_t)(y + ymin) * W2 + xx] * kk[y];
+    dst[(size_t)yy * W2 + xx] = (float)ss;
+}
+
+// Spline pass along x: tmp[r][x] = sum_t w[x][t] * src[r][start[x] + t]  (double)
+__global__ void spline_x(const float* __restrict__ src, int Wc, double* __restrict__ tmp, int W,
+                         int Hc, const double* __restrict__ w, const int* __restrict__ st, int width) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (x >= W || r >= Hc) return;
+    const double* ww = w + (size_t)x * width;
+    const float* row = src + (size_t)r * Wc + st[x];
+    double s = 0.0;
+    for (int t = 0; t < width; ++t) s = s + ww[t] * (double)row[t];
+    tmp[(size_t)r * W + x] = s;
+}
+
+// Spline pass along y, rounding to float and scaling in float32.
+__global__ void spline_y(const double* __restrict__ tmp, int W, float* __restrict__ dst, int H,
+                         const double* __restrict__ w, const int* __restrict__ st, int width,
+                         float scale) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const double* ww = w + (size_t)y * width;
+    const double* col = tmp + (size_t)st[y] * W + x;
+    double s = 0.0;
+    for (int t = 0; t < width; ++t) s = s + ww[t] * col[(size_t)t * W];
+    float o = (float)s;
+    dst[(size_t)y * W + x] = o * scale;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Bilinear sample with the reference's rounding (GenericPyramidalOpticalFlow.py:69-110).
+__device__ __forceinline__ float warp_px(const float* __restrict__ img, int H, int W, float cy,
+                                         float cx) {
+    double ry = rint((double)cy), rx = rint((double)cx);
+    int iy = (int)ry, ix = (int)rx;
+    double dy = (double)cy - ry, dx = (double)cx - rx;
+    int iyn = dy < 0 ? iy - 1 : iy + 1;
+    int ixn = dx < 0 ? ix - 1 : ix + 1;
+    dy = fabs(dy);
+    dx = fabs(dx);
+    iy = clampi(iy, 0, H - 1);
+    iyn = clampi(iyn, 0, H - 1);
+    ix = clampi(ix, 0, W - 1);
+    ixn = clampi(ixn, 0, W - 1);
+    double r = (1.0 - dy) * (1.0 - dx) * (double)img[(size_t)iy * W + ix];
+    r = r + (1.0 - dy) * dx * (double)img[(size_t)iy * W + ixn];
+    r = r + dy * (1.0 - dx) * (double)img[(size_t)iyn * W + ix];
+    r = r + dy * dx * (double)img[(size_t)iyn * W + ixn];
+    return (float)r;
+}
+
+__global__ void warp_sym(const float* __restrict__ im1, const float* __restrict__ im2,
+                         const float* __restrict__ U, const float* __restrict__ V, int H, int W,
+                         float* __restrict__ o1, float* __restrict__ o2) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    size_t i = (size_t)y * W + x;
+    float hu = U[i] / 2.0f, hv = V[i] / 2.0f;
+    float cy1 = (float)((double)y - (double)hv), cx1 = (float)((double)x - (double)hu);
+    float cy2 = (float)((double)y + (double)hv), cx2 = (float)((double)x + (double)hu);
+    o1[i] = warp_px(im1, H, W, cy1, cx1);
+    o2[i] = warp_px(im2, H, W, cy2, cx2);
+}
+
+// Bilinear sample at given float32 coordinates (doBiLinearWarping).
+__global__ void warp_coords_k(const float* __restrict__ img, const float* __restrict__ cy,
+                              const float* __restrict__ cx, int H, int W, float* __restrict__ out) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    size_t i = (size_t)y * W + x;
+    out[i] = warp_px(img, H, W, cy[i], cx[i]);
+}
+
+__global__ void accumulate_k(float* __restrict__ acc, const float* __restrict__ x, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) acc[i] = acc[i] + x[i];
+}
+
+__global__ void fill_zero_k(float* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+// s0 may alias d (the result may already be in place).
+__global__ void select_copy_k(const SolveCtrl* c, const float* s0, const float* s1, float* d,
+                              size_t n) {
+    const float* s = c->final_buf ? s1 : s0;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+// Stage 1 of a deterministic max: per-block maxima.
+__global__ void max_stage1(const float* __restrict__ x, size_t n, float* __restrict__ part) {
+    __shared__ float sm[kRedThreads];
+    float m = -INFINITY;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, x[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void max_stage2(const float* __restrict__ part, int np, float* __restrict__ out) {
+    __shared__ float sm[kRedThreads];
+    float m = -INFINITY;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmaxf(m, part[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// Stage 1 of a deterministic pair of squared-difference sums.
+__global__ void dnorm_stage1(const float* __restrict__ A, const float* __restrict__ A0,
+                             const float* __restrict__ B, const float* __restrict__ B0, size_t n,
+                             double* __restrict__ part) {
+    __shared__ double sa[kRedThreads], sb[kRedThreads];
+    double a = 0.0, b = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        float da = A[i] - A0[i], db = B[i] - B0[i];
+        a = a + (double)da * (double)da;
+        b = b + (double)db * (double)db;
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+            sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa[0];
+        part[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+__global__ void dnorm_stage2(const double* __restrict__ part, int np, size_t n,
+                             float* __restrict__ out) {
+    __shared__ double sa[kRedThreads], sb[kRedThreads];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        a = a + part[2 * i];
+        b = b + part[2 * i + 1];
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+            sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)((sqrt(sa[0]) + sqrt(sb[0])) / (double)n);
+}
+
+dim3 grid_rows(int W, int H, int bx = 256) { return dim3((W + bx - 1) / bx, H); }
+
+int grid1d(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+
+// ---------------- host-side tables ----------------
+
+double pil_filter(int filter, double x) {
+    if (x < 0.0) x = -x;
+    if (filter == 3) {
+        const double a = -0.5;
+        if (x < 1.0) return ((a + 2.0) * x - (a + 3.0)) * x * x + 1;
+        if (x < 2.0) return (((x - 5) * x + 8) * x - 4) * a;
+        return 0.0;
+    }
+    return x < 1.0 ? 1.0 - x : 0.0;
+}
+
+// Pillow's coefficient table for one axis (inSize -> outSize).
+void pil_table(int inSize, int outSize, int filter, std::vector<double>& k, std::vector<int>& b,
+               int& ks) {
+    double scale = (double)inSize / outSize;
+    double fs = scale < 1.0 ? 1.0 : scale;
+    double support = (filter == 3 ? 2.0 : 1.0) * fs;
+    ks = (int)std::ceil(support) * 2 + 1;
+    k.assign((size_t)outSize * ks, 0.0);
+    b.assign((size_t)outSize * 2, 0);
+    for (int xx = 0; xx < outSize; ++xx) {
+        double center = (xx + 0.5) * scale;
+        double ss = 1.0 / fs;
+        int xmin = (int)(center - support + 0.5);
+        if (xmin < 0) xmin = 0;
+        int xmax = (int)(center + support + 0.5);
+        if (xmax > inSize) xmax = inSize;
+        xmax -= xmin;
+        double* kk = k.data() + (size_t)xx * ks;
+        double ww = 0.0;
+        for (int x = 0; x < xmax; ++x) {
+            double w = pil_filter(filter, (x + xmin - center + 0.5) * ss);
+            kk[x] = w;
+            ww += w;
+        }
+        for (int x = 0; x < xmax; ++x)
+            if (ww != 0.0) kk[x] /= ww;
+        b[2 * xx] = xmin;
+        b[2 * xx + 1] = xmax;
+    }
+}
+
+int get_resize_table(ofri_ctx* ctx, int in, int out, int filter, ResizeTable** t) {
+    auto key = std::make_tuple(in, out, filter);
+    auto it = ctx->resize_tables.find(key);
+    if (it != ctx->resize_tables.end()) {
+        *t = &it->second;
+        return OFRI_OK;
+    }
+    ResizeTable r{};
+    r.in = in; r.out = out; r.filter = filter;
+    std::vector<double> k;
+    std::vector<int> b;
+    pil_table(in, out, filter, k, b, r.ksize);
+    r.first = b[0];
+    r.last = b[2 * (out - 1)] + b[2 * (out - 1) + 1];
+    OFRI_HIP(hipMalloc(&r.d_k, k.size() * sizeof(double)));
+    OFRI_HIP(hipMalloc(&r.d_bounds, b.size() * sizeof(int)));
+    OFRI_HIP(hipMemcpy(r.d_k, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
+    OFRI_HIP(hipMemcpy(r.d_bounds, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+    auto res = ctx->resize_tables.emplace(key, r);
+    *t = &res.first->second;
+    return OFRI_OK;
+}
+
+// Band of the not-a-knot interpolation weights, BAND coarse samples either
+// side of the interval; the cardinal functions decay by (2 - sqrt 3) per
+// sample, so the dropped tail is below 1e-18 of the kept weights.
+constexpr int kSplineBand = 32;
+
+void spline_weights(int n, int N, std::vector<double>& w, std::vector<int>& st, int& width) {
+    // slopes S (n x n) of the not-a-knot spline through unit-spaced samples
+    std::vector<double> S((size_t)n * n), cp(n), d(n);
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) d[i] = 0.0;
+        if (j <= 2) d[0] = (j == 0 ? -5.0 : (j == 1 ? 4.0 : 1.0)) / 2.0;
+        for (int i = 1; i < n - 1; ++i) d[i] = 3.0 * ((j == i + 1) - (j == i - 1));
+        if (j >= n - 3) d[n - 1] = (j == n - 1 ? 5.0 : (j == n - 2 ? -4.0 : -1.0)) / 2.0;
+        cp[0] = 2.0;
+        for (int i = 1; i < n; ++i) {
+            double a = (i == n - 1) ? 2.0 : 1.0;
+            double bb = (i == n - 1) ? 1.0 : 4.0;
+            double c = (i == n - 1) ? 0.0 : 1.0;
+            double m = bb - a * cp[i - 1];
+            cp[i] = c / m;
+            d[i] = (d[i] - a * d[i - 1]) / m;
+        }
+        for (int i = n - 2; i >= 0; --i) d[i] -= cp[i] * d[i + 1];
+        for (int i = 0; i < n; ++i) S[(size_t)i * n + j] = d[i];
+    }
+    width = n < 2 * kSplineBand + 2 ? n : 2 * kSplineBand + 2;
+    w.assign((size_t)N * width, 0.0);
+    st.assign(N, 0);
+    std::vector<double> row(n);
+    for (int k = 0; k < N; ++k) {
+        bool over = (long long)k * n > (long long)(n - 1) * N;
+        double u = over ? (double)(n - 1) : ((double)k * (double)n) / (double)N;
+        int i = (int)std::floor(u);
+        if (i > n - 2) i = n - 2;
+        double t = u - i, t2 = t * t, t3 = t2 * t;
+        double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+        for (int j = 0; j < n; ++j) row[j] = h10 * S[(size_t)i * n + j] + h11 * S[(size_t)(i + 1) * n + j];
+        row[i] += h00;
+        row[i + 1] += h01;
+        int s0 = i - kSplineBand;
+        if (s0 < 0) s0 = 0;
+        if (s0 > n - width) s0 = n - width;
+        st[k] = s0;
+        for (int t2i = 0; t2i < width; ++t2i) w[(size_t)k * width + t2i] = row[s0 + t2i];
+    }
+}
+
+int get_spline_table(ofri_ctx* ctx, int n, int N, SplineTable** t) {
+    auto key = std::make_pair(n, N);
+    auto it = ctx->spline_tables.find(key);
+    if (it != ctx->spline_tables.end()) {
+        *t = &it->second;
+        return OFRI_OK;
+    }
+    SplineTable s{};
+    s.n = n; s.N = N;
+    std::vector<double> w;
+    std::vector<int> st;
+    spline_weights(n, N, w, st, s.width);
+    OFRI_HIP(hipMalloc(&s.d_w, w.size() * sizeof(double)));
+    OFRI_HIP(hipMalloc(&s.d_start, st.size() * sizeof(int)));
+    OFRI_HIP(hipMemcpy(s.d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    OFRI_HIP(hipMemcpy(s.d_start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    auto res = ctx->spline_tables.emplace(key, s);
+    *t = &res.first->second;
+    return OFRI_OK;
+}
+
+}  // namespace
+
+int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n) {
+    if (n < 1 || n > kMaxTaps || (n & 1) == 0)
+        return fail(OFRI_E_ARG, "gauss_sep: tap count %d must be odd and <= %d", n, kMaxTaps);
+    if (H < n || W < n) return fail(OFRI_E_ARG, "gauss_sep: image %dx%d smaller than %d taps", H, W, n);
+    Taps t{};
+    for (int i = 0; i < n; ++i) t.w[i] = w[i];
+    t.n = n;
+    void* tmp;
+    OFRI_TRY(scratch(ctx, "gauss_tmp", (size_t)H * W * sizeof(float), &tmp));
+    gauss_rows<<<grid_rows(W, H), 256, 0, ctx->stream>>>(src, (float*)tmp, H, W, t);
+    gauss_cols<<<grid_rows(W, H), 256, 0, ctx->stream>>>((const float*)tmp, dst, H, W, t);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter) {
+    if (filter != 2 && filter != 3) return fail(OFRI_E_ARG, "resize: filter %d not supported", filter);
+    if (H < 1 || W < 1 || H2 < 1 || W2 < 1) return fail(OFRI_E_ARG, "resize: empty image");
+    ResizeTable *th, *tv;
+    OFRI_TRY(get_resize_table(ctx, W, W2, filter, &th));
+    OFRI_TRY(get_resize_table(ctx, H, H2, filter, &tv));
+    int Ht = tv->last - tv->first;
+    void* tmp;
+    OFRI_TRY(scratch(ctx, "resize_tmp", (size_t)Ht * W2 * sizeof(float), &tmp));
+    resize_h<<<grid_rows(W2, Ht), 256, 0, ctx->stream>>>(src, W, (float*)tmp, W2, Ht, tv->first,
+                                                        th->d_k, th->d_bounds, th->ksize);
+    resize_v<<<grid_rows(W2, H2), 256, 0, ctx->stream>>>((const float*)tmp, W2, dst, H2, tv->first,
+                                                        tv->d_k, tv->d_bounds, tv->ksize);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int spline_upsample(ofri_ctx* ctx, const float* src, int Hc, int Wc, float* dst, int H, int W,
+                    float scale) {
+    if (Hc < 4 || Wc < 4) return fail(OFRI_E_ARG, "spline_upsample: coarse grid %dx%d below 4x4", Hc, Wc);
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "spline_upsample: empty output");
+    SplineTable *tx, *ty;
+    OFRI_TRY(get_spline_table(ctx, Wc, W, &tx));
+    OFRI_TRY(get_spline_table(ctx, Hc, H, &ty));
+    void* tmp;
+    OFRI_TRY(scratch(ctx, "spline_tmp", (size_t)Hc * W * sizeof(double), &tmp));
+    spline_x<<<grid_rows(W, Hc), 256, 0, ctx->stream>>>(src, Wc, (double*)tmp, W, Hc, tx->d_w,
+                                                       tx->d_start, tx->width);
+    spline_y<<<grid_rows(W, H), 256, 0, ctx->stream>>>((const double*)tmp, W, dst, H, ty->d_w,
+                                                      ty->d_start, ty->width, scale);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const float* U,
+                   const float* V, int H, int W, float* o1, float* o2) {
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "warp: empty image");
+    warp_sym<<<grid_rows(W, H), 256, 0, ctx->stream>>>(im1, im2, U, V, H, W, o1, o2);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* cx, int H, int W,
+                float* out) {
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "warp: empty image");
+    warp_coords_k<<<grid_rows(W, H), 256, 0, ctx->stream>>>(img, cy, cx, H, W, out);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n) {
+    if (n == 0) return OFRI_OK;
+    accumulate_k<<<grid1d(n), 256, 0, ctx->stream>>>(acc, x, n);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int fill_zero(ofri_ctx* ctx, float* p, size_t n) {
+    if (n == 0) return OFRI_OK;
+    fill_zero_k<<<grid1d(n), 256, 0, ctx->stream>>>(p, n);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const float* s1, float* dst,
+                size_t n) {
+    select_copy_k<<<grid1d(n), 256, 0, ctx->stream>>>(ctrl, s0, s1, dst, n);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out) {
+    void* part;
+    OFRI_TRY(scratch(ctx, "max_part", kRedBlocks * sizeof(float), &part));
+    int nb = (int)((n + kRedThreads - 1) / kRedThreads);
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb < 1) nb = 1;
+    max_stage1<<<nb, kRedThreads, 0, ctx->stream>>>(x, n, (float*)part);
+    max_stage2<<<1, kRedThreads, 0, ctx->stream>>>((const float*)part, nb, d_out);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
+                  size_t n, float* d_out) {
+    void* part;
+    OFRI_TRY(scratch(ctx, "dnorm_part", 2 * kRedBlocks * sizeof(double), &part));
+    int nb = (int)((n + kRedThreads - 1) / kRedThreads);
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb < 1) nb = 1;
+    dnorm_stage1<<<nb, kRedThreads, 0, ctx->stream>>>(A, A0, B, B0, n, (double*)part);
+    dnorm_stage2<<<1, kRedThreads, 0, ctx->stream>>>((const double*)part, nb, n, d_out);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+}  // namespace ofri

@@ -1,0 +1,386 @@
+// Liu-Shen physics-based refinement (reference:
+// PhysicsBasedOpticalFlowLiuShen.py:47-158).
+//
+// Setup: one per-pixel kernel forms the eight per-pixel coefficient fields
+// IIx, IIy, II, Ixt, Iyt, B11, B12, B22 (the images normalised by their own
+// maxima, generate_invmatrix's 2x2 inverse).  Sweeps: temporally blocked like
+// Horn-Schunck (64-column extended tiles, R rows per lane in registers with
+// the eight coefficients, DPP for horizontal neighbours, LDS for the edge rows
+// of each wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries
+// as explicit selects.  Each launch also reduces, per fused sweep, the
+// squared update of the pixels it owns; a one-block check kernel turns the
+// per-sweep sums into the reference's norm (||du||_2 + ||dv||_2)/(r*c),
+// applies `while err > tol and k < maxnum` on the device, and marks a
+// re-run of the stopping launch with fewer sweeps when the test held inside
+// it.  The launch sequence is fixed, so a whole solve can be captured in a
+// HIP graph.
+#include <cmath>
+
+#include "ofri_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ofri {
+namespace {
+
+__device__ __forceinline__ float from_left(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128,138 and :47-73).
+__global__ void ls_setup_k(const float* __restrict__ im1, const float* __restrict__ im2, int H,
+                           int W, const float* __restrict__ mx, float h, float* __restrict__ F,
+                           size_t N) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= W) return;
+    const float m1 = mx[0], m2 = mx[1];
+    int im = clampi(i - 1, 0, H - 1), ip = clampi(i + 1, 0, H - 1);
+    int jm = clampi(j - 1, 0, W - 1), jp = clampi(j + 1, 0, W - 1);
+    auto I = [&](int a, int b) -> float { return im1[(size_t)a * W + b] / m1; };
+    auto It = [&](int a, int b) -> float {
+        float i2 = im2[(size_t)a * W + b] / m2;
+        return i2 - I(a, b);
+    };
+    float x = I(i, j);
+    float dr = (float)(-0.5 * (double)I(im, j) + 0.5 * (double)I(ip, j));
+    float dc = (float)(-0.5 * (double)I(i, jm) + 0.5 * (double)I(i, jp));
+    float dtr = (float)(-0.5 * (double)It(im, j) + 0.5 * (double)It(ip, j));
+    float dtc = (float)(-0.5 * (double)It(i, jm) + 0.5 * (double)It(i, jp));
+    int cnt = (i > 0) + (i < H - 1);
+    int cnt_c = (j > 0) + (j < W - 1);
+    float cm = (float)((cnt + 1) * (cnt_c + 1) - 1);
+    float d2r = (float)(1.0 * (double)I(im, j) - 2.0 * (double)x + 1.0 * (double)I(ip, j));
+    float d2c = (float)(1.0 * (double)I(i, jm) - 2.0 * (double)x + 1.0 * (double)I(i, jp));
+    float mm = (float)(0.25 * (double)I(im, jm) - 0.25 * (double)I(im, jp) -
+                       0.25 * (double)I(ip, jm) + 0.25 * (double)I(ip, jp));
+    float two_x = 2.0f * x;
+    float hc = h * cm;
+    float A11 = x * (d2r - two_x) - hc;
+    float A22 = x * (d2c - two_x) - hc;
+    float A12 = x * mm;
+    float det = A11 * A22 - A12 * A12;
+    size_t q = (size_t)i * W + j;
+    F[q] = x * dr;              // IIx
+    F[N + q] = x * dc;          // IIy
+    F[2 * N + q] = x * x;       // II
+    F[3 * N + q] = x * dtr;     // Ixt
+    F[4 * N + q] = x * dtc;     // Iyt
+    F[5 * N + q] = A22 / det;   // B11
+    F[6 * N + q] = (-A12) / det;  // B12
+    F[7 * N + q] = A11 / det;   // B22
+}
+
+struct LsPass {
+    const float* in[2][2];  // [ping-pong][u, v]
+    float* out[2][2];
+    const float* F;         // 8 planes of N
+    size_t N;
+    int H, W;
+    int K;
+    int nsweeps;
+    int src;                // ping-pong index read (normal launch)
+    float h;
+    double* part;           // [K][nblocks][2] squared-update sums (nullptr: none)
+    SolveCtrl* ctrl;
+    int redo;               // 1: re-run launch driven by ctrl->redo
+};
+
+struct Raw {
+    float l, c, r;
+};
+
+__device__ __forceinline__ Raw raw(float x) { return Raw{from_left(x), x, from_right(x)}; }
+
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
+    int nsw = a.nsweeps, src = a.src;
+    if (a.redo) {
+        nsw = a.ctrl->redo;
+        src = a.ctrl->redo_in;
+        if (nsw <= 0) return;
+    } else if (a.ctrl->done) {
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K = a.K, H = a.H, W = a.W;
+    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
+    const int gx = (int)blockIdx.x * outW - K + lane;
+    const int gy0 = (int)blockIdx.y * outH - K + w * R;
+    const int cx = clampi(gx, 0, W - 1);
+    const bool el = gx == 0, er = gx == W - 1;
+    const bool own_x = lane >= K && lane < 64 - K && gx < W;
+    const size_t N = a.N;
+
+    float u[R], v[R], IIX[R], IIY[R], II[R], IXT[R], IYT[R], B11[R], B12[R], B22[R];
+    const float* uin = a.in[src][0];
+    const float* vin = a.in[src][1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int cy = clampi(gy0 + r, 0, H - 1);
+        size_t q = (size_t)cy * W + cx;
+        u[r] = uin[q];
+        v[r] = vin[q];
+        IIX[r] = a.F[q];
+        IIY[r] = a.F[N + q];
+        II[r] = a.F[2 * N + q];
+        IXT[r] = a.F[3 * N + q];
+        IYT[r] = a.F[4 * N + q];
+        B11[r] = a.F[5 * N + q];
+        B12[r] = a.F[6 * N + q];
+        B22[r] = a.F[7 * N + q];
+    }
+
+    __shared__ float edge[2][WAVES][2][2][64];
+    __shared__ double red[WAVES][2];
+    const float h = a.h;
+    const int nblk = gridDim.x * gridDim.y;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int s = 0; s < nsw; ++s) {
+        const int b = s & 1;
+        edge[b][w][0][0][lane] = u[0];
+        edge[b][w][0][1][lane] = v[0];
+        edge[b][w][1][0][lane] = u[R - 1];
+        edge[b][w][1][1][lane] = v[R - 1];
+        __syncthreads();
+        float uup = w > 0 ? edge[b][w - 1][1][0][lane] : u[0];
+        float vup = w > 0 ? edge[b][w - 1][1][1][lane] : v[0];
+        float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : u[R - 1];
+        float vdn = w < WAVES - 1 ? edge[b][w + 1][0][1][lane] : v[R - 1];
+        Raw pu = raw(uup), pv = raw(vup);
+        Raw cu = raw(u[0]), cv = raw(v[0]);
+        double su = 0.0, sv = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Raw nu = raw(r + 1 < R ? u[r + 1] : udn);
+            Raw nv = raw(r + 1 < R ? v[r + 1] : vdn);
+            const int gy = gy0 + r;
+            const bool top = gy == 0, bot = gy == H - 1;
+            // clamped rows ('nearest') and zero-padded rows ('constant')
+            const Raw& uu = top ? cu : pu;
+            const Raw& ud = bot ? cu : nu;
+            const Raw& vu = top ? cv : pv;
+            const Raw& vd = bot ? cv : nv;
+            // clamped columns
+            float uu_l = el ? uu.c : uu.l, uu_r = er ? uu.c : uu.r;
+            float ud_l = el ? ud.c : ud.l, ud_r = er ? ud.c : ud.r;
+            float vu_l = el ? vu.c : vu.l, vu_r = er ? vu.c : vu.r;
+            float vd_l = el ? vd.c : vd.l, vd_r = er ? vd.c : vd.r;
+            float cu_l = el ? cu.c : cu.l, cu_r = er ? cu.c : cu.r;
+            float cv_l = el ? cv.c : cv.l, cv_r = er ? cv.c : cv.r;
+            float dru = (float)(-0.5 * (double)uu.c + 0.5 * (double)ud.c);
+            float dcu = (float)(-0.5 * (double)cu_l + 0.5 * (double)cu_r);
+            float drv = (float)(-0.5 * (double)vu.c + 0.5 * (double)vd.c);
+            float dcv = (float)(-0.5 * (double)cv_l + 0.5 * (double)cv_r);
+            float fru = (float)(1.0 * (double)uu.c + 1.0 * (double)ud.c);
+            float fcv = (float)(1.0 * (double)cv_l + 1.0 * (double)cv_r);
+            float mu = (float)(0.25 * (double)uu_l - 0.25 * (double)uu_r - 0.25 * (double)ud_l +
+                               0.25 * (double)ud_r);
+            float mv = (float)(0.25 * (double)vu_l - 0.25 * (double)vu_r - 0.25 * (double)vd_l +
+                               0.25 * (double)vd_r);
+            // 8-neighbour sums with zero outside the image
+            float zl_pu = (el || top) ? 0.0f : pu.l, zc_pu = top ? 0.0f : pu.c, zr_pu = (er || top) ? 0.0f : pu.r;
+            float zl_nu = (el || bot) ? 0.0f : nu.l, zc_nu = bot ? 0.0f : nu.c, zr_nu = (er || bot) ? 0.0f : nu.r;
+            float zl_pv = (el || top) ? 0.0f : pv.l, zc_pv = top ? 0.0f : pv.c, zr_pv = (er || top) ? 0.0f : pv.r;
+            float zl_nv = (el || bot) ? 0.0f : nv.l, zc_nv = bot ? 0.0f : nv.c, zr_nv = (er || bot) ? 0.0f : nv.r;
+            float zl_cu = el ? 0.0f : cu.l, zr_cu = er ? 0.0f : cu.r;
+            float zl_cv = el ? 0.0f : cv.l, zr_cv = er ? 0.0f : cv.r;
+            double hu = 0.0;
+            hu = hu + (double)zl_pu; hu = hu + (double)zc_pu; hu = hu + (double)zr_pu;
+            hu = hu + (double)zl_cu; hu = hu + (double)zr_cu;
+            hu = hu + (double)zl_nu; hu = hu + (double)zc_nu; hu = hu + (double)zr_nu;
+            double hv = 0.0;
+            hv = hv + (double)zl_pv; hv = hv + (double)zc_pv; hv = hv + (double)zr_pv;
+            hv = hv + (double)zl_cv; hv = hv + (double)zr_cv;
+            hv = hv + (double)zl_nv; hv = hv + (double)zc_nv; hv = hv + (double)zr_nv;
+            float h0u = (float)hu, h0v = (float)hv;
+            float iix = IIX[r], iiy = IIY[r], ii2 = II[r];
+            float bu = (2.0f * iix) * dru;
+            bu = bu + iix * dcv;
+            bu = bu + iiy * drv;
+            bu = bu + ii2 * fru;
+            bu = bu + ii2 * mv;
+            bu = bu + h * h0u;
+            bu = bu + IXT[r];
+            float bv = iiy * dru;
+            bv = bv + iix * dcu;
+            bv = bv + (2.0f * iiy) * dcv;
+            bv = bv + ii2 * mu;
+            bv = bv + ii2 * fcv;
+            bv = bv + h * h0v;
+            bv = bv + IYT[r];
+            float t1 = B11[r] * bu, t2 = B12[r] * bv;
+            float un = -(t1 + t2);
+            float t3 = B12[r] * bu, t4 = B22[r] * bv;
+            float vn = -(t3 + t4);
+            if (a.part) {
+                const int ry = w * R + r;
+                if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
+                    float du = un - u[r], dv = vn - v[r];
+                    su = su + (double)du * (double)du;
+                    sv = sv + (double)dv * (double)dv;
+                }
+            }
+            u[r] = un;
+            v[r] = vn;
+            pu = cu; pv = cv;
+            cu = nu; cv = nv;
+        }
+        if (a.part) {
+            // fixed-order wave reduction, then waves in order
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                su = su + __shfl_xor(su, off);
+                sv = sv + __shfl_xor(sv, off);
+            }
+            if (lane == 0) {
+                red[w][0] = su;
+                red[w][1] = sv;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tu = 0.0, tv = 0.0;
+                for (int q = 0; q < WAVES; ++q) {
+                    tu = tu + red[q][0];
+                    tv = tv + red[q][1];
+                }
+                a.part[((size_t)s * nblk + blk) * 2] = tu;
+                a.part[((size_t)s * nblk + blk) * 2 + 1] = tv;
+            }
+        }
+    }
+
+    float* uo = a.out[src ^ 1][0];
+    float* vo = a.out[src ^ 1][1];
+    if (own_x) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int ry = w * R + r, gy = gy0 + r;
+            if (ry >= K && ry < WAVES * R - K && gy < H) {
+                size_t q = (size_t)gy * W + gx;
+                uo[q] = u[r];
+                vo[q] = v[r];
+            }
+        }
+    }
+}
+
+// Convergence test of one launch: `while err > tol and k < maxnum`
+// (PhysicsBasedOpticalFlowLiuShen.py:141,150; helper :79).
+__global__ void ls_check_k(const double* __restrict__ part, int nblk, int nsw, int dst, double tol,
+                           int maxnum, size_t N, SolveCtrl* ctrl) {
+    __shared__ double sa[256], sb[256];
+    __shared__ int stop;
+    if (ctrl->done) return;
+    if (threadIdx.x == 0) stop = 0;
+    for (int s = 0; s < nsw; ++s) {
+        double a = 0.0, b = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+            a = a + part[((size_t)s * nblk + i) * 2];
+            b = b + part[((size_t)s * nblk + i) * 2 + 1];
+        }
+        sa[threadIdx.x] = a;
+        sb[threadIdx.x] = b;
+        __syncthreads();
+        for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+            if ((int)threadIdx.x < k) {
+                sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + k];
+                sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + k];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            float err = (float)((sqrt(sa[0]) + sqrt(sb[0])) / (double)N);
+            ctrl->sweeps += 1;
+            ctrl->last_err = err;
+            ctrl->final_buf = dst;
+            if (!((double)err > tol)) {
+                ctrl->done = 1;
+                if (s + 1 < nsw) {
+                    ctrl->redo = s + 1;
+                    ctrl->redo_in = dst ^ 1;
+                }
+                stop = 1;
+            } else if (ctrl->sweeps >= maxnum) {
+                ctrl->done = 1;
+            }
+        }
+        __syncthreads();
+        if (stop) return;
+    }
+}
+
+__global__ void ctrl_init_k(SolveCtrl* c) {
+    c->done = 0;
+    c->sweeps = 0;
+    c->final_buf = 0;
+    c->redo = 0;
+    c->redo_in = 0;
+    c->last_err = 0.0f;
+}
+
+constexpr int kR = 8;
+constexpr int kWaves = 8;
+constexpr int kDefaultDepth = 6;
+
+}  // namespace
+
+int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl) {
+    if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
+    if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
+    size_t N = (size_t)H * W;
+    int depth = ctx->tuning.ls_depth > 0 ? ctx->tuning.ls_depth : kDefaultDepth;
+    if (depth > 24) depth = 24;
+    const int outW = 64 - 2 * depth, outH = kWaves * kR - 2 * depth;
+    dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
+    int nblk = grid.x * grid.y;
+    void *fbuf, *pbuf, *mbuf;
+    OFRI_TRY(scratch(ctx, "ls_fields", 10 * N * sizeof(float), &fbuf));
+    OFRI_TRY(scratch(ctx, "ls_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
+    OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
+    float* F = (float*)fbuf;
+    float* ub = F + 8 * N;
+    float* vb = F + 9 * N;
+    float* mx = (float*)mbuf;
+    OFRI_TRY(reduce_max(ctx, im1, N, mx));
+    OFRI_TRY(reduce_max(ctx, im2, N, mx + 1));
+    dim3 g((W + 255) / 256, H);
+    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N);
+    ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
+    OFRI_HIP(hipGetLastError());
+
+    LsPass p{};
+    p.in[0][0] = u; p.in[0][1] = v; p.in[1][0] = ub; p.in[1][1] = vb;
+    p.out[0][0] = u; p.out[0][1] = v; p.out[1][0] = ub; p.out[1][1] = vb;
+    p.F = F; p.N = N; p.H = H; p.W = W; p.K = depth; p.h = h;
+    p.part = (double*)pbuf;
+    p.ctrl = d_ctrl;
+    int done = 0, src = 0;
+    while (done < maxnum) {
+        int ns = maxnum - done < depth ? maxnum - done : depth;
+        p.nsweeps = ns;
+        p.src = src;
+        p.redo = 0;
+        ls_pass_k<kR, kWaves><<<grid, 64 * kWaves, 0, ctx->stream>>>(p);
+        ls_check_k<<<1, 256, 0, ctx->stream>>>(p.part, nblk, ns, src ^ 1, tol, maxnum, N, d_ctrl);
+        OFRI_HIP(hipGetLastError());
+        src ^= 1;
+        done += ns;
+    }
+    if (maxnum > 0) {
+        p.redo = 1;
+        p.part = nullptr;
+        ls_pass_k<kR, kWaves><<<grid, 64 * kWaves, 0, ctx->stream>>>(p);
+        OFRI_TRY(select_copy(ctx, d_ctrl, u, ub, u, N));
+        OFRI_TRY(select_copy(ctx, d_ctrl, v, vb, v, N));
+    }
+    return OFRI_OK;
+}
+
+}  // namespace ofri

@@ -1,0 +1,377 @@
+// C ABI of libofri: context, errors, device memory, the per-operation entry
+// points and the whole-pyramid LS(HS) driver.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "ofri_internal.h"
+
+namespace ofri {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out) {
+    DevBuf& b = ctx->ws[name];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            OFRI_HIP(hipStreamSynchronize(ctx->stream));
+            OFRI_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        size_t sz = (bytes + 255) & ~(size_t)255;
+        if (hipMalloc(&b.p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(OFRI_E_NOMEM, "scratch '%s': cannot allocate %zu bytes", name.c_str(), sz);
+        }
+        b.bytes = sz;
+    }
+    *out = b.p;
+    return OFRI_OK;
+}
+
+// Level sizes of the pyramid (GenericPyramidalOpticalFlow.py:291,335-347,414):
+// level l < L is resized to round(dim * 2^-(L-l)) (half-to-even), level L keeps
+// the input size.
+static void level_sizes(int H, int W, int L, std::vector<int>& hs, std::vector<int>& ws) {
+    double scale = 1.0 / std::pow(2.0, L - 1);
+    hs.clear();
+    ws.clear();
+    for (int level = 1; level <= L; ++level) {
+        if (scale < 1.0 && level != L) {
+            hs.push_back((int)std::nearbyint(H * scale));
+            ws.push_back((int)std::nearbyint(W * scale));
+        } else {
+            hs.push_back(H);
+            ws.push_back(W);
+        }
+        scale *= 2;
+    }
+}
+
+}  // namespace ofri
+
+using namespace ofri;
+
+#define CHECK_CTX(ctx) \
+    if (!(ctx)) return fail(OFRI_E_STATE, "null context")
+
+extern "C" {
+
+const char* ofri_last_error(void) { return g_err.c_str(); }
+
+int ofri_version(void) { return 100; }
+
+int ofri_create(int device, ofri_ctx** out) {
+    if (!out) return fail(OFRI_E_ARG, "ofri_create: null output");
+    int n = 0;
+    OFRI_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(OFRI_E_ARG, "ofri_create: device %d of %d", device, n);
+    OFRI_HIP(hipSetDevice(device));
+    ofri_ctx* c = new ofri_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(OFRI_E_HIP, "ofri_create: stream/event creation failed");
+    }
+    *out = c;
+    return OFRI_OK;
+}
+
+int ofri_destroy(ofri_ctx* ctx) {
+    if (!ctx) return OFRI_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->ws)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& kv : ctx->resize_tables) {
+        (void)hipFree(kv.second.d_k);
+        (void)hipFree(kv.second.d_bounds);
+    }
+    for (auto& kv : ctx->spline_tables) {
+        (void)hipFree(kv.second.d_w);
+        (void)hipFree(kv.second.d_start);
+    }
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OFRI_OK;
+}
+
+int ofri_sync(ofri_ctx* ctx) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_malloc(ofri_ctx* ctx, size_t bytes, void** d_out) {
+    CHECK_CTX(ctx);
+    if (!d_out) return fail(OFRI_E_ARG, "ofri_malloc: null output");
+    OFRI_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(d_out, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(OFRI_E_NOMEM, "ofri_malloc: %zu bytes", bytes);
+    }
+    return OFRI_OK;
+}
+
+int ofri_free(ofri_ctx* ctx, void* d_ptr) {
+    CHECK_CTX(ctx);
+    if (d_ptr) {
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        OFRI_HIP(hipFree(d_ptr));
+    }
+    return OFRI_OK;
+}
+
+int ofri_upload(ofri_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_download(ofri_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_copy(ofri_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_memset(ofri_ctx* ctx, void* d_dst, int value, size_t bytes) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_timer_start(ofri_ctx* ctx) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    return OFRI_OK;
+}
+
+int ofri_timer_stop(ofri_ctx* ctx, float* ms_out) {
+    CHECK_CTX(ctx);
+    OFRI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    OFRI_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    OFRI_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ms_out) *ms_out = ms;
+    return OFRI_OK;
+}
+
+int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t) {
+    CHECK_CTX(ctx);
+    if (!t) return fail(OFRI_E_ARG, "ofri_set_tuning: null");
+    if (t->hs_depth < 0 || t->hs_depth > 24 || t->ls_depth < 0 || t->ls_depth > 24)
+        return fail(OFRI_E_ARG, "ofri_set_tuning: depth outside 0..24");
+    ctx->tuning = *t;
+    return OFRI_OK;
+}
+
+int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t) {
+    CHECK_CTX(ctx);
+    if (!t) return fail(OFRI_E_ARG, "ofri_get_tuning: null");
+    *t = ctx->tuning;
+    return OFRI_OK;
+}
+
+int ofri_gauss_sep(ofri_ctx* ctx, const float* d_src, float* d_dst, int H, int W, const float* h_w,
+                   int n) {
+    CHECK_CTX(ctx);
+    if (!d_src || !d_dst || !h_w) return fail(OFRI_E_ARG, "ofri_gauss_sep: null pointer");
+    return gauss_sep(ctx, d_src, d_dst, H, W, h_w, n);
+}
+
+int ofri_resize(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, int H2, int W2,
+                int filter) {
+    CHECK_CTX(ctx);
+    if (!d_src || !d_dst) return fail(OFRI_E_ARG, "ofri_resize: null pointer");
+    return resize(ctx, d_src, H, W, d_dst, H2, W2, filter);
+}
+
+int ofri_spline_upsample(ofri_ctx* ctx, const float* d_src, int Hc, int Wc, float* d_dst, int H,
+                         int W, float scale) {
+    CHECK_CTX(ctx);
+    if (!d_src || !d_dst) return fail(OFRI_E_ARG, "ofri_spline_upsample: null pointer");
+    return spline_upsample(ctx, d_src, Hc, Wc, d_dst, H, W, scale);
+}
+
+int ofri_warp_symmetric(ofri_ctx* ctx, const float* d_im1, const float* d_im2, const float* d_U,
+                        const float* d_V, int H, int W, float* d_out1, float* d_out2) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_U || !d_V || !d_out1 || !d_out2)
+        return fail(OFRI_E_ARG, "ofri_warp_symmetric: null pointer");
+    return warp_symmetric(ctx, d_im1, d_im2, d_U, d_V, H, W, d_out1, d_out2);
+}
+
+int ofri_warp_coords(ofri_ctx* ctx, const float* d_img, const float* d_cy, const float* d_cx, int H,
+                     int W, float* d_out) {
+    CHECK_CTX(ctx);
+    if (!d_img || !d_cy || !d_cx || !d_out) return fail(OFRI_E_ARG, "ofri_warp_coords: null pointer");
+    return warp_coords(ctx, d_img, d_cy, d_cx, H, W, d_out);
+}
+
+int ofri_accumulate(ofri_ctx* ctx, float* d_acc, const float* d_x, size_t n) {
+    CHECK_CTX(ctx);
+    if (!d_acc || !d_x) return fail(OFRI_E_ARG, "ofri_accumulate: null pointer");
+    return accumulate(ctx, d_acc, d_x, n);
+}
+
+int ofri_hs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
+            int niter, float* d_U, float* d_V, float* h_err) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_U || !d_V) return fail(OFRI_E_ARG, "ofri_hs: null pointer");
+    void* e = nullptr;
+    if (h_err) OFRI_TRY(scratch(ctx, "api_err", sizeof(float), &e));
+    OFRI_TRY(hs_solve(ctx, d_im1, d_im2, H, W, alpha, niter, d_U, d_V, (float*)e));
+    if (h_err) {
+        OFRI_HIP(hipMemcpyAsync(h_err, e, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return OFRI_OK;
+}
+
+int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+            int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_ls: null pointer");
+    void* c;
+    OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
+    OFRI_TRY(ls_solve(ctx, d_im1, d_im2, H, W, h, maxnum, tol, d_u, d_v, (SolveCtrl*)c));
+    if (h_err || h_iters) {
+        SolveCtrl hc;
+        OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (h_err) *h_err = hc.last_err;
+        if (h_iters) *h_iters = hc.sweeps;
+    }
+    return OFRI_OK;
+}
+
+int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                  const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
+                  ofri_level_stats* h_stats) {
+    CHECK_CTX(ctx);
+    if (!p || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_lshs: null pointer");
+    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_lshs: levels %d", p->levels);
+    if (!p->alphas) return fail(OFRI_E_ARG, "ofri_pyr_lshs: alphas missing");
+    const int L = p->levels;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    for (int l = 0; l < L; ++l)
+        if (lh[l] < 4 || lw[l] < 4)
+            return fail(OFRI_E_ARG, "ofri_pyr_lshs: level %d size %dx%d below 4x4", l + 1, lh[l], lw[l]);
+    const size_t NF = (size_t)H * W;
+    // workspace: n1 n2 (resized) w1 w2 (warped) f1 f2 (filtered) o1 o2 (opt-filtered)
+    //            U V (solver) A B (accumulated, ping-pong with upsampled)
+    void* ws;
+    OFRI_TRY(scratch(ctx, "pyr_ws", 14 * NF * sizeof(float), &ws));
+    float* base = (float*)ws;
+    float *n1 = base, *n2 = base + NF, *w1 = base + 2 * NF, *w2 = base + 3 * NF;
+    float *f1 = base + 4 * NF, *f2 = base + 5 * NF, *o1 = base + 6 * NF, *o2 = base + 7 * NF;
+    float *U = base + 8 * NF, *V = base + 9 * NF;
+    float *Ua = base + 10 * NF, *Va = base + 11 * NF, *Ub = base + 12 * NF, *Vb = base + 13 * NF;
+    void* st;
+    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (sizeof(SolveCtrl) + 16), &st));
+    SolveCtrl* ctrls = (SolveCtrl*)st;
+    float* hs_errs = (float*)(ctrls + L);
+
+    float *acc_u = Ua, *acc_v = Va;
+    int ph = 0, pw = 0;
+    for (int l = 0; l < L; ++l) {
+        const int h = lh[l], w = lw[l];
+        const size_t N = (size_t)h * w;
+        const bool last = l == L - 1;
+        const float *c1 = d_im1, *c2 = d_im2;
+        if (h != H || w != W) {
+            OFRI_TRY(resize(ctx, d_im1, H, W, n1, h, w, 3));
+            OFRI_TRY(resize(ctx, d_im2, H, W, n2, h, w, 3));
+            c1 = n1;
+            c2 = n2;
+        }
+        const float *s1 = c1, *s2 = c2;  // images the main solver sees
+        float* out_u = last ? d_Uacc : (acc_u == Ua ? Ub : Ua);
+        float* out_v = last ? d_Vacc : (acc_v == Va ? Vb : Va);
+        if (l > 0) {
+            // updateNextPyramidalLevel with warping, biLinear and scaling
+            if (h != ph || w != pw) {
+                float sx = (float)w / (float)pw, sy = (float)h / (float)ph;
+                OFRI_TRY(spline_upsample(ctx, acc_u, ph, pw, out_u, h, w, sx));
+                OFRI_TRY(spline_upsample(ctx, acc_v, ph, pw, out_v, h, w, sy));
+            } else {
+                OFRI_HIP(hipMemcpyAsync(out_u, acc_u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_HIP(hipMemcpyAsync(out_v, acc_v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            OFRI_TRY(warp_symmetric(ctx, c1, c2, out_u, out_v, h, w, w1, w2));
+            s1 = w1;
+            s2 = w2;
+        } else {
+            OFRI_TRY(fill_zero(ctx, out_u, N));
+            OFRI_TRY(fill_zero(ctx, out_v, N));
+        }
+        if (p->filt_w && p->filt_n > 0) {
+            OFRI_TRY(gauss_sep(ctx, s1, f1, h, w, p->filt_w, p->filt_n));
+            OFRI_TRY(gauss_sep(ctx, s2, f2, h, w, p->filt_w, p->filt_n));
+            s1 = f1;
+            s2 = f2;
+        }
+        OFRI_TRY(fill_zero(ctx, U, N));
+        OFRI_TRY(fill_zero(ctx, V, N));
+        OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l));
+        if (p->ls_alpha > 0.0f) {
+            const float *q1 = c1, *q2 = c2;
+            if (p->opt_w && p->opt_n > 0) {
+                OFRI_TRY(gauss_sep(ctx, c1, o1, h, w, p->opt_w, p->opt_n));
+                OFRI_TRY(gauss_sep(ctx, c2, o2, h, w, p->opt_w, p->opt_n));
+                q1 = o1;
+                q2 = o2;
+            }
+            // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
+            OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
+        }
+        OFRI_TRY(accumulate(ctx, out_u, U, N));
+        OFRI_TRY(accumulate(ctx, out_v, V, N));
+        acc_u = out_u;
+        acc_v = out_v;
+        ph = h;
+        pw = w;
+    }
+    if (h_stats) {
+        std::vector<SolveCtrl> hc(L);
+        std::vector<float> he(L);
+        OFRI_HIP(hipMemcpyAsync(hc.data(), ctrls, L * sizeof(SolveCtrl), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(he.data(), hs_errs, L * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        for (int l = 0; l < L; ++l) {
+            h_stats[l].H = lh[l];
+            h_stats[l].W = lw[l];
+            h_stats[l].hs_err = he[l];
+            h_stats[l].ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
+            h_stats[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
+        }
+    }
+    return OFRI_OK;
+}
+
+}  // extern "C"

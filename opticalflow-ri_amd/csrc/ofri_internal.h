@@ -1,0 +1,109 @@
+// Internal declarations shared by the libofri translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ofri.h"
+
+namespace ofri {
+
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+#define OFRI_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::ofri::fail(OFRI_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define OFRI_TRY(expr)              \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != OFRI_OK) return s_; \
+    } while (0)
+
+// Device-side control word of a solver run with a convergence test.
+struct SolveCtrl {
+    int done;        // 1 once the stop condition held or the sweep budget ran out
+    int sweeps;      // sweeps taken so far
+    int final_buf;   // ping-pong index holding the result
+    int redo;        // >0: the stopping pass must be re-run with this many sweeps
+    int redo_in;     // ping-pong index the redo reads
+    float last_err;  // last update norm (float32, like the reference)
+    int pad[2];
+};
+
+// A device allocation owned by the context, grown on demand.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ResizeTable {
+    int in, out, filter, ksize, first, last;
+    double* d_k = nullptr;
+    int* d_bounds = nullptr;
+};
+
+struct SplineTable {
+    int n, N, width;
+    double* d_w = nullptr;  // N x width
+    int* d_start = nullptr;  // N
+};
+
+}  // namespace ofri
+
+struct ofri_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ofri_tuning tuning{0, 0};
+    std::map<std::string, ofri::DevBuf> ws;
+    std::map<std::tuple<int, int, int>, ofri::ResizeTable> resize_tables;
+    std::map<std::pair<int, int>, ofri::SplineTable> spline_tables;
+};
+
+namespace ofri {
+
+// Named scratch buffer of at least `bytes` bytes (grows, never shrinks).
+int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out);
+
+// ---- image kernels (k_image.hip) ----
+int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n);
+int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter);
+int spline_upsample(ofri_ctx* ctx, const float* src, int Hc, int Wc, float* dst, int H, int W,
+                    float scale);
+int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const float* U,
+                   const float* V, int H, int W, float* o1, float* o2);
+int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* cx, int H, int W,
+                float* out);
+int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n);
+int fill_zero(ofri_ctx* ctx, float* p, size_t n);
+// max over n floats -> d_out[0]
+int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
+// d_out[0] = (||A-A0|| + ||B-B0||) / n as float32 (differences in float32)
+int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
+                  size_t n, float* d_out);
+// dst[i] = src[ctrl->final_buf][i] for a ping-pong pair
+int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const float* s1, float* dst,
+                size_t n);
+
+// ---- Horn-Schunck (k_hs.hip) ----
+int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
+             int niter, float* U, float* V, float* d_err);
+
+// ---- Liu-Shen (k_ls.hip) ----
+int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl);
+
+}  // namespace ofri

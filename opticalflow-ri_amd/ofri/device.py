@@ -1,0 +1,125 @@
+"""Device context and device-resident float32 images.
+
+One process-wide context (one HIP stream) drives all drop-in classes; the
+pyramid driver keeps every intermediate image and flow in device memory and
+only crosses to the host at the API boundary, where the reference's callers
+hand over and receive numpy arrays.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from . import _capi
+
+_ctx = None
+_lock = threading.Lock()
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = _capi.load()
+        h = C.c_void_p()
+        _capi.check(self.lib.ofri_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        _capi.check(self.lib.ofri_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.ofri_destroy(self.handle)
+            self.handle = None
+
+    def set_tuning(self, hs_depth=0, ls_depth=0):
+        t = _capi.Tuning(int(hs_depth), int(ls_depth))
+        _capi.check(self.lib.ofri_set_tuning(self.handle, C.byref(t)))
+
+    def get_tuning(self):
+        t = _capi.Tuning()
+        _capi.check(self.lib.ofri_get_tuning(self.handle, C.byref(t)))
+        return t.hs_depth, t.ls_depth
+
+    def timer_start(self):
+        _capi.check(self.lib.ofri_timer_start(self.handle))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _capi.check(self.lib.ofri_timer_stop(self.handle, C.byref(ms)))
+        return ms.value
+
+
+def context():
+    """The process-wide context on device OFRI_DEVICE (default 0)."""
+    global _ctx
+    with _lock:
+        if _ctx is None:
+            _ctx = Context(int(os.environ.get("OFRI_DEVICE", "0")))
+        return _ctx
+
+
+class DeviceArray:
+    """A float32 H x W array in device memory owned by this object."""
+
+    __slots__ = ("ctx", "shape", "ptr", "nbytes")
+
+    def __init__(self, shape, ctx=None):
+        self.ctx = ctx or context()
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        p = C.c_void_p()
+        _capi.check(self.ctx.lib.ofri_malloc(self.ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a, ctx=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        d = cls(a.shape, ctx)
+        d.upload(a)
+        return d
+
+    @classmethod
+    def zeros(cls, shape, ctx=None):
+        d = cls(shape, ctx)
+        _capi.check(d.ctx.lib.ofri_memset(d.ctx.handle, d.ptr, 0, d.nbytes))
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.shape != self.shape:
+            raise ValueError("shape %s != %s" % (a.shape, self.shape))
+        _capi.check(self.ctx.lib.ofri_upload(self.ctx.handle, self.ptr, a.ctypes.data_as(C.c_void_p),
+                                             self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, np.float32)
+        _capi.check(self.ctx.lib.ofri_download(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                               self.nbytes))
+        return out
+
+    def copy(self):
+        d = DeviceArray(self.shape, self.ctx)
+        _capi.check(self.ctx.lib.ofri_copy(self.ctx.handle, d.ptr, self.ptr, self.nbytes))
+        return d
+
+    @property
+    def H(self):
+        return self.shape[0]
+
+    @property
+    def W(self):
+        return self.shape[1]
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.ctx.lib.ofri_free(self.ctx.handle, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def as_device(a, ctx=None):
+    return a if isinstance(a, DeviceArray) else DeviceArray.from_host(a, ctx)

@@ -1,0 +1,131 @@
+"""Device-array operations: one call per libofri entry point.
+
+Every function takes and returns DeviceArray objects (host numpy arrays are
+uploaded first) and runs on the process-wide context's stream.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _capi
+from .device import DeviceArray, as_device, context
+
+# Reference-like console output (the reference prints per level and per
+# Liu-Shen sweep); OFRI_VERBOSE=0 silences it.
+VERBOSE = os.environ.get("OFRI_VERBOSE", "1") != "0"
+
+
+def log(*args):
+    if VERBOSE:
+        print(*args)
+
+
+def _lib():
+    return context().lib
+
+
+def _h():
+    return context().handle
+
+
+def gauss_sep(img, weights, out=None):
+    """Separable pre-filter with float32 weights (gaussian_filter.py:54-94)."""
+    img = as_device(img)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    out = out if out is not None else DeviceArray(img.shape)
+    _capi.check(_lib().ofri_gauss_sep(_h(), img.ptr, out.ptr, img.H, img.W, _capi.fptr(w), len(w)))
+    return out
+
+
+BICUBIC = 3
+BILINEAR = 2
+
+
+def resize(img, H2, W2, filt=BICUBIC):
+    """Pillow-compatible float-image resize (GenericPyramidalOpticalFlow.py:66-67)."""
+    img = as_device(img)
+    out = DeviceArray((H2, W2))
+    _capi.check(_lib().ofri_resize(_h(), img.ptr, img.H, img.W, out.ptr, int(H2), int(W2), int(filt)))
+    return out
+
+
+def spline_upsample(flow, H, W, scale=1.0):
+    """RectBivariateSpline upsample + scaling (GenericPyramidalOpticalFlow.py:151-171)."""
+    flow = as_device(flow)
+    out = DeviceArray((H, W))
+    _capi.check(_lib().ofri_spline_upsample(_h(), flow.ptr, flow.H, flow.W, out.ptr, int(H), int(W),
+                                            float(np.float32(scale))))
+    return out
+
+
+def warp_symmetric(im1, im2, U, V):
+    """Symmetric bilinear warp (GenericPyramidalOpticalFlow.py:69-115,197-200)."""
+    im1, im2, U, V = (as_device(a) for a in (im1, im2, U, V))
+    o1, o2 = DeviceArray(im1.shape), DeviceArray(im1.shape)
+    _capi.check(_lib().ofri_warp_symmetric(_h(), im1.ptr, im2.ptr, U.ptr, V.ptr, im1.H, im1.W, o1.ptr, o2.ptr))
+    return o1, o2
+
+
+def warp_coords(img, cy, cx):
+    """Bilinear sample at float32 coordinates (GenericPyramidalOpticalFlow.py:69-115)."""
+    img, cy, cx = (as_device(a) for a in (img, cy, cx))
+    out = DeviceArray(img.shape)
+    _capi.check(_lib().ofri_warp_coords(_h(), img.ptr, cy.ptr, cx.ptr, img.H, img.W, out.ptr))
+    return out
+
+
+def accumulate(acc, x):
+    _capi.check(_lib().ofri_accumulate(_h(), acc.ptr, x.ptr, int(np.prod(acc.shape))))
+    return acc
+
+
+def hs(im1, im2, alpha, niter, U, V):
+    """Horn-Schunck; U, V are updated in place.  Returns the reference's error."""
+    im1, im2 = as_device(im1), as_device(im2)
+    err = C.c_float()
+    _capi.check(_lib().ofri_hs(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(alpha)), int(niter),
+                               U.ptr, V.ptr, C.byref(err)))
+    return np.float32(err.value)
+
+
+def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
+    """Liu-Shen in the solver's orientation (u = row component); u, v in place.
+    Returns (err, sweeps)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    err = C.c_float()
+    it = C.c_int()
+    _capi.check(_lib().ofri_ls(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(h)), int(maxnum),
+                               float(tol), u.ptr, v.ptr, C.byref(err), C.byref(it)))
+    return np.float32(err.value), it.value
+
+
+def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
+             Uacc=None, Vacc=None, stats=True):
+    """Whole LS(HS) pyramid on the device.  Returns (Uacc, Vacc, stats list)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    H, W = im1.shape
+    al = np.ascontiguousarray(alphas, dtype=np.float32)
+    if len(al) < levels:
+        raise ValueError("need one alpha per level")
+    fw = None if filt_w is None else np.ascontiguousarray(filt_w, dtype=np.float32)
+    ow = None if opt_w is None else np.ascontiguousarray(opt_w, dtype=np.float32)
+    p = _capi.LshsParams()
+    p.levels = int(levels)
+    p.alphas = _capi.fptr(al)
+    p.niter = int(niter)
+    p.filt_w = _capi.fptr(fw) if fw is not None else None
+    p.filt_n = 0 if fw is None else len(fw)
+    p.opt_w = _capi.fptr(ow) if ow is not None else None
+    p.opt_n = 0 if ow is None else len(ow)
+    p.ls_alpha = float(ls_alpha) if ls_alpha else 0.0
+    p.ls_maxnum = int(ls_maxnum)
+    p.ls_tol = float(ls_tol)
+    Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
+    Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
+    st = (_capi.LevelStats * levels)() if stats else None
+    _capi.check(_lib().ofri_pyr_lshs(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
+                                     st if stats else None))
+    out = [dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err), ls_iters=s.ls_iters)
+           for s in st] if stats else []
+    return Uacc, Vacc, out

@@ -1,0 +1,212 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this
+module, and only as the checker.  The shipped library never calls it.
+
+Every function takes numpy arrays and returns new arrays; `prec` selects the
+instance: "f32" is the reference dataflow (float32 storage, float64 stencil
+sums), "f64" the same method in float64.  See ofri_oracle.c for the reference
+file:line each entry follows.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _declare(L):
+    for suf, P, R in (("f32", _fp, C.c_float), ("f64", _dp, C.c_double)):
+        getattr(L, "orc_gauss_sep_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int]
+        getattr(L, "orc_hs_derivs_" + suf).argtypes = [P, P, C.c_int, C.c_int, P, P, P]
+        getattr(L, "orc_hs_sweep_" + suf).argtypes = [P, P, P, R, C.c_int, C.c_int, P, P, P, P]
+        f = getattr(L, "orc_hs_" + suf)
+        f.argtypes = [P, P, C.c_int, C.c_int, R, C.c_int, P, P]
+        f.restype = C.c_double
+        getattr(L, "orc_ls_setup_" + suf).argtypes = [P, P, C.c_int, C.c_int, R] + [P] * 8
+        f = getattr(L, "orc_ls_sweep_" + suf)
+        f.argtypes = [P] * 8 + [R, C.c_int, C.c_int, P, P, P, P]
+        f.restype = C.c_double
+        f = getattr(L, "orc_ls_" + suf)
+        f.argtypes = [P, P, C.c_int, C.c_int, R, C.c_int, C.c_double, P, P, _ip]
+        f.restype = C.c_double
+        getattr(L, "orc_resize_bicubic_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int]
+        getattr(L, "orc_spline_upsample_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, R, _dp]
+        getattr(L, "orc_warp_symmetric_" + suf).argtypes = [P, P, P, P, C.c_int, C.c_int, P, P]
+        getattr(L, "orc_pyr_lshs_" + suf).argtypes = [
+            P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, R, C.c_int,
+            C.c_double, P, P, _dp]
+    L.orc_spline_matrix.argtypes = [C.c_int, C.c_int, _dp]
+    L.orc_pil_coeffs.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip]
+    L.orc_pil_coeffs.restype = _dp
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def _dt(prec):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def _arr(a, prec):
+    return np.ascontiguousarray(a, dtype=_dt(prec))
+
+
+def _p(a):
+    return a.ctypes.data_as(_fp if a.dtype == np.float32 else _dp)
+
+
+def gauss_sep(img, weights, prec="f32"):
+    """gaussian_filterPx with the given weights; returns a new array."""
+    out = _arr(img, prec).copy()
+    w = _arr(weights, prec)
+    H, W = out.shape
+    getattr(lib(), "orc_gauss_sep_" + prec)(_p(out), H, W, _p(w), len(w))
+    return out
+
+
+def hs_derivs(I0, I1, prec="f32"):
+    I0 = _arr(I0, prec); I1 = _arr(I1, prec)
+    H, W = I0.shape
+    fx, fy, ft = (np.empty_like(I0) for _ in range(3))
+    getattr(lib(), "orc_hs_derivs_" + prec)(_p(I0), _p(I1), H, W, _p(fx), _p(fy), _p(ft))
+    return fx, fy, ft
+
+
+def hs_sweep(fx, fy, ft, alpha, U, V, prec="f32"):
+    fx, fy, ft, U, V = (_arr(a, prec) for a in (fx, fy, ft, U, V))
+    H, W = U.shape
+    Un, Vn = np.empty_like(U), np.empty_like(V)
+    getattr(lib(), "orc_hs_sweep_" + prec)(_p(fx), _p(fy), _p(ft), alpha, H, W, _p(U), _p(V), _p(Un), _p(Vn))
+    return Un, Vn
+
+
+def hs(I0, I1, alpha, niter, U=None, V=None, prec="f32"):
+    """Returns (U, V, total_error)."""
+    I0 = _arr(I0, prec); I1 = _arr(I1, prec)
+    H, W = I0.shape
+    U = np.zeros((H, W), _dt(prec)) if U is None else _arr(U, prec).copy()
+    V = np.zeros((H, W), _dt(prec)) if V is None else _arr(V, prec).copy()
+    e = getattr(lib(), "orc_hs_" + prec)(_p(I0), _p(I1), H, W, alpha, int(niter), _p(U), _p(V))
+    return U, V, e
+
+
+def ls_setup(im1, im2, h, prec="f32"):
+    """Returns dict of IIx, IIy, II, Ixt, Iyt, B11, B12, B22."""
+    im1 = _arr(im1, prec); im2 = _arr(im2, prec)
+    H, W = im1.shape
+    names = ("IIx", "IIy", "II", "Ixt", "Iyt", "B11", "B12", "B22")
+    f = {n: np.empty_like(im1) for n in names}
+    getattr(lib(), "orc_ls_setup_" + prec)(_p(im1), _p(im2), H, W, h, *[_p(f[n]) for n in names])
+    return f
+
+
+def ls_sweep(fields, h, u, v, prec="f32"):
+    names = ("IIx", "IIy", "II", "Ixt", "Iyt", "B11", "B12", "B22")
+    fs = [_arr(fields[n], prec) for n in names]
+    u = _arr(u, prec); v = _arr(v, prec)
+    H, W = u.shape
+    un, vn = np.empty_like(u), np.empty_like(v)
+    e = getattr(lib(), "orc_ls_sweep_" + prec)(*[_p(a) for a in fs], h, H, W, _p(u), _p(v), _p(un), _p(vn))
+    return un, vn, e
+
+
+def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8, prec="f32"):
+    """Solver-internal orientation: u = row component, v = column component.
+    Returns (u, v, err, iters)."""
+    im1 = _arr(im1, prec); im2 = _arr(im2, prec)
+    H, W = im1.shape
+    u = _arr(u, prec).copy(); v = _arr(v, prec).copy()
+    it = C.c_int(0)
+    e = getattr(lib(), "orc_ls_" + prec)(_p(im1), _p(im2), H, W, h, int(maxnum), float(tol), _p(u), _p(v), C.byref(it))
+    return u, v, e, it.value
+
+
+def resize_bicubic(src, H2, W2, prec="f32"):
+    src = _arr(src, prec)
+    H, W = src.shape
+    dst = np.empty((H2, W2), _dt(prec))
+    getattr(lib(), "orc_resize_bicubic_" + prec)(_p(src), H, W, _p(dst), H2, W2)
+    return dst
+
+
+def spline_upsample(src, H, W, scale=1.0, prec="f32", want_f64=False):
+    src = _arr(src, prec)
+    Hc, Wc = src.shape
+    dst = np.empty((H, W), _dt(prec))
+    d64 = np.empty((H, W), np.float64) if want_f64 else None
+    getattr(lib(), "orc_spline_upsample_" + prec)(
+        _p(src), Hc, Wc, _p(dst), H, W, scale, d64.ctypes.data_as(_dp) if want_f64 else None)
+    return (dst, d64) if want_f64 else dst
+
+
+def spline_matrix(n, N):
+    out = np.empty((N, n), np.float64)
+    lib().orc_spline_matrix(n, N, out.ctypes.data_as(_dp))
+    return out
+
+
+def pil_coeffs(in_size, out_size, filt=3):
+    ks = C.c_int(0)
+    bounds = np.zeros(2 * out_size, np.int32)
+    p = lib().orc_pil_coeffs(in_size, out_size, filt, C.byref(ks), bounds.ctypes.data_as(_ip))
+    k = np.ctypeslib.as_array(p, shape=(out_size * ks.value,)).copy().reshape(out_size, ks.value)
+    C.CDLL(None).free(C.cast(p, C.c_void_p))
+    return k, bounds.reshape(out_size, 2)
+
+
+def warp_symmetric(im1, im2, U, V, prec="f32"):
+    im1, im2, U, V = (_arr(a, prec) for a in (im1, im2, U, V))
+    H, W = im1.shape
+    o1, o2 = np.empty_like(im1), np.empty_like(im2)
+    getattr(lib(), "orc_warp_symmetric_" + prec)(_p(im1), _p(im2), _p(U), _p(V), H, W, _p(o1), _p(o2))
+    return o1, o2
+
+
+def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
+             prec="f32"):
+    """LS(HS) pyramid.  alphas: per level in the order the HS adapter pops them
+    (coarse level first).  filt_w / opt_w: pre-filter weights or None.
+    Returns (Uaccum, Vaccum, stats[levels, 3] = hs_err, ls_err, ls_iters)."""
+    im1 = _arr(im1, prec); im2 = _arr(im2, prec)
+    H, W = im1.shape
+    al = _arr(np.asarray(alphas), prec)
+    fw = _arr(filt_w if filt_w is not None else np.zeros(1), prec)
+    ow = _arr(opt_w if opt_w is not None else np.zeros(1), prec)
+    U = np.empty((H, W), _dt(prec)); V = np.empty((H, W), _dt(prec))
+    st = np.zeros((levels, 3), np.float64)
+    getattr(lib(), "orc_pyr_lshs_" + prec)(
+        _p(im1), _p(im2), H, W, levels, _p(al), int(niter), _p(fw), 0 if filt_w is None else len(fw),
+        _p(ow), 0 if opt_w is None else len(ow), ls_alpha if ls_alpha else 0.0, int(ls_maxnum),
+        float(ls_tol), _p(U), _p(V), st.ctypes.data_as(_dp))
+    return U, V, st

@@ -1,0 +1,257 @@
+"""Pins the CPU oracle against the library-probed golden vectors, brute force on
+tiny inputs and the mathematics (fixed points, closed forms).  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, -(a & 0x7FFFFFFF), a)
+    b = np.where(b < 0, -(b & 0x7FFFFFFF), b)
+    return np.abs(a - b).max()
+
+
+# --------------------------------------------------------------- stencils
+def test_hs_derivative_stencils_match_scipy(golden):
+    g = golden("stencils")
+    X = g["X"]
+    z = np.zeros_like(X)
+    fx, fy, ft = orc.hs_derivs(X, z)           # I0 = X, I1 = 0
+    assert np.array_equal(fx, g["hs_cx"])
+    assert np.array_equal(fy, g["hs_cy"])
+    assert np.array_equal(ft, g["hs_ct"])
+    fx, fy, ft = orc.hs_derivs(z, X)           # I1 = X: ft = -CT[X]
+    assert np.array_equal(fx, g["hs_cx"])
+    assert np.array_equal(ft, -g["hs_ct"])
+
+
+def test_hs_average_matches_scipy(golden):
+    g = golden("stencils")
+    Y = g["Y"]
+    z = np.zeros_like(Y)
+    un, vn = orc.hs_sweep(z, z, z, np.float32(21), Y, 2 * Y)   # der = 0 -> ubar
+    assert np.array_equal(un, g["hs_avg"])
+    assert np.array_equal(vn, (2 * Y).astype(np.float32) * 0 + orc.hs_sweep(z, z, z, 21, 2 * Y, Y)[0])
+
+
+def _ls_probe(Y, **fields):
+    H, W = Y.shape
+    base = {k: np.zeros((H, W), np.float32) for k in ("IIx", "IIy", "II", "Ixt", "Iyt", "B11", "B12", "B22")}
+    base["B11"][:] = -1.0                    # u' = -(B11 bu) = bu
+    base.update({k: np.full((H, W), v, np.float32) for k, v in fields.items()})
+    return base
+
+
+def test_ls_sweep_stencils_match_scipy(golden):
+    """Each LS sweep stencil isolated by choosing the coefficient fields so that
+    u' equals one term of bu (PhysicsBasedOpticalFlowLiuShen.py:142-144)."""
+    g = golden("stencils")
+    Y = g["Y"]
+    z = np.zeros_like(Y)
+    un, _, _ = orc.ls_sweep(_ls_probe(Y, IIx=0.5), 0, Y, z)          # 2*.5*Dr[u]
+    assert np.array_equal(un, g["ls_dr"])
+    un, _, _ = orc.ls_sweep(_ls_probe(Y, IIx=1.0), 0, z, Y)          # Dc[v]
+    assert np.array_equal(un, g["ls_dc"])
+    un, _, _ = orc.ls_sweep(_ls_probe(Y, IIy=1.0), 0, z, Y)          # Dr[v]
+    assert np.array_equal(un, g["ls_dr"])
+    un, _, _ = orc.ls_sweep(_ls_probe(Y, II=1.0), 0, Y, z)           # Fr[u]
+    assert np.array_equal(un, g["ls_fr"])
+    un, _, _ = orc.ls_sweep(_ls_probe(Y, II=1.0), 0, z, Y)           # M[v]
+    assert np.array_equal(un, g["ls_m"])
+    un, _, _ = orc.ls_sweep(_ls_probe(Y), 1, Y, z)                   # H0[u], zero padded
+    assert np.array_equal(un, g["ls_h0"])
+
+
+# --------------------------------------------------------------- solvers vs scipy-driven runs
+def test_hs_small_matches_scipy_driven_run(golden):
+    g = golden("hs_small")
+    fx, fy, ft = orc.hs_derivs(g["I0"], g["I1"])
+    assert np.array_equal(fx, g["fx"]) and np.array_equal(fy, g["fy"]) and np.array_equal(ft, g["ft"])
+    U, V, _ = orc.hs(g["I0"], g["I1"], g["alpha"], int(g["niter"]), g["U0"], g["V0"])
+    assert ulp_diff(U, g["U"]) <= 1 and ulp_diff(V, g["V"]) <= 1
+
+
+def test_ls_small_matches_scipy_driven_run(golden):
+    g = golden("ls_small")
+    f = orc.ls_setup(g["im1"], g["im2"], np.float32(g["h"]))
+    for k in ("IIx", "IIy", "II", "Ixt", "Iyt", "B11", "B12", "B22"):
+        assert ulp_diff(f[k], g[k]) <= 1, k
+    u, v = g["u0"], g["v0"]
+    errs = []
+    for _ in range(int(g["nsweep"])):
+        u, v, e = orc.ls_sweep(f, float(g["h"]), u, v)
+        errs.append(e)
+    assert ulp_diff(u, g["u"]) <= 2 and ulp_diff(v, g["v"]) <= 2
+    np.testing.assert_allclose(errs, g["errs"], rtol=1e-6)
+    # driver: fixed sweep count through maxnum, tolerance 0 never stops early
+    u2, v2, e2, it = orc.ls(g["im1"], g["im2"], float(g["h"]), g["u0"], g["v0"], maxnum=int(g["nsweep"]), tol=0.0)
+    assert it == int(g["nsweep"])
+    assert np.array_equal(u2, u) and np.array_equal(v2, v)
+
+
+def test_ls_stops_on_tolerance():
+    rng = np.random.default_rng(3)
+    a = (rng.random((20, 24)) * 200 + 10).astype(np.float32)
+    b = np.roll(a, 1, axis=1)
+    z = np.zeros_like(a)
+    _, _, e_all, it_all = orc.ls(a, b, 5.0, z, z, maxnum=60, tol=0.0)
+    assert it_all == 60
+    # a tolerance just above the 10th sweep's norm stops there
+    errs = []
+    u, v = z, z
+    f = orc.ls_setup(a, b, np.float32(5))
+    for _ in range(12):
+        u, v, e = orc.ls_sweep(f, 5.0, u, v)
+        errs.append(np.float32(e))
+    tol = float(errs[9]) * 1.0000001
+    _, _, e10, it10 = orc.ls(a, b, 5.0, z, z, maxnum=60, tol=tol)
+    assert it10 <= 10 and e10 <= tol
+
+
+# --------------------------------------------------------------- fixed points (mathematics)
+def test_hs_fixed_point_satisfies_discrete_equations():
+    """After many FP64 sweeps, (U, V) satisfy the HS Euler-Lagrange equations
+    (alpha^2 + fx^2 + fy^2) U = alpha^2 ... in Jacobi form: U = ubar - fx*der."""
+    rng = np.random.default_rng(5)
+    H, W = 12, 10
+    I0 = rng.random((H, W)) * 50
+    I1 = np.roll(I0, 1, axis=1) + rng.random((H, W))
+    U, V, _ = orc.hs(I0, I1, 3.0, 4000, prec="f64")
+    fx, fy, ft = orc.hs_derivs(I0, I1, prec="f64")
+    ub, vb = orc.hs_sweep(np.zeros((H, W)), np.zeros((H, W)), np.zeros((H, W)), 3.0, U, V, prec="f64")
+    # residual of  fx (fx U + fy V + ft) + alpha^2 (U - ubar) = 0
+    r = fx * (fx * U + fy * V + ft) + 9.0 * (U - ub)
+    assert np.abs(r).max() < 1e-8 * (1 + np.abs(fx).max() ** 2)
+    r = fy * (fx * U + fy * V + ft) + 9.0 * (V - vb)
+    assert np.abs(r).max() < 1e-8 * (1 + np.abs(fy).max() ** 2)
+
+
+def test_ls_fixed_point_solves_linear_system():
+    """LS Jacobi fixed point: A [u; v] = -b(u, v) per pixel, where the sweep is
+    u' = -(B bu): at convergence u = u'."""
+    rng = np.random.default_rng(6)
+    H, W = 6, 7
+    a = (rng.random((H, W)) * 200 + 10)
+    b = np.roll(a, 1, axis=0) * 0.98 + 3
+    z = np.zeros_like(a)
+    f = orc.ls_setup(a, b, 5.0, prec="f64")
+    # the sweep is affine, x' = G x + c: probe G column by column, solve directly
+    c = np.concatenate([t.ravel() for t in orc.ls_sweep(f, 5.0, z, z, prec="f64")[:2]])
+    n = H * W
+    G = np.zeros((2 * n, 2 * n))
+    for q in range(2 * n):
+        e = np.zeros(2 * n); e[q] = 1.0
+        un, vn, _ = orc.ls_sweep(f, 5.0, e[:n].reshape(H, W), e[n:].reshape(H, W), prec="f64")
+        G[:, q] = np.concatenate([un.ravel(), vn.ravel()]) - c
+    xs = np.linalg.solve(np.eye(2 * n) - G, c)
+    assert np.abs(np.linalg.eigvals(G)).max() < 1.0          # Jacobi contraction
+    u, v, e, it = orc.ls(a, b, 5.0, z, z, maxnum=5000, tol=1e-16, prec="f64")
+    np.testing.assert_allclose(np.concatenate([u.ravel(), v.ravel()]), xs, atol=1e-9)
+
+
+def test_f32_and_f64_instances_agree():
+    rng = np.random.default_rng(7)
+    I0 = (rng.random((32, 40)) * 255).astype(np.float32)
+    I1 = np.roll(I0, 1, axis=1)
+    U32, V32, e32 = orc.hs(I0, I1, 21.0, 200)
+    U64, V64, e64 = orc.hs(I0, I1, 21.0, 200, prec="f64")
+    assert np.abs(U32 - U64).max() < 1e-4 and np.abs(V32 - V64).max() < 1e-4
+    assert abs(e32 - e64) < 1e-6 * max(1, abs(e64)) + 1e-9
+
+
+# --------------------------------------------------------------- pre-filter brute force
+def _prefilter_literal(img, k):
+    """The reference's own construction (gaussian_filter.py:54-85): padded line
+    buffer, full 1-D convolution accumulated in float32, then slicing."""
+    img = img.astype(np.float32).copy()
+    n = len(k); h = n // 2
+    H, W = img.shape
+
+    def line(v):
+        L = len(v)
+        tmp = np.zeros(L + n - 1, np.float32)
+        tmp[h:h + L] = v
+        for j in range(h):
+            tmp[h - 1 - j] = v[j]
+            tmp[L + n - 2 - j] = v[L - 1 - j]
+        full = np.zeros(L + n - 1 + 2 * h, np.float32)
+        for i in range(len(tmp) + n - 1):
+            for j in range(n):
+                if 0 <= i - j < len(tmp):
+                    full[i] = np.float32(full[i] + np.float32(tmp[i - j] * k[j]))
+        res = full[h:h + len(tmp)]
+        return res[h:-h]
+    for i in range(H):
+        img[i, :] = line(img[i, :])
+    for j in range(W):
+        img[:, j] = line(img[:, j])
+    return img
+
+
+@pytest.mark.parametrize("key", ["s3.4_n3", "s0.48_n5", "s2_n3", "s2.5_n7"])
+def test_prefilter_matches_literal_construction(golden, key):
+    k = golden("gauss_w")[key]
+    rng = np.random.default_rng(8)
+    img = (rng.random((9, 13)) * 255).astype(np.float32)
+    ref = _prefilter_literal(img, k)
+    out = orc.gauss_sep(img, k)
+    assert np.array_equal(out, ref)
+
+
+def test_prefilter_constant_and_impulse():
+    k = np.array([0.25, 0.5, 0.25], np.float32)
+    out = orc.gauss_sep(np.full((6, 7), 8.0, np.float32), k)
+    assert np.all(out == 8.0)
+    imp = np.zeros((7, 7), np.float32); imp[3, 3] = 1
+    out = orc.gauss_sep(imp, k)
+    assert np.array_equal(out[2:5, 2:5], np.outer(k, k).astype(np.float32))
+
+
+# --------------------------------------------------------------- resize / spline / warp
+def test_resize_bicubic_matches_pillow(golden):
+    g = golden("pil")
+    for t, (H, W, H2, W2) in enumerate(g["cases"]):
+        out = orc.resize_bicubic(g["in%d" % t], int(H2), int(W2))
+        assert np.array_equal(out, g["bicubic%d" % t]), (H, W, H2, W2)
+
+
+def test_pil_coefficients_impulse(golden):
+    g = golden("pil")
+    k, b = orc.pil_coeffs(21, 10, 3)
+    # column response to an impulse at 10 equals the coefficient of input 10
+    resp = np.array([k[xx, 10 - b[xx, 0]] if 0 <= 10 - b[xx, 0] < b[xx, 1] else 0.0 for xx in range(10)])
+    assert np.array_equal(resp.astype(np.float32), g["impulse_bicubic"][0])
+
+
+def test_spline_matches_rectbivariatespline(golden):
+    g = golden("spline")
+    for t, (hc, wc, h, w) in enumerate(g["cases"]):
+        _, d64 = orc.spline_upsample(g["in%d" % t], int(h), int(w), 1.0, want_f64=True)
+        np.testing.assert_allclose(d64, g["out%d" % t], rtol=0, atol=1e-11)
+
+
+def test_spline_reproduces_cubics_and_interpolates():
+    n, N = 9, 18
+    Wm = orc.spline_matrix(n, N)
+    x = np.arange(n, dtype=np.float64)
+    u = np.minimum(np.arange(N) * n / N, n - 1)
+    for p in (lambda t: 1 + 0 * t, lambda t: t, lambda t: t ** 2 - 3 * t, lambda t: 0.1 * t ** 3 - t):
+        np.testing.assert_allclose(Wm @ p(x), p(u), atol=1e-10)
+    assert np.allclose(Wm[::2], np.eye(n)[: N // 2 + 0][: len(Wm[::2])], atol=1e-13)
+
+
+def test_warp_matches_numpy(golden):
+    g = golden("warp")
+    o1, o2 = orc.warp_symmetric(g["im1"], g["im2"], g["U"], g["V"])
+    assert np.array_equal(o1, g["o1"]) and np.array_equal(o2, g["o2"])
+
+
+def test_warp_zero_flow_is_identity():
+    rng = np.random.default_rng(9)
+    a = (rng.random((8, 9)) * 255).astype(np.float32)
+    z = np.zeros_like(a)
+    o1, o2 = orc.warp_symmetric(a, a * 2, z, z)
+    assert np.array_equal(o1, a) and np.array_equal(o2, a * 2)
