@@ -1,0 +1,157 @@
+"""Benchmark: LS(HS) 2-level dense flow at 1024x1024 on one MI355X.
+
+One step = one synthetic PIV pair (1024^2, Lamb-Oseen, density 0.04) solved
+coarse-to-fine: bicubic 1/2 downsample, FILTER 3.4 (3 taps) and HS with 600
+sweeps (alpha 45 at 512^2, 21 at 1024^2), FILTER_OPT 0.48 (5 taps) and
+Liu-Shen alpha 5 (tol 1e-8, at most 60 sweeps), spline upsample x2, symmetric
+bilinear warp, accumulation.  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  `value` is Mpix-pairs/s (finest-level H*W/1e6 per pair
+per second).  `roofline` is measured live for the dominant kernel (the
+temporally blocked Horn-Schunck sweep) with HIP events on its stream;
+`cpu_baseline` times the CPU oracle (the reference dataflow restated in C) on
+this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "opticalflow-ri_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+os.environ.setdefault("OFRI_VERBOSE", "0")
+
+import numpy as np  # noqa: E402
+
+METRIC = "Mpix-pairs/s to fixed residual, LiuShen-HS 1024² learned vs plain; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+HS_BYTES_PER_PX_SWEEP = 28.0     # read U, V, fx, fy, ft; write U, V
+HS_SETUP_BYTES_PER_PX = 20.0     # read 2 images, write fx, fy, ft
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per HS launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "hs_pass_1024_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(im1, im2, fw, ow):
+    from oracle import oracle as orc
+    threads = orc.get_threads()
+    t0 = time.perf_counter()
+    orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+    dt = time.perf_counter() - t0
+    H, W = im1.shape
+    return {"value": round(H * W / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": threads, "kind": "port",
+            "sample": "1 pair %dx%d LS(HS) 2-level, C oracle (float32 fields, float64 stencil sums), "
+                      "OpenMP %d threads, %.1f s" % (H, W, threads, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (pairs are independent, "
+              "see the sharding notes)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from gaussian_filter import prepareGaussianKernel
+    from ofri import ops, synth
+    from ofri.device import DeviceArray, context
+
+    ctx = context()
+    n = args.size
+    im1, im2, _, _ = synth.config_pair(4, size=n)
+    fw, ow = prepareGaussianKernel(3.4, 3), prepareGaussianKernel(0.48, 5)
+    d1, d2 = DeviceArray.from_host(im1), DeviceArray.from_host(im2)
+    Ua, Va = DeviceArray((n, n)), DeviceArray((n, n))
+
+    def step():
+        ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 600, fw, ow, 5.0, Uacc=Ua, Vacc=Va, stats=False)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.timer_start()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    gpu_ms = ctx.timer_stop()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    mpix = n * n / 1e6
+    value = mpix * args.steps / wall
+
+    # dominant kernel: HS sweeps at the finest level, timed alone on the context stream
+    depth = ctx.get_tuning()[0] or 6
+    niter = 600
+    launches = -(-niter // depth)
+    f1 = ops.gauss_sep(d1, fw)
+    f2 = ops.gauss_sep(d2, fw)
+    U, V = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
+    lib, h = ctx.lib, ctx.handle
+    lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)  # warm
+    reps = 3
+    ctx.timer_start()
+    for _ in range(reps):
+        lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)
+    hs_ms = ctx.timer_stop() / reps
+    alg_bytes = (niter * HS_BYTES_PER_PX_SWEEP + HS_SETUP_BYTES_PER_PX) * n * n
+    launch_ms = hs_ms / launches
+    achieved = alg_bytes / (hs_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+
+    # the headline run's per-level statistics (for the record)
+    _, _, st = ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpix-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "gpu_ms_per_step": round(gpu_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "LS(HS) 2-level %dx%d plain form: HS 600 sweeps (alpha 45 / 21), LS alpha 5 "
+                        "(tol 1e-8, max 60), FILTER 3.4/3 taps, FILTER_OPT 0.48/5 taps" % (n, n),
+            "pairs_per_step": 1, "height": n, "width": n, "levels": 2,
+            "ls_sweeps": [s["ls_iters"] for s in st], "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "hs_pass_k (temporally blocked HS sweep, %d sweeps/launch)" % depth,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "launch_ms": round(launch_ms, 5),
+            "algorithmic_bytes_per_launch": round(alg_bytes / launches),
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(im1, im2, fw, ow)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -98,9 +98,6 @@ def genericPyramidalOpticalFlow(im1, im2, FILTER, mainOFlowAlgoAdapter, pyramida
     """Coarse-to-fine optical flow with a main and an optional adapter
     (GenericPyramidalOpticalFlow.py:237-415).  Returns (Uaccum, Vaccum)."""
     main, optional = mainOFlowAlgoAdapter, optionalOFlowAlgoAdapter
-    scale = 1.0 / (2.0 ** (pyramidalLevels - 1))
-    if scale > 1.0:
-        raise Exception('Invalid scale level: ' + str(scale))
 
     if main.hasGenericPyramidalDefaults():
         defaults = main.getGenericPyramidalDefaults()
@@ -122,6 +119,10 @@ def genericPyramidalOpticalFlow(im1, im2, FILTER, mainOFlowAlgoAdapter, pyramida
                 else:
                     pyramidalScaling = val
 
+    if pyramidalLevels < 1:
+        # the reference's level loop is empty: nothing is computed
+        # (its 'Invalid scale level' check sits inside that loop)
+        return None, None
     im1 = np.ascontiguousarray(im1, dtype=np.float32)
     im2 = np.ascontiguousarray(im2, dtype=np.float32)
     H, W = im1.shape

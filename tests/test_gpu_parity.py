@@ -1,0 +1,338 @@
+"""GPU parity: libofri (through its C ABI) against the CPU oracle on the same
+seeded inputs.  Bar: bit-exact for weights, sampling indices and the
+deterministic stencil pipelines; flow max |du| <= 1e-3 px and mean-EPE
+difference <= 1e-4 px against the oracle's reference dataflow, with the
+FP32-vs-FP64 error printed for every solver result."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FLOW_TOL = 1e-3      # max |delta u| px, against the oracle's reference dataflow
+EPE_TOL = 1e-4       # mean end-point-error difference, px
+
+
+@pytest.fixture(scope="module")
+def ops():
+    from ofri import ops as _ops
+    from ofri.device import context
+    context()  # fails loudly if libofri.so or the GPU is missing
+    return _ops
+
+
+def _img(rng, H, W, lo=0.0, hi=255.0):
+    return (rng.random((H, W)) * (hi - lo) + lo).astype(np.float32)
+
+
+def _epe(u1, v1, u2, v2):
+    return np.sqrt((u1 - u2) ** 2 + (v1 - v2) ** 2)
+
+
+def report(name, U, V, U32, V32, U64, V64):
+    d32 = max(np.abs(U - U32).max(), np.abs(V - V32).max())
+    d64 = max(np.abs(U - U64).max(), np.abs(V - V64).max())
+    e64 = _epe(U, V, U64, V64).mean()
+    print("[%s] max|d| vs oracle-f32 %.3g px, vs oracle-f64 %.3g px (mean EPE to f64 %.3g px)" % (name, d32, d64, e64))
+    return d32, d64
+
+
+# ------------------------------------------------------------- pre-filter
+@pytest.mark.parametrize("shape", [(256, 256), (37, 53), (5, 9)])
+@pytest.mark.parametrize("sigma,n", [(3.4, 3), (0.48, 5), (2.0, 3), (1.8, 7)])
+def test_prefilter_bitexact(ops, golden, shape, sigma, n):
+    from gaussian_filter import prepareGaussianKernel
+    if min(shape) < n:
+        pytest.skip("image smaller than kernel")
+    w = prepareGaussianKernel(sigma, n)
+    rng = np.random.default_rng(1)
+    img = _img(rng, *shape)
+    out = ops.gauss_sep(img, w).download()
+    assert np.array_equal(out, orc.gauss_sep(img, w))
+
+
+def test_prefilter_dropin_mutates_and_returns(ops):
+    from gaussian_filter import gaussian_filterPx, prepareGaussianKernel
+    rng = np.random.default_rng(2)
+    img = _img(rng, 31, 40)
+    ref = orc.gauss_sep(img, prepareGaussianKernel(3.4, 3))
+    out = gaussian_filterPx(img, 3.4, 3)
+    assert out is img and np.array_equal(img, ref)
+
+
+# ------------------------------------------------------------- resize / spline / warp
+def test_resize_bicubic_matches_pillow_fixture(ops, golden):
+    g = golden("pil")
+    for t, (H, W, H2, W2) in enumerate(g["cases"]):
+        out = ops.resize(g["in%d" % t], int(H2), int(W2), ops.BICUBIC).download()
+        assert np.array_equal(out, g["bicubic%d" % t]), (H, W, H2, W2)
+        out = ops.resize(g["in%d" % t], int(H2), int(W2), ops.BILINEAR).download()
+        assert np.array_equal(out, g["bilinear%d" % t]), (H, W, H2, W2)
+
+
+@pytest.mark.parametrize("H,W", [(1024, 1024), (2048, 768), (385, 257)])
+def test_resize_half_matches_oracle(ops, H, W):
+    rng = np.random.default_rng(3)
+    img = _img(rng, H, W)
+    H2, W2 = int(np.round(H * 0.5)), int(np.round(W * 0.5))
+    assert np.array_equal(ops.resize(img, H2, W2).download(), orc.resize_bicubic(img, H2, W2))
+
+
+def test_spline_matches_scipy_fixture(ops, golden):
+    g = golden("spline")
+    for t, (hc, wc, h, w) in enumerate(g["cases"]):
+        out = ops.spline_upsample(g["in%d" % t], int(h), int(w), 1.0).download()
+        np.testing.assert_allclose(out, g["out%d" % t].astype(np.float32), rtol=2e-7, atol=1e-7)
+
+
+@pytest.mark.parametrize("hc,wc,h,w", [(512, 512, 1024, 1024), (96, 160, 192, 320), (100, 70, 201, 141)])
+def test_spline_matches_oracle(ops, hc, wc, h, w):
+    rng = np.random.default_rng(4)
+    c = (rng.standard_normal((hc, wc)) * 3).astype(np.float32)
+    sx = np.float32(w) / np.float32(wc)
+    out = ops.spline_upsample(c, h, w, sx).download()
+    ref = orc.spline_upsample(c, h, w, sx)
+    # double sums in a different order: at most one float32 rounding step apart
+    assert np.abs(out - ref).max() <= 4e-7 * max(1.0, np.abs(ref).max())
+    assert (out != ref).mean() < 1e-3
+
+
+def test_warp_matches_fixture_and_oracle(ops, golden):
+    g = golden("warp")
+    o1, o2 = ops.warp_symmetric(g["im1"], g["im2"], g["U"], g["V"])
+    assert np.array_equal(o1.download(), g["o1"]) and np.array_equal(o2.download(), g["o2"])
+    rng = np.random.default_rng(5)
+    H, W = 300, 411
+    a, b = _img(rng, H, W), _img(rng, H, W)
+    U = (rng.standard_normal((H, W)) * 4).astype(np.float32)
+    V = (rng.standard_normal((H, W)) * 4).astype(np.float32)
+    r1, r2 = orc.warp_symmetric(a, b, U, V)
+    o1, o2 = ops.warp_symmetric(a, b, U, V)
+    assert np.array_equal(o1.download(), r1) and np.array_equal(o2.download(), r2)
+
+
+def test_warp_coords_dropin(ops, golden):
+    from GenericPyramidalOpticalFlow import doBiLinearWarping
+    g = golden("warp")
+    H, W = g["im1"].shape
+    ys, xs = np.meshgrid(np.arange(H, dtype=np.int32), np.arange(W, dtype=np.int32), indexing="ij")
+    out = doBiLinearWarping(g["im1"], np.float32(ys - g["V"] / 2.0), np.float32(xs - g["U"] / 2.0), 1, "nearest")
+    assert np.array_equal(out, g["o1"])
+
+
+# ------------------------------------------------------------- Horn-Schunck
+def _piv(H, W, seed=7, shift=(1.3, 0.6)):
+    from ofri import synth
+    rng = np.random.default_rng(seed)
+    im1, im2, _, _ = synth.piv_pair(H, W, 0.05, lambda x, y: (shift[0] + 0 * x, shift[1] + 0 * y), rng)
+    return im1, im2
+
+
+@pytest.mark.parametrize("H,W,niter", [(256, 256, 600), (97, 131, 50), (12, 10, 7), (300, 70, 33)])
+def test_hs_matches_oracle(ops, H, W, niter):
+    from ofri.device import DeviceArray
+    im1, im2 = _piv(H, W)
+    U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+    err = ops.hs(im1, im2, 21.0, niter, U, V)
+    Ug, Vg = U.download(), V.download()
+    U32, V32, e32 = orc.hs(im1, im2, 21.0, niter)
+    U64, V64, _ = orc.hs(im1, im2, 21.0, niter, prec="f64")
+    d32, d64 = report("hs %dx%d/%d" % (H, W, niter), Ug, Vg, U32, V32, U64, V64)
+    assert d32 <= 1e-5
+    assert abs(_epe(Ug, Vg, U64, V64).mean() - _epe(U32, V32, U64, V64).mean()) <= EPE_TOL
+    assert d64 <= FLOW_TOL
+    assert abs(float(err) - e32) <= 1e-5 * abs(e32) + 1e-12
+
+
+def test_hs_nonzero_initial_flow_and_small_golden(ops, golden):
+    from ofri.device import DeviceArray
+    g = golden("hs_small")
+    U = DeviceArray.from_host(g["U0"]); V = DeviceArray.from_host(g["V0"])
+    ops.hs(g["I0"], g["I1"], float(g["alpha"]), int(g["niter"]), U, V)
+    assert np.abs(U.download() - g["U"]).max() <= 2e-6 * np.abs(g["U"]).max()
+    assert np.abs(V.download() - g["V"]).max() <= 2e-6 * np.abs(g["V"]).max()
+
+
+@pytest.mark.parametrize("depth", [1, 2, 5, 8, 12])
+def test_hs_temporal_depth_does_not_change_result(ops, depth):
+    from ofri.device import DeviceArray, context
+    im1, im2 = _piv(200, 333, seed=11)
+    ctx = context()
+    res = []
+    for d in (depth, 6):
+        ctx.set_tuning(hs_depth=d)
+        U = DeviceArray.zeros(im1.shape); V = DeviceArray.zeros(im1.shape)
+        ops.hs(im1, im2, 45.0, 61, U, V)
+        res.append((U.download(), V.download()))
+    ctx.set_tuning()
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_hs_dropin_matches_oracle(ops):
+    from HornSchunck import HS, HSOpticalFlowAlgoAdapter
+    im1, im2 = _piv(64, 80, seed=12)
+    z = np.zeros(im1.shape, np.float32)
+    U, V, err = HS(im1, im2, 21, 100, z, z)
+    U32, V32, e32 = orc.hs(im1, im2, 21.0, 100)
+    assert np.abs(U - U32).max() <= 1e-5 and np.abs(V - V32).max() <= 1e-5
+    ad = HSOpticalFlowAlgoAdapter([21, 45], 100)
+    U2, V2, _ = ad.compute(im1, im2, z, z)        # pops 45
+    assert ad.alphas == [21]
+    U45, _, _ = orc.hs(im1, im2, 45.0, 100)
+    assert np.abs(U2 - U45).max() <= 1e-5
+
+
+# ------------------------------------------------------------- Liu-Shen
+@pytest.mark.parametrize("H,W", [(256, 256), (131, 97), (9, 14)])
+def test_ls_matches_oracle(ops, H, W):
+    from ofri.device import DeviceArray
+    im1, im2 = _piv(H, W, seed=21)
+    rng = np.random.default_rng(22)
+    u0 = (rng.standard_normal((H, W)) * 0.3 + 0.6).astype(np.float32)
+    v0 = (rng.standard_normal((H, W)) * 0.3 + 1.3).astype(np.float32)
+    du, dv = DeviceArray.from_host(u0), DeviceArray.from_host(v0)
+    err, it = ops.ls(im1, im2, 5.0, du, dv)
+    u32, v32, e32, it32 = orc.ls(im1, im2, 5.0, u0, v0)
+    u64, v64, _, _ = orc.ls(im1, im2, 5.0, u0, v0, prec="f64")
+    ug, vg = du.download(), dv.download()
+    d32, d64 = report("ls %dx%d" % (H, W), ug, vg, u32, v32, u64, v64)
+    assert it == it32
+    assert d32 <= 1e-5
+    assert d64 <= FLOW_TOL
+    assert abs(float(err) - e32) <= 1e-5 * abs(e32)
+
+
+def test_ls_stops_inside_a_fused_launch(ops):
+    """A tolerance between the 8th and 7th sweep's norm stops after sweep 8,
+    which sits inside a fused launch: the re-run must give exactly 8 sweeps."""
+    from ofri.device import DeviceArray
+    H, W = 96, 120
+    im1, im2 = _piv(H, W, seed=23)
+    z = np.zeros((H, W), np.float32)
+    f = orc.ls_setup(im1, im2, np.float32(5))
+    u, v, errs = z, z, []
+    for _ in range(10):
+        u, v, e = orc.ls_sweep(f, 5.0, u, v)
+        errs.append(float(np.float32(e)))
+    assert errs[7] < errs[6]
+    tol = 0.5 * (errs[7] + errs[6])
+    u8, v8, e8, it8 = orc.ls(im1, im2, 5.0, z, z, maxnum=60, tol=tol)
+    assert it8 == 8
+    du, dv = DeviceArray.zeros((H, W)), DeviceArray.zeros((H, W))
+    err, it = ops.ls(im1, im2, 5.0, du, dv, maxnum=60, tol=tol)
+    assert it == 8
+    assert np.abs(du.download() - u8).max() <= 1e-5 and np.abs(dv.download() - v8).max() <= 1e-5
+    # maxnum smaller than the launch depth and zero
+    du, dv = DeviceArray.zeros((H, W)), DeviceArray.zeros((H, W))
+    err, it = ops.ls(im1, im2, 5.0, du, dv, maxnum=3, tol=0.0)
+    u3, v3, _, _ = orc.ls(im1, im2, 5.0, z, z, maxnum=3, tol=0.0)
+    assert it == 3 and np.abs(du.download() - u3).max() <= 1e-5
+    du = DeviceArray.from_host(u3)
+    err, it = ops.ls(im1, im2, 5.0, du, dv, maxnum=0)
+    assert it == 0 and np.array_equal(du.download(), u3)
+
+
+def test_ls_small_golden(ops, golden):
+    from ofri.device import DeviceArray
+    g = golden("ls_small")
+    du, dv = DeviceArray.from_host(g["u0"]), DeviceArray.from_host(g["v0"])
+    err, it = ops.ls(g["im1"], g["im2"], float(g["h"]), du, dv, maxnum=int(g["nsweep"]), tol=0.0)
+    assert it == int(g["nsweep"])
+    np.testing.assert_allclose(du.download(), g["u"], rtol=2e-6, atol=2e-6 * np.abs(g["u"]).max())
+    np.testing.assert_allclose(float(err), g["errs"][-1], rtol=1e-5)
+
+
+def test_ls_adapter_swaps_components(ops):
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    im1, im2 = _piv(40, 56, seed=24)
+    rng = np.random.default_rng(25)
+    U = rng.standard_normal(im1.shape).astype(np.float32)
+    V = rng.standard_normal(im1.shape).astype(np.float32)
+    resU, resV, err = LiuShenOpticalFlowAlgoAdapter(5).compute(im1, im2, U, V)
+    rv, ru, _, _ = orc.ls(im1, im2, 5.0, V, U)   # solver u = caller V
+    assert np.abs(resU - ru).max() <= 1e-5 and np.abs(resV - rv).max() <= 1e-5
+    with pytest.raises(TypeError):
+        LiuShenOpticalFlowAlgoAdapter(5).hasGenericPyramidalDefaults()
+
+
+# ------------------------------------------------------------- whole LS(HS) pyramid
+def _lshs_params():
+    from gaussian_filter import prepareGaussianKernel
+    return prepareGaussianKernel(3.4, 3), prepareGaussianKernel(0.48, 5)
+
+
+@pytest.mark.parametrize("size,levels", [(256, 2), (256, 1), (200, 3)])
+def test_pyramid_lshs_matches_oracle(ops, size, levels):
+    from ofri import synth
+    fw, ow = _lshs_params()
+    im1, im2, ut, vt = synth.config_pair(4, size=size)
+    alphas = [45.0, 21.0, 21.0][:levels] if levels > 1 else [21.0]
+    U, V, st = ops.pyr_lshs(im1, im2, levels, alphas, 600, fw, ow, 5.0)
+    Ug, Vg = U.download(), V.download()
+    U32, V32, s32 = orc.pyr_lshs(im1, im2, levels, alphas, 600, fw, ow, 5.0)
+    U64, V64, _ = orc.pyr_lshs(im1, im2, levels, alphas, 600, fw, ow, 5.0, prec="f64")
+    d32, d64 = report("lshs %d^2 L%d" % (size, levels), Ug, Vg, U32, V32, U64, V64)
+    assert d32 <= FLOW_TOL
+    epe_g, epe_o = _epe(Ug, Vg, ut, vt).mean(), _epe(U32, V32, ut, vt).mean()
+    assert abs(epe_g - epe_o) <= EPE_TOL
+    for l in range(levels):
+        assert st[l]["ls_iters"] == int(s32[l, 2])
+        assert abs(float(st[l]["hs_err"]) - s32[l, 0]) <= 1e-5 * abs(s32[l, 0]) + 1e-12
+    print("  EPE to truth: gpu %.4f px, oracle %.4f px" % (epe_g, epe_o))
+
+
+def test_dropin_driver_on_reference_piv_pair(ops, golden):
+    """The north-star harness (examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py)
+    on the reference's own TIFF pair, through the drop-in API."""
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from HornSchunck import HSOpticalFlowAlgoAdapter
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    g = golden("piv_pair")
+    Iold, Inew = g["im0"].astype(np.float32), g["im1"].astype(np.float32)
+    hs = HSOpticalFlowAlgoAdapter([21, 45], 600)
+    U, V = genericPyramidalOpticalFlow(Iold, Inew, 3.4, hs, 2, 1, 0.48, LiuShenOpticalFlowAlgoAdapter(5))
+    assert hs.alphas == []
+    fw, ow = _lshs_params()
+    U32, V32, _ = orc.pyr_lshs(Iold, Inew, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+    U64, V64, _ = orc.pyr_lshs(Iold, Inew, 2, [45.0, 21.0], 600, fw, ow, 5.0, prec="f64")
+    d32, _ = report("reference TIFF pair", U, V, U32, V32, U64, V64)
+    assert d32 <= FLOW_TOL
+    # Poiseuille profile: u(y) positive, peaked mid-channel
+    prof = U.mean(axis=1)
+    assert prof[200:312].mean() > prof[:40].mean()
+
+
+def test_generic_path_equals_device_path(ops):
+    """An adapter wrapper forces the level loop (numpy at the adapter
+    boundary); it must reproduce the single device-resident call."""
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from HornSchunck import HSOpticalFlowAlgoAdapter
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    from ofri import synth
+    im1, im2, _, _ = synth.config_pair(4, size=128)
+
+    class Wrapped(HSOpticalFlowAlgoAdapter):
+        pass
+
+    Ua, Va = genericPyramidalOpticalFlow(im1, im2, 3.4, HSOpticalFlowAlgoAdapter([21, 45], 200), 2, 1, 0.48,
+                                         LiuShenOpticalFlowAlgoAdapter(5))
+    Ub, Vb = genericPyramidalOpticalFlow(im1, im2, 3.4, Wrapped([21, 45], 200), 2, 1, 0.48,
+                                         LiuShenOpticalFlowAlgoAdapter(5))
+    assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
+
+
+def test_full_size_pipeline_properties(ops):
+    """1024^2 LS(HS) 2-level: deterministic across runs, finite, and close to
+    the synthetic ground truth away from the borders."""
+    from ofri import synth
+    fw, ow = _lshs_params()
+    im1, im2, ut, vt = synth.config_pair(4)
+    U1, V1, st = ops.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+    U2, V2, _ = ops.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+    a, b = U1.download(), V1.download()
+    assert np.array_equal(a, U2.download()) and np.array_equal(b, V2.download())
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    epe = _epe(a, b, ut, vt)[64:-64, 64:-64].mean()
+    print("  1024^2 EPE to truth %.4f px, stats %s" % (epe, st))
+    assert epe < 1.0
