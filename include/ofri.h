@@ -100,6 +100,28 @@ int ofri_hs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
 int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
             int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
 
+/* ---- Farneback ----------------------------------------------------------- */
+typedef struct ofri_fb_params {
+    int poly_n;               /* polyN (5 or 7) */
+    const float* g;           /* poly_n+1 taps of the expansion Gaussian, g[0] at the centre */
+    const float* xg;          /* x * g */
+    const float* xxg;         /* x^2 * g */
+    const float* ig;          /* 4 entries of the inverse Gram matrix: ig11, ig03, ig33, ig55 */
+    const float* pre_kernel;  /* pre_kh+1 half taps of the pre-blur (centre first) */
+    int pre_kh;
+    const float* win_kernel;  /* win_kh+1 half taps of the window blur */
+    int win_kh;
+    int iters;                /* flow iterations */
+} ofri_fb_params;
+
+/* One Farneback level (the reference's compute() with pyramidalLevels = 1):
+ * reflect-101 pre-blur, polynomial expansion of both frames, then `iters`
+ * rounds of window blur, flow update and matrices update.  d_flowx / d_flowy:
+ * initial flow in, result out.  Replaces Farneback_PyCL.compute
+ * (Farneback_PyCL.py:461-603) and the kernels of optical_flow_farneback.cl. */
+int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                   float* d_flowx, float* d_flowy, const ofri_fb_params* p);
+
 /* ---- whole LS(HS) pyramid ------------------------------------------------ */
 typedef struct ofri_lshs_params {
     int levels;            /* pyramidalLevels */

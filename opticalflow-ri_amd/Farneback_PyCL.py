@@ -1,0 +1,131 @@
+"""Drop-in for the reference's Farneback_PyCL.py: Farneback dense optical flow
+on the GPU (libofri ofri_farneback) instead of PyOpenCL.
+
+Reference: Farneback_PyCL.py:63-615 and optical_flow_farneback.cl.  The
+constructor keeps the reference's signature (platformID / deviceID are
+accepted and ignored: the context's GPU is used).  The expansion constants
+(FarnebackPrepareGaussian) and the blur kernels (getGaussianKernelBitExact)
+are formed on the host with the reference's dtype sequence; the device runs
+pre-blur, polynomial expansion and the flow iterations with all buffers
+resident, without the reference's per-kernel host copies.
+"""
+import numpy as np
+
+from GaussianKernelBitExact import getGaussianKernelBitExact
+from ofri import ops
+from ofri.device import DeviceArray
+
+FLT_EPSILON = 1.19209289550781250000000000000000000e-7
+
+
+class Farneback_PyCL(object):
+    """Farneback adapter for genericPyramidalOpticalFlow (Farneback_PyCL.py:63-615)."""
+
+    def __init__(self, windowSize=33, Niters=5, polyN=7, polySigma=1.5, useGaussian=True, pyrScale=0.5,
+                 pyramidalLevels=1, platformID=0, deviceID=0, provideGenericPyramidalDefaults=True):
+        self.useDouble = False
+        self.windowSize = windowSize
+        self.numIters = Niters
+        self.polyN = int(polyN)
+        self.polySigma = polySigma
+        self.useGaussianFilter = useGaussian
+        self.pyramidalLevels = pyramidalLevels - 1
+        self.fastPyramids = False
+        self.pyrScale = pyrScale
+        self.provideGenericPyramidalDefaults = provideGenericPyramidalDefaults
+        if windowSize & 1 == 0:
+            raise Exception('windowSize must be an odd value')
+        ops.log('Runnning on device: libofri (gfx950)')
+
+    # ---- host-side constants (Farneback_PyCL.py:123-208) -----------------
+    def FarnebackPrepareGaussian(self):
+        """Expansion taps g, xg, xxg (float32, 2n+1) and the four inverse Gram
+        entries (float64)."""
+        n = self.polyN
+        sigma = self.polySigma
+        if sigma < FLT_EPSILON:
+            sigma = n * 0.3
+        g = np.zeros(2 * n + 1, dtype=np.float32)
+        xg = np.zeros(2 * n + 1, dtype=np.float32)
+        xxg = np.zeros(2 * n + 1, dtype=np.float32)
+        total = np.float64(0.0)
+        for x in range(-n, n + 1):
+            g[x + n] = np.exp(-x * x / (2 * sigma * sigma))
+            total += g[x + n]
+        total = 1.0 / total
+        for x in range(-n, n + 1):
+            g[x + n] = np.float32(g[x + n] * total)
+            xg[x + n] = np.float32(x * g[x + n])
+            xxg[x + n] = np.float32(x * x * g[x + n])
+        G = np.zeros((6, 6), np.float64)
+        for y in range(-n, n + 1):
+            gy = g[y + n]
+            for x in range(-n, n + 1):
+                gx = g[x + n]
+                G[0, 0] += gy * gx
+                G[1, 1] += gy * gx * x * x
+                G[3, 3] += gy * gx * x * x * x * x
+                G[5, 5] += gy * gx * x * x * y * y
+        G[2, 2] = G[0, 3] = G[0, 4] = G[3, 0] = G[4, 0] = G[1, 1]
+        G[4, 4] = G[3, 3]
+        G[3, 4] = G[4, 3] = G[5, 5]
+        invG = np.linalg.inv(G)
+        return g, xg, xxg, invG[1, 1], invG[0, 3], invG[3, 3], invG[5, 5]
+
+    def setPolynomialExpansionConsts(self):
+        n = self.polyN
+        g, xg, xxg, i11, i03, i33, i55 = self.FarnebackPrepareGaussian()
+        self.matrixG = g[n:].reshape([1, n + 1]).copy()
+        self.matrixXG = xg[n:].reshape([1, n + 1]).copy()
+        self.matrixXXG = xxg[n:].reshape([1, n + 1]).copy()
+        self.matrixIGD = np.array([i11, i03, i33, i55], dtype=np.float64)
+        self.matrixIG = self.matrixIGD.astype(np.float32)
+
+    def getGaussianKernel(self, n, sigma):
+        _, kernel = getGaussianKernelBitExact(n, sigma)
+        return np.float32(kernel).reshape([1, n])
+
+    def setGaussianBlurKernel(self, smoothSize, sigma):
+        g = self.getGaussianKernel(smoothSize, sigma)
+        half = int(smoothSize / 2)
+        self.matrixGKernel = g[0, half:].reshape([1, half + 1]).copy()
+
+    def _consts(self):
+        self.setPolynomialExpansionConsts()
+        self.setGaussianBlurKernel(3, 0.0)                 # smoothSize 3, sigma 0 at scale 1
+        pre = self.matrixGKernel[0].copy()
+        self.setGaussianBlurKernel(self.windowSize, self.windowSize / 2 * 0.3)
+        win = self.matrixGKernel[0].copy()
+        return {"g": self.matrixG[0], "xg": self.matrixXG[0], "xxg": self.matrixXXG[0], "ig": self.matrixIG,
+                "pre_kernel": pre, "win_kernel": win, "iters": int(self.numIters)}
+
+    # ---- adapter protocol -------------------------------------------------
+    def compute(self, im1, im2, U, V):
+        assert self.polyN == 5 or self.polyN == 7
+        assert np.shape(im1) == np.shape(im2) and self.pyrScale < 1
+        assert np.shape(U) == np.shape(im1) and np.shape(V) == np.shape(im1)
+        if not self.useGaussianFilter:
+            raise NotImplementedError("box-filter window (useGaussian=False) is not implemented")
+        size = np.shape(im1)
+        levels, scale = 0, 1.0
+        while levels < self.pyramidalLevels:
+            scale *= self.pyrScale
+            if size[1] * scale < 32 or size[0] * scale < 32:
+                break
+            levels += 1
+        if levels > 0:
+            raise NotImplementedError("Farneback internal pyramid levels > 1 are not implemented")
+        fx = DeviceArray.from_host(np.ascontiguousarray(U, dtype=np.float32))
+        fy = DeviceArray.from_host(np.ascontiguousarray(V, dtype=np.float32))
+        ops.farneback(np.ascontiguousarray(im1, dtype=np.float32), np.ascontiguousarray(im2, dtype=np.float32),
+                      fx, fy, self._consts())
+        return fx.download(), fy.download(), 'Unknown'
+
+    def getAlgoName(self):
+        return 'Farneback CL'
+
+    def hasGenericPyramidalDefaults(self):
+        return self.provideGenericPyramidalDefaults
+
+    def getGenericPyramidalDefaults(self):
+        return {'warping': False, 'scaling': True}

@@ -269,6 +269,16 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
     return OFRI_OK;
 }
 
+int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                   float* d_flowx, float* d_flowy, const ofri_fb_params* p) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_flowx || !d_flowy || !p || !p->g || !p->xg || !p->xxg || !p->ig ||
+        !p->pre_kernel || !p->win_kernel)
+        return fail(OFRI_E_ARG, "ofri_farneback: null pointer");
+    if (p->iters < 0) return fail(OFRI_E_ARG, "ofri_farneback: negative iteration count");
+    return farneback(ctx, d_im1, d_im2, H, W, d_flowx, d_flowy, p);
+}
+
 int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                   const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
                   ofri_level_stats* h_stats) {

@@ -102,6 +102,10 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err);
 
+// ---- Farneback (k_fb.hip) ----
+int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* flx, float* fly,
+              const ofri_fb_params* p);
+
 // ---- Liu-Shen (k_ls.hip) ----
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl);

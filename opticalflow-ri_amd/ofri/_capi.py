@@ -45,6 +45,21 @@ class LevelStats(C.Structure):
                 ("ls_iters", C.c_int)]
 
 
+class FbParams(C.Structure):
+    _fields_ = [
+        ("poly_n", C.c_int),
+        ("g", C.POINTER(C.c_float)),
+        ("xg", C.POINTER(C.c_float)),
+        ("xxg", C.POINTER(C.c_float)),
+        ("ig", C.POINTER(C.c_float)),
+        ("pre_kernel", C.POINTER(C.c_float)),
+        ("pre_kh", C.c_int),
+        ("win_kernel", C.POINTER(C.c_float)),
+        ("win_kh", C.c_int),
+        ("iters", C.c_int),
+    ]
+
+
 class Tuning(C.Structure):
     _fields_ = [("hs_depth", C.c_int), ("ls_depth", C.c_int)]
 
@@ -76,6 +91,7 @@ SIGNATURES = {
     "ofri_accumulate": [_vp, _vp, _vp, _sz],
     "ofri_hs": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp, _fp],
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
+    "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],

@@ -100,6 +100,26 @@ def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
     return np.float32(err.value), it.value
 
 
+def farneback(im1, im2, flowx, flowy, consts):
+    """One Farneback level (Farneback_PyCL.py:461-603); flowx / flowy device
+    arrays updated in place.  consts: dict of float32 arrays g, xg, xxg (polyN+1
+    taps), ig (4), pre_kernel, win_kernel (half kernels, centre first), and
+    the int iters."""
+    im1, im2 = as_device(im1), as_device(im2)
+    arrs = {k: np.ascontiguousarray(consts[k], dtype=np.float32)
+            for k in ("g", "xg", "xxg", "ig", "pre_kernel", "win_kernel")}
+    p = _capi.FbParams()
+    p.poly_n = len(arrs["g"]) - 1
+    p.g, p.xg, p.xxg, p.ig = (_capi.fptr(arrs[k]) for k in ("g", "xg", "xxg", "ig"))
+    p.pre_kernel = _capi.fptr(arrs["pre_kernel"])
+    p.pre_kh = len(arrs["pre_kernel"]) - 1
+    p.win_kernel = _capi.fptr(arrs["win_kernel"])
+    p.win_kh = len(arrs["win_kernel"]) - 1
+    p.iters = int(consts["iters"])
+    _capi.check(_lib().ofri_farneback(_h(), im1.ptr, im2.ptr, im1.H, im1.W, flowx.ptr, flowy.ptr, C.byref(p)))
+    return flowx, flowy
+
+
 def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
              Uacc=None, Vacc=None, stats=True):
     """Whole LS(HS) pyramid on the device.  Returns (Uacc, Vacc, stats list)."""

@@ -210,3 +210,70 @@ def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum
         _p(ow), 0 if opt_w is None else len(ow), ls_alpha if ls_alpha else 0.0, int(ls_maxnum),
         float(ls_tol), _p(U), _p(V), st.ctypes.data_as(_dp))
     return U, V, st
+
+
+# ----------------------------------------------------------------- Farneback
+def _decl_fb(L):
+    for suf, P, R in (("f32", _fp, C.c_float), ("f64", _dp, C.c_double)):
+        getattr(L, "orc_fb_blur_" + suf).argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
+        getattr(L, "orc_fb_polyexp_" + suf).argtypes = [P, C.c_int, C.c_int, P, P, P, C.c_int, P, P]
+        getattr(L, "orc_fb_update_matrices_" + suf).argtypes = [P, P, P, P, C.c_int, C.c_int, P]
+        getattr(L, "orc_fb_update_flow_" + suf).argtypes = [P, C.c_int, C.c_int, P, P]
+        getattr(L, "orc_farneback_" + suf).argtypes = [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int, P, P,
+                                                       C.c_int, C.c_int, P, P]
+
+
+def _fblib():
+    L = lib()
+    if not getattr(L, "_fb_declared", False):
+        _decl_fb(L)
+        L._fb_declared = True
+    return L
+
+
+def fb_blur(src, half_kernel, planes=1, prec="f32"):
+    src = _arr(src, prec)
+    H = src.shape[0] // planes
+    W = src.shape[1]
+    gk = _arr(half_kernel, prec)
+    dst = np.empty_like(src)
+    getattr(_fblib(), "orc_fb_blur_" + prec)(_p(src), _p(dst), H, W, planes, _p(gk), len(gk) - 1)
+    return dst
+
+
+def fb_polyexp(src, g, xg, xxg, ig, prec="f32"):
+    src = _arr(src, prec)
+    H, W = src.shape
+    g, xg, xxg, ig = (_arr(a, prec) for a in (g, xg, xxg, ig))
+    dst = np.empty((5 * H, W), _dt(prec))
+    getattr(_fblib(), "orc_fb_polyexp_" + prec)(_p(src), H, W, _p(g), _p(xg), _p(xxg), len(g) - 1, _p(ig), _p(dst))
+    return dst
+
+
+def fb_update_matrices(fx, fy, R0, R1, prec="f32"):
+    fx, fy, R0, R1 = (_arr(a, prec) for a in (fx, fy, R0, R1))
+    H, W = fx.shape
+    M = np.empty((5 * H, W), _dt(prec))
+    getattr(_fblib(), "orc_fb_update_matrices_" + prec)(_p(fx), _p(fy), _p(R0), _p(R1), H, W, _p(M))
+    return M
+
+
+def fb_update_flow(M, prec="f32"):
+    M = _arr(M, prec)
+    H, W = M.shape[0] // 5, M.shape[1]
+    fx, fy = np.empty((H, W), _dt(prec)), np.empty((H, W), _dt(prec))
+    getattr(_fblib(), "orc_fb_update_flow_" + prec)(_p(M), H, W, _p(fx), _p(fy))
+    return fx, fy
+
+
+def farneback(im1, im2, U, V, consts, prec="f32"):
+    """One Farneback compute() at a single internal level.  consts: dict with
+    pre_kernel, g, xg, xxg, ig, win_kernel (half kernels) and iters."""
+    im1, im2 = _arr(im1, prec), _arr(im2, prec)
+    H, W = im1.shape
+    fx, fy = _arr(U, prec).copy(), _arr(V, prec).copy()
+    pk, g, xg, xxg, ig, wk = (_arr(consts[k], prec) for k in ("pre_kernel", "g", "xg", "xxg", "ig", "win_kernel"))
+    getattr(_fblib(), "orc_farneback_" + prec)(_p(im1), _p(im2), H, W, _p(pk), len(pk) - 1, _p(g), _p(xg), _p(xxg),
+                                                len(g) - 1, _p(ig), _p(wk), len(wk) - 1, int(consts["iters"]),
+                                                _p(fx), _p(fy))
+    return fx, fy

@@ -568,4 +568,194 @@ void NAME(orc_pyr_lshs)(const REAL* im1, const REAL* im2, int H, int W, int leve
     free(Uacc); free(Vacc);
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* Farneback (Farneback_PyCL.py:123-603, optical_flow_farneback.cl:72-429). */
+/* Float arithmetic as the OpenCL source is written, with the `acc +=      */
+/* a*b` accumulations fused (OpenCL contracts by default): FMA(a, b, c).    */
+/* Planes of the 5-plane buffers are stored one after another (5H x W).    */
+/* ------------------------------------------------------------------------ */
+static inline REAL NAME(fma_)(REAL a, REAL b, REAL c) {
+    return sizeof(REAL) == 4 ? (REAL)fmaf((float)a, (float)b, (float)c) : (REAL)fma(a, b, c);
+}
+/* reflect-101 of the OpenCL helpers idx_row_low / idx_row_high / idx_col (.cl:135-158) */
+static inline int NAME(r101_low)(int y, int last) { return abs(y) % (last + 1); }
+static inline int NAME(r101_high)(int y, int last) { return abs(last - abs(last - y)) % (last + 1); }
+static inline int NAME(r101_col)(int x, int last) {
+    return NAME(r101_low)(NAME(r101_high)(x, last), last);
+}
+
+/* gaussianBlur over `planes` stacked planes (.cl:160-253): vertical pass    */
+/* v = s*g0, v = fma(s[y-j] + s[y+j], g[j], v); then horizontal pass on the */
+/* vertical results with reflect-101 columns.  gk: kh+1 half-kernel taps.   */
+void NAME(orc_fb_blur)(const REAL* src, REAL* dst, int H, int W, int planes, const REAL* gk, int kh) {
+    REAL* t = (REAL*)malloc(sizeof(REAL) * (size_t)H * W);
+    for (int k = 0; k < planes; ++k) {
+        const REAL* s = src + (size_t)k * H * W;
+        REAL* d = dst + (size_t)k * H * W;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                REAL v = s[IDX(y, x)] * gk[0];
+                for (int j = 1; j <= kh; ++j) {
+                    REAL a = s[IDX(NAME(r101_low)(y - j, H - 1), x)] + s[IDX(NAME(r101_high)(y + j, H - 1), x)];
+                    v = NAME(fma_)(a, gk[j], v);
+                }
+                t[IDX(y, x)] = v;
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                REAL r = t[IDX(y, x)] * gk[0];
+                for (int i = 1; i <= kh; ++i) {
+                    REAL a = t[IDX(y, NAME(r101_col)(x - i, W - 1))] + t[IDX(y, NAME(r101_col)(x + i, W - 1))];
+                    r = NAME(fma_)(a, gk[i], r);
+                }
+                d[IDX(y, x)] = r;
+            }
+    }
+    free(t);
+}
+
+/* polynomialExpansion (.cl:72-133), clamped borders.  g/xg/xxg: n+1 taps,  */
+/* ig[4] = ig11, ig03, ig33, ig55.  dst: 5 planes.                           */
+void NAME(orc_fb_polyexp)(const REAL* src, int H, int W, const REAL* g, const REAL* xg,
+                          const REAL* xxg, int n, const REAL* ig, REAL* dst) {
+    REAL* r0 = (REAL*)malloc(sizeof(REAL) * (size_t)H * W * 3);
+    REAL* r1 = r0 + (size_t)H * W;
+    REAL* r2 = r1 + (size_t)H * W;
+    size_t N = (size_t)H * W;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            REAL a = src[IDX(y, x)] * g[0], b = 0, c = 0;
+            for (int k = 1; k <= n; ++k) {
+                REAL t0 = src[IDX(y - k < 0 ? 0 : y - k, x)];
+                REAL t1 = src[IDX(y + k > H - 1 ? H - 1 : y + k, x)];
+                a = NAME(fma_)(g[k], t0 + t1, a);
+                b = NAME(fma_)(xg[k], t1 - t0, b);
+                c = NAME(fma_)(xxg[k], t0 + t1, c);
+            }
+            r0[IDX(y, x)] = a; r1[IDX(y, x)] = b; r2[IDX(y, x)] = c;
+        }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            REAL b1 = g[0] * r0[IDX(y, x)], b3 = g[0] * r1[IDX(y, x)], b5 = g[0] * r2[IDX(y, x)];
+            REAL b2 = 0, b4 = 0, b6 = 0;
+            for (int k = 1; k <= n; ++k) {
+                int xp = x + k > W - 1 ? W - 1 : x + k, xm = x - k < 0 ? 0 : x - k;
+                REAL p0 = r0[IDX(y, xp)], m0 = r0[IDX(y, xm)];
+                REAL p1 = r1[IDX(y, xp)], m1 = r1[IDX(y, xm)];
+                REAL p2 = r2[IDX(y, xp)], m2 = r2[IDX(y, xm)];
+                b1 = NAME(fma_)(p0 + m0, g[k], b1);
+                b4 = NAME(fma_)(p0 + m0, xxg[k], b4);
+                b2 = NAME(fma_)(p0 - m0, xg[k], b2);
+                b3 = NAME(fma_)(p1 + m1, g[k], b3);
+                b6 = NAME(fma_)(p1 - m1, xg[k], b6);
+                b5 = NAME(fma_)(p2 + m2, g[k], b5);
+            }
+            size_t q = IDX(y, x);
+            dst[q] = b3 * ig[0];
+            dst[N + q] = b2 * ig[0];
+            dst[2 * N + q] = NAME(fma_)(b1, ig[1], b5 * ig[2]);
+            dst[3 * N + q] = NAME(fma_)(b1, ig[1], b4 * ig[2]);
+            dst[4 * N + q] = b6 * ig[3];
+        }
+    free(r0);
+}
+
+/* updateMatrices (.cl:256-348). */
+void NAME(orc_fb_update_matrices)(const REAL* fxw, const REAL* fyw, const REAL* R0, const REAL* R1,
+                                  int H, int W, REAL* M) {
+    static const float border[6] = {0.14f, 0.14f, 0.4472f, 0.4472f, 0.4472f, 1.f};
+    size_t N = (size_t)H * W;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t q = IDX(y, x);
+            REAL dx = fxw[q], dy = fyw[q];
+            REAL fx = (REAL)x + dx, fy = (REAL)y + dy;
+            REAL flx = floor(fx), fly = floor(fy);
+            int x1 = (int)flx, y1 = (int)fly;
+            fx -= (REAL)x1;
+            fy -= (REAL)y1;
+            REAL r2, r3, r4, r5, r6;
+            if (x1 >= 0 && y1 >= 0 && x1 < W - 1 && y1 < H - 1) {
+                REAL a00 = (1 - fx) * (1 - fy), a01 = fx * (1 - fy), a10 = (1 - fx) * fy, a11 = fx * fy;
+                REAL r[5];
+                for (int k = 0; k < 5; ++k) {
+                    const REAL* P = R1 + (size_t)k * N;
+                    REAL s = a00 * P[IDX(y1, x1)];
+                    s = NAME(fma_)(a01, P[IDX(y1, x1 + 1)], s);
+                    s = NAME(fma_)(a10, P[IDX(y1 + 1, x1)], s);
+                    s = NAME(fma_)(a11, P[IDX(y1 + 1, x1 + 1)], s);
+                    r[k] = s;
+                }
+                r2 = r[0]; r3 = r[1];
+                r4 = (R0[2 * N + q] + r[2]) * (REAL)0.5;
+                r5 = (R0[3 * N + q] + r[3]) * (REAL)0.5;
+                r6 = (R0[4 * N + q] + r[4]) * (REAL)0.25;
+            } else {
+                r2 = r3 = 0;
+                r4 = R0[2 * N + q];
+                r5 = R0[3 * N + q];
+                r6 = R0[4 * N + q] * (REAL)0.5;
+            }
+            r2 = (R0[q] - r2) * (REAL)0.5;
+            r3 = (R0[N + q] - r3) * (REAL)0.5;
+            r2 = r2 + NAME(fma_)(r4, dy, r6 * dx);
+            r3 = r3 + NAME(fma_)(r6, dy, r5 * dx);
+            int bx0 = x < 5 ? x : 5, by0 = y < 5 ? y : 5;
+            int bx1 = W - x - 1 < 5 ? W - x - 1 : 5, by1 = H - y - 1 < 5 ? H - y - 1 : 5;
+            REAL sc = (REAL)border[bx0] * (REAL)border[by0];
+            sc = sc * (REAL)border[bx1];
+            sc = sc * (REAL)border[by1];
+            r2 *= sc; r3 *= sc; r4 *= sc; r5 *= sc; r6 *= sc;
+            M[q] = NAME(fma_)(r4, r4, r6 * r6);
+            M[N + q] = (r4 + r5) * r6;
+            M[2 * N + q] = NAME(fma_)(r5, r5, r6 * r6);
+            M[3 * N + q] = NAME(fma_)(r4, r2, r6 * r3);
+            M[4 * N + q] = NAME(fma_)(r6, r2, r5 * r3);
+        }
+}
+
+/* updateFlow (.cl:408-429). */
+void NAME(orc_fb_update_flow)(const REAL* M, int H, int W, REAL* fxw, REAL* fyw) {
+    size_t N = (size_t)H * W;
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; ++q) {
+        REAL g11 = M[q], g12 = M[N + q], g22 = M[2 * N + q], h1 = M[3 * N + q], h2 = M[4 * N + q];
+        REAL det = NAME(fma_)(g11, g22, -(g12 * g12)) + (REAL)1e-3f;
+        REAL inv = (REAL)1 / det;
+        fxw[q] = NAME(fma_)(g11, h2, -(g12 * h1)) * inv;
+        fyw[q] = NAME(fma_)(g22, h1, -(g12 * h2)) * inv;
+    }
+}
+
+/* One Farneback compute() with pyramidalLevels = 1 (Farneback_PyCL.py:461-603):
+ * pre-blur both frames with pk (pkh+1 half taps), polynomial expansion,
+ * updateMatrices from the initial flow, then `iters` x [window blur (wk,
+ * wkh+1 half taps) -> updateFlow -> updateMatrices (all but the last)].
+ * fxw / fyw: initial flow in, result out. */
+void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const REAL* pk, int pkh,
+                         const REAL* g, const REAL* xg, const REAL* xxg, int n, const REAL* ig,
+                         const REAL* wk, int wkh, int iters, REAL* fxw, REAL* fyw) {
+    size_t N = (size_t)H * W;
+    REAL* buf = (REAL*)malloc(sizeof(REAL) * N * 21);
+    REAL *bl = buf, *RA = buf + N, *RB = RA + 5 * N, *M = RB + 5 * N, *Mb = M + 5 * N;
+    NAME(orc_fb_blur)(im1, bl, H, W, 1, pk, pkh);
+    NAME(orc_fb_polyexp)(bl, H, W, g, xg, xxg, n, ig, RA);
+    NAME(orc_fb_blur)(im2, bl, H, W, 1, pk, pkh);
+    NAME(orc_fb_polyexp)(bl, H, W, g, xg, xxg, n, ig, RB);
+    NAME(orc_fb_update_matrices)(fxw, fyw, RA, RB, H, W, M);
+    for (int i = 0; i < iters; ++i) {
+        NAME(orc_fb_blur)(M, Mb, H, W, 5, wk, wkh);
+        REAL* t = M; M = Mb; Mb = t;
+        NAME(orc_fb_update_flow)(M, H, W, fxw, fyw);
+        if (i < iters - 1) NAME(orc_fb_update_matrices)(fxw, fyw, RA, RB, H, W, M);
+    }
+    free(buf);
+}
+
 #undef IDX
