@@ -157,8 +157,10 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
 
 /* Tuning knobs of the temporally blocked sweeps (0 = library default). */
 typedef struct ofri_tuning {
-    int hs_depth;   /* sweeps fused per launch, Horn-Schunck */
-    int ls_depth;   /* sweeps fused per launch, Liu-Shen */
+    int hs_depth;    /* sweeps fused per launch, Horn-Schunck */
+    int ls_depth;    /* sweeps fused per launch, Liu-Shen */
+    int hs_variant;  /* Horn-Schunck tile shape index (0 = default) */
+    int ls_variant;  /* Liu-Shen tile shape index (0 = default) */
 } ofri_tuning;
 int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t);
 int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t);

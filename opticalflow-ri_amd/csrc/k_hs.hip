@@ -71,33 +71,42 @@ struct HsPass {
     float a2;     // alpha*alpha in float32
 };
 
+// Reflect-101 index of any integer position (period 2n-2): the extended
+// tile is the mirror extension of the image, as scipy's 'mirror' mode sees it.
+__device__ __forceinline__ int mirror_idx(int p, int n) {
+    const int period = 2 * n - 2;
+    p %= period;
+    if (p < 0) p += period;
+    return p < n ? p : period - p;
+}
+
+// A row's value at this lane and its left / right lane neighbours, in double.
 struct Trip {
-    float l, c, r;  // left / centre / right neighbour of one row, mirrored at the image edge
+    double l, c, r;
 };
 
-__device__ __forceinline__ Trip trip(float x, bool edge_l, bool edge_r) {
-    Trip t;
-    float l = from_left(x), r = from_right(x);
-    t.c = x;
-    t.l = edge_l ? r : l;   // column -1 mirrors to column 1
-    t.r = edge_r ? l : r;   // column W mirrors to column W-2
-    return t;
+__device__ __forceinline__ Trip trip(float x) {
+    return Trip{(double)from_left(x), (double)x, (double)from_right(x)};
 }
 
 // Average of the 8 neighbours with weights wc (corners) and we (edges),
-// summed in double and rounded to float.
+// summed in double and rounded to float.  Every addition pairs mirror-image
+// operands, so a ghost cell's average equals its mirror pixel's bit for bit.
 __device__ __forceinline__ float hs_avg(const Trip& up, const Trip& mid, const Trip& dn) {
     const double wc = (double)(float)(1.0 / 12.0);
     const double we = (double)(float)(1.0 / 6.0);
-    double vl = (double)up.l + (double)dn.l;
-    double vc = (double)up.c + (double)dn.c;
-    double vr = (double)up.r + (double)dn.r;
+    double vl = up.l + dn.l;
+    double vc = up.c + dn.c;
+    double vr = up.r + dn.r;
     double corners = vl + vr;
-    double edges = vc + ((double)mid.l + (double)mid.r);
+    double edges = vc + (mid.l + mid.r);
     double s = wc * corners + we * edges;
     return (float)s;
 }
 
+// The extended tile is loaded as the mirror extension of the image (every
+// field, including fx, fy, ft), so the image border needs no special case:
+// the mirror symmetry of the extension is preserved exactly by every sweep.
 template <int R, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     const int lane = threadIdx.x & 63;
@@ -106,15 +115,12 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
     const int gx = (int)blockIdx.x * outW - K + lane;
     const int gy0 = (int)blockIdx.y * outH - K + w * R;
-    const int cx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
-    const bool edge_l = gx == 0, edge_r = gx == W - 1;
+    const int cx = mirror_idx(gx, W);
 
     float U[R], V[R], FX[R], FY[R], FT[R], DEN[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        int gy = gy0 + r;
-        int cy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
-        size_t q = (size_t)cy * W + cx;
+        size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
         U[r] = a.Uin[q];
         V[r] = a.Vin[q];
         FX[r] = a.fx[q];
@@ -137,21 +143,14 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : U[R - 1];
         float vdn = w < WAVES - 1 ? edge[b][w + 1][0][1][lane] : V[R - 1];
 
-        Trip up_u = trip(uup, edge_l, edge_r), up_v = trip(vup, edge_l, edge_r);
-        Trip cu = trip(U[0], edge_l, edge_r), cv = trip(V[0], edge_l, edge_r);
+        Trip up_u = trip(uup), up_v = trip(vup);
+        Trip cu = trip(U[0]), cv = trip(V[0]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float nxu = r + 1 < R ? U[r + 1] : udn;
-            float nxv = r + 1 < R ? V[r + 1] : vdn;
-            Trip dn_u = trip(nxu, edge_l, edge_r), dn_v = trip(nxv, edge_l, edge_r);
-            const int gy = gy0 + r;
-            // mirror at the top / bottom image rows (wave-uniform test)
-            const Trip& tu = gy == 0 ? dn_u : up_u;
-            const Trip& tv = gy == 0 ? dn_v : up_v;
-            const Trip& bu = gy == H - 1 ? up_u : dn_u;
-            const Trip& bv = gy == H - 1 ? up_v : dn_v;
-            float ub = hs_avg(tu, cu, bu);
-            float vb = hs_avg(tv, cv, bv);
+            Trip dn_u = trip(r + 1 < R ? U[r + 1] : udn);
+            Trip dn_v = trip(r + 1 < R ? V[r + 1] : vdn);
+            float ub = hs_avg(up_u, cu, dn_u);
+            float vb = hs_avg(up_v, cv, dn_v);
             float x = FX[r], y = FY[r];
             float num = x * ub;
             num = num + y * vb;
@@ -178,8 +177,18 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     }
 }
 
-constexpr int kR = 16;
-constexpr int kWaves = 4;
+// Tile shapes (rows per lane, wavefronts per workgroup) compiled in; the
+// tuning picks one by index.
+struct HsVariant {
+    int R, waves;
+    void (*kernel)(HsPass);
+};
+const HsVariant kHsVariants[] = {
+    {16, 4, hs_pass_k<16, 4>}, {8, 4, hs_pass_k<8, 4>}, {8, 8, hs_pass_k<8, 8>},
+    {12, 4, hs_pass_k<12, 4>}, {16, 2, hs_pass_k<16, 2>}, {4, 8, hs_pass_k<4, 8>},
+    {24, 2, hs_pass_k<24, 2>}, {12, 8, hs_pass_k<12, 8>},
+};
+constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
 constexpr int kDefaultDepth = 6;
 
 }  // namespace
@@ -207,13 +216,17 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
     int depth = ctx->tuning.hs_depth > 0 ? ctx->tuning.hs_depth : kDefaultDepth;
-    if (depth > 24) depth = 24;
+    int vi = ctx->tuning.hs_variant;
+    if (vi < 0 || vi >= kNumHsVariants) vi = 0;
+    const HsVariant& var = kHsVariants[vi];
+    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
+        return fail(OFRI_E_ARG, "hs: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
     HsPass p{};
     p.fx = fx; p.fy = fy; p.ft = ft;
     p.H = H; p.W = W;
     p.a2 = alpha * alpha;
     p.K = depth;
-    const int outW = 64 - 2 * depth, outH = kWaves * kR - 2 * depth;
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
     dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
     float *cu = U, *cv = V, *nu = Ub, *nv = Vb;
     int done = 0;
@@ -221,7 +234,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         int ns = niter - done < depth ? niter - done : depth;
         p.Uin = cu; p.Vin = cv; p.Uout = nu; p.Vout = nv;
         p.nsweeps = ns;
-        hs_pass_k<kR, kWaves><<<grid, 64 * kWaves, 0, ctx->stream>>>(p);
+        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_HIP(hipGetLastError());
         std::swap(cu, nu);
         std::swap(cv, nv);

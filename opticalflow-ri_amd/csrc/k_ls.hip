@@ -96,48 +96,88 @@ struct Raw {
 
 __device__ __forceinline__ Raw raw(float x) { return Raw{from_left(x), x, from_right(x)}; }
 
-template <int R, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
-    int nsw = a.nsweeps, src = a.src;
-    if (a.redo) {
-        nsw = a.ctrl->redo;
-        src = a.ctrl->redo_in;
-        if (nsw <= 0) return;
-    } else if (a.ctrl->done) {
-        return;
-    }
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int K = a.K, H = a.H, W = a.W;
-    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
-    const int gx = (int)blockIdx.x * outW - K + lane;
-    const int gy0 = (int)blockIdx.y * outH - K + w * R;
-    const int cx = clampi(gx, 0, W - 1);
-    const bool el = gx == 0, er = gx == W - 1;
-    const bool own_x = lane >= K && lane < 64 - K && gx < W;
-    const size_t N = a.N;
+// One LS update at a pixel from the 3x3 neighbourhoods of u and v (rows
+// pu/cu/nu etc.).  EDGE adds the 'nearest' / 'constant' boundary handling;
+// tiles that do not touch the image border skip it.
+//   Dr, Dc, Fr, Fc: two-term sums scaled by a power of two are formed in
+//   float32, which rounds exactly as the double sum rounded to float does;
+//   M and the 8-neighbour sum H0 are summed in double as the reference does.
+template <bool EDGE>
+__device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Raw& nu, const Raw& pv,
+                                          const Raw& cv, const Raw& nv, bool top, bool bot, bool el,
+                                          bool er, float h, float iix, float iiy, float ii2, float ixt,
+                                          float iyt, float b11, float b12, float b22, float& un,
+                                          float& vn) {
+    // clamped rows / columns ('nearest')
+    const Raw& uu = (EDGE && top) ? cu : pu;
+    const Raw& ud = (EDGE && bot) ? cu : nu;
+    const Raw& vu = (EDGE && top) ? cv : pv;
+    const Raw& vd = (EDGE && bot) ? cv : nv;
+    float uu_l = (EDGE && el) ? uu.c : uu.l, uu_r = (EDGE && er) ? uu.c : uu.r;
+    float ud_l = (EDGE && el) ? ud.c : ud.l, ud_r = (EDGE && er) ? ud.c : ud.r;
+    float vu_l = (EDGE && el) ? vu.c : vu.l, vu_r = (EDGE && er) ? vu.c : vu.r;
+    float vd_l = (EDGE && el) ? vd.c : vd.l, vd_r = (EDGE && er) ? vd.c : vd.r;
+    float cu_l = (EDGE && el) ? cu.c : cu.l, cu_r = (EDGE && er) ? cu.c : cu.r;
+    float cv_l = (EDGE && el) ? cv.c : cv.l, cv_r = (EDGE && er) ? cv.c : cv.r;
+    float dru = 0.5f * (ud.c - uu.c);
+    float dcu = 0.5f * (cu_r - cu_l);
+    float drv = 0.5f * (vd.c - vu.c);
+    float dcv = 0.5f * (cv_r - cv_l);
+    float fru = uu.c + ud.c;
+    float fcv = cv_l + cv_r;
+    float mu = (float)(0.25 * (double)uu_l - 0.25 * (double)uu_r - 0.25 * (double)ud_l + 0.25 * (double)ud_r);
+    float mv = (float)(0.25 * (double)vu_l - 0.25 * (double)vu_r - 0.25 * (double)vd_l + 0.25 * (double)vd_r);
+    // 8-neighbour sums with zero outside the image ('constant')
+    float zl_pu = (EDGE && (el || top)) ? 0.0f : pu.l, zc_pu = (EDGE && top) ? 0.0f : pu.c;
+    float zr_pu = (EDGE && (er || top)) ? 0.0f : pu.r;
+    float zl_nu = (EDGE && (el || bot)) ? 0.0f : nu.l, zc_nu = (EDGE && bot) ? 0.0f : nu.c;
+    float zr_nu = (EDGE && (er || bot)) ? 0.0f : nu.r;
+    float zl_pv = (EDGE && (el || top)) ? 0.0f : pv.l, zc_pv = (EDGE && top) ? 0.0f : pv.c;
+    float zr_pv = (EDGE && (er || top)) ? 0.0f : pv.r;
+    float zl_nv = (EDGE && (el || bot)) ? 0.0f : nv.l, zc_nv = (EDGE && bot) ? 0.0f : nv.c;
+    float zr_nv = (EDGE && (er || bot)) ? 0.0f : nv.r;
+    float zl_cu = (EDGE && el) ? 0.0f : cu.l, zr_cu = (EDGE && er) ? 0.0f : cu.r;
+    float zl_cv = (EDGE && el) ? 0.0f : cv.l, zr_cv = (EDGE && er) ? 0.0f : cv.r;
+    double hu = 0.0;
+    hu = hu + (double)zl_pu; hu = hu + (double)zc_pu; hu = hu + (double)zr_pu;
+    hu = hu + (double)zl_cu; hu = hu + (double)zr_cu;
+    hu = hu + (double)zl_nu; hu = hu + (double)zc_nu; hu = hu + (double)zr_nu;
+    double hv = 0.0;
+    hv = hv + (double)zl_pv; hv = hv + (double)zc_pv; hv = hv + (double)zr_pv;
+    hv = hv + (double)zl_cv; hv = hv + (double)zr_cv;
+    hv = hv + (double)zl_nv; hv = hv + (double)zc_nv; hv = hv + (double)zr_nv;
+    float h0u = (float)hu, h0v = (float)hv;
+    float bu = (2.0f * iix) * dru;
+    bu = bu + iix * dcv;
+    bu = bu + iiy * drv;
+    bu = bu + ii2 * fru;
+    bu = bu + ii2 * mv;
+    bu = bu + h * h0u;
+    bu = bu + ixt;
+    float bv = iiy * dru;
+    bv = bv + iix * dcu;
+    bv = bv + (2.0f * iiy) * dcv;
+    bv = bv + ii2 * mu;
+    bv = bv + ii2 * fcv;
+    bv = bv + h * h0v;
+    bv = bv + iyt;
+    float t1 = b11 * bu, t2 = b12 * bv;
+    un = -(t1 + t2);
+    float t3 = b12 * bu, t4 = b22 * bv;
+    vn = -(t3 + t4);
+}
 
-    float u[R], v[R], IIX[R], IIY[R], II[R], IXT[R], IYT[R], B11[R], B12[R], B22[R];
-    const float* uin = a.in[src][0];
-    const float* vin = a.in[src][1];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        int cy = clampi(gy0 + r, 0, H - 1);
-        size_t q = (size_t)cy * W + cx;
-        u[r] = uin[q];
-        v[r] = vin[q];
-        IIX[r] = a.F[q];
-        IIY[r] = a.F[N + q];
-        II[r] = a.F[2 * N + q];
-        IXT[r] = a.F[3 * N + q];
-        IYT[r] = a.F[4 * N + q];
-        B11[r] = a.F[5 * N + q];
-        B12[r] = a.F[6 * N + q];
-        B22[r] = a.F[7 * N + q];
-    }
-
+template <int R, int WAVES, bool EDGE>
+__device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int src, int lane, int w, int gx,
+                                          int gy0, float (&u)[R], float (&v)[R], const float (&IIX)[R],
+                                          const float (&IIY)[R], const float (&II)[R], const float (&IXT)[R],
+                                          const float (&IYT)[R], const float (&B11)[R], const float (&B12)[R],
+                                          const float (&B22)[R]) {
     __shared__ float edge[2][WAVES][2][2][64];
     __shared__ double red[WAVES][2];
+    const int K = a.K, H = a.H, W = a.W;
+    const bool el = gx == 0, er = gx == W - 1;
+    const bool own_x = lane >= K && lane < 64 - K && gx < W;
     const float h = a.h;
     const int nblk = gridDim.x * gridDim.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
@@ -160,64 +200,9 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
             Raw nu = raw(r + 1 < R ? u[r + 1] : udn);
             Raw nv = raw(r + 1 < R ? v[r + 1] : vdn);
             const int gy = gy0 + r;
-            const bool top = gy == 0, bot = gy == H - 1;
-            // clamped rows ('nearest') and zero-padded rows ('constant')
-            const Raw& uu = top ? cu : pu;
-            const Raw& ud = bot ? cu : nu;
-            const Raw& vu = top ? cv : pv;
-            const Raw& vd = bot ? cv : nv;
-            // clamped columns
-            float uu_l = el ? uu.c : uu.l, uu_r = er ? uu.c : uu.r;
-            float ud_l = el ? ud.c : ud.l, ud_r = er ? ud.c : ud.r;
-            float vu_l = el ? vu.c : vu.l, vu_r = er ? vu.c : vu.r;
-            float vd_l = el ? vd.c : vd.l, vd_r = er ? vd.c : vd.r;
-            float cu_l = el ? cu.c : cu.l, cu_r = er ? cu.c : cu.r;
-            float cv_l = el ? cv.c : cv.l, cv_r = er ? cv.c : cv.r;
-            float dru = (float)(-0.5 * (double)uu.c + 0.5 * (double)ud.c);
-            float dcu = (float)(-0.5 * (double)cu_l + 0.5 * (double)cu_r);
-            float drv = (float)(-0.5 * (double)vu.c + 0.5 * (double)vd.c);
-            float dcv = (float)(-0.5 * (double)cv_l + 0.5 * (double)cv_r);
-            float fru = (float)(1.0 * (double)uu.c + 1.0 * (double)ud.c);
-            float fcv = (float)(1.0 * (double)cv_l + 1.0 * (double)cv_r);
-            float mu = (float)(0.25 * (double)uu_l - 0.25 * (double)uu_r - 0.25 * (double)ud_l +
-                               0.25 * (double)ud_r);
-            float mv = (float)(0.25 * (double)vu_l - 0.25 * (double)vu_r - 0.25 * (double)vd_l +
-                               0.25 * (double)vd_r);
-            // 8-neighbour sums with zero outside the image
-            float zl_pu = (el || top) ? 0.0f : pu.l, zc_pu = top ? 0.0f : pu.c, zr_pu = (er || top) ? 0.0f : pu.r;
-            float zl_nu = (el || bot) ? 0.0f : nu.l, zc_nu = bot ? 0.0f : nu.c, zr_nu = (er || bot) ? 0.0f : nu.r;
-            float zl_pv = (el || top) ? 0.0f : pv.l, zc_pv = top ? 0.0f : pv.c, zr_pv = (er || top) ? 0.0f : pv.r;
-            float zl_nv = (el || bot) ? 0.0f : nv.l, zc_nv = bot ? 0.0f : nv.c, zr_nv = (er || bot) ? 0.0f : nv.r;
-            float zl_cu = el ? 0.0f : cu.l, zr_cu = er ? 0.0f : cu.r;
-            float zl_cv = el ? 0.0f : cv.l, zr_cv = er ? 0.0f : cv.r;
-            double hu = 0.0;
-            hu = hu + (double)zl_pu; hu = hu + (double)zc_pu; hu = hu + (double)zr_pu;
-            hu = hu + (double)zl_cu; hu = hu + (double)zr_cu;
-            hu = hu + (double)zl_nu; hu = hu + (double)zc_nu; hu = hu + (double)zr_nu;
-            double hv = 0.0;
-            hv = hv + (double)zl_pv; hv = hv + (double)zc_pv; hv = hv + (double)zr_pv;
-            hv = hv + (double)zl_cv; hv = hv + (double)zr_cv;
-            hv = hv + (double)zl_nv; hv = hv + (double)zc_nv; hv = hv + (double)zr_nv;
-            float h0u = (float)hu, h0v = (float)hv;
-            float iix = IIX[r], iiy = IIY[r], ii2 = II[r];
-            float bu = (2.0f * iix) * dru;
-            bu = bu + iix * dcv;
-            bu = bu + iiy * drv;
-            bu = bu + ii2 * fru;
-            bu = bu + ii2 * mv;
-            bu = bu + h * h0u;
-            bu = bu + IXT[r];
-            float bv = iiy * dru;
-            bv = bv + iix * dcu;
-            bv = bv + (2.0f * iiy) * dcv;
-            bv = bv + ii2 * mu;
-            bv = bv + ii2 * fcv;
-            bv = bv + h * h0v;
-            bv = bv + IYT[r];
-            float t1 = B11[r] * bu, t2 = B12[r] * bv;
-            float un = -(t1 + t2);
-            float t3 = B12[r] * bu, t4 = B22[r] * bv;
-            float vn = -(t3 + t4);
+            float un, vn;
+            ls_update<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r], II[r],
+                            IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
             if (a.part) {
                 const int ry = w * R + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
@@ -254,10 +239,56 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
             }
         }
     }
+}
+
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
+    int nsw = a.nsweeps, src = a.src;
+    if (a.redo) {
+        nsw = a.ctrl->redo;
+        src = a.ctrl->redo_in;
+        if (nsw <= 0) return;
+    } else if (a.ctrl->done) {
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K = a.K, H = a.H, W = a.W;
+    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
+    const int x0 = (int)blockIdx.x * outW - K, y0 = (int)blockIdx.y * outH - K;
+    const int gx = x0 + lane;
+    const int gy0 = y0 + w * R;
+    const int cx = clampi(gx, 0, W - 1);
+    const size_t N = a.N;
+    // does any pixel of the extended tile lie on the image border?
+    const bool edge_tile = x0 <= 0 || y0 <= 0 || x0 + 63 >= W - 1 || y0 + WAVES * R - 1 >= H - 1;
+
+    float u[R], v[R], IIX[R], IIY[R], II[R], IXT[R], IYT[R], B11[R], B12[R], B22[R];
+    const float* uin = a.in[src][0];
+    const float* vin = a.in[src][1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int cy = clampi(gy0 + r, 0, H - 1);
+        size_t q = (size_t)cy * W + cx;
+        u[r] = uin[q];
+        v[r] = vin[q];
+        IIX[r] = a.F[q];
+        IIY[r] = a.F[N + q];
+        II[r] = a.F[2 * N + q];
+        IXT[r] = a.F[3 * N + q];
+        IYT[r] = a.F[4 * N + q];
+        B11[r] = a.F[5 * N + q];
+        B12[r] = a.F[6 * N + q];
+        B22[r] = a.F[7 * N + q];
+    }
+    if (edge_tile)
+        ls_sweeps<R, WAVES, true>(a, nsw, src, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12, B22);
+    else
+        ls_sweeps<R, WAVES, false>(a, nsw, src, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12, B22);
 
     float* uo = a.out[src ^ 1][0];
     float* vo = a.out[src ^ 1][1];
-    if (own_x) {
+    if (lane >= K && lane < 64 - K && gx < W) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             int ry = w * R + r, gy = gy0 + r;
@@ -324,8 +355,14 @@ __global__ void ctrl_init_k(SolveCtrl* c) {
     c->last_err = 0.0f;
 }
 
-constexpr int kR = 8;
-constexpr int kWaves = 8;
+struct LsVariant {
+    int R, waves;
+    void (*kernel)(LsPass);
+};
+const LsVariant kLsVariants[] = {
+    {8, 8, ls_pass_k<8, 8>}, {8, 4, ls_pass_k<8, 4>}, {4, 8, ls_pass_k<4, 8>}, {12, 4, ls_pass_k<12, 4>},
+};
+constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
 constexpr int kDefaultDepth = 6;
 
 }  // namespace
@@ -336,8 +373,12 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
     size_t N = (size_t)H * W;
     int depth = ctx->tuning.ls_depth > 0 ? ctx->tuning.ls_depth : kDefaultDepth;
-    if (depth > 24) depth = 24;
-    const int outW = 64 - 2 * depth, outH = kWaves * kR - 2 * depth;
+    int vi = ctx->tuning.ls_variant;
+    if (vi < 0 || vi >= kNumLsVariants) vi = 0;
+    const LsVariant& var = kLsVariants[vi];
+    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
+        return fail(OFRI_E_ARG, "ls: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
     dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
     int nblk = grid.x * grid.y;
     void *fbuf, *pbuf, *mbuf;
@@ -367,7 +408,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.nsweeps = ns;
         p.src = src;
         p.redo = 0;
-        ls_pass_k<kR, kWaves><<<grid, 64 * kWaves, 0, ctx->stream>>>(p);
+        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         ls_check_k<<<1, 256, 0, ctx->stream>>>(p.part, nblk, ns, src ^ 1, tol, maxnum, N, d_ctrl);
         OFRI_HIP(hipGetLastError());
         src ^= 1;
@@ -376,7 +417,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (maxnum > 0) {
         p.redo = 1;
         p.part = nullptr;
-        ls_pass_k<kR, kWaves><<<grid, 64 * kWaves, 0, ctx->stream>>>(p);
+        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_TRY(select_copy(ctx, d_ctrl, u, ub, u, N));
         OFRI_TRY(select_copy(ctx, d_ctrl, v, vb, v, N));
     }

@@ -185,6 +185,8 @@ int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t) {
     if (!t) return fail(OFRI_E_ARG, "ofri_set_tuning: null");
     if (t->hs_depth < 0 || t->hs_depth > 24 || t->ls_depth < 0 || t->ls_depth > 24)
         return fail(OFRI_E_ARG, "ofri_set_tuning: depth outside 0..24");
+    if (t->hs_variant < 0 || t->hs_variant >= 64 || t->ls_variant < 0 || t->ls_variant >= 64)
+        return fail(OFRI_E_ARG, "ofri_set_tuning: variant index outside 0..63");
     ctx->tuning = *t;
     return OFRI_OK;
 }

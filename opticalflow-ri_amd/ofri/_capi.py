@@ -61,7 +61,7 @@ class FbParams(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("hs_depth", C.c_int), ("ls_depth", C.c_int)]
+    _fields_ = [("hs_depth", C.c_int), ("ls_depth", C.c_int), ("hs_variant", C.c_int), ("ls_variant", C.c_int)]
 
 
 _vp = C.c_void_p

@@ -33,14 +33,14 @@ class Context:
             self.lib.ofri_destroy(self.handle)
             self.handle = None
 
-    def set_tuning(self, hs_depth=0, ls_depth=0):
-        t = _capi.Tuning(int(hs_depth), int(ls_depth))
+    def set_tuning(self, hs_depth=0, ls_depth=0, hs_variant=0, ls_variant=0):
+        t = _capi.Tuning(int(hs_depth), int(ls_depth), int(hs_variant), int(ls_variant))
         _capi.check(self.lib.ofri_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self):
         t = _capi.Tuning()
         _capi.check(self.lib.ofri_get_tuning(self.handle, C.byref(t)))
-        return t.hs_depth, t.ls_depth
+        return t.hs_depth, t.ls_depth, t.hs_variant, t.ls_variant
 
     def timer_start(self):
         _capi.check(self.lib.ofri_timer_start(self.handle))

@@ -758,4 +758,96 @@ void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const R
     free(buf);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Dense pyramidal-LK, one level (denseLucasKanade_PyCL.py:113-169 with     */
+/* pyrlkDenseLargeW.cl:304-669), symmetric window of 2*hw+1 pixels.         */
+/* For output pixel (i, j): window pixels (i-hw+r, j-hw+c); the patch I and */
+/* its Scharr derivatives come from clamp-to-edge reads (the LINEAR sampler */
+/* at pixel centres), Dx = 3(I[-1,+1]+I[+1,+1]-I[-1,-1]-I[+1,-1]) +         */
+/* 10(I[0,+1]-I[0,-1]), Dy likewise; A = sum of the derivative products;   */
+/* D = A11 A22 - A12^2, stop (status 0, flow unchanged) if D < FLT_EPSILON; */
+/* up to niter Gauss-Newton steps with J sampled bilinearly (OpenCL LINEAR, */
+/* clamp-to-edge) at the displaced window, delta = 32 A^-1 b, early exit    */
+/* when |delta| < 0.01 in both components.  Window sums in double.          */
+/* ------------------------------------------------------------------------ */
+static inline REAL NAME(lk_tex)(const REAL* T, int H, int W, REAL px, REAL py) {
+    /* OpenCL LINEAR filter at unnormalised coordinate (px + .5, py + .5) */
+    REAL fx = floor(px), fy = floor(py);
+    REAL a = px - fx, b = py - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = NAME(clp)(x0, W); x1 = NAME(clp)(x1, W);
+    y0 = NAME(clp)(y0, H); y1 = NAME(clp)(y1, H);
+    REAL t = (1 - a) * (1 - b) * T[IDX(y0, x0)];
+    t = t + a * (1 - b) * T[IDX(y0, x1)];
+    t = t + (1 - a) * b * T[IDX(y1, x0)];
+    t = t + a * b * T[IDX(y1, x1)];
+    return t;
+}
+
+void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int niter, REAL* u, REAL* v,
+                        unsigned char* status) {
+    const int win = 2 * hw + 1;
+#pragma omp parallel
+    {
+        REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)win * win * 3);
+        REAL *Dx = P + (size_t)win * win, *Dy = Dx + (size_t)win * win;
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < H; ++i)
+            for (int j = 0; j < W; ++j) {
+                size_t g = IDX(i, j);
+                double a11 = 0, a12 = 0, a22 = 0;
+                for (int r = 0; r < win; ++r)
+                    for (int c = 0; c < win; ++c) {
+                        int y = i - hw + r, x = j - hw + c;
+                        int ym = NAME(clp)(y - 1, H), y0 = NAME(clp)(y, H), yp = NAME(clp)(y + 1, H);
+                        int xm = NAME(clp)(x - 1, W), x0 = NAME(clp)(x, W), xp = NAME(clp)(x + 1, W);
+                        REAL dx = (REAL)3 * (I[IDX(ym, xp)] + I[IDX(yp, xp)] - I[IDX(ym, xm)] - I[IDX(yp, xm)]) +
+                                  (REAL)10 * (I[IDX(y0, xp)] - I[IDX(y0, xm)]);
+                        REAL dy = (REAL)3 * (I[IDX(yp, xm)] + I[IDX(yp, xp)] - I[IDX(ym, xm)] - I[IDX(ym, xp)]) +
+                                  (REAL)10 * (I[IDX(yp, x0)] - I[IDX(ym, x0)]);
+                        size_t k = (size_t)r * win + c;
+                        P[k] = I[IDX(y0, x0)];
+                        Dx[k] = dx;
+                        Dy[k] = dy;
+                        a11 += (double)dx * dx;
+                        a12 += (double)dx * dy;
+                        a22 += (double)dy * dy;
+                    }
+                REAL A11 = (REAL)a11, A12 = (REAL)a12, A22 = (REAL)a22;
+                REAL D = A11 * A22 - A12 * A12;
+                if (status) status[g] = 1;
+                if (D < (REAL)1.192092896e-07f) {
+                    if (status) status[g] = 0;
+                    continue;
+                }
+                A11 /= D; A12 /= D; A22 /= D;
+                REAL px = (REAL)j + u[g] - (REAL)hw, py = (REAL)i + v[g] - (REAL)hw;
+                for (int k = 0; k < niter; ++k) {
+                    if (px < (REAL)-hw || px >= (REAL)W || py < (REAL)-hw || py >= (REAL)H) {
+                        if (status) status[g] = 0;
+                        break;
+                    }
+                    double b1 = 0, b2 = 0;
+                    for (int r = 0; r < win; ++r)
+                        for (int c = 0; c < win; ++c) {
+                            size_t q = (size_t)r * win + c;
+                            REAL diff = NAME(lk_tex)(J, H, W, px + (REAL)c, py + (REAL)r) - P[q];
+                            b1 += (double)diff * Dx[q];
+                            b2 += (double)diff * Dy[q];
+                        }
+                    REAL B1 = (REAL)b1, B2 = (REAL)b2;
+                    REAL ddx = (A12 * B2 - A22 * B1) * (REAL)32;
+                    REAL ddy = (A12 * B1 - A11 * B2) * (REAL)32;
+                    px += ddx;
+                    py += ddy;
+                    if (fabs((double)ddx) < 0.01 && fabs((double)ddy) < 0.01) break;
+                }
+                u[g] = px + (REAL)hw - (REAL)j;
+                v[g] = py + (REAL)hw - (REAL)i;
+            }
+        free(P);
+    }
+}
+
 #undef IDX
