@@ -98,7 +98,7 @@ def main():
     value = mpix * args.steps / wall
 
     # dominant kernel: HS sweeps at the finest level, timed alone on the context stream
-    depth = ctx.get_tuning()[0] or 6
+    depth, _, hs_variant, _ = ctx.query_tuning(n, n)
     niter = 600
     launches = -(-niter // depth)
     f1 = ops.gauss_sep(d1, fw)
@@ -140,7 +140,7 @@ def main():
             "ls_sweeps": [s["ls_iters"] for s in st], "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm", "kernel": "hs_pass_k (temporally blocked HS sweep, %d sweeps/launch)" % depth,
+            "bound": "hbm", "kernel": "hs_pass_k (temporally blocked HS sweep, tile variant %d, %d sweeps/launch)" % (hs_variant, depth),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,

@@ -155,15 +155,25 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
                   const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
                   ofri_level_stats* h_stats);
 
-/* Tuning knobs of the temporally blocked sweeps (0 = library default). */
+/* Tuning knobs of the temporally blocked sweeps.  0 = the library's
+ * per-image-size default (measured on gfx950); a variant k >= 1 selects the
+ * k-th compiled tile shape. */
 typedef struct ofri_tuning {
     int hs_depth;    /* sweeps fused per launch, Horn-Schunck */
     int ls_depth;    /* sweeps fused per launch, Liu-Shen */
-    int hs_variant;  /* Horn-Schunck tile shape index (0 = default) */
-    int ls_variant;  /* Liu-Shen tile shape index (0 = default) */
+    int hs_variant;  /* Horn-Schunck tile shape, 1-based (0 = default) */
+    int ls_variant;  /* Liu-Shen tile shape, 1-based (0 = default) */
 } ofri_tuning;
 int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t);
 int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t);
+/* The depths and 1-based tile shapes the sweeps actually use for an H x W image. */
+int ofri_query_tuning(ofri_ctx* ctx, int H, int W, ofri_tuning* out);
+
+/* HIP-graph replay of ofri_pyr_lshs (default on): the first call with a given
+ * set of pointers, sizes, parameters and tuning runs eagerly, the second
+ * captures the launch sequence, later calls replay it.  A graph is rebuilt
+ * after any internal scratch buffer is reallocated. */
+int ofri_set_graphs(ofri_ctx* ctx, int on);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,17 @@ __device__ __forceinline__ float hs_avg(const Trip& up, const Trip& mid, const T
     return (float)s;
 }
 
+// num / den correctly rounded, from rcp = RN(1/den): q = RN(num*rcp) is within
+// an ulp of the quotient and one FMA residual correction rounds it exactly
+// (Markstein's theorem for round-to-nearest binary arithmetic).  Tiny
+// numerators, where the residual could underflow, take the IEEE division.
+__device__ __forceinline__ float quotient(float num, float den, float rcp) {
+    if (__builtin_expect(fabsf(num) < 0x1p-100f, 0)) return num / den;
+    float q = num * rcp;
+    float res = fmaf(-den, q, num);
+    return fmaf(res, rcp, q);
+}
+
 // The extended tile is loaded as the mirror extension of the image (every
 // field, including fx, fy, ft), so the image border needs no special case:
 // the mirror symmetry of the extension is preserved exactly by every sweep.
@@ -117,7 +128,7 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     const int gy0 = (int)blockIdx.y * outH - K + w * R;
     const int cx = mirror_idx(gx, W);
 
-    float U[R], V[R], FX[R], FY[R], FT[R], DEN[R];
+    float U[R], V[R], FX[R], FY[R], FT[R], DEN[R], RCP[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
@@ -128,6 +139,7 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
         FT[r] = a.ft[q];
         float d = a.a2 + FX[r] * FX[r];
         DEN[r] = d + FY[r] * FY[r];
+        RCP[r] = 1.0f / DEN[r];   // correctly rounded reciprocal
     }
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
             float num = x * ub;
             num = num + y * vb;
             num = num + FT[r];
-            float der = num / DEN[r];
+            float der = quotient(num, DEN[r], RCP[r]);
             U[r] = ub - x * der;
             V[r] = vb - y * der;
             up_u = cu; up_v = cv;
@@ -189,7 +201,18 @@ const HsVariant kHsVariants[] = {
     {24, 2, hs_pass_k<24, 2>}, {12, 8, hs_pass_k<12, 8>},
 };
 constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
-constexpr int kDefaultDepth = 6;
+// Defaults measured by the autotuner on gfx950 (per-sweep time, 600 sweeps):
+// large images favour 12-row lanes in 8-wavefront tiles, small ones 4-row
+// lanes (more workgroups to spread over the 256 CUs).
+void hs_default(size_t npx, int* variant, int* depth) {
+    if (npx >= (size_t)700000) {
+        *variant = 7;
+        *depth = 5;
+    } else {
+        *variant = 5;
+        *depth = 5;
+    }
+}
 
 }  // namespace
 
@@ -215,9 +238,11 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    int depth = ctx->tuning.hs_depth > 0 ? ctx->tuning.hs_depth : kDefaultDepth;
-    int vi = ctx->tuning.hs_variant;
-    if (vi < 0 || vi >= kNumHsVariants) vi = 0;
+    int vi, depth;
+    hs_default((size_t)H * W, &vi, &depth);
+    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
+    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
     const HsVariant& var = kHsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
         return fail(OFRI_E_ARG, "hs: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
@@ -248,4 +273,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     return OFRI_OK;
 }
 
+}  // namespace ofri
+
+namespace ofri {
+// Effective Horn-Schunck launch shape for an image of npx pixels (1-based variant).
+void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
+    int vi, d;
+    hs_default(npx, &vi, &d);
+    if (ctx->tuning.hs_depth > 0) d = ctx->tuning.hs_depth;
+    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    *variant1 = vi + 1;
+    *depth = d;
+}
 }  // namespace ofri

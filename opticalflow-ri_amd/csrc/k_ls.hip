@@ -7,13 +7,14 @@
 // Horn-Schunck (64-column extended tiles, R rows per lane in registers with
 // the eight coefficients, DPP for horizontal neighbours, LDS for the edge rows
 // of each wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries
-// as explicit selects.  Each launch also reduces, per fused sweep, the
-// squared update of the pixels it owns; a one-block check kernel turns the
-// per-sweep sums into the reference's norm (||du||_2 + ||dv||_2)/(r*c),
-// applies `while err > tol and k < maxnum` on the device, and marks a
-// re-run of the stopping launch with fewer sweeps when the test held inside
-// it.  The launch sequence is fixed, so a whole solve can be captured in a
-// HIP graph.
+// as explicit selects in the workgroups whose tile touches the image border
+// (interior tiles skip them).  Each launch also reduces, per fused sweep, the
+// squared update of the pixels it owns; the last workgroup to finish turns
+// the per-sweep sums into the reference's norm (||du||_2 + ||dv||_2)/(r*c),
+// applies `while err > tol and k < maxnum` on the device, and marks a re-run
+// of the stopping launch with fewer sweeps when the test held inside it.
+// The launch sequence is fixed, so a whole solve can be captured in a HIP
+// graph.
 #include <cmath>
 
 #include "ofri_internal.h"
@@ -86,9 +87,14 @@ struct LsPass {
     int src;                // ping-pong index read (normal launch)
     float h;
     double* part;           // [K][nblocks][2] squared-update sums (nullptr: none)
+    double tol;             // stopping tolerance on the update norm
+    int maxnum;             // sweep budget
     SolveCtrl* ctrl;
     int redo;               // 1: re-run launch driven by ctrl->redo
 };
+
+__device__ void ls_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
+                         size_t N, SolveCtrl* ctrl);
 
 struct Raw {
     float l, c, r;
@@ -299,26 +305,49 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
             }
         }
     }
+    if (a.part) {
+        // the last workgroup to finish runs the convergence test
+        __shared__ int last;
+        const int nblk = gridDim.x * gridDim.y;
+        if (threadIdx.x == 0) {
+            __threadfence();
+            last = atomicAdd(&a.ctrl->arrived, 1) == nblk - 1;
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            ls_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl);
+            if (threadIdx.x == 0) a.ctrl->arrived = 0;
+        }
+    }
 }
 
 // Convergence test of one launch: `while err > tol and k < maxnum`
-// (PhysicsBasedOpticalFlowLiuShen.py:141,150; helper :79).
-__global__ void ls_check_k(const double* __restrict__ part, int nblk, int nsw, int dst, double tol,
-                           int maxnum, size_t N, SolveCtrl* ctrl) {
-    __shared__ double sa[256], sb[256];
+// (PhysicsBasedOpticalFlowLiuShen.py:141,150; helper :79).  Run by the last
+// workgroup of the launch to finish, with all its threads: the per-sweep
+// squared-update sums of every workgroup are added in a fixed order, the
+// float32 norm of each sweep is tested in turn, and the control word records
+// the sweep count, the last norm, the buffer holding the result and, when
+// the test held before the launch's last sweep, how many sweeps to re-run.
+__device__ void ls_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
+                         size_t N, SolveCtrl* ctrl) {
+    __shared__ double sa[512], sb[512];
     __shared__ int stop;
-    if (ctrl->done) return;
+    const int T = blockDim.x;
     if (threadIdx.x == 0) stop = 0;
     for (int s = 0; s < nsw; ++s) {
         double a = 0.0, b = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-            a = a + part[((size_t)s * nblk + i) * 2];
-            b = b + part[((size_t)s * nblk + i) * 2 + 1];
+        for (int i = threadIdx.x; i < nblk; i += T) {
+            // other workgroups' sums: read at device scope, past this CU's cache
+            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            b = b + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2 + 1, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
         }
         sa[threadIdx.x] = a;
         sb[threadIdx.x] = b;
         __syncthreads();
-        for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        for (int k = T / 2; k > 0; k >>= 1) {
             if ((int)threadIdx.x < k) {
                 sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + k];
                 sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + k];
@@ -353,6 +382,7 @@ __global__ void ctrl_init_k(SolveCtrl* c) {
     c->redo = 0;
     c->redo_in = 0;
     c->last_err = 0.0f;
+    c->arrived = 0;
 }
 
 struct LsVariant {
@@ -363,7 +393,16 @@ const LsVariant kLsVariants[] = {
     {8, 8, ls_pass_k<8, 8>}, {8, 4, ls_pass_k<8, 4>}, {4, 8, ls_pass_k<4, 8>}, {12, 4, ls_pass_k<12, 4>},
 };
 constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
-constexpr int kDefaultDepth = 6;
+// Defaults measured by the autotuner on gfx950.
+void ls_default(size_t npx, int* variant, int* depth) {
+    if (npx >= (size_t)700000) {
+        *variant = 3;
+        *depth = 5;
+    } else {
+        *variant = 2;
+        *depth = 5;
+    }
+}
 
 }  // namespace
 
@@ -372,9 +411,11 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
     size_t N = (size_t)H * W;
-    int depth = ctx->tuning.ls_depth > 0 ? ctx->tuning.ls_depth : kDefaultDepth;
-    int vi = ctx->tuning.ls_variant;
-    if (vi < 0 || vi >= kNumLsVariants) vi = 0;
+    int vi, depth;
+    ls_default((size_t)H * W, &vi, &depth);
+    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
+    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
     const LsVariant& var = kLsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
         return fail(OFRI_E_ARG, "ls: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
@@ -402,6 +443,8 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.F = F; p.N = N; p.H = H; p.W = W; p.K = depth; p.h = h;
     p.part = (double*)pbuf;
     p.ctrl = d_ctrl;
+    p.tol = tol;
+    p.maxnum = maxnum;
     int done = 0, src = 0;
     while (done < maxnum) {
         int ns = maxnum - done < depth ? maxnum - done : depth;
@@ -409,7 +452,6 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.src = src;
         p.redo = 0;
         var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
-        ls_check_k<<<1, 256, 0, ctx->stream>>>(p.part, nblk, ns, src ^ 1, tol, maxnum, N, d_ctrl);
         OFRI_HIP(hipGetLastError());
         src ^= 1;
         done += ns;
@@ -424,4 +466,16 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     return OFRI_OK;
 }
 
+}  // namespace ofri
+
+namespace ofri {
+// Effective Liu-Shen launch shape for an image of npx pixels (1-based variant).
+void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
+    int vi, d;
+    ls_default(npx, &vi, &d);
+    if (ctx->tuning.ls_depth > 0) d = ctx->tuning.ls_depth;
+    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    *variant1 = vi + 1;
+    *depth = d;
+}
 }  // namespace ofri

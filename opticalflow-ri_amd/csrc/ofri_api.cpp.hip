@@ -37,6 +37,7 @@ int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out) {
             return fail(OFRI_E_NOMEM, "scratch '%s': cannot allocate %zu bytes", name.c_str(), sz);
         }
         b.bytes = sz;
+        ++ctx->ws_gen;
     }
     *out = b.p;
     return OFRI_OK;
@@ -105,6 +106,8 @@ int ofri_destroy(ofri_ctx* ctx) {
         (void)hipFree(kv.second.d_w);
         (void)hipFree(kv.second.d_start);
     }
+    for (auto& kv : ctx->graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -198,6 +201,15 @@ int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t) {
     return OFRI_OK;
 }
 
+int ofri_query_tuning(ofri_ctx* ctx, int H, int W, ofri_tuning* out) {
+    CHECK_CTX(ctx);
+    if (!out || H < 1 || W < 1) return fail(OFRI_E_ARG, "ofri_query_tuning: bad argument");
+    size_t n = (size_t)H * W;
+    hs_effective(ctx, n, &out->hs_variant, &out->hs_depth);
+    ls_effective(ctx, n, &out->ls_variant, &out->ls_depth);
+    return OFRI_OK;
+}
+
 int ofri_gauss_sep(ofri_ctx* ctx, const float* d_src, float* d_dst, int H, int W, const float* h_w,
                    int n) {
     CHECK_CTX(ctx);
@@ -281,13 +293,14 @@ int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H,
     return farneback(ctx, d_im1, d_im2, H, W, d_flowx, d_flowy, p);
 }
 
-int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
-                  const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
-                  ofri_level_stats* h_stats) {
-    CHECK_CTX(ctx);
-    if (!p || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_lshs: null pointer");
-    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_lshs: levels %d", p->levels);
-    if (!p->alphas) return fail(OFRI_E_ARG, "ofri_pyr_lshs: alphas missing");
+}  // extern "C"
+
+namespace ofri {
+
+// Enqueues the whole LS(HS) pyramid on the context stream.  Per-level
+// statistics land in the "pyr_stats" scratch (L SolveCtrl + L floats).
+static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                            const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc) {
     const int L = p->levels;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
@@ -369,6 +382,92 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
         ph = h;
         pw = w;
     }
+    return OFRI_OK;
+}
+
+// Key of a pyramid call: everything that is baked into its launch sequence.
+static std::string lshs_key(ofri_ctx* ctx, const float* a, const float* b, int H, int W,
+                            const ofri_lshs_params* p, const float* u, const float* v) {
+    std::string k;
+    auto put = [&](const void* x, size_t n) { k.append((const char*)x, n); };
+    const void* ptrs[4] = {a, b, u, v};
+    put(ptrs, sizeof ptrs);
+    put(&H, sizeof H);
+    put(&W, sizeof W);
+    put(&p->levels, sizeof p->levels);
+    put(p->alphas, sizeof(float) * p->levels);
+    put(&p->niter, sizeof p->niter);
+    put(&p->filt_n, sizeof p->filt_n);
+    if (p->filt_w && p->filt_n > 0) put(p->filt_w, sizeof(float) * p->filt_n);
+    put(&p->opt_n, sizeof p->opt_n);
+    if (p->opt_w && p->opt_n > 0) put(p->opt_w, sizeof(float) * p->opt_n);
+    put(&p->ls_alpha, sizeof p->ls_alpha);
+    put(&p->ls_maxnum, sizeof p->ls_maxnum);
+    put(&p->ls_tol, sizeof p->ls_tol);
+    put(&ctx->tuning, sizeof ctx->tuning);
+    return k;
+}
+
+}  // namespace ofri
+
+extern "C" {
+
+int ofri_set_graphs(ofri_ctx* ctx, int on) {
+    CHECK_CTX(ctx);
+    ctx->graphs_on = on != 0;
+    return OFRI_OK;
+}
+
+int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                  const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
+                  ofri_level_stats* h_stats) {
+    CHECK_CTX(ctx);
+    if (!p || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_lshs: null pointer");
+    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_lshs: levels %d", p->levels);
+    if (!p->alphas) return fail(OFRI_E_ARG, "ofri_pyr_lshs: alphas missing");
+    const int L = p->levels;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    for (int l = 0; l < L; ++l)
+        if (lh[l] < 4 || lw[l] < 4)
+            return fail(OFRI_E_ARG, "ofri_pyr_lshs: level %d size %dx%d below 4x4", l + 1, lh[l], lw[l]);
+    if (!ctx->graphs_on) {
+        OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc));
+    } else {
+        std::string key = lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
+        GraphEntry& g = ctx->graphs[key];
+        if (g.exec && g.ws_gen == ctx->ws_gen) {
+            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
+            // first sight (or buffers moved): run eagerly, which also sizes every scratch buffer
+            if (g.exec) {
+                (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+            }
+            OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc));
+            g.eager_runs += 1;
+            g.ws_gen = ctx->ws_gen;
+        } else {
+            hipGraph_t graph;
+            OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int st = pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            if (st != OFRI_OK) return st;
+            if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                g.exec = nullptr;
+                return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+            }
+            g.ws_gen = ctx->ws_gen;
+            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        }
+    }
+    void* st;
+    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (sizeof(SolveCtrl) + 16), &st));
+    SolveCtrl* ctrls = (SolveCtrl*)st;
+    float* hs_errs = (float*)(ctrls + L);
     if (h_stats) {
         std::vector<SolveCtrl> hc(L);
         std::vector<float> he(L);

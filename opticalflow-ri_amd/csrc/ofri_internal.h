@@ -40,7 +40,8 @@ struct SolveCtrl {
     int redo;        // >0: the stopping pass must be re-run with this many sweeps
     int redo_in;     // ping-pong index the redo reads
     float last_err;  // last update norm (float32, like the reference)
-    int pad[2];
+    int arrived;     // workgroups of the current launch that have finished
+    int pad;
 };
 
 // A device allocation owned by the context, grown on demand.
@@ -63,11 +64,23 @@ struct SplineTable {
 
 }  // namespace ofri
 
+namespace ofri {
+// A captured launch sequence, valid while no scratch buffer is reallocated.
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    uint64_t ws_gen = 0;
+    int eager_runs = 0;  // calls seen with this key before capture
+};
+}  // namespace ofri
+
 struct ofri_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    ofri_tuning tuning{0, 0};
+    ofri_tuning tuning{0, 0, 0, 0};
+    bool graphs_on = true;
+    uint64_t ws_gen = 1;  // bumped whenever a scratch buffer is (re)allocated
+    std::map<std::string, ofri::GraphEntry> graphs;
     std::map<std::string, ofri::DevBuf> ws;
     std::map<std::tuple<int, int, int>, ofri::ResizeTable> resize_tables;
     std::map<std::pair<int, int>, ofri::SplineTable> spline_tables;
@@ -101,6 +114,9 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // ---- Horn-Schunck (k_hs.hip) ----
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err);
+
+void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
+void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 
 // ---- Farneback (k_fb.hip) ----
 int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* flx, float* fly,

@@ -96,6 +96,8 @@ SIGNATURES = {
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],
     "ofri_get_tuning": [_vp, C.POINTER(Tuning)],
+    "ofri_query_tuning": [_vp, C.c_int, C.c_int, C.POINTER(Tuning)],
+    "ofri_set_graphs": [_vp, C.c_int],
 }
 
 _lib = None

@@ -37,7 +37,7 @@ def time_hs(n, variant, depth, sweeps=120, reps=3):
     im1, im2 = _pair(n)
     d1, d2 = DeviceArray.from_host(im1), DeviceArray.from_host(im2)
     U, V = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
-    ctx.set_tuning(hs_depth=depth, hs_variant=variant)
+    ctx.set_tuning(hs_depth=depth, hs_variant=variant + 1)
     try:
         st = ctx.lib.ofri_hs(ctx.handle, d1.ptr, d2.ptr, n, n, 21.0, sweeps, U.ptr, V.ptr, None)
         if st != 0:
@@ -55,7 +55,7 @@ def time_ls(n, variant, depth, sweeps=60, reps=3):
     im1, im2 = _pair(n)
     d1, d2 = DeviceArray.from_host(im1), DeviceArray.from_host(im2)
     u, v = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
-    ctx.set_tuning(ls_depth=depth, ls_variant=variant)
+    ctx.set_tuning(ls_depth=depth, ls_variant=variant + 1)
     try:
         st = ctx.lib.ofri_ls(ctx.handle, d1.ptr, d2.ptr, n, n, 5.0, sweeps, 0.0, u.ptr, v.ptr, None, None)
         if st != 0:
@@ -70,7 +70,8 @@ def time_ls(n, variant, depth, sweeps=60, reps=3):
 
 
 def tune(n, depths=(3, 4, 5, 6, 8, 10), verbose=True):
-    """Returns {'hs': (variant, depth, ms/sweep), 'ls': (...)} for an n x n image."""
+    """Returns {'hs': (variant, depth, ms/sweep), 'ls': (...)} for an n x n image;
+    variant is the 0-based index of the compiled tile shape (tuning value - 1)."""
     out = {}
     for kind, nv, fn in (("hs", HS_VARIANTS, time_hs), ("ls", LS_VARIANTS, time_ls)):
         best = None

@@ -42,6 +42,15 @@ class Context:
         _capi.check(self.lib.ofri_get_tuning(self.handle, C.byref(t)))
         return t.hs_depth, t.ls_depth, t.hs_variant, t.ls_variant
 
+    def query_tuning(self, H, W):
+        """(hs_depth, ls_depth, hs_variant, ls_variant) the sweeps use for H x W."""
+        t = _capi.Tuning()
+        _capi.check(self.lib.ofri_query_tuning(self.handle, int(H), int(W), C.byref(t)))
+        return t.hs_depth, t.ls_depth, t.hs_variant, t.ls_variant
+
+    def set_graphs(self, on):
+        _capi.check(self.lib.ofri_set_graphs(self.handle, 1 if on else 0))
+
     def timer_start(self):
         _capi.check(self.lib.ofri_timer_start(self.handle))
 

@@ -176,7 +176,7 @@ def test_hs_tile_variants_identical(ops, variant):
     ctx = context()
     U32, V32, _ = orc.hs(im1, im2, 21.0, 40)
     for depth in (3, 7):
-        ctx.set_tuning(hs_depth=depth, hs_variant=variant)
+        ctx.set_tuning(hs_depth=depth, hs_variant=variant + 1)
         U = DeviceArray.zeros(im1.shape); V = DeviceArray.zeros(im1.shape)
         try:
             ops.hs(im1, im2, 21.0, 40, U, V)
@@ -193,7 +193,7 @@ def test_ls_tile_variants_identical(ops, variant):
     u32, v32, e32, it32 = orc.ls(im1, im2, 5.0, np.zeros((H, W)), np.zeros((H, W)), maxnum=17, tol=0.0)
     ctx = context()
     for depth in (2, 5):
-        ctx.set_tuning(ls_depth=depth, ls_variant=variant)
+        ctx.set_tuning(ls_depth=depth, ls_variant=variant + 1)
         du, dv = DeviceArray.zeros((H, W)), DeviceArray.zeros((H, W))
         try:
             err, it = ops.ls(im1, im2, 5.0, du, dv, maxnum=17, tol=0.0)
@@ -370,3 +370,26 @@ def test_full_size_pipeline_properties(ops):
     epe = _epe(a, b, ut, vt)[64:-64, 64:-64].mean()
     print("  1024^2 EPE to truth %.4f px, stats %s" % (epe, st))
     assert epe < 1.0
+
+
+def test_pyramid_graph_replays_match_eager(ops):
+    """ofri_pyr_lshs runs eagerly, then captures, then replays a HIP graph:
+    every replay (with unrelated eager work and syncs in between) must equal
+    the eager result bit for bit."""
+    from ofri import synth
+    from ofri.device import DeviceArray, context
+    fw, ow = _lshs_params()
+    im1, im2, _, _ = synth.config_pair(4, size=192)
+    ctx = context()
+    d1, d2 = DeviceArray.from_host(im1), DeviceArray.from_host(im2)
+    Ua, Va = DeviceArray((192, 192)), DeviceArray((192, 192))
+    ctx.set_graphs(False)
+    ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 120, fw, ow, 5.0, Uacc=Ua, Vacc=Va)
+    ref = (Ua.download(), Va.download())
+    ctx.set_graphs(True)
+    for rep in range(5):
+        ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 120, fw, ow, 5.0, Uacc=Ua, Vacc=Va)
+        got = (Ua.download(), Va.download())
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), rep
+        ops.gauss_sep(d1, fw).download()         # unrelated eager work between replays
+        Ua.upload(np.full((192, 192), 7.0, np.float32))
