@@ -199,6 +199,9 @@ const HsVariant kHsVariants[] = {
     {16, 4, hs_pass_k<16, 4>}, {8, 4, hs_pass_k<8, 4>}, {8, 8, hs_pass_k<8, 8>},
     {12, 4, hs_pass_k<12, 4>}, {16, 2, hs_pass_k<16, 2>}, {4, 8, hs_pass_k<4, 8>},
     {24, 2, hs_pass_k<24, 2>}, {12, 8, hs_pass_k<12, 8>},
+    // same tile areas spread over 16 wavefronts: fewer rows per lane, more waves in flight
+    {6, 16, hs_pass_k<6, 16>}, {4, 16, hs_pass_k<4, 16>}, {2, 16, hs_pass_k<2, 16>},
+    {8, 16, hs_pass_k<8, 16>},
 };
 constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
 // Defaults measured by the autotuner on gfx950 (per-sweep time, 600 sweeps):

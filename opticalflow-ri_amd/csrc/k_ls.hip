@@ -131,8 +131,10 @@ __device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Ra
     float dcv = 0.5f * (cv_r - cv_l);
     float fru = uu.c + ud.c;
     float fcv = cv_l + cv_r;
-    float mu = (float)(0.25 * (double)uu_l - 0.25 * (double)uu_r - 0.25 * (double)ud_l + 0.25 * (double)ud_r);
-    float mv = (float)(0.25 * (double)vu_l - 0.25 * (double)vu_r - 0.25 * (double)vd_l + 0.25 * (double)vd_r);
+    // the reference's double sum of quarter-weighted corners, with the exact
+    // power-of-two scale taken out of the sum (it commutes with rounding)
+    float mu = (float)((((double)uu_l - (double)uu_r) - (double)ud_l) + (double)ud_r) * 0.25f;
+    float mv = (float)((((double)vu_l - (double)vu_r) - (double)vd_l) + (double)vd_r) * 0.25f;
     // 8-neighbour sums with zero outside the image ('constant')
     float zl_pu = (EDGE && (el || top)) ? 0.0f : pu.l, zc_pu = (EDGE && top) ? 0.0f : pu.c;
     float zr_pu = (EDGE && (er || top)) ? 0.0f : pu.r;
@@ -391,6 +393,7 @@ struct LsVariant {
 };
 const LsVariant kLsVariants[] = {
     {8, 8, ls_pass_k<8, 8>}, {8, 4, ls_pass_k<8, 4>}, {4, 8, ls_pass_k<4, 8>}, {12, 4, ls_pass_k<12, 4>},
+    {2, 16, ls_pass_k<2, 16>}, {4, 16, ls_pass_k<4, 16>}, {6, 8, ls_pass_k<6, 8>},
 };
 constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
 // Defaults measured by the autotuner on gfx950.

@@ -19,8 +19,8 @@ import numpy as np
 from . import _capi
 from .device import DeviceArray, context
 
-HS_VARIANTS = 8     # number of compiled Horn-Schunck tile shapes
-LS_VARIANTS = 4     # number of compiled Liu-Shen tile shapes
+HS_VARIANTS = 12    # number of compiled Horn-Schunck tile shapes
+LS_VARIANTS = 7     # number of compiled Liu-Shen tile shapes
 CACHE = os.environ.get("OFRI_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "ofri", "tuning.json"))
 
 

@@ -169,7 +169,7 @@ def test_hs_temporal_depth_does_not_change_result(ops, depth):
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(12))
 def test_hs_tile_variants_identical(ops, variant):
     from ofri.device import DeviceArray, context
     im1, im2 = _piv(150, 211, seed=13)
@@ -185,7 +185,7 @@ def test_hs_tile_variants_identical(ops, variant):
         assert np.array_equal(U.download(), U32) and np.array_equal(V.download(), V32), (variant, depth)
 
 
-@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("variant", range(7))
 def test_ls_tile_variants_identical(ops, variant):
     from ofri.device import DeviceArray, context
     H, W = 170, 230
