@@ -122,6 +122,19 @@ typedef struct ofri_fb_params {
 int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                    float* d_flowx, float* d_flowy, const ofri_fb_params* p);
 
+/* ---- dense Lucas-Kanade --------------------------------------------------- */
+/* One level of dense pyramidal LK with a (2*half_win+1)^2 window (half_win
+ * 1..16): Scharr structure tensor of I over the window, up to niter
+ * Gauss-Newton steps with J sampled bilinearly at the displaced window,
+ * step 32*A^-1*b, early exit when both step components are below 0.01 px.
+ * d_u / d_v: initial flow in, result out (left unchanged where the tensor is
+ * singular).  d_status (may be NULL): 1 = tracked, 0 = singular tensor or the
+ * window left the image.  Replaces denseLucasKanade_PyCl.compute
+ * (denseLucasKanade_PyCL.py:113-169) and pyrlkDenseLargeW.cl lkDense (:304-669)
+ * with the symmetric window (vorticity enhancement off). */
+int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
+                  float* d_v, int half_win, int niter, unsigned char* d_status);
+
 /* ---- whole LS(HS) pyramid ------------------------------------------------ */
 typedef struct ofri_lshs_params {
     int levels;            /* pyramidalLevels */

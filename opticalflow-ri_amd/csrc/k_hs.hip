@@ -204,13 +204,15 @@ const HsVariant kHsVariants[] = {
     {8, 16, hs_pass_k<8, 16>},
 };
 constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
-// Defaults measured by the autotuner on gfx950 (per-sweep time, 600 sweeps):
-// large images favour 12-row lanes in 8-wavefront tiles, small ones 4-row
-// lanes (more workgroups to spread over the 256 CUs).
+// Defaults measured by the autotuner on gfx950 (ms per sweep over 120
+// sweeps): about 1M pixels and up favour 8-row lanes in 16-wavefront tiles
+// fused 10 sweeps deep (fewer launches outweigh the wider halo); smaller
+// images 4-row lanes in 8-wavefront tiles, 5 deep (more workgroups to spread
+// over the 256 CUs).
 void hs_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
-        *variant = 7;
-        *depth = 5;
+        *variant = 11;
+        *depth = 10;
     } else {
         *variant = 5;
         *depth = 5;

@@ -1,0 +1,151 @@
+// Dense pyramidal Lucas-Kanade, one level (reference:
+// denseLucasKanade_PyCL.py:113-169 with pyrlkDenseLargeW.cl:304-669).
+//
+// The reference launches one 8x8 workgroup per output pixel and reduces the
+// window sums across its 64 work-items.  Here one thread owns one output
+// pixel and a 16x16 workgroup owns a 16x16 block of outputs: the block's
+// whole window region of I (block + 2*hw, plus the one-pixel Scharr ring,
+// clamp-to-edge) is staged once in LDS together with its Scharr derivatives,
+// and every pixel's window is read from there.  The structure tensor A and
+// the per-iteration right-hand side b are summed in double; J is sampled
+// bilinearly (the OpenCL LINEAR sampler at pixel centres, clamp-to-edge)
+// with the pixel's fractional weights formed once per iteration.
+#include "ofri_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ofri {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kMaxHalf = 16;
+constexpr int kReg = kTile + 2 * kMaxHalf;       // window region edge
+constexpr int kRegI = kReg + 2;                  // with the Scharr ring
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restrict__ I,
+                                                            const float* __restrict__ J, int H, int W,
+                                                            int hw, int niter, float* __restrict__ u,
+                                                            float* __restrict__ v,
+                                                            unsigned char* __restrict__ status) {
+    __shared__ float sI[kRegI * kRegI];
+    __shared__ float sDx[kReg * kReg];
+    __shared__ float sDy[kReg * kReg];
+    const int win = 2 * hw + 1;
+    const int reg = kTile + 2 * hw;       // region edge actually used
+    const int regI = reg + 2;
+    const int bx0 = blockIdx.x * kTile, by0 = blockIdx.y * kTile;
+    const int tid = threadIdx.y * kTile + threadIdx.x;
+    // stage I over the region with its ring: image rows by0-hw-1 .. , clamped
+    for (int k = tid; k < regI * regI; k += kTile * kTile) {
+        int ry = k / regI, rx = k % regI;
+        int y = clampi(by0 - hw - 1 + ry, 0, H - 1), x = clampi(bx0 - hw - 1 + rx, 0, W - 1);
+        sI[ry * kRegI + rx] = I[(size_t)y * W + x];
+    }
+    __syncthreads();
+    // Scharr derivatives at every region position from the staged (clamped) I
+    for (int k = tid; k < reg * reg; k += kTile * kTile) {
+        int ry = k / reg + 1, rx = k % reg + 1;
+        const float* c = sI + ry * kRegI + rx;
+        float dx = 3.0f * (c[-kRegI + 1] + c[kRegI + 1] - c[-kRegI - 1] - c[kRegI - 1]) +
+                   10.0f * (c[1] - c[-1]);
+        float dy = 3.0f * (c[kRegI - 1] + c[kRegI + 1] - c[-kRegI - 1] - c[-kRegI + 1]) +
+                   10.0f * (c[kRegI] - c[-kRegI]);
+        sDx[(ry - 1) * kReg + (rx - 1)] = dx;
+        sDy[(ry - 1) * kReg + (rx - 1)] = dy;
+    }
+    __syncthreads();
+
+    const int j = bx0 + threadIdx.x, i = by0 + threadIdx.y;
+    if (j >= W || i >= H) return;
+    const size_t g = (size_t)i * W + j;
+    // window of pixel (i, j) starts at region (threadIdx.y, threadIdx.x)
+    const int o = threadIdx.y * kReg + threadIdx.x;
+    const int oI = (threadIdx.y + 1) * kRegI + threadIdx.x + 1;
+    double a11 = 0.0, a12 = 0.0, a22 = 0.0;
+    for (int r = 0; r < win; ++r) {
+        const float* dxr = sDx + o + r * kReg;
+        const float* dyr = sDy + o + r * kReg;
+        for (int c = 0; c < win; ++c) {
+            double dx = (double)dxr[c], dy = (double)dyr[c];
+            a11 = a11 + dx * dx;
+            a12 = a12 + dx * dy;
+            a22 = a22 + dy * dy;
+        }
+    }
+    float A11 = (float)a11, A12 = (float)a12, A22 = (float)a22;
+    float D = A11 * A22 - A12 * A12;
+    unsigned char st = 1;
+    if (D < 1.192092896e-07f) {
+        if (status) status[g] = 0;
+        return;   // flow left as it came in
+    }
+    A11 /= D;
+    A12 /= D;
+    A22 /= D;
+    float px = (float)j + u[g] - (float)hw, py = (float)i + v[g] - (float)hw;
+    for (int k = 0; k < niter; ++k) {
+        if (px < (float)-hw || px >= (float)W || py < (float)-hw || py >= (float)H) {
+            st = 0;
+            break;
+        }
+        const float fx = floorf(px), fy = floorf(py);
+        const float a = px - fx, b = py - fy;
+        const int x0 = (int)fx, y0 = (int)fy;
+        const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
+        const float w01 = (1.0f - a) * b, w11 = a * b;
+        const bool inside = x0 >= 0 && y0 >= 0 && x0 + win < W && y0 + win < H;
+        double b1 = 0.0, b2 = 0.0;
+        for (int r = 0; r < win; ++r) {
+            const float* pr = sI + oI + r * kRegI;
+            const float* dxr = sDx + o + r * kReg;
+            const float* dyr = sDy + o + r * kReg;
+            int ya = y0 + r, yb = ya + 1;
+            if (!inside) {
+                ya = clampi(ya, 0, H - 1);
+                yb = clampi(yb, 0, H - 1);
+            }
+            const float* Ja = J + (size_t)ya * W;
+            const float* Jb = J + (size_t)yb * W;
+            for (int c = 0; c < win; ++c) {
+                int xa = x0 + c, xb = xa + 1;
+                if (!inside) {
+                    xa = clampi(xa, 0, W - 1);
+                    xb = clampi(xb, 0, W - 1);
+                }
+                float t = w00 * Ja[xa];
+                t = t + w10 * Ja[xb];
+                t = t + w01 * Jb[xa];
+                t = t + w11 * Jb[xb];
+                float diff = t - pr[c];
+                b1 = b1 + (double)diff * (double)dxr[c];
+                b2 = b2 + (double)diff * (double)dyr[c];
+            }
+        }
+        float B1 = (float)b1, B2 = (float)b2;
+        float ddx = (A12 * B2 - A22 * B1) * 32.0f;
+        float ddy = (A12 * B1 - A11 * B2) * 32.0f;
+        px = px + ddx;
+        py = py + ddy;
+        if (fabsf(ddx) < 0.01f && fabsf(ddy) < 0.01f) break;
+    }
+    u[g] = px + (float)hw - (float)j;
+    v[g] = py + (float)hw - (float)i;
+    if (status) status[g] = st;
+}
+
+}  // namespace
+
+int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw, int niter, float* u,
+             float* v, unsigned char* status) {
+    if (hw < 1 || hw > kMaxHalf) return fail(OFRI_E_ARG, "lk: half window %d outside 1..%d", hw, kMaxHalf);
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "lk: empty image");
+    if (niter < 0) return fail(OFRI_E_ARG, "lk: negative iteration count");
+    dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
+    lk_dense_k<<<grid, dim3(kTile, kTile), 0, ctx->stream>>>(I, J, H, W, hw, niter, u, v, status);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+}  // namespace ofri

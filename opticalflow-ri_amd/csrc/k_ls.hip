@@ -400,9 +400,9 @@ constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
 void ls_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
         *variant = 3;
-        *depth = 5;
+        *depth = 4;
     } else {
-        *variant = 2;
+        *variant = 1;
         *depth = 5;
     }
 }

@@ -283,6 +283,13 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
     return OFRI_OK;
 }
 
+int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
+                  float* d_v, int half_win, int niter, unsigned char* d_status) {
+    CHECK_CTX(ctx);
+    if (!d_I || !d_J || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_lk_dense: null pointer");
+    return lk_dense(ctx, d_I, d_J, H, W, half_win, niter, d_u, d_v, d_status);
+}
+
 int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                    float* d_flowx, float* d_flowy, const ofri_fb_params* p) {
     CHECK_CTX(ctx);

@@ -122,6 +122,10 @@ void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* flx, float* fly,
               const ofri_fb_params* p);
 
+// ---- dense Lucas-Kanade (k_lk.hip) ----
+int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw, int niter, float* u,
+             float* v, unsigned char* status);
+
 // ---- Liu-Shen (k_ls.hip) ----
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl);

@@ -120,6 +120,41 @@ def farneback(im1, im2, flowx, flowy, consts):
     return flowx, flowy
 
 
+def lk_dense(I, J, u, v, half_win=13, niter=5, status=None):
+    """One dense-LK level (denseLucasKanade_PyCL.py:113-169); u, v device
+    arrays updated in place; status: optional device buffer of H*W bytes."""
+    I, J = as_device(I), as_device(J)
+    _capi.check(_lib().ofri_lk_dense(_h(), I.ptr, J.ptr, I.H, I.W, u.ptr, v.ptr, int(half_win), int(niter),
+                                     status.ptr if status is not None else None))
+    return u, v
+
+
+class ByteArray:
+    """An H x W uint8 device buffer (status masks)."""
+
+    def __init__(self, shape):
+        self.ctx = context()
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape))
+        p = C.c_void_p()
+        _capi.check(self.ctx.lib.ofri_malloc(self.ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def download(self):
+        out = np.empty(self.shape, np.uint8)
+        _capi.check(self.ctx.lib.ofri_download(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                               self.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.ctx.lib.ofri_free(self.ctx.handle, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
 def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
              Uacc=None, Vacc=None, stats=True):
     """Whole LS(HS) pyramid on the device.  Returns (Uacc, Vacc, stats list)."""

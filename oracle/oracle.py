@@ -277,3 +277,18 @@ def farneback(im1, im2, U, V, consts, prec="f32"):
                                                 len(g) - 1, _p(ig), _p(wk), len(wk) - 1, int(consts["iters"]),
                                                 _p(fx), _p(fy))
     return fx, fy
+
+
+# ----------------------------------------------------------------- dense LK
+def lk_dense(I, J, u, v, half_win=13, niter=5, prec="f32"):
+    """One dense-LK level.  Returns (u, v, status uint8)."""
+    L = lib()
+    f = getattr(L, "orc_lk_dense_" + prec)
+    P = _fp if prec == "f32" else _dp
+    f.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.POINTER(C.c_ubyte)]
+    I, J = _arr(I, prec), _arr(J, prec)
+    H, W = I.shape
+    u, v = _arr(u, prec).copy(), _arr(v, prec).copy()
+    st = np.zeros((H, W), np.uint8)
+    f(_p(I), _p(J), H, W, int(half_win), int(niter), _p(u), _p(v), st.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return u, v, st

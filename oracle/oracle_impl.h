@@ -770,21 +770,6 @@ void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const R
 /* clamp-to-edge) at the displaced window, delta = 32 A^-1 b, early exit    */
 /* when |delta| < 0.01 in both components.  Window sums in double.          */
 /* ------------------------------------------------------------------------ */
-static inline REAL NAME(lk_tex)(const REAL* T, int H, int W, REAL px, REAL py) {
-    /* OpenCL LINEAR filter at unnormalised coordinate (px + .5, py + .5) */
-    REAL fx = floor(px), fy = floor(py);
-    REAL a = px - fx, b = py - fy;
-    int x0 = (int)fx, y0 = (int)fy;
-    int x1 = x0 + 1, y1 = y0 + 1;
-    x0 = NAME(clp)(x0, W); x1 = NAME(clp)(x1, W);
-    y0 = NAME(clp)(y0, H); y1 = NAME(clp)(y1, H);
-    REAL t = (1 - a) * (1 - b) * T[IDX(y0, x0)];
-    t = t + a * (1 - b) * T[IDX(y0, x1)];
-    t = t + (1 - a) * b * T[IDX(y1, x0)];
-    t = t + a * b * T[IDX(y1, x1)];
-    return t;
-}
-
 void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int niter, REAL* u, REAL* v,
                         unsigned char* status) {
     const int win = 2 * hw + 1;
@@ -828,14 +813,27 @@ void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int 
                         if (status) status[g] = 0;
                         break;
                     }
+                    /* bilinear weights are those of the window origin: sample
+                     * (r, c) sits at integer offsets from it */
+                    REAL flx = floor(px), fly = floor(py);
+                    REAL fa = px - flx, fb = py - fly;
+                    int x0 = (int)flx, y0 = (int)fly;
+                    REAL w00 = (1 - fa) * (1 - fb), w10 = fa * (1 - fb), w01 = (1 - fa) * fb, w11 = fa * fb;
                     double b1 = 0, b2 = 0;
-                    for (int r = 0; r < win; ++r)
+                    for (int r = 0; r < win; ++r) {
+                        int ya = NAME(clp)(y0 + r, H), yb = NAME(clp)(y0 + r + 1, H);
                         for (int c = 0; c < win; ++c) {
                             size_t q = (size_t)r * win + c;
-                            REAL diff = NAME(lk_tex)(J, H, W, px + (REAL)c, py + (REAL)r) - P[q];
+                            int xa = NAME(clp)(x0 + c, W), xb = NAME(clp)(x0 + c + 1, W);
+                            REAL t = w00 * J[IDX(ya, xa)];
+                            t = t + w10 * J[IDX(ya, xb)];
+                            t = t + w01 * J[IDX(yb, xa)];
+                            t = t + w11 * J[IDX(yb, xb)];
+                            REAL diff = t - P[q];
                             b1 += (double)diff * Dx[q];
                             b2 += (double)diff * Dy[q];
                         }
+                    }
                     REAL B1 = (REAL)b1, B2 = (REAL)b2;
                     REAL ddx = (A12 * B2 - A22 * B1) * (REAL)32;
                     REAL ddy = (A12 * B1 - A11 * B2) * (REAL)32;
