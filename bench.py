@@ -30,6 +30,7 @@ METRIC = "Mpix-pairs/s to fixed residual, LiuShen-HS 1024² learned vs plain; HB
 HBM_PEAK_GBS = 8000.0
 HS_BYTES_PER_PX_SWEEP = 28.0     # read U, V, fx, fy, ft; write U, V
 HS_SETUP_BYTES_PER_PX = 20.0     # read 2 images, write fx, fy, ft
+CALIB_BYTES = 64 << 20           # size of the counter-calibration copy
 
 
 def parse():
@@ -118,6 +119,12 @@ def main():
 
     # the headline run's per-level statistics (for the record)
     _, _, st = ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+
+    # calibration dispatch for the HBM counters: one plain device-to-device
+    # copy of CALIB_BYTES (read CALIB_BYTES, write CALIB_BYTES)
+    calib_src = DeviceArray((CALIB_BYTES // 4 // 1024, 1024))
+    calib_src.copy()
+    ctx.sync()
 
     out = {
         "metric": METRIC,

@@ -13,9 +13,12 @@
 //
 // Numerics follow the reference exactly: the 3x3 average is formed in double
 // from the float32 field and rounded to float32 (scipy.ndimage.convolve), the
-// update is float32 without FMA contraction and with IEEE division
-// (HS_helper2).  The double sum is grouped symmetrically (vertical pairs,
-// corners, edges), which also makes the mirror boundary an exact select.
+// update is float32 without FMA contraction and with a correctly rounded
+// quotient (HS_helper2; formed from a correctly rounded reciprocal and one
+// FMA correction, which rounds identically to IEEE division).  The double sum
+// is grouped symmetrically (vertical pairs, corners, edges), so a tile loaded
+// as the mirror extension of the image stays an exact mirror extension under
+// the sweeps and the border needs no special case.
 #include "ofri_internal.h"
 
 #pragma clang fp contract(off)
