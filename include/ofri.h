@@ -89,6 +89,19 @@ int ofri_accumulate(ofri_ctx* ctx, float* d_acc, const float* d_x, size_t n);
 int ofri_hs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
             int niter, float* d_U, float* d_V, float* h_err);
 
+/* Horn-Schunck with a stopping rule: sweeps until the per-sweep update norm
+ * (||dU||_2 + ||dV||_2)/(H*W) <= tol, or maxnum sweeps (tol < 0: exactly
+ * maxnum).  d_hist (device, maxnum floats, may be NULL) receives each sweep's
+ * norm; *h_err / *h_iters (may be NULL) the last norm and the sweep count.
+ * The sweeps are those of ofri_hs bit for bit: after k sweeps the flow equals
+ * ofri_hs with niter = k.  The reference's HS always runs Niter sweeps
+ * (HornSchunck.py:86-104); the rule is the Liu-Shen one
+ * (PhysicsBasedOpticalFlowLiuShen.py:79,141) applied to the same update, for
+ * callers that solve to a residual instead of a sweep count (warm starts). */
+int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
+                int maxnum, double tol, float* d_U, float* d_V, float* d_hist, float* h_err,
+                int* h_iters);
+
 /* ---- Liu-Shen ------------------------------------------------------------ */
 /* Liu-Shen physics-based refinement in the solver's own orientation: d_u is
  * the row (y) component, d_v the column (x) component, both in/out.  Sweeps

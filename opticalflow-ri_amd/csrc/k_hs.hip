@@ -20,6 +20,7 @@
 // as the mirror extension of the image stays an exact mirror extension under
 // the sweeps and the border needs no special case.
 #include "ofri_internal.h"
+#include "solve_check.h"
 
 #pragma clang fp contract(off)
 
@@ -61,10 +62,17 @@ __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restric
 }
 
 struct HsPass {
-    const float* Uin;
-    const float* Vin;
-    float* Uout;
-    float* Vout;
+    const float* in[2][2];   // [ping-pong][U, V]
+    float* out[2][2];
+    int src;                 // ping-pong index read (normal launch)
+    // residual mode (hs_pass_k<..., true>): per-sweep norms and stop test
+    double* part;            // [K][nblocks][2] squared-update sums
+    SolveCtrl* ctrl;
+    int redo;                // 1: re-run launch driven by ctrl->redo
+    double tol;
+    int maxnum;
+    size_t N;
+    float* hist;             // per-sweep residuals (nullptr: not recorded)
     const float* fx;
     const float* fy;
     const float* ft;
@@ -121,8 +129,18 @@ __device__ __forceinline__ float quotient(float num, float den, float rcp) {
 // The extended tile is loaded as the mirror extension of the image (every
 // field, including fx, fy, ft), so the image border needs no special case:
 // the mirror symmetry of the extension is preserved exactly by every sweep.
-template <int R, int WAVES>
+template <int R, int WAVES, bool NORM>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
+    int nsw = a.nsweeps, src = a.src;
+    if (NORM) {
+        if (a.redo) {
+            nsw = a.ctrl->redo;
+            src = a.ctrl->redo_in;
+            if (nsw <= 0) return;
+        } else if (a.ctrl->done) {
+            return;
+        }
+    }
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int K = a.K, H = a.H, W = a.W;
@@ -130,13 +148,16 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     const int gx = (int)blockIdx.x * outW - K + lane;
     const int gy0 = (int)blockIdx.y * outH - K + w * R;
     const int cx = mirror_idx(gx, W);
+    const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
     float U[R], V[R], FX[R], FY[R], FT[R], DEN[R], RCP[R];
+    const float* uin = a.in[src][0];
+    const float* vin = a.in[src][1];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
-        U[r] = a.Uin[q];
-        V[r] = a.Vin[q];
+        U[r] = uin[q];
+        V[r] = vin[q];
         FX[r] = a.fx[q];
         FY[r] = a.fy[q];
         FT[r] = a.ft[q];
@@ -146,7 +167,10 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     }
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
-    for (int s = 0; s < a.nsweeps; ++s) {
+    __shared__ double red[WAVES][2];
+    const int nblk = gridDim.x * gridDim.y;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
         edge[b][w][0][0][lane] = U[0];
         edge[b][w][0][1][lane] = V[0];
@@ -160,6 +184,7 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
 
         Trip up_u = trip(uup), up_v = trip(vup);
         Trip cu = trip(U[0]), cv = trip(V[0]);
+        double su = 0.0, sv = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Trip dn_u = trip(r + 1 < R ? U[r + 1] : udn);
@@ -171,41 +196,76 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
             num = num + y * vb;
             num = num + FT[r];
             float der = quotient(num, DEN[r], RCP[r]);
-            U[r] = ub - x * der;
-            V[r] = vb - y * der;
+            float un = ub - x * der;
+            float vn = vb - y * der;
+            if (NORM && !a.redo) {
+                const int ry = w * R + r, gy = gy0 + r;
+                if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
+                    float du = un - U[r], dv = vn - V[r];
+                    su = su + (double)du * (double)du;
+                    sv = sv + (double)dv * (double)dv;
+                }
+            }
+            U[r] = un;
+            V[r] = vn;
             up_u = cu; up_v = cv;
             cu = dn_u; cv = dn_v;
         }
+        if (NORM && !a.redo) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                su = su + __shfl_xor(su, off);
+                sv = sv + __shfl_xor(sv, off);
+            }
+            if (lane == 0) {
+                red[w][0] = su;
+                red[w][1] = sv;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tu = 0.0, tv = 0.0;
+                for (int q = 0; q < WAVES; ++q) {
+                    tu = tu + red[q][0];
+                    tv = tv + red[q][1];
+                }
+                a.part[((size_t)s * nblk + blk) * 2] = tu;
+                a.part[((size_t)s * nblk + blk) * 2 + 1] = tv;
+            }
+        }
     }
 
-    if (lane >= K && lane < 64 - K && gx < W) {
+    float* uo = a.out[src ^ 1][0];
+    float* vo = a.out[src ^ 1][1];
+    if (own_x) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             int ry = w * R + r;
             int gy = gy0 + r;
             if (ry >= K && ry < WAVES * R - K && gy < H) {
                 size_t q = (size_t)gy * W + gx;
-                a.Uout[q] = U[r];
-                a.Vout[q] = V[r];
+                uo[q] = U[r];
+                vo[q] = V[r];
             }
         }
     }
+    if (NORM && !a.redo) arrive_and_check(a.part, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
 }
 
 // Tile shapes (rows per lane, wavefronts per workgroup) compiled in; the
 // tuning picks one by index.
 struct HsVariant {
     int R, waves;
-    void (*kernel)(HsPass);
+    void (*kernel)(HsPass);   // fixed sweep count
+    void (*normed)(HsPass);   // with per-sweep residuals and the stop test
 };
+#define HS_SHAPE(R, WV) {R, WV, hs_pass_k<R, WV, false>, hs_pass_k<R, WV, true>}
 const HsVariant kHsVariants[] = {
-    {16, 4, hs_pass_k<16, 4>}, {8, 4, hs_pass_k<8, 4>}, {8, 8, hs_pass_k<8, 8>},
-    {12, 4, hs_pass_k<12, 4>}, {16, 2, hs_pass_k<16, 2>}, {4, 8, hs_pass_k<4, 8>},
-    {24, 2, hs_pass_k<24, 2>}, {12, 8, hs_pass_k<12, 8>},
+    HS_SHAPE(16, 4), HS_SHAPE(8, 4), HS_SHAPE(8, 8), HS_SHAPE(12, 4), HS_SHAPE(16, 2),
+    HS_SHAPE(4, 8), HS_SHAPE(24, 2), HS_SHAPE(12, 8),
     // same tile areas spread over 16 wavefronts: fewer rows per lane, more waves in flight
-    {6, 16, hs_pass_k<6, 16>}, {4, 16, hs_pass_k<4, 16>}, {2, 16, hs_pass_k<2, 16>},
-    {8, 16, hs_pass_k<8, 16>},
+    HS_SHAPE(6, 16), HS_SHAPE(4, 16), HS_SHAPE(2, 16), HS_SHAPE(8, 16),
 };
+#undef HS_SHAPE
 constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
 // Defaults measured by the autotuner on gfx950 (ms per sweep over 120
 // sweeps): about 1M pixels and up favour 8-row lanes in 16-wavefront tiles
@@ -225,10 +285,23 @@ void hs_default(size_t npx, int* variant, int* depth) {
 }  // namespace
 
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
-             int niter, float* U, float* V, float* d_err) {
+             int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
+             float* d_hist) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
     size_t N = (size_t)H * W;
+    int vi, depth;
+    hs_default((size_t)H * W, &vi, &depth);
+    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
+    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
+    const HsVariant& var = kHsVariants[vi];
+    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
+        return fail(OFRI_E_ARG, "hs: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
+    dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
+    const int nblk = grid.x * grid.y;
+
     void* buf;
     OFRI_TRY(scratch(ctx, "hs_fields", 7 * N * sizeof(float), &buf));
     float* fx = (float*)buf;
@@ -238,6 +311,8 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* Vb = Ub + N;
     float* U0 = Vb + N;
     float* V0 = U0 + N;
+    void* pbuf = nullptr;
+    if (d_ctrl) OFRI_TRY(scratch(ctx, "hs_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
     dim3 g((W + 255) / 256, H);
     // computeDerivatives(I1=im2, I0=im1)
     hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft);
@@ -246,36 +321,46 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    int vi, depth;
-    hs_default((size_t)H * W, &vi, &depth);
-    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
-    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
-    if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
-    const HsVariant& var = kHsVariants[vi];
-    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
-        return fail(OFRI_E_ARG, "hs: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
+    if (d_ctrl) {
+        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
+        OFRI_HIP(hipGetLastError());
+    }
     HsPass p{};
+    p.in[0][0] = U; p.in[0][1] = V; p.in[1][0] = Ub; p.in[1][1] = Vb;
+    p.out[0][0] = U; p.out[0][1] = V; p.out[1][0] = Ub; p.out[1][1] = Vb;
     p.fx = fx; p.fy = fy; p.ft = ft;
     p.H = H; p.W = W;
     p.a2 = alpha * alpha;
     p.K = depth;
-    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
-    dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
-    float *cu = U, *cv = V, *nu = Ub, *nv = Vb;
-    int done = 0;
+    p.part = (double*)pbuf;
+    p.ctrl = d_ctrl;
+    p.tol = tol;
+    p.maxnum = niter;
+    p.N = N;
+    p.hist = d_hist;
+    auto kern = d_ctrl ? var.normed : var.kernel;
+    int done = 0, src = 0;
     while (done < niter) {
         int ns = niter - done < depth ? niter - done : depth;
-        p.Uin = cu; p.Vin = cv; p.Uout = nu; p.Vout = nv;
         p.nsweeps = ns;
-        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+        p.src = src;
+        p.redo = 0;
+        kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_HIP(hipGetLastError());
-        std::swap(cu, nu);
-        std::swap(cv, nv);
+        src ^= 1;
         done += ns;
     }
-    if (cu != U) {
-        OFRI_HIP(hipMemcpyAsync(U, cu, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        OFRI_HIP(hipMemcpyAsync(V, cv, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    if (d_ctrl) {
+        if (niter > 0) {
+            p.redo = 1;
+            kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+            OFRI_HIP(hipGetLastError());
+            OFRI_TRY(select_copy(ctx, d_ctrl, U, Ub, U, N));
+            OFRI_TRY(select_copy(ctx, d_ctrl, V, Vb, V, N));
+        }
+    } else if (src != 0) {
+        OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(V, Vb, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
     if (d_err) OFRI_TRY(diff_norm_err(ctx, U, U0, V, V0, N, d_err));
     return OFRI_OK;

@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "ofri_internal.h"
+#include "solve_check.h"
 
 #pragma clang fp contract(off)
 
@@ -89,12 +90,10 @@ struct LsPass {
     double* part;           // [K][nblocks][2] squared-update sums (nullptr: none)
     double tol;             // stopping tolerance on the update norm
     int maxnum;             // sweep budget
+    float* hist;            // per-sweep norms (nullptr: not recorded)
     SolveCtrl* ctrl;
     int redo;               // 1: re-run launch driven by ctrl->redo
 };
-
-__device__ void ls_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
-                         size_t N, SolveCtrl* ctrl);
 
 struct Raw {
     float l, c, r;
@@ -307,85 +306,9 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
             }
         }
     }
-    if (a.part) {
-        // the last workgroup to finish runs the convergence test
-        __shared__ int last;
-        const int nblk = gridDim.x * gridDim.y;
-        if (threadIdx.x == 0) {
-            __threadfence();
-            last = atomicAdd(&a.ctrl->arrived, 1) == nblk - 1;
-        }
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            ls_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl);
-            if (threadIdx.x == 0) a.ctrl->arrived = 0;
-        }
-    }
+    if (a.part) arrive_and_check(a.part, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
 }
 
-// Convergence test of one launch: `while err > tol and k < maxnum`
-// (PhysicsBasedOpticalFlowLiuShen.py:141,150; helper :79).  Run by the last
-// workgroup of the launch to finish, with all its threads: the per-sweep
-// squared-update sums of every workgroup are added in a fixed order, the
-// float32 norm of each sweep is tested in turn, and the control word records
-// the sweep count, the last norm, the buffer holding the result and, when
-// the test held before the launch's last sweep, how many sweeps to re-run.
-__device__ void ls_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
-                         size_t N, SolveCtrl* ctrl) {
-    __shared__ double sa[512], sb[512];
-    __shared__ int stop;
-    const int T = blockDim.x;
-    if (threadIdx.x == 0) stop = 0;
-    for (int s = 0; s < nsw; ++s) {
-        double a = 0.0, b = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += T) {
-            // other workgroups' sums: read at device scope, past this CU's cache
-            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-            b = b + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2 + 1, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-        }
-        sa[threadIdx.x] = a;
-        sb[threadIdx.x] = b;
-        __syncthreads();
-        for (int k = T / 2; k > 0; k >>= 1) {
-            if ((int)threadIdx.x < k) {
-                sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + k];
-                sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + k];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            float err = (float)((sqrt(sa[0]) + sqrt(sb[0])) / (double)N);
-            ctrl->sweeps += 1;
-            ctrl->last_err = err;
-            ctrl->final_buf = dst;
-            if (!((double)err > tol)) {
-                ctrl->done = 1;
-                if (s + 1 < nsw) {
-                    ctrl->redo = s + 1;
-                    ctrl->redo_in = dst ^ 1;
-                }
-                stop = 1;
-            } else if (ctrl->sweeps >= maxnum) {
-                ctrl->done = 1;
-            }
-        }
-        __syncthreads();
-        if (stop) return;
-    }
-}
-
-__global__ void ctrl_init_k(SolveCtrl* c) {
-    c->done = 0;
-    c->sweeps = 0;
-    c->final_buf = 0;
-    c->redo = 0;
-    c->redo_in = 0;
-    c->last_err = 0.0f;
-    c->arrived = 0;
-}
 
 struct LsVariant {
     int R, waves;

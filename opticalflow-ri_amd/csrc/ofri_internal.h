@@ -112,8 +112,11 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
                 size_t n);
 
 // ---- Horn-Schunck (k_hs.hip) ----
+// With d_ctrl: per-sweep update norms, stop once the norm <= tol (tol < 0:
+// never), the sweep history in d_hist (niter floats, may be null).
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
-             int niter, float* U, float* V, float* d_err);
+             int niter, float* U, float* V, float* d_err, double tol = -1.0,
+             SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);

@@ -1,0 +1,97 @@
+// Device-side convergence test shared by the Jacobi solvers (Horn-Schunck and
+// Liu-Shen).  Included by k_hs.hip and k_ls.hip.
+#pragma once
+
+#include "ofri_internal.h"
+
+namespace ofri {
+namespace {
+
+// Test of one launch of `nsw` fused sweeps: `while err > tol and k < maxnum`
+// with err = (||du||_2 + ||dv||_2) / (H*W) (the reference's Liu-Shen update
+// norm, PhysicsBasedOpticalFlowLiuShen.py:79,141; the same residual is used
+// for Horn-Schunck when a tolerance is given).  Run by the last workgroup of
+// the launch to finish, with all its threads: the per-sweep squared-update
+// sums of every workgroup are added in a fixed order, the float32 norm of
+// each sweep is tested in turn (and recorded in `hist` when given), and the
+// control word records the sweep count, the last norm, the buffer holding
+// the result and, when the test held before the launch's last sweep, how
+// many sweeps the stopping launch must be re-run with.
+__device__ void solve_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
+                            size_t N, SolveCtrl* ctrl, float* hist) {
+    __shared__ double sa[1024], sb[1024];
+    __shared__ int stop;
+    const int T = blockDim.x * blockDim.y;
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if (t == 0) stop = 0;
+    for (int s = 0; s < nsw; ++s) {
+        double a = 0.0, b = 0.0;
+        for (int i = t; i < nblk; i += T) {
+            // other workgroups' sums: read at device scope, past this CU's cache
+            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            b = b + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2 + 1, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        }
+        sa[t] = a;
+        sb[t] = b;
+        __syncthreads();
+        for (int k = T / 2; k > 0; k >>= 1) {
+            if (t < k) {
+                sa[t] = sa[t] + sa[t + k];
+                sb[t] = sb[t] + sb[t + k];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            float err = (float)((sqrt(sa[0]) + sqrt(sb[0])) / (double)N);
+            if (hist) hist[ctrl->sweeps] = err;
+            ctrl->sweeps += 1;
+            ctrl->last_err = err;
+            ctrl->final_buf = dst;
+            if (!((double)err > tol)) {
+                ctrl->done = 1;
+                if (s + 1 < nsw) {
+                    ctrl->redo = s + 1;
+                    ctrl->redo_in = dst ^ 1;
+                }
+                stop = 1;
+            } else if (ctrl->sweeps >= maxnum) {
+                ctrl->done = 1;
+            }
+        }
+        __syncthreads();
+        if (stop) return;
+    }
+}
+
+// Called by every workgroup at the end of a launch with per-sweep partial
+// sums: the last one to arrive runs solve_check and re-arms the counter.
+__device__ __forceinline__ void arrive_and_check(const double* part, int nsw, int dst, double tol,
+                                                 int maxnum, size_t N, SolveCtrl* ctrl, float* hist) {
+    __shared__ int last;
+    const int nblk = gridDim.x * gridDim.y;
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        __threadfence();
+        last = atomicAdd(&ctrl->arrived, 1) == nblk - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist);
+        if (threadIdx.x == 0 && threadIdx.y == 0) ctrl->arrived = 0;
+    }
+}
+
+__global__ void ctrl_init_k(SolveCtrl* c) {
+    c->done = 0;
+    c->sweeps = 0;
+    c->final_buf = 0;
+    c->redo = 0;
+    c->redo_in = 0;
+    c->last_err = 0.0f;
+    c->arrived = 0;
+}
+
+}  // namespace
+}  // namespace ofri

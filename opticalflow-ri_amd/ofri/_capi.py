@@ -90,6 +90,8 @@ SIGNATURES = {
     "ofri_warp_coords": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_accumulate": [_vp, _vp, _vp, _sz],
     "ofri_hs": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp, _fp],
+    "ofri_hs_tol": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _vp, _fp,
+                    _ip],
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],

@@ -89,6 +89,21 @@ def hs(im1, im2, alpha, niter, U, V):
     return np.float32(err.value)
 
 
+def hs_tol(im1, im2, alpha, maxnum, tol, U, V, hist=None):
+    """Horn-Schunck until the per-sweep update norm <= tol or maxnum sweeps
+    (tol < 0: exactly maxnum); U, V updated in place.  hist: optional device
+    array of >= maxnum floats receiving each sweep's norm.  Returns (err, sweeps)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    if hist is not None and hist.nbytes // 4 < maxnum:
+        raise ValueError("hs_tol: history buffer holds %d < %d sweeps" % (hist.nbytes // 4, maxnum))
+    err = C.c_float()
+    it = C.c_int()
+    _capi.check(_lib().ofri_hs_tol(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(alpha)), int(maxnum),
+                                   float(tol), U.ptr, V.ptr, hist.ptr if hist is not None else None,
+                                   C.byref(err), C.byref(it)))
+    return np.float32(err.value), it.value
+
+
 def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
     """Liu-Shen in the solver's orientation (u = row component); u, v in place.
     Returns (err, sweeps)."""

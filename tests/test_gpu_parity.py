@@ -267,6 +267,76 @@ def test_ls_stops_inside_a_fused_launch(ops):
     assert it == 0 and np.array_equal(du.download(), u3)
 
 
+def _hs_norm_history(im1, im2, alpha, n):
+    """Per-sweep (||dU||_2 + ||dV||_2)/(H*W) of the oracle's float32 sweeps."""
+    fx, fy, ft = orc.hs_derivs(im1, im2)
+    U = np.zeros(im1.shape, np.float32); V = np.zeros(im1.shape, np.float32)
+    hist = []
+    for _ in range(n):
+        Un, Vn = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        du = (Un - U).astype(np.float64); dv = (Vn - V).astype(np.float64)
+        hist.append(np.float32((np.sqrt((du * du).sum()) + np.sqrt((dv * dv).sum())) / U.size))
+        U, V = Un, Vn
+    return np.array(hist, np.float32), U, V
+
+
+@pytest.mark.parametrize("H,W,depth", [(160, 230, 0), (97, 131, 3), (64, 64, 8)])
+def test_hs_residual_history_matches_oracle(ops, H, W, depth):
+    """ofri_hs_tol with no tolerance: the per-sweep update norms recorded on the
+    device equal the oracle's, and the flow equals ofri_hs's bit for bit."""
+    from ofri.device import DeviceArray, context
+    im1, im2 = _piv(H, W, seed=H)
+    n = 23
+    ref_hist, U32, V32 = _hs_norm_history(im1, im2, 21.0, n)
+    Uo, Vo, _ = orc.hs(im1, im2, 21.0, n)
+    assert np.array_equal(U32, Uo) and np.array_equal(V32, Vo)
+    ctx = context()
+    ctx.set_tuning(hs_depth=depth)
+    try:
+        U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+        hist = DeviceArray.zeros((n,))
+        err, it = ops.hs_tol(im1, im2, 21.0, n, -1.0, U, V, hist)
+    finally:
+        ctx.set_tuning()
+    assert it == n
+    assert np.array_equal(U.download(), U32) and np.array_equal(V.download(), V32)
+    h = hist.download()
+    assert np.allclose(h, ref_hist, rtol=1e-6, atol=0), np.abs(h / ref_hist - 1).max()
+    assert err == h[-1]
+
+
+def test_hs_tol_stops_inside_a_fused_launch(ops):
+    """A tolerance equal to the 9th sweep's norm stops after the first sweep
+    whose norm is at or below it; that sweep sits inside a fused launch and the
+    result must equal ofri_hs run for exactly that many sweeps."""
+    from ofri.device import DeviceArray, context
+    H, W = 120, 150
+    im1, im2 = _piv(H, W, seed=31)
+    ref_hist, _, _ = _hs_norm_history(im1, im2, 45.0, 40)
+    ctx = context()
+    ctx.set_tuning(hs_depth=5)
+    try:
+        hist = DeviceArray.zeros((40,))
+        U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+        ops.hs_tol(im1, im2, 45.0, 40, -1.0, U, V, hist)
+        h = hist.download()
+        tol = float(h[8])
+        stop = int(np.argmax(h <= np.float32(tol))) + 1
+        assert stop % 5 != 0            # not at a launch boundary
+        U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+        err, it = ops.hs_tol(im1, im2, 45.0, 40, tol, U, V)
+        assert it == stop and err == h[stop - 1]
+        Ur = DeviceArray.zeros((H, W)); Vr = DeviceArray.zeros((H, W))
+        ops.hs(im1, im2, 45.0, stop, Ur, Vr)
+        assert np.array_equal(U.download(), Ur.download()) and np.array_equal(V.download(), Vr.download())
+        # zero sweeps leaves the flow untouched
+        U0 = DeviceArray.from_host(np.full((H, W), 0.5, np.float32))
+        err, it = ops.hs_tol(im1, im2, 45.0, 0, 0.0, U0, V)
+        assert it == 0 and np.all(U0.download() == 0.5)
+    finally:
+        ctx.set_tuning()
+
+
 def test_ls_small_golden(ops, golden):
     from ofri.device import DeviceArray
     g = golden("ls_small")
