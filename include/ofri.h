@@ -181,6 +181,28 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
                   const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
                   ofri_level_stats* h_stats);
 
+/* One image pair of a batch: device frames at t and t+1 (H x W) and the
+ * device outputs U, V (H x W) the pair's accumulated flow is written to. */
+typedef struct ofri_pair {
+    const float* im1;
+    const float* im2;
+    int H, W;
+    float* U;
+    float* V;
+} ofri_pair;
+
+/* Batched LS(HS): ofri_pyr_lshs for n pairs of any sizes in one call, with
+ * the same parameters for every pair.  Each pair's result equals what
+ * ofri_pyr_lshs returns for it alone, bit for bit.  The Horn-Schunck and
+ * Liu-Shen sweeps of all pairs at a level run in shared launches (each pair
+ * keeps its own Liu-Shen stop test), and repeated calls on the same buffers
+ * replay one HIP graph of the whole batch.  h_stats (may be NULL) receives
+ * n * levels entries, pair-major.  The reference has no batch entry point: a
+ * caller loops genericPyramidalOpticalFlow (GenericPyramidalOpticalFlow.py:237)
+ * over the pairs. */
+int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lshs_params* p,
+                    ofri_level_stats* h_stats);
+
 /* Tuning knobs of the temporally blocked sweeps.  0 = the library's
  * per-image-size default (measured on gfx950); a variant k >= 1 selects the
  * k-th compiled tile shape. */

@@ -152,6 +152,45 @@ def _lshs_device(im1, im2, FILTER, hs, levels, FILTER_OPT, ls):
     return Ud.download(), Vd.download()
 
 
+def genericPyramidalOpticalFlowBatch(pairs, FILTER, mainOFlowAlgoAdapter, pyramidalLevels=1, kLevels=1,
+                                     FILTER_OPT=None, optionalOFlowAlgoAdapter=None, warping=True, biLinear=True,
+                                     pyramidalIntermediateScaling=True, pyramidalScaling=False):
+    """genericPyramidalOpticalFlow for many (im1, im2) pairs of any sizes with one
+    set of parameters.  Returns a list of (Uaccum, Vaccum), each equal to what
+    genericPyramidalOpticalFlow returns for that pair alone.
+
+    With the Horn-Schunck main adapter and the Liu-Shen (or no) optional
+    adapter on the adapters' default options, the whole batch is one device
+    call (ofri_batch_lshs) and the main adapter's alphas are popped once for
+    the batch, as one driver call pops them.  Any other combination runs the
+    driver per pair on a copy of each adapter."""
+    import copy
+    main, optional = mainOFlowAlgoAdapter, optionalOFlowAlgoAdapter
+    pairs = [(np.ascontiguousarray(a, dtype=np.float32), np.ascontiguousarray(b, dtype=np.float32))
+             for a, b in pairs]
+    for a, b in pairs:
+        if a.shape != b.shape:
+            raise ValueError("images differ in shape: %s vs %s" % (a.shape, b.shape))
+    # the HS adapter's defaults switch warping, biLinear and final scaling on
+    native = (_native_lshs(main, optional) and main.hasGenericPyramidalDefaults() and kLevels == 1 and
+              pyramidalLevels >= 1 and (pyramidalLevels == 1 or pyramidalIntermediateScaling) and
+              len(main.alphas) >= pyramidalLevels and
+              all(min(min(s) for s in _level_shapes(a.shape[0], a.shape[1], pyramidalLevels)) >= 4
+                  for a, _ in pairs))
+    if not native:
+        return [genericPyramidalOpticalFlow(a, b, FILTER, copy.deepcopy(main), pyramidalLevels, kLevels,
+                                            FILTER_OPT, copy.deepcopy(optional), warping, biLinear,
+                                            pyramidalIntermediateScaling, pyramidalScaling) for a, b in pairs]
+    alphas = [main.alphas.pop() for _ in range(pyramidalLevels)]
+    filt = prepareGaussianKernel(FILTER, 3) if FILTER > 1e-3 else None
+    opt = None
+    if optional is not None and FILTER_OPT is not None and FILTER_OPT > 1e-3:
+        opt = prepareGaussianKernel(FILTER_OPT, 5)
+    out, _ = ops.batch_lshs(pairs, pyramidalLevels, alphas, main.Niter, filt, opt,
+                            optional.alpha if optional is not None else 0.0, stats=False)
+    return [(U.download(), V.download()) for U, V in out]
+
+
 def _generic_levels(im1, im2, FILTER, main, levels, kLevels, FILTER_OPT, optional, warping, biLinear,
                     interScaling, finalScaling, shapes):
     Uaccum = Vaccum = U = V = None

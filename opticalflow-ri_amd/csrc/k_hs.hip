@@ -77,6 +77,7 @@ struct HsPass {
     const float* fy;
     const float* ft;
     int H, W;
+    int tiles_x, tiles_y;  // tile grid of this image (batched launches)
     int K;        // halo depth of the tiling (>= nsweeps)
     int nsweeps;  // sweeps in this launch
     float a2;     // alpha*alpha in float32
@@ -129,24 +130,17 @@ __device__ __forceinline__ float quotient(float num, float den, float rcp) {
 // The extended tile is loaded as the mirror extension of the image (every
 // field, including fx, fy, ft), so the image border needs no special case:
 // the mirror symmetry of the extension is preserved exactly by every sweep.
+// One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
+// one being number blk; nsw sweeps from ping-pong buffer src.
 template <int R, int WAVES, bool NORM>
-__global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
-    int nsw = a.nsweeps, src = a.src;
-    if (NORM) {
-        if (a.redo) {
-            nsw = a.ctrl->redo;
-            src = a.ctrl->redo_in;
-            if (nsw <= 0) return;
-        } else if (a.ctrl->done) {
-            return;
-        }
-    }
+__device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int bx, int by, int blk,
+                                        int nblk) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int K = a.K, H = a.H, W = a.W;
     const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
-    const int gx = (int)blockIdx.x * outW - K + lane;
-    const int gy0 = (int)blockIdx.y * outH - K + w * R;
+    const int gx = bx * outW - K + lane;
+    const int gy0 = by * outH - K + w * R;
     const int cx = mirror_idx(gx, W);
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
@@ -168,8 +162,6 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
     __shared__ double red[WAVES][2];
-    const int nblk = gridDim.x * gridDim.y;
-    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
         edge[b][w][0][0][lane] = U[0];
@@ -248,7 +240,50 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
             }
         }
     }
-    if (NORM && !a.redo) arrive_and_check(a.part, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+    if (NORM && !a.redo) arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+}
+
+// Start of a launch in residual mode: the re-run launch takes its sweep count
+// and input buffer from the control word; a finished solve does nothing.
+__device__ __forceinline__ bool hs_launch_args(const HsPass& a, int& nsw, int& src) {
+    nsw = a.nsweeps;
+    src = a.src;
+    if (a.redo) {
+        nsw = a.ctrl->redo;
+        src = a.ctrl->redo_in;
+        return nsw > 0;
+    }
+    return !a.ctrl->done;
+}
+
+template <int R, int WAVES, bool NORM>
+__global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
+    int nsw = a.nsweeps, src = a.src;
+    if (NORM && !hs_launch_args(a, nsw, src)) return;
+    hs_tile<R, WAVES, NORM>(a, nsw, src, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
+                            gridDim.x * gridDim.y);
+}
+
+// Many images in one launch: workgroup b belongs to the image p with
+// blk0[p] <= b < blk0[p+1] (blk0 ascending, blk0[0] = 0) and is tile
+// b - blk0[p] of it, row-major over its tiles_x columns of tiles.  Every image
+// shares the tile shape, depth and sweep count; pointers and sizes come from
+// its own HsPass in the device table.
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void hs_batch_k(const HsPass* __restrict__ tab,
+                                                         const int* __restrict__ blk0, int n,
+                                                         int nsweeps, int src) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const HsPass& a = tab[lo];
+    const int local = b - blk0[lo];
+    hs_tile<R, WAVES, false>(a, nsweeps, src, local % a.tiles_x, local / a.tiles_x, local,
+                             a.tiles_x * a.tiles_y);
 }
 
 // Tile shapes (rows per lane, wavefronts per workgroup) compiled in; the
@@ -257,8 +292,9 @@ struct HsVariant {
     int R, waves;
     void (*kernel)(HsPass);   // fixed sweep count
     void (*normed)(HsPass);   // with per-sweep residuals and the stop test
+    void (*batched)(const HsPass*, const int*, int, int, int);
 };
-#define HS_SHAPE(R, WV) {R, WV, hs_pass_k<R, WV, false>, hs_pass_k<R, WV, true>}
+#define HS_SHAPE(R, WV) {R, WV, hs_pass_k<R, WV, false>, hs_pass_k<R, WV, true>, hs_batch_k<R, WV>}
 const HsVariant kHsVariants[] = {
     HS_SHAPE(16, 4), HS_SHAPE(8, 4), HS_SHAPE(8, 8), HS_SHAPE(12, 4), HS_SHAPE(16, 2),
     HS_SHAPE(4, 8), HS_SHAPE(24, 2), HS_SHAPE(12, 8),
@@ -369,6 +405,93 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
 }  // namespace ofri
 
 namespace ofri {
+
+size_t hs_batch_table_bytes(int n) {
+    return align256((size_t)n * sizeof(HsPass)) + align256((size_t)(n + 1) * sizeof(int));
+}
+
+// Batched Horn-Schunck: derivatives per image, then every sweep launch covers
+// the tiles of all n images (one tile shape and depth for the batch, chosen
+// for the batch's total pixel count unless tuned).
+int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter, void* d_tab,
+             bool upload) {
+    if (n < 1) return fail(OFRI_E_ARG, "hs_batch: no images");
+    if (niter < 0) return fail(OFRI_E_ARG, "hs_batch: negative iteration count");
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (im[i].H < 2 || im[i].W < 2)
+            return fail(OFRI_E_ARG, "hs_batch: image %d is %dx%d", i, im[i].H, im[i].W);
+        total += (size_t)im[i].H * im[i].W;
+    }
+    int vi, depth;
+    hs_default(total, &vi, &depth);
+    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
+    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
+    const HsVariant& var = kHsVariants[vi];
+    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
+        return fail(OFRI_E_ARG, "hs: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
+
+    void* buf;
+    OFRI_TRY(scratch(ctx, "hsb_fields", 7 * total * sizeof(float), &buf));
+    std::vector<HsPass> tab(n);
+    std::vector<int> blk0(n + 1);
+    float* f = (float*)buf;
+    blk0[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const int H = im[i].H, W = im[i].W;
+        const size_t N = (size_t)H * W;
+        float *fx = f, *fy = f + N, *ft = f + 2 * N, *Ub = f + 3 * N, *Vb = f + 4 * N;
+        float *U0 = f + 5 * N, *V0 = f + 6 * N;
+        f += 7 * N;
+        dim3 g((W + 255) / 256, H);
+        hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, fx, fy, ft);
+        OFRI_HIP(hipGetLastError());
+        if (im[i].d_err) {
+            OFRI_HIP(hipMemcpyAsync(U0, im[i].U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            OFRI_HIP(hipMemcpyAsync(V0, im[i].V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        HsPass& p = tab[i];
+        p = HsPass{};
+        p.in[0][0] = im[i].U; p.in[0][1] = im[i].V; p.in[1][0] = Ub; p.in[1][1] = Vb;
+        p.out[0][0] = im[i].U; p.out[0][1] = im[i].V; p.out[1][0] = Ub; p.out[1][1] = Vb;
+        p.fx = fx; p.fy = fy; p.ft = ft;
+        p.H = H; p.W = W; p.N = N;
+        p.a2 = alpha * alpha;
+        p.K = depth;
+        p.tiles_x = (W + outW - 1) / outW;
+        p.tiles_y = (H + outH - 1) / outH;
+        blk0[i + 1] = blk0[i] + p.tiles_x * p.tiles_y;
+    }
+    HsPass* d_pass = (HsPass*)d_tab;
+    int* d_blk0 = (int*)((char*)d_tab + align256((size_t)n * sizeof(HsPass)));
+    if (upload) {
+        OFRI_HIP(hipMemcpyAsync(d_pass, tab.data(), n * sizeof(HsPass), hipMemcpyHostToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(d_blk0, blk0.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    int done = 0, src = 0;
+    while (done < niter) {
+        const int ns = niter - done < depth ? niter - done : depth;
+        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, ns, src);
+        OFRI_HIP(hipGetLastError());
+        src ^= 1;
+        done += ns;
+    }
+    f = (float*)buf;
+    for (int i = 0; i < n; ++i) {
+        const size_t N = (size_t)im[i].H * im[i].W;
+        if (src != 0) {
+            OFRI_HIP(hipMemcpyAsync(im[i].U, f + 3 * N, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            OFRI_HIP(hipMemcpyAsync(im[i].V, f + 4 * N, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        if (im[i].d_err) OFRI_TRY(diff_norm_err(ctx, im[i].U, f + 5 * N, im[i].V, f + 6 * N, N, im[i].d_err));
+        f += 7 * N;
+    }
+    return OFRI_OK;
+}
+
 // Effective Horn-Schunck launch shape for an image of npx pixels (1-based variant).
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
     int vi, d;

@@ -93,6 +93,7 @@ struct LsPass {
     float* hist;            // per-sweep norms (nullptr: not recorded)
     SolveCtrl* ctrl;
     int redo;               // 1: re-run launch driven by ctrl->redo
+    int tiles_x, tiles_y;   // tile grid of this image (batched launches)
 };
 
 struct Raw {
@@ -175,8 +176,8 @@ __device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Ra
 }
 
 template <int R, int WAVES, bool EDGE>
-__device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int src, int lane, int w, int gx,
-                                          int gy0, float (&u)[R], float (&v)[R], const float (&IIX)[R],
+__device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int nblk, int lane, int w,
+                                          int gx, int gy0, float (&u)[R], float (&v)[R], const float (&IIX)[R],
                                           const float (&IIY)[R], const float (&II)[R], const float (&IXT)[R],
                                           const float (&IYT)[R], const float (&B11)[R], const float (&B12)[R],
                                           const float (&B22)[R]) {
@@ -186,8 +187,6 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int src, int
     const bool el = gx == 0, er = gx == W - 1;
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
     const float h = a.h;
-    const int nblk = gridDim.x * gridDim.y;
-    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
         edge[b][w][0][0][lane] = u[0];
@@ -248,21 +247,29 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int src, int
     }
 }
 
-template <int R, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
-    int nsw = a.nsweeps, src = a.src;
+// Start of a launch: the re-run launch takes its sweep count and input buffer
+// from the control word; a finished solve does nothing.
+__device__ __forceinline__ bool ls_launch_args(const LsPass& a, int& nsw, int& src) {
+    nsw = a.nsweeps;
+    src = a.src;
     if (a.redo) {
         nsw = a.ctrl->redo;
         src = a.ctrl->redo_in;
-        if (nsw <= 0) return;
-    } else if (a.ctrl->done) {
-        return;
+        return nsw > 0;
     }
+    return !a.ctrl->done;
+}
+
+// One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
+// one being number blk.
+template <int R, int WAVES>
+__device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int bx, int by, int blk,
+                                        int nblk) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int K = a.K, H = a.H, W = a.W;
     const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
-    const int x0 = (int)blockIdx.x * outW - K, y0 = (int)blockIdx.y * outH - K;
+    const int x0 = bx * outW - K, y0 = by * outH - K;
     const int gx = x0 + lane;
     const int gy0 = y0 + w * R;
     const int cx = clampi(gx, 0, W - 1);
@@ -289,9 +296,11 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
         B22[r] = a.F[7 * N + q];
     }
     if (edge_tile)
-        ls_sweeps<R, WAVES, true>(a, nsw, src, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12, B22);
+        ls_sweeps<R, WAVES, true>(a, nsw, blk, nblk, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12,
+                                  B22);
     else
-        ls_sweeps<R, WAVES, false>(a, nsw, src, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12, B22);
+        ls_sweeps<R, WAVES, false>(a, nsw, blk, nblk, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12,
+                                   B22);
 
     float* uo = a.out[src ^ 1][0];
     float* vo = a.out[src ^ 1][1];
@@ -306,18 +315,55 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
             }
         }
     }
-    if (a.part) arrive_and_check(a.part, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+    if (a.part) arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+}
+
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
+    int nsw, src;
+    if (!ls_launch_args(a, nsw, src)) return;
+    ls_tile<R, WAVES>(a, nsw, src, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
+                      gridDim.x * gridDim.y);
+}
+
+// Many images in one launch (workgroup b is tile b - blk0[p] of image p,
+// blk0 ascending from 0): each image has its own coefficient fields, buffers,
+// partial sums and control word in its LsPass, so each stops on its own; the
+// launch-wide sweep count, input buffer and re-run flag come as arguments.
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ls_batch_k(const LsPass* __restrict__ tab,
+                                                         const int* __restrict__ blk0, int n,
+                                                         int nsweeps, int src0, int redo) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    LsPass a = tab[lo];
+    a.nsweeps = nsweeps;
+    a.src = src0;
+    a.redo = redo;
+    if (redo) a.part = nullptr;
+    int nsw, src;
+    if (!ls_launch_args(a, nsw, src)) return;
+    const int local = b - blk0[lo];
+    ls_tile<R, WAVES>(a, nsw, src, local % a.tiles_x, local / a.tiles_x, local, a.tiles_x * a.tiles_y);
 }
 
 
 struct LsVariant {
     int R, waves;
     void (*kernel)(LsPass);
+    void (*batched)(const LsPass*, const int*, int, int, int, int);
 };
+#define LS_SHAPE(R, WV) {R, WV, ls_pass_k<R, WV>, ls_batch_k<R, WV>}
 const LsVariant kLsVariants[] = {
-    {8, 8, ls_pass_k<8, 8>}, {8, 4, ls_pass_k<8, 4>}, {4, 8, ls_pass_k<4, 8>}, {12, 4, ls_pass_k<12, 4>},
-    {2, 16, ls_pass_k<2, 16>}, {4, 16, ls_pass_k<4, 16>}, {6, 8, ls_pass_k<6, 8>},
+    LS_SHAPE(8, 8), LS_SHAPE(8, 4), LS_SHAPE(4, 8), LS_SHAPE(12, 4),
+    LS_SHAPE(2, 16), LS_SHAPE(4, 16), LS_SHAPE(6, 8),
 };
+#undef LS_SHAPE
 constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
 // Defaults measured by the autotuner on gfx950.
 void ls_default(size_t npx, int* variant, int* depth) {
@@ -395,6 +441,104 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
 }  // namespace ofri
 
 namespace ofri {
+
+size_t ls_batch_table_bytes(int n) {
+    return align256((size_t)n * sizeof(LsPass)) + align256((size_t)(n + 1) * sizeof(int));
+}
+
+// Batched Liu-Shen: setup per image, then every sweep launch covers the tiles
+// of all n images; each image runs its own stop test on its own control word
+// (a finished image's workgroups return at once), and one re-run launch
+// finishes every image that stopped inside a fused launch.
+int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, double tol,
+             SolveCtrl* d_ctrls, void* d_tab, bool upload) {
+    if (n < 1) return fail(OFRI_E_ARG, "ls_batch: no images");
+    if (maxnum < 0) return fail(OFRI_E_ARG, "ls_batch: negative sweep budget");
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (im[i].H < 2 || im[i].W < 2)
+            return fail(OFRI_E_ARG, "ls_batch: image %d is %dx%d", i, im[i].H, im[i].W);
+        total += (size_t)im[i].H * im[i].W;
+    }
+    int vi, depth;
+    ls_default(total, &vi, &depth);
+    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
+    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
+    const LsVariant& var = kLsVariants[vi];
+    if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
+        return fail(OFRI_E_ARG, "ls: depth %d too large for tile %dx%d", depth, 64, var.R * var.waves);
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
+    size_t nparts = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t tiles = (size_t)((im[i].W + outW - 1) / outW) * ((im[i].H + outH - 1) / outH);
+        nparts += (size_t)depth * tiles * 2;
+    }
+    void *fbuf, *pbuf, *mbuf;
+    OFRI_TRY(scratch(ctx, "lsb_fields", 10 * total * sizeof(float), &fbuf));
+    OFRI_TRY(scratch(ctx, "lsb_part", nparts * sizeof(double), &pbuf));
+    OFRI_TRY(scratch(ctx, "lsb_max", 2 * (size_t)n * sizeof(float), &mbuf));
+    std::vector<LsPass> tab(n);
+    std::vector<int> blk0(n + 1);
+    float* f = (float*)fbuf;
+    double* part = (double*)pbuf;
+    blk0[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const int H = im[i].H, W = im[i].W;
+        const size_t N = (size_t)H * W;
+        float* F = f;
+        float *ub = F + 8 * N, *vb = F + 9 * N;
+        f += 10 * N;
+        float* mx = (float*)mbuf + 2 * i;
+        OFRI_TRY(reduce_max(ctx, im[i].im1, N, mx));
+        OFRI_TRY(reduce_max(ctx, im[i].im2, N, mx + 1));
+        dim3 g((W + 255) / 256, H);
+        ls_setup_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, mx, h, F, N);
+        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrls + i);
+        OFRI_HIP(hipGetLastError());
+        LsPass& p = tab[i];
+        p = LsPass{};
+        p.in[0][0] = im[i].U; p.in[0][1] = im[i].V; p.in[1][0] = ub; p.in[1][1] = vb;
+        p.out[0][0] = im[i].U; p.out[0][1] = im[i].V; p.out[1][0] = ub; p.out[1][1] = vb;
+        p.F = F; p.N = N; p.H = H; p.W = W; p.K = depth; p.h = h;
+        p.tol = tol;
+        p.maxnum = maxnum;
+        p.ctrl = d_ctrls + i;
+        p.tiles_x = (W + outW - 1) / outW;
+        p.tiles_y = (H + outH - 1) / outH;
+        p.part = part;
+        part += (size_t)depth * p.tiles_x * p.tiles_y * 2;
+        blk0[i + 1] = blk0[i] + p.tiles_x * p.tiles_y;
+    }
+    LsPass* d_pass = (LsPass*)d_tab;
+    int* d_blk0 = (int*)((char*)d_tab + align256((size_t)n * sizeof(LsPass)));
+    if (upload) {
+        OFRI_HIP(hipMemcpyAsync(d_pass, tab.data(), n * sizeof(LsPass), hipMemcpyHostToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(d_blk0, blk0.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    int done = 0, src = 0;
+    while (done < maxnum) {
+        const int ns = maxnum - done < depth ? maxnum - done : depth;
+        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, ns, src, 0);
+        OFRI_HIP(hipGetLastError());
+        src ^= 1;
+        done += ns;
+    }
+    if (maxnum > 0) {
+        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, 0, 0, 1);
+        OFRI_HIP(hipGetLastError());
+        f = (float*)fbuf;
+        for (int i = 0; i < n; ++i) {
+            const size_t N = (size_t)im[i].H * im[i].W;
+            OFRI_TRY(select_copy(ctx, d_ctrls + i, im[i].U, f + 8 * N, im[i].U, N));
+            OFRI_TRY(select_copy(ctx, d_ctrls + i, im[i].V, f + 9 * N, im[i].V, N));
+            f += 10 * N;
+        }
+    }
+    return OFRI_OK;
+}
+
 // Effective Liu-Shen launch shape for an image of npx pixels (1-based variant).
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
     int vi, d;

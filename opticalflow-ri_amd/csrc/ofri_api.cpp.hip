@@ -106,8 +106,10 @@ int ofri_destroy(ofri_ctx* ctx) {
         (void)hipFree(kv.second.d_w);
         (void)hipFree(kv.second.d_start);
     }
-    for (auto& kv : ctx->graphs)
+    for (auto& kv : ctx->graphs) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.tab.p) (void)hipFree(kv.second.tab.p);
+    }
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -434,6 +436,129 @@ static std::string lshs_key(ofri_ctx* ctx, const float* a, const float* b, int H
     return k;
 }
 
+// Enqueues a batch of LS(HS) pyramids.  Every pair is taken through the same
+// steps as pyr_lshs_enqueue; the light per-pixel steps run per pair, the
+// Horn-Schunck and Liu-Shen sweeps of a level in launches shared by all pairs.
+// Statistics land in "batch_stats": [L][n] SolveCtrl, then [L][n] HS errors.
+static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const ofri_lshs_params* p,
+                              void* d_tab, bool upload) {
+    const int L = p->levels;
+    std::vector<std::vector<int>> lh(n), lw(n);
+    size_t totalF = 0;
+    for (int i = 0; i < n; ++i) {
+        level_sizes(pr[i].H, pr[i].W, L, lh[i], lw[i]);
+        totalF += (size_t)pr[i].H * pr[i].W;
+    }
+    void* ws;
+    OFRI_TRY(scratch(ctx, "batch_ws", 14 * totalF * sizeof(float), &ws));
+    void* st;
+    OFRI_TRY(scratch(ctx, "batch_stats", (size_t)L * n * (sizeof(SolveCtrl) + sizeof(float)), &st));
+    SolveCtrl* ctrls = (SolveCtrl*)st;
+    float* hs_errs = (float*)(ctrls + (size_t)L * n);
+    const size_t tab_level = align256(hs_batch_table_bytes(n)) + align256(ls_batch_table_bytes(n));
+
+    struct PairWs {
+        float *n1, *n2, *w1, *w2, *f1, *f2, *o1, *o2, *U, *V, *Ua, *Va, *Ub, *Vb;
+        float *acc_u, *acc_v;
+        const float *c1, *c2;
+        int ph, pw;
+    };
+    std::vector<PairWs> P(n);
+    float* base = (float*)ws;
+    for (int i = 0; i < n; ++i) {
+        const size_t NF = (size_t)pr[i].H * pr[i].W;
+        float** planes[14] = {&P[i].n1, &P[i].n2, &P[i].w1, &P[i].w2, &P[i].f1, &P[i].f2, &P[i].o1,
+                              &P[i].o2, &P[i].U,  &P[i].V,  &P[i].Ua, &P[i].Va, &P[i].Ub, &P[i].Vb};
+        for (int k = 0; k < 14; ++k) *planes[k] = base + k * NF;
+        base += 14 * NF;
+        P[i].acc_u = P[i].Ua;
+        P[i].acc_v = P[i].Va;
+        P[i].ph = P[i].pw = 0;
+    }
+    std::vector<BatchImage> hsim(n), lsim(n);
+    for (int l = 0; l < L; ++l) {
+        const bool last = l == L - 1;
+        for (int i = 0; i < n; ++i) {
+            PairWs& q = P[i];
+            const int H = pr[i].H, W = pr[i].W, h = lh[i][l], w = lw[i][l];
+            const size_t N = (size_t)h * w;
+            q.c1 = pr[i].im1;
+            q.c2 = pr[i].im2;
+            if (h != H || w != W) {
+                OFRI_TRY(resize(ctx, pr[i].im1, H, W, q.n1, h, w, 3));
+                OFRI_TRY(resize(ctx, pr[i].im2, H, W, q.n2, h, w, 3));
+                q.c1 = q.n1;
+                q.c2 = q.n2;
+            }
+            const float *s1 = q.c1, *s2 = q.c2;
+            float* out_u = last ? pr[i].U : (q.acc_u == q.Ua ? q.Ub : q.Ua);
+            float* out_v = last ? pr[i].V : (q.acc_v == q.Va ? q.Vb : q.Va);
+            if (l > 0) {
+                if (h != q.ph || w != q.pw) {
+                    float sx = (float)w / (float)q.pw, sy = (float)h / (float)q.ph;
+                    OFRI_TRY(spline_upsample(ctx, q.acc_u, q.ph, q.pw, out_u, h, w, sx));
+                    OFRI_TRY(spline_upsample(ctx, q.acc_v, q.ph, q.pw, out_v, h, w, sy));
+                } else {
+                    OFRI_HIP(hipMemcpyAsync(out_u, q.acc_u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                    OFRI_HIP(hipMemcpyAsync(out_v, q.acc_v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                }
+                OFRI_TRY(warp_symmetric(ctx, q.c1, q.c2, out_u, out_v, h, w, q.w1, q.w2));
+                s1 = q.w1;
+                s2 = q.w2;
+            } else {
+                OFRI_TRY(fill_zero(ctx, out_u, N));
+                OFRI_TRY(fill_zero(ctx, out_v, N));
+            }
+            if (p->filt_w && p->filt_n > 0) {
+                OFRI_TRY(gauss_sep(ctx, s1, q.f1, h, w, p->filt_w, p->filt_n));
+                OFRI_TRY(gauss_sep(ctx, s2, q.f2, h, w, p->filt_w, p->filt_n));
+                s1 = q.f1;
+                s2 = q.f2;
+            }
+            OFRI_TRY(fill_zero(ctx, q.U, N));
+            OFRI_TRY(fill_zero(ctx, q.V, N));
+            hsim[i] = BatchImage{s1, s2, h, w, q.U, q.V, hs_errs + (size_t)l * n + i};
+            q.acc_u = out_u;  // the accumulation target of this level
+            q.acc_v = out_v;
+        }
+        char* tab = (char*)d_tab + l * tab_level;
+        OFRI_TRY(hs_batch(ctx, hsim.data(), n, p->alphas[l], p->niter, tab, upload));
+        if (p->ls_alpha > 0.0f) {
+            for (int i = 0; i < n; ++i) {
+                PairWs& q = P[i];
+                const int h = lh[i][l], w = lw[i][l];
+                const float *q1 = q.c1, *q2 = q.c2;
+                if (p->opt_w && p->opt_n > 0) {
+                    OFRI_TRY(gauss_sep(ctx, q.c1, q.o1, h, w, p->opt_w, p->opt_n));
+                    OFRI_TRY(gauss_sep(ctx, q.c2, q.o2, h, w, p->opt_w, p->opt_n));
+                    q1 = q.o1;
+                    q2 = q.o2;
+                }
+                // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
+                lsim[i] = BatchImage{q1, q2, h, w, q.V, q.U, nullptr};
+            }
+            OFRI_TRY(ls_batch(ctx, lsim.data(), n, p->ls_alpha, p->ls_maxnum, p->ls_tol, ctrls + (size_t)l * n,
+                              tab + align256(hs_batch_table_bytes(n)), upload));
+        }
+        for (int i = 0; i < n; ++i) {
+            PairWs& q = P[i];
+            const size_t N = (size_t)lh[i][l] * lw[i][l];
+            OFRI_TRY(accumulate(ctx, q.acc_u, q.U, N));
+            OFRI_TRY(accumulate(ctx, q.acc_v, q.V, N));
+            q.ph = lh[i][l];
+            q.pw = lw[i][l];
+        }
+    }
+    return OFRI_OK;
+}
+
+static std::string batch_key(ofri_ctx* ctx, const ofri_pair* pr, int n, const ofri_lshs_params* p) {
+    std::string k = "batch";
+    k.append((const char*)pr, sizeof(ofri_pair) * n);
+    k += lshs_key(ctx, nullptr, nullptr, 0, 0, p, nullptr, nullptr);
+    return k;
+}
+
 }  // namespace ofri
 
 extern "C" {
@@ -506,6 +631,93 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
             h_stats[l].hs_err = he[l];
             h_stats[l].ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
             h_stats[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
+        }
+    }
+    return OFRI_OK;
+}
+
+int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lshs_params* p,
+                    ofri_level_stats* h_stats) {
+    CHECK_CTX(ctx);
+    if (!p || !pairs) return fail(OFRI_E_ARG, "ofri_batch_lshs: null pointer");
+    if (n < 1) return fail(OFRI_E_ARG, "ofri_batch_lshs: %d pairs", n);
+    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_batch_lshs: levels %d", p->levels);
+    if (!p->alphas) return fail(OFRI_E_ARG, "ofri_batch_lshs: alphas missing");
+    const int L = p->levels;
+    for (int i = 0; i < n; ++i) {
+        const ofri_pair& q = pairs[i];
+        if (!q.im1 || !q.im2 || !q.U || !q.V) return fail(OFRI_E_ARG, "ofri_batch_lshs: pair %d: null pointer", i);
+        std::vector<int> lh, lw;
+        level_sizes(q.H, q.W, L, lh, lw);
+        for (int l = 0; l < L; ++l)
+            if (lh[l] < 4 || lw[l] < 4)
+                return fail(OFRI_E_ARG, "ofri_batch_lshs: pair %d level %d size %dx%d below 4x4", i, l + 1, lh[l],
+                            lw[l]);
+    }
+    const std::string key = batch_key(ctx, pairs, n, p);
+    GraphEntry& g = ctx->graphs[key];
+    const size_t tab_bytes = (size_t)L * (align256(hs_batch_table_bytes(n)) + align256(ls_batch_table_bytes(n)));
+    if (g.tab.bytes < tab_bytes) {
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (g.tab.p) OFRI_HIP(hipFree(g.tab.p));
+        g.tab.p = nullptr;
+        g.tab.bytes = 0;
+        OFRI_HIP(hipMalloc(&g.tab.p, tab_bytes));
+        g.tab.bytes = tab_bytes;
+    }
+    if (!ctx->graphs_on) {
+        OFRI_TRY(batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, true));
+    } else if (g.exec && g.ws_gen == ctx->ws_gen) {
+        OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
+        if (g.exec) {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        // The job tables hold scratch addresses; when a scratch buffer grew during
+        // this run, tables written before the growth are stale, so the next call
+        // runs eagerly again instead of capturing.
+        const uint64_t gen0 = ctx->ws_gen;
+        OFRI_TRY(batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, true));
+        g.eager_runs = ctx->ws_gen == gen0 ? g.eager_runs + 1 : 0;
+        g.ws_gen = ctx->ws_gen;
+    } else {
+        hipGraph_t graph;
+        OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, false);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (st != OFRI_OK) return st;
+        if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            g.exec = nullptr;
+            return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+        }
+        g.ws_gen = ctx->ws_gen;
+        OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    }
+    if (h_stats) {
+        void* st;
+        OFRI_TRY(scratch(ctx, "batch_stats", (size_t)L * n * (sizeof(SolveCtrl) + sizeof(float)), &st));
+        std::vector<SolveCtrl> hc((size_t)L * n);
+        std::vector<float> he((size_t)L * n);
+        OFRI_HIP(hipMemcpyAsync(hc.data(), st, hc.size() * sizeof(SolveCtrl), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(he.data(), (SolveCtrl*)st + hc.size(), he.size() * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i) {
+            std::vector<int> lh, lw;
+            level_sizes(pairs[i].H, pairs[i].W, L, lh, lw);
+            for (int l = 0; l < L; ++l) {
+                ofri_level_stats& o = h_stats[(size_t)i * L + l];
+                const size_t k = (size_t)l * n + i;
+                o.H = lh[l];
+                o.W = lw[l];
+                o.hs_err = he[k];
+                o.ls_err = p->ls_alpha > 0.0f ? hc[k].last_err : 0.0f;
+                o.ls_iters = p->ls_alpha > 0.0f ? hc[k].sweeps : 0;
+            }
         }
     }
     return OFRI_OK;

@@ -70,6 +70,7 @@ struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     uint64_t ws_gen = 0;
     int eager_runs = 0;  // calls seen with this key before capture
+    DevBuf tab;          // device job tables of a batched call (owned)
 };
 }  // namespace ofri
 
@@ -132,5 +133,28 @@ int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw
 // ---- Liu-Shen (k_ls.hip) ----
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl);
+
+// ---- batched solves: every image's tiles in the same launches ----
+// One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets
+// HS's returned error.  LS: U, V are the solver's u (row) and v (column).
+struct BatchImage {
+    const float* im1;
+    const float* im2;
+    int H, W;
+    float* U;
+    float* V;
+    float* d_err;
+};
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+// Bytes of the device job table of n images (HS / LS).  The table is written
+// when `upload` is true (never while a graph is being captured; a captured
+// launch sequence reads the table its eager run wrote).
+size_t hs_batch_table_bytes(int n);
+size_t ls_batch_table_bytes(int n);
+int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter, void* d_tab,
+             bool upload);
+// d_ctrls: n control words (results: sweeps, last norm)
+int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, double tol,
+             SolveCtrl* d_ctrls, void* d_tab, bool upload);
 
 }  // namespace ofri

@@ -65,12 +65,13 @@ __device__ void solve_check(const double* part, int nblk, int nsw, int dst, doub
     }
 }
 
-// Called by every workgroup at the end of a launch with per-sweep partial
-// sums: the last one to arrive runs solve_check and re-arms the counter.
-__device__ __forceinline__ void arrive_and_check(const double* part, int nsw, int dst, double tol,
-                                                 int maxnum, size_t N, SolveCtrl* ctrl, float* hist) {
+// Called by every workgroup of an image (nblk of them) at the end of a launch
+// with per-sweep partial sums: the last one to arrive runs solve_check and
+// re-arms the counter.
+__device__ __forceinline__ void arrive_and_check(const double* part, int nblk, int nsw, int dst,
+                                                 double tol, int maxnum, size_t N, SolveCtrl* ctrl,
+                                                 float* hist) {
     __shared__ int last;
-    const int nblk = gridDim.x * gridDim.y;
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         __threadfence();
         last = atomicAdd(&ctrl->arrived, 1) == nblk - 1;

@@ -45,6 +45,11 @@ class LevelStats(C.Structure):
                 ("ls_iters", C.c_int)]
 
 
+class Pair(C.Structure):
+    _fields_ = [("im1", C.c_void_p), ("im2", C.c_void_p), ("H", C.c_int), ("W", C.c_int),
+                ("U", C.c_void_p), ("V", C.c_void_p)]
+
+
 class FbParams(C.Structure):
     _fields_ = [
         ("poly_n", C.c_int),
@@ -95,6 +100,7 @@ SIGNATURES = {
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
+    "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],

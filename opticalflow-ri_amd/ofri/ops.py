@@ -170,6 +170,57 @@ class ByteArray:
             pass
 
 
+def _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol):
+    """LshsParams plus the arrays it points into (kept alive by the caller)."""
+    al = np.ascontiguousarray(alphas, dtype=np.float32)
+    if len(al) < levels:
+        raise ValueError("need one alpha per level")
+    fw = None if filt_w is None else np.ascontiguousarray(filt_w, dtype=np.float32)
+    ow = None if opt_w is None else np.ascontiguousarray(opt_w, dtype=np.float32)
+    p = _capi.LshsParams()
+    p.levels = int(levels)
+    p.alphas = _capi.fptr(al)
+    p.niter = int(niter)
+    p.filt_w = _capi.fptr(fw) if fw is not None else None
+    p.filt_n = 0 if fw is None else len(fw)
+    p.opt_w = _capi.fptr(ow) if ow is not None else None
+    p.opt_n = 0 if ow is None else len(ow)
+    p.ls_alpha = float(ls_alpha) if ls_alpha else 0.0
+    p.ls_maxnum = int(ls_maxnum)
+    p.ls_tol = float(ls_tol)
+    return p, (al, fw, ow)
+
+
+def batch_lshs(pairs, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8, out=None,
+               stats=True):
+    """LS(HS) pyramids of many pairs of any sizes in one call (ofri_batch_lshs).
+    pairs: sequence of (im1, im2) (numpy or DeviceArray).  out: optional list of
+    (U, V) DeviceArrays.  Returns (list of (U, V), per-pair list of per-level stats)."""
+    dev = [(as_device(a), as_device(b)) for a, b in pairs]
+    n = len(dev)
+    if n == 0:
+        raise ValueError("batch_lshs: no pairs")
+    for a, b in dev:
+        if a.shape != b.shape:
+            raise ValueError("images differ in shape: %s vs %s" % (a.shape, b.shape))
+    if out is None:
+        out = [(DeviceArray(a.shape), DeviceArray(a.shape)) for a, _ in dev]
+    p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol)
+    tab = (_capi.Pair * n)()
+    for i, ((a, b), (U, V)) in enumerate(zip(dev, out)):
+        tab[i].im1, tab[i].im2 = a.ptr, b.ptr
+        tab[i].H, tab[i].W = a.shape
+        tab[i].U, tab[i].V = U.ptr, V.ptr
+    st = (_capi.LevelStats * (n * levels))() if stats else None
+    _capi.check(_lib().ofri_batch_lshs(_h(), tab, n, C.byref(p), st))
+    res = []
+    if stats:
+        for i in range(n):
+            res.append([dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err),
+                             ls_iters=s.ls_iters) for s in st[i * levels:(i + 1) * levels]])
+    return out, res
+
+
 def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
              Uacc=None, Vacc=None, stats=True):
     """Whole LS(HS) pyramid on the device.  Returns (Uacc, Vacc, stats list)."""

@@ -100,8 +100,10 @@ def render(H, W, xs, ys, d, I0, rng, noise=2.0):
     return img.astype(np.float32)
 
 
-def piv_pair(H, W, density, flow, rng, noise=2.0):
-    """Returns (im1, im2, u, v) with float32 images and the dense flow.
+def piv_pair(H, W, density, flow, rng, noise=2.0, truth=True):
+    """Returns (im1, im2, u, v) with float32 images and the dense flow
+    (u = v = None when truth is False: the dense field of the turbulence
+    family costs far more than rendering).
 
     flow: 'poiseuille' | 'lamb_oseen' | 'turbulence' or a callable
     (x, y) -> (u, v) evaluated at particle positions."""
@@ -123,6 +125,8 @@ def piv_pair(H, W, density, flow, rng, noise=2.0):
     up, vp = f(xs, ys)
     im1 = render(H, W, xs, ys, d, I0, rng, noise)
     im2 = render(H, W, xs + up, ys + vp, d, I0, rng, noise)
+    if not truth:
+        return im1, im2, None, None
     yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
     u, v = f(xx, yy)
     return im1, im2, np.asarray(u, np.float32), np.asarray(v, np.float32)
@@ -148,17 +152,29 @@ def config_pair(cfg, seed=MASTER_SEED, size=None):
     raise ValueError("no single-pair configuration %r" % cfg)
 
 
-def batch_pairs(n_pairs=64, seed=MASTER_SEED, sizes=None):
+def _batch_item(args):
+    H, W, dens, fam, child, truth = args
+    return piv_pair(H, W, dens, fam, np.random.default_rng(child), truth=truth)
+
+
+def batch_pairs(n_pairs=64, seed=MASTER_SEED, sizes=None, truth=False, workers=1):
     """Uneven batch: H, W from {256, 384, ..., 2048}, density U[0.02, 0.08],
-    flow family uniform over the three families (max 4 px)."""
+    flow family uniform over the three families (max 4 px).  Items are
+    (im1, im2, u, v); u, v are None unless truth is True.  workers > 1 renders
+    the pairs in that many processes (same pairs: every pair has its own
+    child seed)."""
     ss = np.random.SeedSequence(seed).spawn(8)[3]
     rng = np.random.default_rng(ss)
     choices = sizes if sizes is not None else np.arange(256, 2049, 128)
-    out = []
+    specs = []
     for child in ss.spawn(n_pairs):
-        r = np.random.default_rng(child)
         H, W = int(rng.choice(choices)), int(rng.choice(choices))
         dens = rng.uniform(0.02, 0.08)
         fam = ["poiseuille", "lamb_oseen", "turbulence"][rng.integers(0, 3)]
-        out.append(piv_pair(H, W, dens, fam, r))
-    return out
+        specs.append((H, W, dens, fam, child, truth))
+    if workers <= 1:
+        return [_batch_item(a) for a in specs]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_batch_item, specs))
