@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="lshs", choices=["lshs", "batch", "fbls", "lk"],
+                    help="lshs: 1024^2 LS(HS) 2-level (default); batch: 64 uneven pairs LS(HS); "
+                         "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level")
+    ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
     return ap.parse_args()
 
 
@@ -65,12 +69,165 @@ def cpu_baseline(im1, im2, fw, ow):
                       "OpenMP %d threads, %.1f s" % (H, W, threads, dt)}
 
 
+def _timed(step, ctx, steps, warmup):
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    ctx.timer_start()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    gpu_ms = ctx.timer_stop()
+    return time.perf_counter() - t0, gpu_ms
+
+
+def _line(metric, value, unit, args, wall, gpu_ms, config, roofline, cpu=None, dtype="f32"):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+           "gpu_ms_per_step": round(gpu_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config, "roofline": roofline}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    return out
+
+
+def bench_batch(args):
+    """64 uneven LS(HS) 2-level pairs in one call per step (ofri_batch_lshs)."""
+    from gaussian_filter import prepareGaussianKernel
+    from ofri import ops, synth
+    pairs = synth.batch_pairs(args.pairs, workers=16)   # rendered before the GPU is touched
+    from ofri.device import DeviceArray, context
+    ctx = context()
+    fw, ow = prepareGaussianKernel(3.4, 3), prepareGaussianKernel(0.48, 5)
+    prm = dict(levels=2, alphas=[45.0, 21.0], niter=600, filt_w=fw, opt_w=ow, ls_alpha=5.0)
+    dev = [(DeviceArray.from_host(p[0]), DeviceArray.from_host(p[1])) for p in pairs]
+    outs = [(DeviceArray(a.shape), DeviceArray(a.shape)) for a, _ in dev]
+    mpx = sum(a.shape[0] * a.shape[1] for a, _ in dev) / 1e6
+
+    def step():
+        ops.batch_lshs(dev, out=outs, stats=False, **prm)
+
+    wall, gpu_ms = _timed(step, ctx, args.steps, args.warmup)
+    _, st = ops.batch_lshs(dev, out=outs, **prm)
+    ls_sw = sum(s[-1]["ls_iters"] * s[-1]["H"] * s[-1]["W"] + s[0]["ls_iters"] * s[0]["H"] * s[0]["W"] for s in st)
+    px = sum(s[-1]["H"] * s[-1]["W"] + s[0]["H"] * s[0]["W"] for s in st)
+    alg = px * (600 * HS_BYTES_PER_PX_SWEEP + 160.0) + ls_sw * 48.0
+    achieved = alg / (gpu_ms / args.steps * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        sizes = [p[0].size for p in pairs]
+        a, b = pairs[int(np.argsort(sizes)[len(sizes) // 2])][:2]      # the median-size pair
+        t0 = time.perf_counter()
+        orc.pyr_lshs(a, b, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(a.size / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": orc.get_threads(),
+               "kind": "port", "sample": "the batch's median-size pair (%dx%d) through the C oracle's LS(HS) "
+                                         "2-level, %.1f s" % (a.shape + (dt,))}
+    return _line("Mpix-pairs/s, batched LS(HS) 2-level, %d uneven pairs" % len(pairs), mpx * args.steps / wall,
+                 "Mpix-pairs/s", args, wall, gpu_ms,
+                 {"workload": "%d pairs H,W in {256..2048}, density U[0.02,0.08], 3 flow families; LS(HS) "
+                              "2-level as the default workload, all pairs per call (ofri_batch_lshs)" % len(pairs),
+                  "pairs_per_step": len(pairs), "total_mpx": round(mpx, 3), "parallelism": "single GPU"},
+                 {"bound": "hbm", "kernel": "whole batch (all launches)", "achieved": round(achieved, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                  "algorithmic_bytes_per_step": round(alg)}, cpu)
+
+
+def bench_fbls(args):
+    """1024^2 Farneback 2-level + Liu-Shen alpha 10 through the drop-in driver."""
+    from Farneback_PyCL import Farneback_PyCL
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    from ofri import synth
+    from ofri.device import context
+    ctx = context()
+    im1, im2, _, _ = synth.config_pair(1)
+
+    def step():
+        genericPyramidalOpticalFlow(im1, im2, 0, Farneback_PyCL(), 2, 1, 0.48, LiuShenOpticalFlowAlgoAdapter(10))
+
+    wall, gpu_ms = _timed(step, ctx, args.steps, args.warmup)
+    n = im1.shape[0]
+    px = n * n + (n // 2) ** 2
+    alg = px * (744.0 + 60 * 48.0 + 56.0)
+    achieved = alg / (wall / args.steps) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        f = Farneback_PyCL()
+        c = f._consts()
+        h = n
+        a, b = im1[:h, :h].copy(), im2[:h, :h].copy()
+        z = np.zeros_like(a)
+        t0 = time.perf_counter()
+        orc.farneback(a, b, z, z, c)
+        orc.ls(a, b, 10.0, z, z, maxnum=60, tol=1e-8)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(a.size / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": orc.get_threads(),
+               "kind": "port", "sample": "the %dx%d finest level alone: C oracle Farneback (5 iterations) + "
+                                         "Liu-Shen 60 sweeps, %.1f s (pyramid steps and the coarse level left out)"
+                                         % (h, h, dt)}
+    return _line("Mpix-pairs/s, LS(Farneback) 2-level 1024^2", n * n / 1e6 * args.steps / wall, "Mpix-pairs/s", args,
+                 wall, gpu_ms,
+                 {"workload": "1 pair 1024x1024 Lamb-Oseen density 0.04; Farneback (defaults) FILTER 0, 2 levels, "
+                              "Liu-Shen alpha 10, FILTER_OPT 0.48; drop-in driver (numpy at the adapter boundary)",
+                  "pairs_per_step": 1, "parallelism": "single GPU"},
+                 {"bound": "hbm", "kernel": "whole pipeline incl. host round trips", "achieved": round(achieved, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                  "algorithmic_bytes_per_step": round(alg)}, cpu)
+
+
+def bench_lk(args):
+    """2048^2 dense Lucas-Kanade 2-level (halfWindow 13, 5 iterations, FILTER 2, no warping)."""
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from denseLucasKanade_PyCL import denseLucasKanade_PyCl
+    from ofri import synth
+    from ofri.device import context
+    ctx = context()
+    im1, im2, _, _ = synth.config_pair(2)
+
+    def step():
+        genericPyramidalOpticalFlow(im1, im2, 2, denseLucasKanade_PyCl(Niter=5, halfWindow=13), 2, 1, 0.48, None,
+                                    warping=False)
+
+    wall, gpu_ms = _timed(step, ctx, args.steps, args.warmup)
+    n = im1.shape[0]
+    px = n * n + (n // 2) ** 2
+    flops = px * (19000.0 + 8000.0 * 5)      # upper bound: every pixel runs all 5 iterations
+    achieved = flops / (wall / args.steps) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        h = 1024
+        a, b = im1[:h, :h].copy(), im2[:h, :h].copy()
+        z = np.zeros_like(a)
+        t0 = time.perf_counter()
+        orc.lk_dense(a, b, z, z, 13, 5)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(a.size / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": orc.get_threads(),
+               "kind": "port", "sample": "one %dx%d crop: C oracle dense LK level (window 27, 5 iterations), "
+                                         "%.1f s" % (h, h, dt)}
+    return _line("Mpix-pairs/s, dense LK 2-level 2048^2", n * n / 1e6 * args.steps / wall, "Mpix-pairs/s", args,
+                 wall, gpu_ms,
+                 {"workload": "1 pair 2048x2048 shear + turbulence density 0.06; dense LK halfWindow 13, Niter 5, "
+                              "FILTER 2, 2 levels, no warping; drop-in driver (numpy at the adapter boundary)",
+                  "pairs_per_step": 1, "parallelism": "single GPU"},
+                 {"bound": "valu", "kernel": "whole pipeline incl. host round trips (flops upper bound)",
+                  "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
+                  "frac": round(achieved / 157.3, 4), "traffic": None}, cpu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (pairs are independent, "
               "see the sharding notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload != "lshs":
+        fn = {"batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk}[args.workload]
+        print(json.dumps(fn(args)))
+        return
     from gaussian_filter import prepareGaussianKernel
     from ofri import ops, synth
     from ofri.device import DeviceArray, context
