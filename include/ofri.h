@@ -102,6 +102,23 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
                 int maxnum, double tol, float* d_U, float* d_V, float* d_hist, float* h_err,
                 int* h_iters);
 
+/* ofri_hs_tol with Chebyshev semi-iteration over the same sweep: each step
+ * combines the damped plain update with the previous iterate, for a
+ * spectrum of the sweep's linear part in [-1/3, bound] (0 < bound < 1; the
+ * 8-neighbour average bounds it below by -1/3, and a bound below the true
+ * slowest mode only slows that mode down).  The stop test is the plain one:
+ * sweep k records |T(x_k) - x_k| for the plain sweep T, and the accelerated
+ * phase returns T(x_k) for the first iterate x_k with that norm <= its
+ * tolerance.  finish > 1: the accelerated phase runs to finish * tol and
+ * plain sweeps continue to tol (float32 Chebyshev iterates stall near the
+ * rounding level, plain sweeps settle); finish <= 1: accelerated to tol.
+ * maxnum bounds the sweeps of both phases together; d_hist, *h_err and
+ * *h_iters cover both.  Same fixed point as ofri_hs; no counterpart in the
+ * reference (an accelerated form for solving to a residual). */
+int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
+                  int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
+                  float* d_hist, float* h_err, int* h_iters);
+
 /* ---- Liu-Shen ------------------------------------------------------------ */
 /* Liu-Shen physics-based refinement in the solver's own orientation: d_u is
  * the row (y) component, d_v the column (x) component, both in/out.  Sweeps

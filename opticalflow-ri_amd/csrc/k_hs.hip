@@ -73,6 +73,13 @@ struct HsPass {
     int maxnum;
     size_t N;
     float* hist;             // per-sweep residuals (nullptr: not recorded)
+    // Chebyshev mode (hs_pass_k<..., true, true>): previous iterate buffers,
+    // the weight of every global sweep and the damping of the plain step
+    const float* pin[2][2];
+    float* pout[2][2];
+    const float* omega;
+    float gamma;
+    int sweep0;              // global index of this launch's first sweep
     const float* fx;
     const float* fy;
     const float* ft;
@@ -132,9 +139,17 @@ __device__ __forceinline__ float quotient(float num, float den, float rcp) {
 // the mirror symmetry of the extension is preserved exactly by every sweep.
 // One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
 // one being number blk; nsw sweeps from ping-pong buffer src.
-template <int R, int WAVES, bool NORM>
-__device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int bx, int by, int blk,
-                                        int nblk) {
+//
+// CHEB: Chebyshev semi-iteration over the same sweep T.  With the damped step
+// T_g(x) = x + gamma (T(x) - x), whose spectrum lies in [-sigma, sigma],
+// x_{k+1} = x_{k-1} + omega_{k+1} (T_g(x_k) - x_{k-1}); the previous iterate
+// x_{k-1} rides along in registers.  The residual of sweep k is the plain
+// one, |T(x_k) - x_k|, so the stop test is the plain method's.  The re-run
+// of a stopping launch makes its last sweep a plain one: the result is
+// T(x_k) for the first iterate x_k that met the tolerance.
+template <int R, int WAVES, bool NORM, bool CHEB = false>
+__device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int base, int bx, int by,
+                                        int blk, int nblk) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int K = a.K, H = a.H, W = a.W;
@@ -145,6 +160,7 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
     float U[R], V[R], FX[R], FY[R], FT[R], DEN[R], RCP[R];
+    float UP[CHEB ? R : 1], VP[CHEB ? R : 1];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
 #pragma unroll
@@ -152,6 +168,10 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
         U[r] = uin[q];
         V[r] = vin[q];
+        if (CHEB) {
+            UP[r] = a.pin[src][0][q];
+            VP[r] = a.pin[src][1][q];
+        }
         FX[r] = a.fx[q];
         FY[r] = a.fy[q];
         FT[r] = a.ft[q];
@@ -177,6 +197,12 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         Trip up_u = trip(uup), up_v = trip(vup);
         Trip cu = trip(U[0]), cv = trip(V[0]);
         double su = 0.0, sv = 0.0;
+        float om = 1.0f;
+        bool plain_step = true;
+        if (CHEB) {
+            om = a.omega[base + s];
+            plain_step = a.redo && s == nsw - 1;
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Trip dn_u = trip(r + 1 < R ? U[r + 1] : udn);
@@ -197,6 +223,19 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                     su = su + (double)du * (double)du;
                     sv = sv + (double)dv * (double)dv;
                 }
+            }
+            if (CHEB && !plain_step) {
+                const float du = un - U[r], dv = vn - V[r];
+                const float tu = U[r] + a.gamma * du, tv = V[r] + a.gamma * dv;
+                const float nu2 = UP[r] + om * (tu - UP[r]);
+                const float nv2 = VP[r] + om * (tv - VP[r]);
+                UP[r] = U[r];
+                VP[r] = V[r];
+                un = nu2;
+                vn = nv2;
+            } else if (CHEB) {
+                UP[r] = U[r];
+                VP[r] = V[r];
             }
             U[r] = un;
             V[r] = vn;
@@ -237,31 +276,52 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                 size_t q = (size_t)gy * W + gx;
                 uo[q] = U[r];
                 vo[q] = V[r];
+                if (CHEB) {
+                    a.pout[src ^ 1][0][q] = UP[r];
+                    a.pout[src ^ 1][1][q] = VP[r];
+                }
             }
         }
     }
-    if (NORM && !a.redo) arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+    if (NORM && !a.redo)
+        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, CHEB);
 }
 
 // Start of a launch in residual mode: the re-run launch takes its sweep count
 // and input buffer from the control word; a finished solve does nothing.
-__device__ __forceinline__ bool hs_launch_args(const HsPass& a, int& nsw, int& src) {
+__device__ __forceinline__ bool hs_launch_args(const HsPass& a, int& nsw, int& src, int& base) {
     nsw = a.nsweeps;
     src = a.src;
+    base = a.sweep0;
     if (a.redo) {
         nsw = a.ctrl->redo;
         src = a.ctrl->redo_in;
+        base = a.ctrl->sweeps - nsw;
         return nsw > 0;
     }
     return !a.ctrl->done;
 }
 
-template <int R, int WAVES, bool NORM>
+template <int R, int WAVES, bool NORM, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
-    int nsw = a.nsweeps, src = a.src;
-    if (NORM && !hs_launch_args(a, nsw, src)) return;
-    hs_tile<R, WAVES, NORM>(a, nsw, src, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
-                            gridDim.x * gridDim.y);
+    int nsw = a.nsweeps, src = a.src, base = 0;
+    if (NORM && !hs_launch_args(a, nsw, src, base)) return;
+    hs_tile<R, WAVES, NORM, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y,
+                                  blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+}
+
+// Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
+// (omega_1 = 1, omega_2 = 2 / (2 - sigma^2), omega_{k+1} = 1 / (1 - sigma^2
+// omega_k / 4)), formed in double by one thread.
+__global__ void cheb_omega_k(float* w, int n, double sigma) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double s2 = sigma * sigma;
+    double om = 1.0;
+    for (int k = 0; k < n; ++k) {
+        if (k == 1) om = 2.0 / (2.0 - s2);
+        else if (k > 1) om = 1.0 / (1.0 - 0.25 * s2 * om);
+        w[k] = (float)om;
+    }
 }
 
 // Many images in one launch: workgroup b belongs to the image p with
@@ -282,7 +342,7 @@ __global__ __launch_bounds__(64 * WAVES) void hs_batch_k(const HsPass* __restric
     }
     const HsPass& a = tab[lo];
     const int local = b - blk0[lo];
-    hs_tile<R, WAVES, false>(a, nsweeps, src, local % a.tiles_x, local / a.tiles_x, local,
+    hs_tile<R, WAVES, false>(a, nsweeps, src, 0, local % a.tiles_x, local / a.tiles_x, local,
                              a.tiles_x * a.tiles_y);
 }
 
@@ -292,9 +352,12 @@ struct HsVariant {
     int R, waves;
     void (*kernel)(HsPass);   // fixed sweep count
     void (*normed)(HsPass);   // with per-sweep residuals and the stop test
+    void (*cheb)(HsPass);     // Chebyshev-accelerated, with the stop test
     void (*batched)(const HsPass*, const int*, int, int, int);
 };
-#define HS_SHAPE(R, WV) {R, WV, hs_pass_k<R, WV, false>, hs_pass_k<R, WV, true>, hs_batch_k<R, WV>}
+#define HS_SHAPE(R, WV)                                                                     \
+    {R, WV, hs_pass_k<R, WV, false, false>, hs_pass_k<R, WV, true, false>, hs_pass_k<R, WV, true, true>, \
+     hs_batch_k<R, WV>}
 const HsVariant kHsVariants[] = {
     HS_SHAPE(16, 4), HS_SHAPE(8, 4), HS_SHAPE(8, 8), HS_SHAPE(12, 4), HS_SHAPE(16, 2),
     HS_SHAPE(4, 8), HS_SHAPE(24, 2), HS_SHAPE(12, 8),
@@ -322,9 +385,12 @@ void hs_default(size_t npx, int* variant, int* depth) {
 
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
-             float* d_hist) {
+             float* d_hist, float cheb_b, float cheb_switch) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
+    const bool cheb = cheb_b > 0.0f;
+    if (cheb && (!d_ctrl || !(cheb_b < 1.0f)))
+        return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 1), got %g", cheb_b);
     size_t N = (size_t)H * W;
     int vi, depth;
     hs_default((size_t)H * W, &vi, &depth);
@@ -349,6 +415,13 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* V0 = U0 + N;
     void* pbuf = nullptr;
     if (d_ctrl) OFRI_TRY(scratch(ctx, "hs_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
+    float *P0 = nullptr, *omega = nullptr;
+    if (cheb) {
+        void* cb;
+        OFRI_TRY(scratch(ctx, "hs_cheb", (4 * N + (size_t)niter + 64) * sizeof(float), &cb));
+        P0 = (float*)cb;
+        omega = P0 + 4 * N;
+    }
     dim3 g((W + 255) / 256, H);
     // computeDerivatives(I1=im2, I0=im1)
     hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft);
@@ -359,6 +432,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     }
     if (d_ctrl) {
         ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
+        OFRI_HIP(hipGetLastError());
+    }
+    // spectrum of one sweep's linear part lies in [-1/3, b]: the 8-neighbour
+    // average has eigenvalues in [-1/3, 1] and the per-pixel data projection
+    // only shrinks them; b < 1 is the caller's bound on the slowest mode
+    const double lo = -1.0 / 3.0, hi = cheb_b;
+    if (cheb) {
+        OFRI_HIP(hipMemcpyAsync(P0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(P0 + N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        if (niter > 0) cheb_omega_k<<<1, 64, 0, ctx->stream>>>(omega, niter, (hi - lo) / (2.0 - lo - hi));
         OFRI_HIP(hipGetLastError());
     }
     HsPass p{};
@@ -374,29 +457,55 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.maxnum = niter;
     p.N = N;
     p.hist = d_hist;
-    auto kern = d_ctrl ? var.normed : var.kernel;
-    int done = 0, src = 0;
-    while (done < niter) {
-        int ns = niter - done < depth ? niter - done : depth;
-        p.nsweeps = ns;
-        p.src = src;
-        p.redo = 0;
-        kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
-        OFRI_HIP(hipGetLastError());
-        src ^= 1;
-        done += ns;
+    if (cheb) {
+        p.pin[0][0] = P0; p.pin[0][1] = P0 + N; p.pin[1][0] = P0 + 2 * N; p.pin[1][1] = P0 + 3 * N;
+        p.pout[0][0] = P0; p.pout[0][1] = P0 + N; p.pout[1][0] = P0 + 2 * N; p.pout[1][1] = P0 + 3 * N;
+        p.omega = omega;
+        p.gamma = (float)(2.0 / (2.0 - lo - hi));
     }
-    if (d_ctrl) {
-        if (niter > 0) {
-            p.redo = 1;
+    // one phase: up to niter sweeps (fewer when the control word's shared
+    // count reaches the budget), the stopping launch re-run, the result
+    // copied back into U, V
+    auto phase = [&](void (*kern)(HsPass), double t) -> int {
+        p.tol = t;
+        int done = 0, src = 0;
+        while (done < niter) {
+            int ns = niter - done < depth ? niter - done : depth;
+            p.nsweeps = ns;
+            p.src = src;
+            p.sweep0 = done;
+            p.redo = 0;
             kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
             OFRI_HIP(hipGetLastError());
-            OFRI_TRY(select_copy(ctx, d_ctrl, U, Ub, U, N));
-            OFRI_TRY(select_copy(ctx, d_ctrl, V, Vb, V, N));
+            src ^= 1;
+            done += ns;
         }
-    } else if (src != 0) {
-        OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        OFRI_HIP(hipMemcpyAsync(V, Vb, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        if (d_ctrl) {
+            if (niter > 0) {
+                p.redo = 1;
+                kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+                OFRI_HIP(hipGetLastError());
+                OFRI_TRY(select_copy(ctx, d_ctrl, U, Ub, U, N));
+                OFRI_TRY(select_copy(ctx, d_ctrl, V, Vb, V, N));
+            }
+        } else if (src != 0) {
+            OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            OFRI_HIP(hipMemcpyAsync(V, Vb, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return OFRI_OK;
+    };
+    if (!cheb) {
+        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol));
+    } else if (!(cheb_switch > 1.0f)) {
+        OFRI_TRY(phase(var.cheb, tol));
+    } else {
+        // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
+        // still settle: accelerate to cheb_switch * tol, then finish with plain
+        // sweeps to tol, on one sweep count, budget and history
+        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch));
+        ctrl_continue_k<<<1, 1, 0, ctx->stream>>>(d_ctrl, niter);
+        OFRI_HIP(hipGetLastError());
+        OFRI_TRY(phase(var.normed, tol));
     }
     if (d_err) OFRI_TRY(diff_norm_err(ctx, U, U0, V, V0, N, d_err));
     return OFRI_OK;

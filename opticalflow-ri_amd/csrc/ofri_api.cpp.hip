@@ -287,6 +287,26 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
     return OFRI_OK;
 }
 
+int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
+                  int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
+                  float* d_hist, float* h_err, int* h_iters) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_U || !d_V) return fail(OFRI_E_ARG, "ofri_hs_accel: null pointer");
+    if (!(bound > 0.0f && bound < 1.0f)) return fail(OFRI_E_ARG, "ofri_hs_accel: bound %g not in (0, 1)", bound);
+    void* c;
+    OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
+    OFRI_TRY(hs_solve(ctx, d_im1, d_im2, H, W, alpha, maxnum, d_U, d_V, nullptr, tol, (SolveCtrl*)c, d_hist,
+                      bound, finish));
+    if (h_err || h_iters) {
+        SolveCtrl hc;
+        OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (h_err) *h_err = hc.last_err;
+        if (h_iters) *h_iters = hc.sweeps;
+    }
+    return OFRI_OK;
+}
+
 int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
             int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters) {
     CHECK_CTX(ctx);

@@ -115,9 +115,14 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // ---- Horn-Schunck (k_hs.hip) ----
 // With d_ctrl: per-sweep update norms, stop once the norm <= tol (tol < 0:
 // never), the sweep history in d_hist (niter floats, may be null).
+// cheb_b in (0, 1) (needs d_ctrl): Chebyshev-accelerated sweeps for a
+// spectrum bound b on the slowest mode; 0 = plain sweeps.  cheb_switch > 1:
+// accelerate to cheb_switch * tol, then plain sweeps to tol (niter sweeps in
+// all).
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
-             SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr);
+             SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
+             float cheb_switch = 0.0f);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);

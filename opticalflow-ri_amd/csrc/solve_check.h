@@ -15,10 +15,11 @@ namespace {
 // sums of every workgroup are added in a fixed order, the float32 norm of
 // each sweep is tested in turn (and recorded in `hist` when given), and the
 // control word records the sweep count, the last norm, the buffer holding
-// the result and, when the test held before the launch's last sweep, how
-// many sweeps the stopping launch must be re-run with.
+// the result and, when the test held before the launch's last sweep (or at
+// any sweep, with always_redo), how many sweeps the stopping launch must be
+// re-run with.
 __device__ void solve_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
-                            size_t N, SolveCtrl* ctrl, float* hist) {
+                            size_t N, SolveCtrl* ctrl, float* hist, bool always_redo) {
     __shared__ double sa[1024], sb[1024];
     __shared__ int stop;
     const int T = blockDim.x * blockDim.y;
@@ -51,7 +52,7 @@ __device__ void solve_check(const double* part, int nblk, int nsw, int dst, doub
             ctrl->final_buf = dst;
             if (!((double)err > tol)) {
                 ctrl->done = 1;
-                if (s + 1 < nsw) {
+                if (s + 1 < nsw || always_redo) {
                     ctrl->redo = s + 1;
                     ctrl->redo_in = dst ^ 1;
                 }
@@ -70,7 +71,7 @@ __device__ void solve_check(const double* part, int nblk, int nsw, int dst, doub
 // re-arms the counter.
 __device__ __forceinline__ void arrive_and_check(const double* part, int nblk, int nsw, int dst,
                                                  double tol, int maxnum, size_t N, SolveCtrl* ctrl,
-                                                 float* hist) {
+                                                 float* hist, bool always_redo = false) {
     __shared__ int last;
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         __threadfence();
@@ -79,9 +80,19 @@ __device__ __forceinline__ void arrive_and_check(const double* part, int nblk, i
     __syncthreads();
     if (last) {
         __threadfence();
-        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist);
+        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist, always_redo);
         if (threadIdx.x == 0 && threadIdx.y == 0) ctrl->arrived = 0;
     }
+}
+
+// Re-arms a finished solve for a further phase on the same sweep count,
+// budget and history: the phase starts unless the budget is spent.
+__global__ void ctrl_continue_k(SolveCtrl* c, int maxnum) {
+    c->done = c->sweeps >= maxnum;
+    c->final_buf = 0;
+    c->redo = 0;
+    c->redo_in = 0;
+    c->arrived = 0;
 }
 
 __global__ void ctrl_init_k(SolveCtrl* c) {

@@ -104,6 +104,22 @@ def hs_tol(im1, im2, alpha, maxnum, tol, U, V, hist=None):
     return np.float32(err.value), it.value
 
 
+def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0):
+    """Chebyshev-accelerated Horn-Schunck to the plain residual tol
+    (ofri_hs_accel; finish > 1 ends with plain sweeps from finish * tol);
+    U, V in place.  Returns (err, sweeps)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    if hist is not None and hist.nbytes // 4 < maxnum:
+        raise ValueError("hs_accel: history buffer holds %d < %d sweeps" % (hist.nbytes // 4, maxnum))
+    err = C.c_float()
+    it = C.c_int()
+    _capi.check(_lib().ofri_hs_accel(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(alpha)), int(maxnum),
+                                     float(tol), float(bound), float(finish), U.ptr, V.ptr,
+                                     hist.ptr if hist is not None else None,
+                                     C.byref(err), C.byref(it)))
+    return np.float32(err.value), it.value
+
+
 def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
     """Liu-Shen in the solver's orientation (u = row component); u, v in place.
     Returns (err, sweeps)."""

@@ -337,6 +337,75 @@ def test_hs_tol_stops_inside_a_fused_launch(ops):
         ctx.set_tuning()
 
 
+def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
+    """Chebyshev semi-iteration over the oracle's float32 sweep T, with the
+    device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
+    g (T(x_k) - x_k)) - x_{k-1}); stop at the first x_k with plain residual
+    <= tol (finish * tol when finish > 1) and take T(x_k); with finish > 1,
+    plain sweeps then continue to tol on the same count.  Returns (U, V,
+    sweeps, residual history)."""
+    t1 = tol * finish if finish > 1 else tol
+    lo, hi = -1.0 / 3.0, float(np.float32(bound))
+    sig = (hi - lo) / (2.0 - lo - hi)
+    g = np.float32(2.0 / (2.0 - lo - hi))
+    om, w = [], 1.0
+    for k in range(maxnum):
+        w = 1.0 if k == 0 else (2.0 / (2.0 - sig * sig) if k == 1 else 1.0 / (1.0 - 0.25 * sig * sig * w))
+        om.append(np.float32(w))
+    fx, fy, ft = orc.hs_derivs(im1, im2)
+    U = np.zeros(im1.shape, np.float32); V = np.zeros(im1.shape, np.float32)
+    UP, VP = U.copy(), V.copy()
+    hist = []
+    for k in range(maxnum):
+        Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        du, dv = Tu - U, Tv - V
+        r = np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) + np.sqrt((dv.astype(np.float64) ** 2).sum()))
+                       / U.size)
+        hist.append(r)
+        if not r > t1:
+            U, V = Tu, Tv
+            break
+        nu = UP + om[k] * ((U + g * du) - UP)
+        nv = VP + om[k] * ((V + g * dv) - VP)
+        UP, VP, U, V = U, V, nu, nv
+    else:
+        return U, V, maxnum, np.array(hist)
+    if finish > 1:
+        while len(hist) < maxnum and hist[-1] > tol:
+            Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+            du, dv = Tu - U, Tv - V
+            hist.append(np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) +
+                                    np.sqrt((dv.astype(np.float64) ** 2).sum())) / U.size))
+            U, V = Tu, Tv
+    return U, V, len(hist), np.array(hist)
+
+
+@pytest.mark.parametrize("depth,finish", [(0, 0.0), (4, 0.0), (3, 4.0)])
+def test_hs_accel_matches_restated_iteration(ops, depth, finish):
+    """ofri_hs_accel equals the Chebyshev iteration restated over the oracle's
+    sweep bit for bit (sweep count, residual history and the returned T(x_k)),
+    and stops in far fewer sweeps than plain sweeps need for the same norm."""
+    from ofri.device import DeviceArray, context
+    H, W = 96, 120
+    im1, im2 = _piv(H, W, seed=41)
+    plain_hist, _, _ = _hs_norm_history(im1, im2, 45.0, 200)
+    tol = float(plain_hist[199])
+    Ur, Vr, kr, hr = _cheb_restated(im1, im2, 45.0, 0.99, tol, 200, finish)
+    assert kr < 150, kr
+    ctx = context()
+    ctx.set_tuning(hs_depth=depth)
+    try:
+        U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+        hist = DeviceArray.zeros((200,))
+        err, it = ops.hs_accel(im1, im2, 45.0, 200, tol, 0.99, U, V, hist, finish=finish)
+    finally:
+        ctx.set_tuning()
+    print("[hs accel %dx%d finish %g] plain needs 200 sweeps for %.3g, accelerated %d" % (H, W, finish, tol, it))
+    assert it == kr
+    assert np.allclose(hist.download()[:it], hr, rtol=1e-6, atol=0)
+    assert np.array_equal(U.download(), Ur) and np.array_equal(V.download(), Vr)
+
+
 def test_ls_small_golden(ops, golden):
     from ofri.device import DeviceArray
     g = golden("ls_small")
