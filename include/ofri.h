@@ -220,6 +220,36 @@ typedef struct ofri_pair {
 int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lshs_params* p,
                     ofri_level_stats* h_stats);
 
+/* ---- Warm-start network (the learned form) ------------------------------ */
+/* A fixed 4-level convolutional encoder-decoder (3x3 convolutions, BF16
+ * weights and activations, FP32 accumulation on MFMA, about 0.57M
+ * parameters) mapping a pre-filtered frame pair to the plain LS(HS) 2-level
+ * solver's per-level results: at the coarse level (H/2 x W/2) the HS flow and
+ * the Liu-Shen correction on top of it, at the fine level (H x W) the HS
+ * increment and its Liu-Shen correction.  No counterpart in the reference.
+ * Frames are batches of B float32 H x W images (H, W multiples of 16).
+ * Parameters are exchanged packed: layer by layer, weights [cout][tap][cin]
+ * then bias [cout], float32. */
+typedef struct ofri_net ofri_net;
+int ofri_net_create(ofri_ctx* ctx, ofri_net** out);
+int ofri_net_destroy(ofri_net* net);
+int ofri_net_num_params(ofri_net* net, size_t* n);
+int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n);
+int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n);
+/* Planar outputs: d_coarse [B][4][H/2][W/2] = (HS u, HS v, LS-HS u, LS-HS v),
+ * d_fine [B][4][H][W] likewise for the fine level's increment. */
+int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H,
+                     int W, float* d_coarse, float* d_fine);
+/* One training step: forward, weighted Charbonnier loss against planar labels
+ * of the same layout as ofri_net_predict's outputs (h_field_w: 8 weights,
+ * coarse fields then fine fields), backward, and with lr > 0 an Adam step.
+ * *h_loss (may be NULL) receives the loss (synchronises). */
+int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H, int W,
+                   const float* d_lab_coarse, const float* d_lab_fine, const float* h_field_w, float lr,
+                   float* h_loss);
+/* Gradients of the last ofri_net_train call, packed like the parameters. */
+int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
+
 /* Tuning knobs of the temporally blocked sweeps.  0 = the library's
  * per-image-size default (measured on gfx950); a variant k >= 1 selects the
  * k-th compiled tile shape. */

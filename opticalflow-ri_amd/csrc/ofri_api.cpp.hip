@@ -743,4 +743,68 @@ int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lsh
     return OFRI_OK;
 }
 
+// ---- warm-start network ----
+#define NET(p) reinterpret_cast<ofri::Net*>(p)
+
+int ofri_net_create(ofri_ctx* ctx, ofri_net** out) {
+    CHECK_CTX(ctx);
+    if (!out) return fail(OFRI_E_ARG, "ofri_net_create: null pointer");
+    Net* n = nullptr;
+    OFRI_TRY(net_create(ctx, &n));
+    *out = reinterpret_cast<ofri_net*>(n);
+    return OFRI_OK;
+}
+
+int ofri_net_destroy(ofri_net* net) {
+    net_destroy(NET(net));
+    return OFRI_OK;
+}
+
+int ofri_net_num_params(ofri_net* net, size_t* n) {
+    if (!net || !n) return fail(OFRI_E_ARG, "ofri_net_num_params: null pointer");
+    *n = net_num_params(NET(net));
+    return OFRI_OK;
+}
+
+int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n) {
+    CHECK_CTX(ctx);
+    if (!net || !h_params) return fail(OFRI_E_ARG, "ofri_net_set_params: null pointer");
+    return net_set_params(ctx, NET(net), h_params, n);
+}
+
+int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n) {
+    CHECK_CTX(ctx);
+    if (!net || !h_params) return fail(OFRI_E_ARG, "ofri_net_get_params: null pointer");
+    return net_get_params(ctx, NET(net), h_params, n);
+}
+
+int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n) {
+    CHECK_CTX(ctx);
+    if (!net || !h_grads) return fail(OFRI_E_ARG, "ofri_net_get_grads: null pointer");
+    return net_get_grads(ctx, NET(net), h_grads, n);
+}
+
+int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H,
+                     int W, float* d_coarse, float* d_fine) {
+    CHECK_CTX(ctx);
+    if (!net || !d_im1 || !d_im2 || !d_coarse || !d_fine) return fail(OFRI_E_ARG, "ofri_net_predict: null pointer");
+    return net_predict(ctx, NET(net), d_im1, d_im2, B, H, W, d_coarse, d_fine);
+}
+
+int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H, int W,
+                   const float* d_lab_coarse, const float* d_lab_fine, const float* h_field_w, float lr,
+                   float* h_loss) {
+    CHECK_CTX(ctx);
+    if (!net || !d_im1 || !d_im2 || !d_lab_coarse || !d_lab_fine || !h_field_w)
+        return fail(OFRI_E_ARG, "ofri_net_train: null pointer");
+    void* l;
+    OFRI_TRY(scratch(ctx, "net_loss", sizeof(float), &l));
+    OFRI_TRY(net_train(ctx, NET(net), d_im1, d_im2, B, H, W, d_lab_coarse, d_lab_fine, h_field_w, lr, (float*)l));
+    if (h_loss) {
+        OFRI_HIP(hipMemcpyAsync(h_loss, l, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return OFRI_OK;
+}
+
 }  // extern "C"

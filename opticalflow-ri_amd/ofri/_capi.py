@@ -103,6 +103,14 @@ SIGNATURES = {
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],
+    "ofri_net_create": [_vp, C.POINTER(_vp)],
+    "ofri_net_destroy": [_vp],
+    "ofri_net_num_params": [_vp, C.POINTER(C.c_size_t)],
+    "ofri_net_set_params": [_vp, _vp, _fp, C.c_size_t],
+    "ofri_net_get_params": [_vp, _vp, _fp, C.c_size_t],
+    "ofri_net_get_grads": [_vp, _vp, _fp, C.c_size_t],
+    "ofri_net_predict": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp],
+    "ofri_net_train": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_float, _fp],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],
