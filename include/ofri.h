@@ -104,7 +104,7 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
 
 /* ofri_hs_tol with Chebyshev semi-iteration over the same sweep: each step
  * combines the damped plain update with the previous iterate, for a
- * spectrum of the sweep's linear part in [-1/3, bound] (0 < bound < 1; the
+ * spectrum of the sweep's linear part in [-1/3, bound] (0 < bound <= 0.999; the
  * 8-neighbour average bounds it below by -1/3, and a bound below the true
  * slowest mode only slows that mode down).  The stop test is the plain one:
  * sweep k records |T(x_k) - x_k| for the plain sweep T, and the accelerated
@@ -112,8 +112,8 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
  * tolerance.  finish > 1: the accelerated phase runs to finish * tol and
  * plain sweeps continue to tol (float32 Chebyshev iterates stall near the
  * rounding level, plain sweeps settle); finish <= 1: accelerated to tol.
- * maxnum bounds the sweeps of both phases together; d_hist, *h_err and
- * *h_iters cover both.  Same fixed point as ofri_hs; no counterpart in the
+ * maxnum bounds the sweeps of both phases together, the plain phase to at
+ * most maxnum / 2 of them; d_hist, *h_err and *h_iters cover both.  Same fixed point as ofri_hs; no counterpart in the
  * reference (an accelerated form for solving to a residual). */
 int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
                   int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
@@ -182,9 +182,11 @@ typedef struct ofri_lshs_params {
 /* Per-level statistics written by ofri_pyr_lshs (may be NULL). */
 typedef struct ofri_level_stats {
     int H, W;
-    float hs_err;
-    float ls_err;
-    int ls_iters;
+    float hs_err;    /* HS's returned error (change from its initial flow) */
+    float ls_err;    /* Liu-Shen's last update norm */
+    int ls_iters;    /* Liu-Shen sweeps */
+    float hs_res;    /* HS's last update norm (0 unless recorded) */
+    int hs_iters;    /* HS sweeps */
 } ofri_level_stats;
 
 /* Coarse-to-fine LS(HS): bicubic downsample, spline upsample + x2 scaling,
@@ -197,6 +199,16 @@ typedef struct ofri_level_stats {
 int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                   const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc,
                   ofri_level_stats* h_stats);
+
+/* ofri_pyr_lshs run eagerly with every level's solver results exported: d_taps
+ * (device) receives per level, levels in order, the planes HS u, HS v,
+ * LS u - HS u, LS v - HS v at the level's size (4 * sum of level pixels
+ * floats), and the HS sweeps record their update norms (hs_res in h_stats).
+ * The warm-start network's training labels and the residual targets of the
+ * learned form.  No counterpart in the reference. */
+int ofri_pyr_lshs_taps(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                       const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps,
+                       ofri_level_stats* h_stats);
 
 /* One image pair of a batch: device frames at t and t+1 (H x W) and the
  * device outputs U, V (H x W) the pair's accumulated flow is written to. */
@@ -220,6 +232,20 @@ typedef struct ofri_pair {
 int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lshs_params* p,
                     ofri_level_stats* h_stats);
 
+/* ---- Synthetic PIV pairs (training data) --------------------------------- */
+/* Renders a particle-image pair on the device: particles at `density` per
+ * pixel (8x8-pixel cells of 16 slots), d ~ N(2.5, 0.5) in [1, 4], I0 ~
+ * U[150, 255], I0 exp(-8 r^2 / d^2) summed and clipped to [0, 255], frame 2
+ * moved by the flow at each particle, N(0, noise) added.  family 0
+ * Poiseuille (par: umax), 1 Lamb-Oseen (par: cx, cy, rc, vmax), 2 shear +
+ * turbulence (par: shear, dmax; h_modes: kx[n], ky[n], phase[n], amp[n]),
+ * displacements at most 8 px.  A pure function of the seed.  d_u, d_v (may
+ * be NULL) receive the flow field.  The synthetic workloads of the
+ * reference's examples are host-generated; no counterpart in the reference. */
+int ofri_render_pair(ofri_ctx* ctx, int H, int W, int family, float density, unsigned long long seed,
+                     const float* h_par, int npar, const float* h_modes, int nmodes, float noise, float* d_im1,
+                     float* d_im2, float* d_u, float* d_v);
+
 /* ---- Warm-start network (the learned form) ------------------------------ */
 /* A fixed 4-level convolutional encoder-decoder (3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on MFMA, about 0.57M
@@ -236,6 +262,9 @@ int ofri_net_destroy(ofri_net* net);
 int ofri_net_num_params(ofri_net* net, size_t* n);
 int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n);
 int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n);
+/* Random initial parameters (He-normal for the leaky-ReLU layers, heads
+ * scaled down, zero biases) from a seed; resets the optimiser state. */
+int ofri_net_init(ofri_ctx* ctx, ofri_net* net, unsigned long long seed);
 /* Planar outputs: d_coarse [B][4][H/2][W/2] = (HS u, HS v, LS-HS u, LS-HS v),
  * d_fine [B][4][H][W] likewise for the fine level's increment. */
 int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H,
@@ -249,6 +278,29 @@ int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float
                    float* h_loss);
 /* Gradients of the last ofri_net_train call, packed like the parameters. */
 int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
+
+/* The learned form of the LS(HS) 2-level pyramid: the network's predictions
+ * warm-start each level's HS (its flow planes) and Liu-Shen (HS result plus
+ * the predicted correction); HS runs (Chebyshev-accelerated when
+ * cheb_bound > 0) until its update norm is <= hs_tol[level] or hs_budget
+ * sweeps, Liu-Shen until <= ls_tol[level] or ls_budget sweeps.  The targets
+ * are the plain method's residuals at its stopping points (r*).  net NULL:
+ * zero warm starts (the method without the network).  *h_converged (may be
+ * NULL) is 1 when every level met its targets; otherwise the caller falls
+ * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries.  H, W multiples
+ * of 16.  Repeated calls replay a HIP graph.  No counterpart in the
+ * reference. */
+typedef struct ofri_learned_params {
+    const float* hs_tol;   /* per level (coarse, fine) */
+    const float* ls_tol;
+    int hs_budget;
+    int ls_budget;
+    float cheb_bound;      /* e.g. 0.99; 0 = plain sweeps */
+    float cheb_finish;     /* e.g. 4; see ofri_hs_accel */
+} ofri_learned_params;
+int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                          const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,
+                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged);
 
 /* Tuning knobs of the temporally blocked sweeps.  0 = the library's
  * per-image-size default (measured on gfx950); a variant k >= 1 selects the

@@ -22,6 +22,9 @@
 // dw_k: weight and bias gradients, sum over pixels of gated output gradient
 // times shifted input, in float32 on the vector ALUs from LDS-staged tiles,
 // partial sums added atomically (training only).
+#include <cmath>
+#include <random>
+
 #include "ofri_internal.h"
 
 #pragma clang fp contract(off)
@@ -538,13 +541,18 @@ int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
         n->t_off[t] = off;
         off += align256((size_t)B * lh * lw * kTensors[t].C * (kTensors[t].f32 ? 4 : 2));
     }
-    OFRI_HIP(hipStreamSynchronize(ctx->stream));
-    if (n->act) OFRI_HIP(hipFree(n->act));
-    if (n->gact) OFRI_HIP(hipFree(n->gact));
-    n->act = n->gact = nullptr;
-    OFRI_HIP(hipMalloc(&n->act, off));
-    OFRI_HIP(hipMalloc(&n->gact, off));
-    n->act_bytes = off;
+    // the buffers only grow: a smaller batch reuses them with new offsets
+    if (off > n->act_bytes) {
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (n->act) OFRI_HIP(hipFree(n->act));
+        if (n->gact) OFRI_HIP(hipFree(n->gact));
+        n->act = n->gact = nullptr;
+        n->act_bytes = 0;
+        OFRI_HIP(hipMalloc(&n->act, off));
+        OFRI_HIP(hipMalloc(&n->gact, off));
+        n->act_bytes = off;
+        ctx->ws_gen += 1;   // captured graphs holding the old buffers are stale
+    }
     n->B = B;
     n->H = H;
     n->W = W;
@@ -732,6 +740,23 @@ int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count) {
     n->step = 0;
     n->wt_fresh = false;
     return OFRI_OK;
+}
+
+// He initialisation for the leaky-ReLU layers (std sqrt(2 / fan_in)), heads
+// 10x smaller (std 0.1 sqrt(1 / fan_in)), biases zero; Adam state reset.
+int net_init_params(ofri_ctx* ctx, Net* n, unsigned long long seed) {
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<float> h;
+    h.reserve(n->n_valid);
+    for (int l = 0; l < kNumLayers; ++l) {
+        const LayerDef& d = kLayers[l];
+        const double fan = 9.0 * d.cin;
+        const double sd = d.act ? std::sqrt(2.0 / fan) : 0.1 * std::sqrt(1.0 / fan);
+        for (int i = 0; i < 9 * d.cin * d.cout; ++i) h.push_back((float)(nd(gen) * sd));
+        for (int co = 0; co < d.cout; ++co) h.push_back(0.0f);
+    }
+    return net_set_params(ctx, n, h.data(), h.size());
 }
 
 static int unpack(ofri_ctx* ctx, Net* n, const float* d, float* h, size_t count) {

@@ -312,16 +312,29 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
 
 // Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
 // (omega_1 = 1, omega_2 = 2 / (2 - sigma^2), omega_{k+1} = 1 / (1 - sigma^2
-// omega_k / 4)), formed in double by one thread.
-__global__ void cheb_omega_k(float* w, int n, double sigma) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// omega_k / 4)), formed in double on the host.  The recurrence converges
+// geometrically with ratio sigma^2 omega_inf^2 / 4 (at most 0.92 for the
+// bounds accepted here), so past the table's 512 entries every weight equals
+// the last one in float32.
+constexpr int kOmegaTab = 512;
+struct OmegaTab {
+    float w[kOmegaTab];
+};
+
+OmegaTab cheb_omegas(double sigma) {
+    OmegaTab t;
     const double s2 = sigma * sigma;
     double om = 1.0;
-    for (int k = 0; k < n; ++k) {
+    for (int k = 0; k < kOmegaTab; ++k) {
         if (k == 1) om = 2.0 / (2.0 - s2);
         else if (k > 1) om = 1.0 / (1.0 - 0.25 * s2 * om);
-        w[k] = (float)om;
+        t.w[k] = (float)om;
     }
+    return t;
+}
+
+__global__ void cheb_omega_k(OmegaTab t, float* w, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
 }
 
 // Many images in one launch: workgroup b belongs to the image p with
@@ -389,8 +402,8 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
     const bool cheb = cheb_b > 0.0f;
-    if (cheb && (!d_ctrl || !(cheb_b < 1.0f)))
-        return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 1), got %g", cheb_b);
+    if (cheb && (!d_ctrl || !(cheb_b <= 0.999f)))
+        return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 0.999], got %g", cheb_b);
     size_t N = (size_t)H * W;
     int vi, depth;
     hs_default((size_t)H * W, &vi, &depth);
@@ -441,7 +454,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (cheb) {
         OFRI_HIP(hipMemcpyAsync(P0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(P0 + N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        if (niter > 0) cheb_omega_k<<<1, 64, 0, ctx->stream>>>(omega, niter, (hi - lo) / (2.0 - lo - hi));
+        if (niter > 0) cheb_omega_k<<<1, 256, 0, ctx->stream>>>(cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, niter);
         OFRI_HIP(hipGetLastError());
     }
     HsPass p{};
@@ -466,11 +479,11 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     // one phase: up to niter sweeps (fewer when the control word's shared
     // count reaches the budget), the stopping launch re-run, the result
     // copied back into U, V
-    auto phase = [&](void (*kern)(HsPass), double t) -> int {
+    auto phase = [&](void (*kern)(HsPass), double t, int sweeps) -> int {
         p.tol = t;
         int done = 0, src = 0;
-        while (done < niter) {
-            int ns = niter - done < depth ? niter - done : depth;
+        while (done < sweeps) {
+            int ns = sweeps - done < depth ? sweeps - done : depth;
             p.nsweeps = ns;
             p.src = src;
             p.sweep0 = done;
@@ -495,17 +508,19 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         return OFRI_OK;
     };
     if (!cheb) {
-        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol));
+        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol, niter));
     } else if (!(cheb_switch > 1.0f)) {
-        OFRI_TRY(phase(var.cheb, tol));
+        OFRI_TRY(phase(var.cheb, tol, niter));
     } else {
         // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
-        // still settle: accelerate to cheb_switch * tol, then finish with plain
-        // sweeps to tol, on one sweep count, budget and history
-        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch));
-        ctrl_continue_k<<<1, 1, 0, ctx->stream>>>(d_ctrl, niter);
+        // still settle: accelerate to cheb_switch * tol, then finish with at
+        // most niter / 2 plain sweeps to tol, on one sweep count, budget and
+        // history (niter sweeps in all)
+        const int fin = niter / 2 > 0 ? niter / 2 : 1;
+        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter));
+        ctrl_continue_k<<<1, 1, 0, ctx->stream>>>(d_ctrl, niter, fin);
         OFRI_HIP(hipGetLastError());
-        OFRI_TRY(phase(var.normed, tol));
+        OFRI_TRY(phase(var.normed, tol, fin));
     }
     if (d_err) OFRI_TRY(diff_norm_err(ctx, U, U0, V, V0, N, d_err));
     return OFRI_OK;

@@ -164,6 +164,12 @@ __global__ void accumulate_k(float* __restrict__ acc, const float* __restrict__ 
     for (; i < n; i += stride) acc[i] = acc[i] + x[i];
 }
 
+__global__ void sub_k(float* __restrict__ d, const float* __restrict__ a, const float* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = a[i] - b[i];
+}
+
 __global__ void fill_zero_k(float* __restrict__ p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -472,6 +478,13 @@ int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* c
 int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n) {
     if (n == 0) return OFRI_OK;
     accumulate_k<<<grid1d(n), 256, 0, ctx->stream>>>(acc, x, n);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n) {
+    if (n == 0) return OFRI_OK;
+    sub_k<<<grid1d(n), 256, 0, ctx->stream>>>(d, a, b, n);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }

@@ -292,7 +292,8 @@ int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
                   float* d_hist, float* h_err, int* h_iters) {
     CHECK_CTX(ctx);
     if (!d_im1 || !d_im2 || !d_U || !d_V) return fail(OFRI_E_ARG, "ofri_hs_accel: null pointer");
-    if (!(bound > 0.0f && bound < 1.0f)) return fail(OFRI_E_ARG, "ofri_hs_accel: bound %g not in (0, 1)", bound);
+    if (!(bound > 0.0f && bound <= 0.999f))
+        return fail(OFRI_E_ARG, "ofri_hs_accel: bound %g not in (0, 0.999]", bound);
     void* c;
     OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
     OFRI_TRY(hs_solve(ctx, d_im1, d_im2, H, W, alpha, maxnum, d_U, d_V, nullptr, tol, (SolveCtrl*)c, d_hist,
@@ -346,9 +347,25 @@ int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H,
 namespace ofri {
 
 // Enqueues the whole LS(HS) pyramid on the context stream.  Per-level
-// statistics land in the "pyr_stats" scratch (L SolveCtrl + L floats).
+// statistics land in the "pyr_stats" scratch (L Liu-Shen control words, L
+// HS control words, L HS errors).
+// d_taps (may be null): per level, planar [HS u, HS v, LS u - HS u, LS v -
+// HS v] at the level's size, levels one after another.  record_res: the HS
+// sweeps also reduce their update norms (the last one lands in the stats).
+// lrn (may be null): the learned form.  The network's predictions (pred_c:
+// 4 planes at level 1, pred_f: 4 planes at level 2) warm-start every level's
+// HS (planes 0, 1) and its Liu-Shen seed (HS result + planes 2, 3); HS runs
+// accelerated to the level's residual target, Liu-Shen to its own.
+struct LearnedRun {
+    const ofri_learned_params* lp;
+    Net* net;               // null: zero warm starts
+    float* pred_c;
+    float* pred_f;
+};
+
 static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
-                            const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc) {
+                            const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps = nullptr,
+                            bool record_res = false, const LearnedRun* lrn = nullptr) {
     const int L = p->levels;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
@@ -366,12 +383,25 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
     float *U = base + 8 * NF, *V = base + 9 * NF;
     float *Ua = base + 10 * NF, *Va = base + 11 * NF, *Ub = base + 12 * NF, *Vb = base + 13 * NF;
     void* st;
-    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (sizeof(SolveCtrl) + 16), &st));
+    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (2 * sizeof(SolveCtrl) + 16), &st));
     SolveCtrl* ctrls = (SolveCtrl*)st;
-    float* hs_errs = (float*)(ctrls + L);
+    SolveCtrl* hs_ctrls = ctrls + L;
+    float* hs_errs = (float*)(hs_ctrls + L);
 
     float *acc_u = Ua, *acc_v = Va;
+    float* tap = d_taps;
     int ph = 0, pw = 0;
+    if (lrn && lrn->net) {
+        // the network sees the FILTER-pre-filtered full-size pair
+        const float *g1 = d_im1, *g2 = d_im2;
+        if (p->filt_w && p->filt_n > 0) {
+            OFRI_TRY(gauss_sep(ctx, d_im1, f1, H, W, p->filt_w, p->filt_n));
+            OFRI_TRY(gauss_sep(ctx, d_im2, f2, H, W, p->filt_w, p->filt_n));
+            g1 = f1;
+            g2 = f2;
+        }
+        OFRI_TRY(net_predict(ctx, lrn->net, g1, g2, 1, H, W, lrn->pred_c, lrn->pred_f));
+    }
     for (int l = 0; l < L; ++l) {
         const int h = lh[l], w = lw[l];
         const size_t N = (size_t)h * w;
@@ -409,9 +439,32 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             s1 = f1;
             s2 = f2;
         }
-        OFRI_TRY(fill_zero(ctx, U, N));
-        OFRI_TRY(fill_zero(ctx, V, N));
-        OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l));
+        const float* pred = nullptr;
+        if (lrn && lrn->net) pred = l == 0 ? lrn->pred_c : lrn->pred_f;
+        if (pred) {
+            OFRI_HIP(hipMemcpyAsync(U, pred, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            OFRI_HIP(hipMemcpyAsync(V, pred + N, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            OFRI_TRY(fill_zero(ctx, U, N));
+            OFRI_TRY(fill_zero(ctx, V, N));
+        }
+        if (lrn) {
+            const ofri_learned_params* lp = lrn->lp;
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, hs_errs + l, lp->hs_tol[l],
+                              hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish));
+            if (pred) {
+                OFRI_TRY(accumulate(ctx, U, pred + 2 * N, N));
+                OFRI_TRY(accumulate(ctx, V, pred + 3 * N, N));
+            }
+        } else if (record_res) {
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
+        } else {
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l));
+        }
+        if (tap) {
+            OFRI_HIP(hipMemcpyAsync(tap, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            OFRI_HIP(hipMemcpyAsync(tap + N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        }
         if (p->ls_alpha > 0.0f) {
             const float *q1 = c1, *q2 = c2;
             if (p->opt_w && p->opt_n > 0) {
@@ -421,7 +474,16 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
                 q2 = o2;
             }
             // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
-            OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
+            if (lrn)
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
+                                  ctrls + l));
+            else
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
+        }
+        if (tap) {
+            OFRI_TRY(subtract(ctx, tap + 2 * N, U, tap, N));
+            OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
+            tap += 4 * N;
         }
         OFRI_TRY(accumulate(ctx, out_u, U, N));
         OFRI_TRY(accumulate(ctx, out_v, V, N));
@@ -572,6 +634,29 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
     return OFRI_OK;
 }
 
+// Per-level statistics of the last pyramid run (the "pyr_stats" scratch).
+static int pyr_stats_download(ofri_ctx* ctx, int L, const std::vector<int>& lh, const std::vector<int>& lw,
+                              const ofri_lshs_params* p, bool with_res, ofri_level_stats* out) {
+    void* st;
+    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (2 * sizeof(SolveCtrl) + 16), &st));
+    SolveCtrl* ctrls = (SolveCtrl*)st;
+    std::vector<SolveCtrl> hc(2 * (size_t)L);
+    std::vector<float> he(L);
+    OFRI_HIP(hipMemcpyAsync(hc.data(), ctrls, 2 * L * sizeof(SolveCtrl), hipMemcpyDeviceToHost, ctx->stream));
+    OFRI_HIP(hipMemcpyAsync(he.data(), ctrls + 2 * L, L * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    for (int l = 0; l < L; ++l) {
+        out[l].H = lh[l];
+        out[l].W = lw[l];
+        out[l].hs_err = he[l];
+        out[l].ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
+        out[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
+        out[l].hs_res = with_res ? hc[L + l].last_err : 0.0f;
+        out[l].hs_iters = with_res ? hc[L + l].sweeps : p->niter;
+    }
+    return OFRI_OK;
+}
+
 static std::string batch_key(ofri_ctx* ctx, const ofri_pair* pr, int n, const ofri_lshs_params* p) {
     std::string k = "batch";
     k.append((const char*)pr, sizeof(ofri_pair) * n);
@@ -635,24 +720,23 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
             OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
         }
     }
-    void* st;
-    OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (sizeof(SolveCtrl) + 16), &st));
-    SolveCtrl* ctrls = (SolveCtrl*)st;
-    float* hs_errs = (float*)(ctrls + L);
-    if (h_stats) {
-        std::vector<SolveCtrl> hc(L);
-        std::vector<float> he(L);
-        OFRI_HIP(hipMemcpyAsync(hc.data(), ctrls, L * sizeof(SolveCtrl), hipMemcpyDeviceToHost, ctx->stream));
-        OFRI_HIP(hipMemcpyAsync(he.data(), hs_errs, L * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        OFRI_HIP(hipStreamSynchronize(ctx->stream));
-        for (int l = 0; l < L; ++l) {
-            h_stats[l].H = lh[l];
-            h_stats[l].W = lw[l];
-            h_stats[l].hs_err = he[l];
-            h_stats[l].ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
-            h_stats[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
-        }
-    }
+    if (h_stats) OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, false, h_stats));
+    return OFRI_OK;
+}
+
+int ofri_pyr_lshs_taps(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                       const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps,
+                       ofri_level_stats* h_stats) {
+    CHECK_CTX(ctx);
+    if (!p || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: null pointer");
+    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: levels %d", p->levels);
+    if (!p->alphas) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: alphas missing");
+    const int L = p->levels;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, d_taps, true));
+    if (h_stats) OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, h_stats));
+    else OFRI_HIP(hipStreamSynchronize(ctx->stream));
     return OFRI_OK;
 }
 
@@ -737,6 +821,8 @@ int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lsh
                 o.hs_err = he[k];
                 o.ls_err = p->ls_alpha > 0.0f ? hc[k].last_err : 0.0f;
                 o.ls_iters = p->ls_alpha > 0.0f ? hc[k].sweeps : 0;
+                o.hs_res = 0.0f;
+                o.hs_iters = p->niter;
             }
         }
     }
@@ -772,6 +858,12 @@ int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, siz
     return net_set_params(ctx, NET(net), h_params, n);
 }
 
+int ofri_net_init(ofri_ctx* ctx, ofri_net* net, unsigned long long seed) {
+    CHECK_CTX(ctx);
+    if (!net) return fail(OFRI_E_ARG, "ofri_net_init: null pointer");
+    return net_init_params(ctx, NET(net), seed);
+}
+
 int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n) {
     CHECK_CTX(ctx);
     if (!net || !h_params) return fail(OFRI_E_ARG, "ofri_net_get_params: null pointer");
@@ -803,6 +895,90 @@ int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float
     if (h_loss) {
         OFRI_HIP(hipMemcpyAsync(h_loss, l, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return OFRI_OK;
+}
+
+int ofri_render_pair(ofri_ctx* ctx, int H, int W, int family, float density, unsigned long long seed,
+                     const float* h_par, int npar, const float* h_modes, int nmodes, float noise, float* d_im1,
+                     float* d_im2, float* d_u, float* d_v) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2) return fail(OFRI_E_ARG, "ofri_render_pair: null pointer");
+    if ((d_u == nullptr) != (d_v == nullptr)) return fail(OFRI_E_ARG, "ofri_render_pair: give both d_u and d_v");
+    return render_pair(ctx, H, W, family, density, seed, h_par, npar, h_modes, nmodes, noise, d_im1, d_im2, d_u,
+                       d_v);
+}
+
+int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                          const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,
+                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged) {
+    CHECK_CTX(ctx);
+    if (!p || !lp || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc)
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: null pointer");
+    if (p->levels != 2) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: 2 levels only (got %d)", p->levels);
+    if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
+    if (H % 16 || W % 16 || H < 32 || W < 32)
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d (needs multiples of 16)", H, W);
+    if (lp->hs_budget < 1 || lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
+    if (lp->cheb_bound != 0.0f && !(lp->cheb_bound > 0.0f && lp->cheb_bound <= 0.999f))
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g", lp->cheb_bound);
+    const int L = 2;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    void* pr;
+    const size_t nc = 4 * (size_t)(H / 2) * (W / 2), nf = 4 * (size_t)H * W;
+    OFRI_TRY(scratch(ctx, "lrn_pred", (nc + nf) * sizeof(float), &pr));
+    LearnedRun run{lp, NET(net), (float*)pr, (float*)pr + nc};
+    std::string key = "learned" + lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
+    {
+        const void* np = net;
+        key.append((const char*)&np, sizeof np);
+        key.append((const char*)lp->hs_tol, 2 * sizeof(float));
+        key.append((const char*)lp->ls_tol, 2 * sizeof(float));
+        key.append((const char*)&lp->hs_budget, sizeof(int) * 2 + sizeof(float) * 2);
+    }
+    auto enqueue = [&]() { return pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, nullptr, false, &run); };
+    if (!ctx->graphs_on) {
+        OFRI_TRY(enqueue());
+    } else {
+        GraphEntry& g = ctx->graphs[key];
+        if (g.exec && g.ws_gen == ctx->ws_gen) {
+            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
+            if (g.exec) {
+                (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+            }
+            OFRI_TRY(enqueue());
+            g.eager_runs += 1;
+            g.ws_gen = ctx->ws_gen;
+        } else {
+            hipGraph_t graph;
+            OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int st = enqueue();
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            if (st != OFRI_OK) return st;
+            if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                g.exec = nullptr;
+                return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+            }
+            g.ws_gen = ctx->ws_gen;
+            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        }
+    }
+    if (h_stats || h_converged) {
+        ofri_level_stats st[2];
+        OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, st));
+        int ok = 1;
+        for (int l = 0; l < L; ++l) {
+            if (!(st[l].hs_res <= lp->hs_tol[l])) ok = 0;
+            if (p->ls_alpha > 0.0f && !(st[l].ls_err <= lp->ls_tol[l])) ok = 0;
+            if (h_stats) h_stats[l] = st[l];
+        }
+        if (h_converged) *h_converged = ok;
     }
     return OFRI_OK;
 }

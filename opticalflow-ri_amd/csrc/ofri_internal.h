@@ -41,7 +41,7 @@ struct SolveCtrl {
     int redo_in;     // ping-pong index the redo reads
     float last_err;  // last update norm (float32, like the reference)
     int arrived;     // workgroups of the current launch that have finished
-    int pad;
+    int limit;       // >0: the solve also stops at this many sweeps (a later phase's budget)
 };
 
 // A device allocation owned by the context, grown on demand.
@@ -103,6 +103,8 @@ int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* c
                 float* out);
 int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n);
 int fill_zero(ofri_ctx* ctx, float* p, size_t n);
+// d = a - b (float32), n elements; d may alias neither input
+int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n);
 // max over n floats -> d_out[0]
 int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
 // d_out[0] = (||A-A0|| + ||B-B0||) / n as float32 (differences in float32)
@@ -117,8 +119,8 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // never), the sweep history in d_hist (niter floats, may be null).
 // cheb_b in (0, 1) (needs d_ctrl): Chebyshev-accelerated sweeps for a
 // spectrum bound b on the slowest mode; 0 = plain sweeps.  cheb_switch > 1:
-// accelerate to cheb_switch * tol, then plain sweeps to tol (niter sweeps in
-// all).
+// accelerate to cheb_switch * tol, then at most niter / 2 plain sweeps to
+// tol (niter sweeps in all).
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
@@ -162,12 +164,17 @@ int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter,
 int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, double tol,
              SolveCtrl* d_ctrls, void* d_tab, bool upload);
 
+// ---- synthetic pairs (k_synth.hip) ----
+int render_pair(ofri_ctx* ctx, int H, int W, int family, float density, unsigned long long seed, const float* par,
+                int npar, const float* h_modes, int nmodes, float noise, float* im1, float* im2, float* u, float* v);
+
 // ---- warm-start network (k_cnn.hip) ----
 struct Net;
 int net_create(ofri_ctx* ctx, Net** out);
 void net_destroy(Net* n);
 size_t net_num_params(const Net* n);
 int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count);
+int net_init_params(ofri_ctx* ctx, Net* n, unsigned long long seed);
 int net_get_params(ofri_ctx* ctx, Net* n, float* h, size_t count);
 int net_get_grads(ofri_ctx* ctx, Net* n, float* h, size_t count);
 // planar outputs: coarse [B][4][H/2][W/2], fine [B][4][H][W]

@@ -57,7 +57,7 @@ __device__ void solve_check(const double* part, int nblk, int nsw, int dst, doub
                     ctrl->redo_in = dst ^ 1;
                 }
                 stop = 1;
-            } else if (ctrl->sweeps >= maxnum) {
+            } else if (ctrl->sweeps >= maxnum || (ctrl->limit > 0 && ctrl->sweeps >= ctrl->limit)) {
                 ctrl->done = 1;
             }
         }
@@ -85,10 +85,12 @@ __device__ __forceinline__ void arrive_and_check(const double* part, int nblk, i
     }
 }
 
-// Re-arms a finished solve for a further phase on the same sweep count,
-// budget and history: the phase starts unless the budget is spent.
-__global__ void ctrl_continue_k(SolveCtrl* c, int maxnum) {
+// Re-arms a finished solve for a further phase of at most `budget` sweeps
+// on the same sweep count and history (the phase starts unless the overall
+// budget maxnum is spent).
+__global__ void ctrl_continue_k(SolveCtrl* c, int maxnum, int budget) {
     c->done = c->sweeps >= maxnum;
+    c->limit = c->sweeps + budget;
     c->final_buf = 0;
     c->redo = 0;
     c->redo_in = 0;
@@ -103,6 +105,7 @@ __global__ void ctrl_init_k(SolveCtrl* c) {
     c->redo_in = 0;
     c->last_err = 0.0f;
     c->arrived = 0;
+    c->limit = 0;
 }
 
 }  // namespace

@@ -42,12 +42,17 @@ class LshsParams(C.Structure):
 
 class LevelStats(C.Structure):
     _fields_ = [("H", C.c_int), ("W", C.c_int), ("hs_err", C.c_float), ("ls_err", C.c_float),
-                ("ls_iters", C.c_int)]
+                ("ls_iters", C.c_int), ("hs_res", C.c_float), ("hs_iters", C.c_int)]
 
 
 class Pair(C.Structure):
     _fields_ = [("im1", C.c_void_p), ("im2", C.c_void_p), ("H", C.c_int), ("W", C.c_int),
                 ("U", C.c_void_p), ("V", C.c_void_p)]
+
+
+class LearnedParams(C.Structure):
+    _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int),
+                ("ls_budget", C.c_int), ("cheb_bound", C.c_float), ("cheb_finish", C.c_float)]
 
 
 class FbParams(C.Structure):
@@ -103,14 +108,21 @@ SIGNATURES = {
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],
+    "ofri_render_pair": [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp, C.c_int, _fp, C.c_int,
+                         C.c_float, _vp, _vp, _vp, _vp],
     "ofri_net_create": [_vp, C.POINTER(_vp)],
     "ofri_net_destroy": [_vp],
     "ofri_net_num_params": [_vp, C.POINTER(C.c_size_t)],
     "ofri_net_set_params": [_vp, _vp, _fp, C.c_size_t],
+    "ofri_net_init": [_vp, _vp, C.c_ulonglong],
     "ofri_net_get_params": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_get_grads": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_predict": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp],
     "ofri_net_train": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_float, _fp],
+    "ofri_pyr_lshs_taps": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp, _vp,
+                           C.POINTER(LevelStats)],
+    "ofri_pyr_lshs_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), C.POINTER(LearnedParams),
+                              _vp, _vp, _vp, C.POINTER(LevelStats), _ip],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],

@@ -115,11 +115,16 @@ class DeviceArray:
 
     @property
     def H(self):
-        return self.shape[0]
+        return self.shape[-2]
 
     @property
     def W(self):
-        return self.shape[1]
+        return self.shape[-1]
+
+    def view(self, offset, shape):
+        """A float32 window of this allocation starting `offset` elements in
+        (no copy; valid while this array lives)."""
+        return DeviceView(self, offset, shape)
 
     def __del__(self):
         try:
@@ -128,6 +133,24 @@ class DeviceArray:
                 self.ptr = None
         except Exception:
             pass
+
+
+class DeviceView(DeviceArray):
+    """A window of another DeviceArray's memory; frees nothing."""
+
+    __slots__ = ("base",)
+
+    def __init__(self, base, offset, shape):
+        self.ctx = base.ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        if offset < 0 or offset * 4 + self.nbytes > base.nbytes:
+            raise ValueError("view [%d, +%d) outside an allocation of %d bytes" % (offset * 4, self.nbytes, base.nbytes))
+        self.base = base
+        self.ptr = C.c_void_p(base.ptr.value + 4 * int(offset))
+
+    def __del__(self):
+        pass
 
 
 def as_device(a, ctx=None):

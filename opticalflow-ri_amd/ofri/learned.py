@@ -1,0 +1,120 @@
+"""The learned form of LS(HS) 2-level: network warm starts, solves to the
+plain method's residual targets, plain fallback.
+
+    r*  per level: the plain solver's update norm at its stopping point (HS
+        after its 600 sweeps, Liu-Shen at tol or 60 sweeps), from
+        ofri_pyr_lshs_taps on the same pair (plain_targets), or given.
+    run per level: HS from the network's flow (Chebyshev-accelerated, plain
+        finish) until its norm <= r*_HS; Liu-Shen from HS's result plus the
+        network's correction until its norm <= r*_LS.
+    fallback: when a level misses a target within the sweep budgets, the
+        plain pyramid runs instead (its result is returned).
+"""
+import time
+
+import numpy as np
+
+from . import ops
+from .device import DeviceArray, context
+
+WORKLOAD = dict(levels=2, alphas=[45.0, 21.0], niter=600, ls_alpha=5.0, ls_maxnum=60, ls_tol=1e-8)
+# Chebyshev spectrum bound and plain-finish factor of the accelerated HS
+CHEB_BOUND = 0.99
+CHEB_FINISH = 4.0
+
+
+def filters():
+    from gaussian_filter import prepareGaussianKernel
+    return prepareGaussianKernel(3.4, 3), prepareGaussianKernel(0.48, 5)
+
+
+# A residual below the float32 resolution of the solution itself carries no
+# information: the target of a level is the larger of the plain residual and
+# the norm of a 2^-22 relative perturbation of the plain HS solution,
+# (||U||_2 + ||V||_2) * 2^-22 / N, the same norm the residual uses.
+F32_FLOOR = 2.0 ** -22
+
+
+def plain_targets(im1, im2, fw=None, ow=None):
+    """Plain run with residuals recorded.  Returns (U, V, hs_tol[2],
+    ls_tol[2], stats); stats carry the raw plain residuals (hs_res, ls_err)."""
+    if fw is None:
+        fw, ow = filters()
+    U, V, taps, st = ops.pyr_lshs_taps(im1, im2, filt_w=fw, opt_w=ow, **WORKLOAD)
+    t = taps.download()
+    hs_tol, off = [], 0
+    for s in st:
+        n = s["H"] * s["W"]
+        u, v = t[off:off + n].astype(np.float64), t[off + n:off + 2 * n].astype(np.float64)
+        floor = F32_FLOOR * (np.sqrt((u * u).sum()) + np.sqrt((v * v).sum())) / n
+        hs_tol.append(float(max(s["hs_res"], floor)))
+        off += 4 * n
+    return U, V, hs_tol, [float(s["ls_err"]) for s in st], st
+
+
+# Sweep budgets of the learned form: HS at most 200 per level (about 2.5x what
+# the accelerated sweeps need from a zero start), Liu-Shen the plain 60.
+HS_BUDGET = 200
+LS_BUDGET = 60
+
+
+def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None, fallback=True, stats=True,
+          hs_budget=HS_BUDGET, ls_budget=LS_BUDGET):
+    """One pair through the learned form.  Returns (U, V, info) with info:
+    converged, fell_back, per-level stats."""
+    if fw is None:
+        fw, ow = filters()
+    U, V, st, conv = ops.pyr_lshs_learned(im1, im2, net=net, hs_tol=hs_tol, ls_tol=ls_tol, filt_w=fw, opt_w=ow,
+                                          hs_budget=hs_budget, ls_budget=ls_budget, cheb_bound=CHEB_BOUND,
+                                          cheb_finish=CHEB_FINISH, Uacc=Uacc, Vacc=Vacc, stats=stats or fallback,
+                                          **{k: WORKLOAD[k] for k in ("levels", "alphas", "niter", "ls_alpha")})
+    fell = False
+    if fallback and not conv:
+        ops.pyr_lshs(im1, im2, filt_w=fw, opt_w=ow, Uacc=U, Vacc=V, stats=False, **WORKLOAD)
+        fell = True
+    return U, V, dict(converged=conv, fell_back=fell, stats=st)
+
+
+def epe(a, b):
+    e = np.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2)
+    return float(e.mean()), float(e.max())
+
+
+def evaluate(pairs, net, reps=5, with_baseline=True):
+    """Per pair: plain time, learned time (network + solves to r*), sweeps,
+    EPE of the learned result to the plain one; also the method without the
+    network (zero warm starts) when with_baseline.  pairs: list of (im1, im2)."""
+    ctx = context()
+    fw, ow = filters()
+    rows = []
+    for im1, im2 in pairs:
+        d1, d2 = (im1 if hasattr(im1, "ptr") else DeviceArray.from_host(im1),
+                  im2 if hasattr(im2, "ptr") else DeviceArray.from_host(im2))
+        H, W = d1.shape[-2:]
+        Up, Vp, ht, lt, pst = plain_targets(d1, d2, fw, ow)
+        plain = (Up.download(), Vp.download())
+        row = dict(H=H, W=W, hs_tol=ht, ls_tol=lt, plain_ls_iters=[s["ls_iters"] for s in pst])
+        Ua, Va = DeviceArray((H, W)), DeviceArray((H, W))
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            ctx.sync()
+            t = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            ctx.sync()
+            return (time.perf_counter() - t) / reps * 1e3
+
+        row["plain_ms"] = timed(lambda: ops.pyr_lshs(d1, d2, filt_w=fw, opt_w=ow, Uacc=Ua, Vacc=Va, stats=False,
+                                                     **WORKLOAD))
+        for name, nt in (("learned", net), ("nonet", None)) if with_baseline else (("learned", net),):
+            U, V, info = solve(d1, d2, nt, ht, lt, fw, ow, Uacc=Ua, Vacc=Va, fallback=False)
+            res = (U.download(), V.download())
+            row[name] = dict(converged=info["converged"], epe=epe(res, plain),
+                             hs_sweeps=[s["hs_iters"] for s in info["stats"]],
+                             ls_sweeps=[s["ls_iters"] for s in info["stats"]],
+                             ms=timed(lambda: solve(d1, d2, nt, ht, lt, fw, ow, Uacc=Ua, Vacc=Va, fallback=True,
+                                                    stats=False)))
+        rows.append(row)
+    return rows

@@ -15,7 +15,7 @@ from . import _capi
 from .device import DeviceArray, as_device, context
 
 # Loss weights per output field: coarse HS u, v, LS-HS u, v; fine likewise.
-FIELD_WEIGHTS = np.array([1.0, 1.0, 2.0, 2.0, 1.0, 1.0, 2.0, 2.0], np.float32)
+FIELD_WEIGHTS = np.array([1.0, 1.0, 4.0, 4.0, 1.0, 1.0, 4.0, 4.0], np.float32)
 WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
 
 
@@ -45,6 +45,10 @@ class Net(object):
         if p.size != self.num_params:
             raise ValueError("network has %d parameters, got %d" % (self.num_params, p.size))
         _capi.check(self.ctx.lib.ofri_net_set_params(self.ctx.handle, self.handle, _capi.fptr(p), p.size))
+
+    def init(self, seed=0):
+        """Random initial parameters (He-normal, heads scaled down)."""
+        _capi.check(self.ctx.lib.ofri_net_init(self.ctx.handle, self.handle, int(seed)))
 
     def get_params(self):
         out = np.empty(self.num_params, np.float32)

@@ -104,6 +104,24 @@ def hs_tol(im1, im2, alpha, maxnum, tol, U, V, hist=None):
     return np.float32(err.value), it.value
 
 
+def render_pair(H, W, family, density, seed, par, modes=None, noise=2.0, im1=None, im2=None, truth=False):
+    """Synthetic PIV pair rendered on the device (ofri_render_pair).  family 0
+    Poiseuille (par: umax), 1 Lamb-Oseen (cx, cy, rc, vmax), 2 shear +
+    turbulence (shear, dmax; modes: (4, n) array kx, ky, phase, amp).
+    Returns (im1, im2) or (im1, im2, u, v) DeviceArrays."""
+    par = np.ascontiguousarray(par, dtype=np.float32)
+    md = None if modes is None else np.ascontiguousarray(modes, dtype=np.float32)
+    im1 = im1 if im1 is not None else DeviceArray((H, W))
+    im2 = im2 if im2 is not None else DeviceArray((H, W))
+    u = DeviceArray((H, W)) if truth else None
+    v = DeviceArray((H, W)) if truth else None
+    _capi.check(_lib().ofri_render_pair(_h(), H, W, int(family), float(density), int(seed) & ((1 << 64) - 1),
+                                        _capi.fptr(par), par.size, _capi.fptr(md) if md is not None else None,
+                                        0 if md is None else md.shape[-1], float(noise), im1.ptr, im2.ptr,
+                                        u.ptr if truth else None, v.ptr if truth else None))
+    return (im1, im2, u, v) if truth else (im1, im2)
+
+
 def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0):
     """Chebyshev-accelerated Horn-Schunck to the plain residual tol
     (ofri_hs_accel; finish > 1 ends with plain sweeps from finish * tol);
@@ -232,9 +250,71 @@ def batch_lshs(pairs, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=
     res = []
     if stats:
         for i in range(n):
-            res.append([dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err),
-                             ls_iters=s.ls_iters) for s in st[i * levels:(i + 1) * levels]])
+            res.append(_stats_dicts(st[i * levels:(i + 1) * levels]))
     return out, res
+
+
+def _stats_dicts(st):
+    return [dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err), ls_iters=s.ls_iters,
+                 hs_res=np.float32(s.hs_res), hs_iters=s.hs_iters) for s in st]
+
+
+def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
+                  Uacc=None, Vacc=None, taps=None):
+    """ofri_pyr_lshs_taps: the pyramid with every level's HS result and Liu-Shen
+    correction exported.  Returns (Uacc, Vacc, taps, stats); taps is a flat
+    DeviceArray holding, per level, 4 planes of the level's size."""
+    im1, im2 = as_device(im1), as_device(im2)
+    H, W = im1.shape[-2:]
+    p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol)
+    sizes = level_sizes(H, W, levels)
+    n = 4 * sum(h * w for h, w in sizes)
+    taps = taps if taps is not None else DeviceArray((n,))
+    if taps.nbytes < 4 * n:
+        raise ValueError("taps buffer holds %d floats, %d needed" % (taps.nbytes // 4, n))
+    Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
+    Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
+    st = (_capi.LevelStats * levels)()
+    _capi.check(_lib().ofri_pyr_lshs_taps(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
+                                          taps.ptr, st))
+    return Uacc, Vacc, taps, _stats_dicts(st)
+
+
+def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
+                     hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_maxnum=60, ls_tol_plain=1e-8,
+                     Uacc=None, Vacc=None, stats=True):
+    """The learned form (ofri_pyr_lshs_learned): network warm starts, HS and
+    Liu-Shen run to the per-level residual targets hs_tol / ls_tol.  net: an
+    ofri.net.Net or None (zero warm starts).  Returns (Uacc, Vacc, stats,
+    converged); converged is None when stats is False (no synchronisation)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    H, W = im1.shape[-2:]
+    p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol_plain)
+    ht = np.ascontiguousarray(hs_tol, dtype=np.float32)
+    lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
+    lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget), float(cheb_bound),
+                             float(cheb_finish))
+    Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
+    Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
+    st = (_capi.LevelStats * levels)() if stats else None
+    conv = C.c_int(0)
+    _capi.check(_lib().ofri_pyr_lshs_learned(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), C.byref(lp),
+                                             net.handle if net is not None else None, Uacc.ptr, Vacc.ptr, st,
+                                             C.byref(conv) if stats else None))
+    return Uacc, Vacc, (_stats_dicts(st) if stats else []), (bool(conv.value) if stats else None)
+
+
+def level_sizes(H, W, levels):
+    """Level sizes of the pyramid, coarse to fine (GenericPyramidalOpticalFlow.py:291,335-342)."""
+    out = []
+    scale = 1.0 / (2.0 ** (levels - 1))
+    for level in range(1, levels + 1):
+        if scale < 1.0 and level != levels:
+            out.append((int(np.int32(np.round(H * scale))), int(np.int32(np.round(W * scale)))))
+        else:
+            out.append((H, W))
+        scale *= 2
+    return out
 
 
 def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
@@ -263,6 +343,5 @@ def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum
     st = (_capi.LevelStats * levels)() if stats else None
     _capi.check(_lib().ofri_pyr_lshs(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
                                      st if stats else None))
-    out = [dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err), ls_iters=s.ls_iters)
-           for s in st] if stats else []
+    out = _stats_dicts(st) if stats else []
     return Uacc, Vacc, out
