@@ -201,6 +201,134 @@ __global__ __launch_bounds__(256) void conv3_k(ConvArgs a) {
     }
 }
 
+// LDS-tiled form of conv3_k.  A workgroup (4 wavefronts) owns a TS x TS
+// output tile of one image (TS = 16, or 8 for the small low-resolution
+// levels, which would otherwise leave most CUs idle) and NT x 16 output
+// channels; wavefront w computes pixels [16 MT w, 16 MT (w + 1)) of the tile
+// in row-major order, one 16-pixel m-tile each.  Input channels go in chunks
+// of 32: the (TS + 2)^2 input patch of the chunk (halo included, zero outside
+// the image, gated / converted / space-to-depth-read on the way in) and the
+// chunk's weights for all 9 taps are staged in LDS once, then every tap's
+// A and B fragments are 16-byte LDS reads and 9 x MT x NT MFMAs run per
+// wavefront per chunk.  Pixel rows are padded to 40 bf16 (80 bytes) and
+// weight rows to 296 bf16 so that the 16 lanes of a fragment read spread
+// over the LDS banks.
+constexpr int kPixStride = 40;   // bf16 per staged pixel (32 used)
+constexpr int kCoStride = 296;   // bf16 per staged weight row (9 * 32 used)
+
+template <int TS, int NT>
+__global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
+    constexpr int PS = TS + 2;                 // staged patch side
+    constexpr int MT = TS * TS / 64;           // m-tiles per wavefront
+    __shared__ __attribute__((aligned(16))) u16 sx[PS * PS * kPixStride];
+    __shared__ __attribute__((aligned(16))) u16 sw[NT * 16 * kCoStride];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int H = a.H, W = a.W;
+    const int tiles_x = (W + TS - 1) / TS, tiles_y = (H + TS - 1) / TS;
+    int tile = blockIdx.x;
+    const int b = tile / (tiles_x * tiles_y);
+    tile -= b * tiles_x * tiles_y;
+    const int y0 = (tile / tiles_x) * TS, x0 = (tile % tiles_x) * TS;
+    const int cobase = blockIdx.y * (NT * 16);
+
+    f4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const bool plain_x = !a.x.f32 && !a.gate.p;
+    for (int c0 = 0; c0 < a.Cin; c0 += 32) {
+        // stage the input patch: PS x PS pixels x 4 groups of 8 channels
+        for (int idx = t; idx < PS * PS * 4; idx += 256) {
+            const int pix = idx >> 2, part = idx & 3;
+            const int py = pix / PS, px = pix - py * PS;
+            const int gy = y0 - 1 + py, gx = x0 - 1 + px;
+            const int ch = c0 + part * 8;
+            uint4 raw = {0u, 0u, 0u, 0u};
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W && ch < a.Cin) {
+                const long long off = view_off(a.x, b, gy, gx, ch, H, W, a.Cin);
+                if (plain_x) {
+                    raw = *(const uint4*)((const u16*)a.x.p + off);
+                } else {
+                    float f[8];
+                    load8f(a.x, off, f);
+                    if (a.gate.p) {
+                        float gt[8];
+                        load8f(a.gate, view_off(a.gate, b, gy, gx, ch, H, W, a.Cin), gt);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) f[i] = gt[i] > 0.0f ? f[i] : f[i] * kSlope;
+                    }
+                    raw.x = (unsigned)f2bf(f[0]) | ((unsigned)f2bf(f[1]) << 16);
+                    raw.y = (unsigned)f2bf(f[2]) | ((unsigned)f2bf(f[3]) << 16);
+                    raw.z = (unsigned)f2bf(f[4]) | ((unsigned)f2bf(f[5]) << 16);
+                    raw.w = (unsigned)f2bf(f[6]) | ((unsigned)f2bf(f[7]) << 16);
+                }
+            }
+            *(uint4*)(sx + pix * kPixStride + part * 8) = raw;
+        }
+        // stage the weights: NT*16 rows x 9 taps x 4 groups of 8 channels
+        for (int idx = t; idx < NT * 16 * 36; idx += 256) {
+            const int co = idx / 36, rem = idx - co * 36;
+            const int tap = rem >> 2, part = rem & 3;
+            const int ch = c0 + part * 8;
+            uint4 raw = {0u, 0u, 0u, 0u};
+            if (ch < a.Cin) raw = *(const uint4*)(a.w + (size_t)(cobase + co) * a.Kp + tap * a.Cin + ch);
+            *(uint4*)(sw + co * kCoStride + tap * 32 + part * 8) = raw;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap - 3 * (tap / 3);   // patch offsets 0..2
+            bf8 bfr[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                bfr[nt] = __builtin_bit_cast(bf8, *(const uint4*)(sw + (nt * 16 + r16) * kCoStride + tap * 32 + g * 8));
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int p = (wv * MT + mt) * 16 + r16;
+                const int py = p / TS + dy, px = p % TS + dx;
+                const bf8 afr = __builtin_bit_cast(bf8, *(const uint4*)(sx + (py * PS + px) * kPixStride + g * 8));
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // D layout: lane holds m-tile pixels 4g..4g+3, channel r16
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = cobase + nt * 16 + r16;
+        if (co >= a.Cout) continue;
+        const float bv = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = (wv * MT + mt) * 16 + 4 * g + i;
+                const int y = y0 + p / TS, x = x0 + p % TS;
+                if (y >= H || x >= W) continue;
+                float v = acc[mt][nt][i] + bv;
+                if (a.act) v = v > 0.0f ? v : v * kSlope;
+                const long long off = view_off(a.y, b, y, x, co, H, W, a.Cout);
+                if (a.y.f32) {
+                    float* d = (float*)a.y.p + off;
+                    if (a.beta) v = *d + v;
+                    *d = v;
+                } else {
+                    u16* d = (u16*)a.y.p + off;
+                    if (a.beta) v = bf2f(*d) + v;
+                    *d = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
 // Weight / bias gradients of one convolution.  Block (kt, ct, chunk): a
 // 64-wide slice of K, a 32-wide slice of output channels, and `chunk` pixels
 // in stages of 64.  dZ = dY gated by the activation derivative (from the
@@ -572,14 +700,18 @@ View tview(Net* n, void* base, int t, int c0, int mode) {
     return v;
 }
 
+// 16 x 16 tiles while they give a few hundred workgroups, 8 x 8 tiles with
+// 32-channel groups below (low-resolution levels).
 int launch_conv(ofri_ctx* ctx, const ConvArgs& a) {
-    const long long P = (long long)a.B * a.H * a.W;
-    if (a.Cout > 32) {
-        dim3 g((unsigned)((P + 255) / 256), (unsigned)((a.Cout + 63) / 64));
-        conv3_k<4, 4><<<g, 256, 0, ctx->stream>>>(a);
+    const unsigned t16 = (unsigned)(a.B * ((a.H + 15) / 16) * ((a.W + 15) / 16));
+    const unsigned t8 = (unsigned)(a.B * ((a.H + 7) / 8) * ((a.W + 7) / 8));
+    if (t16 >= 512) {
+        if (a.Cout > 32) conv3_lds_k<16, 4><<<dim3(t16, (unsigned)((a.Cout + 63) / 64)), 256, 0, ctx->stream>>>(a);
+        else if (a.Cout > 16) conv3_lds_k<16, 2><<<dim3(t16, 1), 256, 0, ctx->stream>>>(a);
+        else conv3_lds_k<16, 1><<<dim3(t16, 1), 256, 0, ctx->stream>>>(a);
     } else {
-        dim3 g((unsigned)((P + 255) / 256), (unsigned)((a.Cout + 31) / 32));
-        conv3_k<4, 2><<<g, 256, 0, ctx->stream>>>(a);
+        if (a.Cout > 16) conv3_lds_k<8, 2><<<dim3(t8, (unsigned)((a.Cout + 31) / 32)), 256, 0, ctx->stream>>>(a);
+        else conv3_lds_k<8, 1><<<dim3(t8, 1), 256, 0, ctx->stream>>>(a);
     }
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;

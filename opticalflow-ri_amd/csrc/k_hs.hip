@@ -73,6 +73,7 @@ struct HsPass {
     int maxnum;
     size_t N;
     float* hist;             // per-sweep residuals (nullptr: not recorded)
+    int check_last;          // 1: only a launch's last sweep is reduced and tested
     // Chebyshev mode (hs_pass_k<..., true, true>): previous iterate buffers,
     // the weight of every global sweep and the damping of the plain step
     const float* pin[2][2];
@@ -181,7 +182,20 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     }
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
-    __shared__ double red[WAVES][2];
+    // per-wave squared-update sums of a sweep, double-buffered: the next
+    // sweep's barrier publishes them, so the norm costs no barrier of its own
+    __shared__ double red[2][WAVES][2];
+    // sweep sw reduces its update norm (every sweep, or a launch's last)
+    auto normed = [&](int sw) { return NORM && !a.redo && (!a.check_last || sw == nsw - 1); };
+    auto flush = [&](int sw) {
+        double tu = 0.0, tv = 0.0;
+        for (int q = 0; q < WAVES; ++q) {
+            tu = tu + red[sw & 1][q][0];
+            tv = tv + red[sw & 1][q][1];
+        }
+        a.part[((size_t)sw * nblk + blk) * 2] = tu;
+        a.part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
+    };
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
         edge[b][w][0][0][lane] = U[0];
@@ -189,6 +203,8 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         edge[b][w][1][0][lane] = U[R - 1];
         edge[b][w][1][1][lane] = V[R - 1];
         __syncthreads();
+        if (s > 0 && normed(s - 1) && threadIdx.x == 0) flush(s - 1);
+        const bool norm = normed(s);
         float uup = w > 0 ? edge[b][w - 1][1][0][lane] : U[0];
         float vup = w > 0 ? edge[b][w - 1][1][1][lane] : V[0];
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : U[R - 1];
@@ -216,7 +232,7 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             float der = quotient(num, DEN[r], RCP[r]);
             float un = ub - x * der;
             float vn = vb - y * der;
-            if (NORM && !a.redo) {
+            if (norm) {
                 const int ry = w * R + r, gy = gy0 + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
                     float du = un - U[r], dv = vn - V[r];
@@ -242,27 +258,21 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             up_u = cu; up_v = cv;
             cu = dn_u; cv = dn_v;
         }
-        if (NORM && !a.redo) {
+        if (norm) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 su = su + __shfl_xor(su, off);
                 sv = sv + __shfl_xor(sv, off);
             }
             if (lane == 0) {
-                red[w][0] = su;
-                red[w][1] = sv;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double tu = 0.0, tv = 0.0;
-                for (int q = 0; q < WAVES; ++q) {
-                    tu = tu + red[q][0];
-                    tv = tv + red[q][1];
-                }
-                a.part[((size_t)s * nblk + blk) * 2] = tu;
-                a.part[((size_t)s * nblk + blk) * 2 + 1] = tv;
+                red[b][w][0] = su;
+                red[b][w][1] = sv;
             }
         }
+    }
+    if (nsw > 0 && normed(nsw - 1)) {
+        __syncthreads();
+        if (threadIdx.x == 0) flush(nsw - 1);
     }
 
     float* uo = a.out[src ^ 1][0];
@@ -284,7 +294,8 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         }
     }
     if (NORM && !a.redo)
-        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, CHEB);
+        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, CHEB && !a.check_last,
+                         a.check_last ? nsw - 1 : 0);
 }
 
 // Start of a launch in residual mode: the re-run launch takes its sweep count
@@ -384,6 +395,15 @@ constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
 // fused 10 sweeps deep (fewer launches outweigh the wider halo); smaller
 // images 4-row lanes in 8-wavefront tiles, 5 deep (more workgroups to spread
 // over the 256 CUs).
+// Residual-mode defaults (measured on gfx950, Chebyshev and plain sweeps
+// with norms): 8 rows x 16 wavefronts fused 10 deep from about 1M pixels,
+// 4 rows x 16 wavefronts fused 10 deep below (fewer launches than the plain
+// sweep's shape, whose per-launch test would cost more).
+void hs_cheb_default(size_t npx, int* variant, int* depth) {
+    *variant = npx >= (size_t)700000 ? 11 : 9;
+    *depth = 10;
+}
+
 void hs_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
         *variant = 11;
@@ -398,7 +418,7 @@ void hs_default(size_t npx, int* variant, int* depth) {
 
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
-             float* d_hist, float cheb_b, float cheb_switch) {
+             float* d_hist, float cheb_b, float cheb_switch, bool check_last) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
     const bool cheb = cheb_b > 0.0f;
@@ -406,7 +426,8 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 0.999], got %g", cheb_b);
     size_t N = (size_t)H * W;
     int vi, depth;
-    hs_default((size_t)H * W, &vi, &depth);
+    if (cheb) hs_cheb_default((size_t)H * W, &vi, &depth);
+    else hs_default((size_t)H * W, &vi, &depth);
     if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
     if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
     if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
@@ -470,6 +491,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.maxnum = niter;
     p.N = N;
     p.hist = d_hist;
+    p.check_last = check_last ? 1 : 0;
     if (cheb) {
         p.pin[0][0] = P0; p.pin[0][1] = P0 + N; p.pin[1][0] = P0 + 2 * N; p.pin[1][1] = P0 + 3 * N;
         p.pout[0][0] = P0; p.pout[0][1] = P0 + N; p.pout[1][0] = P0 + 2 * N; p.pout[1][1] = P0 + 3 * N;

@@ -94,6 +94,7 @@ struct LsPass {
     SolveCtrl* ctrl;
     int redo;               // 1: re-run launch driven by ctrl->redo
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
+    int check_last;         // 1: only a launch's last sweep is reduced and tested
 };
 
 struct Raw {
@@ -182,11 +183,23 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
                                           const float (&IYT)[R], const float (&B11)[R], const float (&B12)[R],
                                           const float (&B22)[R]) {
     __shared__ float edge[2][WAVES][2][2][64];
-    __shared__ double red[WAVES][2];
+    // per-wave squared-update sums, double-buffered: the next sweep's barrier
+    // publishes them (one barrier per sweep in all)
+    __shared__ double red[2][WAVES][2];
     const int K = a.K, H = a.H, W = a.W;
     const bool el = gx == 0, er = gx == W - 1;
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
     const float h = a.h;
+    auto normed = [&](int sw) { return a.part != nullptr && (!a.check_last || sw == nsw - 1); };
+    auto flush = [&](int sw) {
+        double tu = 0.0, tv = 0.0;
+        for (int q = 0; q < WAVES; ++q) {
+            tu = tu + red[sw & 1][q][0];
+            tv = tv + red[sw & 1][q][1];
+        }
+        a.part[((size_t)sw * nblk + blk) * 2] = tu;
+        a.part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
+    };
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
         edge[b][w][0][0][lane] = u[0];
@@ -194,6 +207,8 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
         edge[b][w][1][0][lane] = u[R - 1];
         edge[b][w][1][1][lane] = v[R - 1];
         __syncthreads();
+        if (s > 0 && normed(s - 1) && threadIdx.x == 0) flush(s - 1);
+        const bool norm = normed(s);
         float uup = w > 0 ? edge[b][w - 1][1][0][lane] : u[0];
         float vup = w > 0 ? edge[b][w - 1][1][1][lane] : v[0];
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : u[R - 1];
@@ -209,7 +224,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
             float un, vn;
             ls_update<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r], II[r],
                             IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
-            if (a.part) {
+            if (norm) {
                 const int ry = w * R + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
                     float du = un - u[r], dv = vn - v[r];
@@ -222,7 +237,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
             pu = cu; pv = cv;
             cu = nu; cv = nv;
         }
-        if (a.part) {
+        if (norm) {
             // fixed-order wave reduction, then waves in order
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -230,20 +245,14 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
                 sv = sv + __shfl_xor(sv, off);
             }
             if (lane == 0) {
-                red[w][0] = su;
-                red[w][1] = sv;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double tu = 0.0, tv = 0.0;
-                for (int q = 0; q < WAVES; ++q) {
-                    tu = tu + red[q][0];
-                    tv = tv + red[q][1];
-                }
-                a.part[((size_t)s * nblk + blk) * 2] = tu;
-                a.part[((size_t)s * nblk + blk) * 2 + 1] = tv;
+                red[b][w][0] = su;
+                red[b][w][1] = sv;
             }
         }
+    }
+    if (nsw > 0 && normed(nsw - 1)) {
+        __syncthreads();
+        if (threadIdx.x == 0) flush(nsw - 1);
     }
 }
 
@@ -315,7 +324,9 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
             }
         }
     }
-    if (a.part) arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist);
+    if (a.part)
+        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, false,
+                         a.check_last ? nsw - 1 : 0);
 }
 
 template <int R, int WAVES>
@@ -379,7 +390,7 @@ void ls_default(size_t npx, int* variant, int* depth) {
 }  // namespace
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
-             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl) {
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
     size_t N = (size_t)H * W;
@@ -417,6 +428,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.ctrl = d_ctrl;
     p.tol = tol;
     p.maxnum = maxnum;
+    p.check_last = check_last ? 1 : 0;
     int done = 0, src = 0;
     while (done < maxnum) {
         int ns = maxnum - done < depth ? maxnum - done : depth;

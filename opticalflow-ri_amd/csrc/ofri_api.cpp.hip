@@ -451,7 +451,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, hs_errs + l, lp->hs_tol[l],
-                              hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish));
+                              hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish, true));
             if (pred) {
                 OFRI_TRY(accumulate(ctx, U, pred + 2 * N, N));
                 OFRI_TRY(accumulate(ctx, V, pred + 3 * N, N));
@@ -476,7 +476,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
             if (lrn)
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
-                                  ctrls + l));
+                                  ctrls + l, true));
             else
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
         }

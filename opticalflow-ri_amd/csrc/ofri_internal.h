@@ -120,11 +120,12 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // cheb_b in (0, 1) (needs d_ctrl): Chebyshev-accelerated sweeps for a
 // spectrum bound b on the slowest mode; 0 = plain sweeps.  cheb_switch > 1:
 // accelerate to cheb_switch * tol, then at most niter / 2 plain sweeps to
-// tol (niter sweeps in all).
+// tol (niter sweeps in all).  check_last: the stop test runs once per fused
+// launch, on its last sweep (no re-run of the stopping launch).
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
-             float cheb_switch = 0.0f);
+             float cheb_switch = 0.0f, bool check_last = false);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
@@ -138,8 +139,10 @@ int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw
              float* v, unsigned char* status);
 
 // ---- Liu-Shen (k_ls.hip) ----
+// check_last: the stop test runs once per fused launch, on its last sweep
+// (the solve may run up to depth - 1 sweeps past the first one that met tol)
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
-             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl);
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets

@@ -8,6 +8,7 @@ namespace ofri {
 namespace {
 
 // Test of one launch of `nsw` fused sweeps: `while err > tol and k < maxnum`
+// (sweeps before s0 carry no partial sums: they are counted, not tested)
 // with err = (||du||_2 + ||dv||_2) / (H*W) (the reference's Liu-Shen update
 // norm, PhysicsBasedOpticalFlowLiuShen.py:79,141; the same residual is used
 // for Horn-Schunck when a tolerance is given).  Run by the last workgroup of
@@ -19,50 +20,52 @@ namespace {
 // any sweep, with always_redo), how many sweeps the stopping launch must be
 // re-run with.
 __device__ void solve_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
-                            size_t N, SolveCtrl* ctrl, float* hist, bool always_redo) {
-    __shared__ double sa[1024], sb[1024];
-    __shared__ int stop;
+                            size_t N, SolveCtrl* ctrl, float* hist, bool always_redo, int s0) {
+    // one pass: wavefront q sums sweeps q, q + nwaves, ... over the
+    // workgroups (lane-strided, then a fixed butterfly), thread 0 then tests
+    // the sweeps in order
+    __shared__ double tot[64][2];
     const int T = blockDim.x * blockDim.y;
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
-    if (t == 0) stop = 0;
-    for (int s = 0; s < nsw; ++s) {
+    const int lane = t & 63, wq = t >> 6, nw = (T + 63) >> 6;
+    for (int s = s0 + wq; s < nsw; s += nw) {
         double a = 0.0, b = 0.0;
-        for (int i = t; i < nblk; i += T) {
+        for (int i = lane; i < nblk; i += 64) {
             // other workgroups' sums: read at device scope, past this CU's cache
-            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             b = b + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2 + 1, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
         }
-        sa[t] = a;
-        sb[t] = b;
-        __syncthreads();
-        for (int k = T / 2; k > 0; k >>= 1) {
-            if (t < k) {
-                sa[t] = sa[t] + sa[t + k];
-                sb[t] = sb[t] + sb[t + k];
-            }
-            __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) {
+            a = a + __shfl_xor(a, off);
+            b = b + __shfl_xor(b, off);
         }
-        if (t == 0) {
-            float err = (float)((sqrt(sa[0]) + sqrt(sb[0])) / (double)N);
-            if (hist) hist[ctrl->sweeps] = err;
-            ctrl->sweeps += 1;
-            ctrl->last_err = err;
-            ctrl->final_buf = dst;
-            if (!((double)err > tol)) {
-                ctrl->done = 1;
-                if (s + 1 < nsw || always_redo) {
-                    ctrl->redo = s + 1;
-                    ctrl->redo_in = dst ^ 1;
-                }
-                stop = 1;
-            } else if (ctrl->sweeps >= maxnum || (ctrl->limit > 0 && ctrl->sweeps >= ctrl->limit)) {
-                ctrl->done = 1;
-            }
+        if (lane == 0) {
+            tot[s][0] = a;
+            tot[s][1] = b;
         }
-        __syncthreads();
-        if (stop) return;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    ctrl->sweeps += s0;   // sweeps run but not reduced
+    for (int s = s0; s < nsw; ++s) {
+        float err = (float)((sqrt(tot[s][0]) + sqrt(tot[s][1])) / (double)N);
+        if (hist) hist[ctrl->sweeps] = err;
+        ctrl->sweeps += 1;
+        ctrl->last_err = err;
+        ctrl->final_buf = dst;
+        if (!((double)err > tol)) {
+            ctrl->done = 1;
+            if (s + 1 < nsw || always_redo) {
+                ctrl->redo = s + 1;
+                ctrl->redo_in = dst ^ 1;
+            }
+            return;
+        }
+        if (ctrl->sweeps >= maxnum || (ctrl->limit > 0 && ctrl->sweeps >= ctrl->limit)) {
+            ctrl->done = 1;
+            return;
+        }
     }
 }
 
@@ -71,7 +74,7 @@ __device__ void solve_check(const double* part, int nblk, int nsw, int dst, doub
 // re-arms the counter.
 __device__ __forceinline__ void arrive_and_check(const double* part, int nblk, int nsw, int dst,
                                                  double tol, int maxnum, size_t N, SolveCtrl* ctrl,
-                                                 float* hist, bool always_redo = false) {
+                                                 float* hist, bool always_redo = false, int s0 = 0) {
     __shared__ int last;
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         __threadfence();
@@ -80,7 +83,7 @@ __device__ __forceinline__ void arrive_and_check(const double* part, int nblk, i
     __syncthreads();
     if (last) {
         __threadfence();
-        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist, always_redo);
+        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist, always_redo, s0);
         if (threadIdx.x == 0 && threadIdx.y == 0) ctrl->arrived = 0;
     }
 }

@@ -52,9 +52,9 @@ def plain_targets(im1, im2, fw=None, ow=None):
     return U, V, hs_tol, [float(s["ls_err"]) for s in st], st
 
 
-# Sweep budgets of the learned form: HS at most 200 per level (about 2.5x what
-# the accelerated sweeps need from a zero start), Liu-Shen the plain 60.
-HS_BUDGET = 200
+# Sweep budgets of the learned form: HS at most 160 per level (about twice
+# what the accelerated sweeps need from a zero start), Liu-Shen the plain 60.
+HS_BUDGET = 160
 LS_BUDGET = 60
 
 
