@@ -285,7 +285,8 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
  * cheb_bound > 0) until its update norm is <= hs_tol[level] or hs_budget
  * sweeps, Liu-Shen until <= ls_tol[level] or ls_budget sweeps.  The targets
  * are the plain method's residuals at its stopping points (r*).  net NULL:
- * zero warm starts (the method without the network).  *h_converged (may be
+ * zero warm starts (the method without the network).  Both solvers test
+ * their targets once per fused launch (at its last sweep).  *h_converged (may be
  * NULL) is 1 when every level met its targets; otherwise the caller falls
  * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries.  H, W multiples
  * of 16.  Repeated calls replay a HIP graph.  No counterpart in the
@@ -295,8 +296,10 @@ typedef struct ofri_learned_params {
     const float* ls_tol;
     int hs_budget;
     int ls_budget;
-    float cheb_bound;      /* e.g. 0.99; 0 = plain sweeps */
-    float cheb_finish;     /* e.g. 4; see ofri_hs_accel */
+    float cheb_bound;      /* HS: e.g. 0.99; 0 = plain sweeps */
+    float cheb_finish;     /* HS: e.g. 4; see ofri_hs_accel */
+    float ls_cheb_lo;      /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
+    float ls_cheb_hi;      /* hi = 0: plain Liu-Shen sweeps */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,

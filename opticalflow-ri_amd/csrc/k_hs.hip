@@ -321,32 +321,6 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
                                   blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
 }
 
-// Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
-// (omega_1 = 1, omega_2 = 2 / (2 - sigma^2), omega_{k+1} = 1 / (1 - sigma^2
-// omega_k / 4)), formed in double on the host.  The recurrence converges
-// geometrically with ratio sigma^2 omega_inf^2 / 4 (at most 0.92 for the
-// bounds accepted here), so past the table's 512 entries every weight equals
-// the last one in float32.
-constexpr int kOmegaTab = 512;
-struct OmegaTab {
-    float w[kOmegaTab];
-};
-
-OmegaTab cheb_omegas(double sigma) {
-    OmegaTab t;
-    const double s2 = sigma * sigma;
-    double om = 1.0;
-    for (int k = 0; k < kOmegaTab; ++k) {
-        if (k == 1) om = 2.0 / (2.0 - s2);
-        else if (k > 1) om = 1.0 / (1.0 - 0.25 * s2 * om);
-        t.w[k] = (float)om;
-    }
-    return t;
-}
-
-__global__ void cheb_omega_k(OmegaTab t, float* w, int n) {
-    for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
-}
 
 // Many images in one launch: workgroup b belongs to the image p with
 // blk0[p] <= b < blk0[p+1] (blk0 ascending, blk0[0] = 0) and is tile

@@ -95,6 +95,13 @@ struct LsPass {
     int redo;               // 1: re-run launch driven by ctrl->redo
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
     int check_last;         // 1: only a launch's last sweep is reduced and tested
+    // Chebyshev mode (ls_pass_k<..., true>): previous iterate buffers, the
+    // weight of every global sweep and the damping of the plain step
+    const float* pin[2][2];
+    float* pout[2][2];
+    const float* omega;
+    float gamma;
+    int sweep0;             // global index of this launch's first sweep
 };
 
 struct Raw {
@@ -176,9 +183,14 @@ __device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Ra
     vn = -(t3 + t4);
 }
 
-template <int R, int WAVES, bool EDGE>
-__device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int nblk, int lane, int w,
-                                          int gx, int gy0, float (&u)[R], float (&v)[R], const float (&IIX)[R],
+// CHEB: Chebyshev semi-iteration over the same sweep T (as in the
+// Horn-Schunck kernel): x_{k+1} = x_{k-1} + omega_{k+1} ((x_k + gamma
+// (T(x_k) - x_k)) - x_{k-1}), the previous iterate in registers; the
+// reduced norm is the plain one, |T(x_k) - x_k|.
+template <int R, int WAVES, bool EDGE, bool CHEB>
+__device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, int blk, int nblk, int lane, int w,
+                                          int gx, int gy0, float (&u)[R], float (&v)[R], float (&up)[R],
+                                          float (&vp)[R], const float (&IIX)[R],
                                           const float (&IIY)[R], const float (&II)[R], const float (&IXT)[R],
                                           const float (&IYT)[R], const float (&B11)[R], const float (&B12)[R],
                                           const float (&B22)[R]) {
@@ -216,6 +228,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
         Raw pu = raw(uup), pv = raw(vup);
         Raw cu = raw(u[0]), cv = raw(v[0]);
         double su = 0.0, sv = 0.0;
+        const float om = CHEB ? a.omega[base + s] : 1.0f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Raw nu = raw(r + 1 < R ? u[r + 1] : udn);
@@ -231,6 +244,14 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
                     su = su + (double)du * (double)du;
                     sv = sv + (double)dv * (double)dv;
                 }
+            }
+            if (CHEB) {
+                const float du = un - u[r], dv = vn - v[r];
+                const float tu = u[r] + a.gamma * du, tv = v[r] + a.gamma * dv;
+                un = up[r] + om * (tu - up[r]);
+                vn = vp[r] + om * (tv - vp[r]);
+                up[r] = u[r];
+                vp[r] = v[r];
             }
             u[r] = un;
             v[r] = vn;
@@ -258,12 +279,14 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int blk, int
 
 // Start of a launch: the re-run launch takes its sweep count and input buffer
 // from the control word; a finished solve does nothing.
-__device__ __forceinline__ bool ls_launch_args(const LsPass& a, int& nsw, int& src) {
+__device__ __forceinline__ bool ls_launch_args(const LsPass& a, int& nsw, int& src, int& base) {
     nsw = a.nsweeps;
     src = a.src;
+    base = a.sweep0;
     if (a.redo) {
         nsw = a.ctrl->redo;
         src = a.ctrl->redo_in;
+        base = a.ctrl->sweeps - nsw;
         return nsw > 0;
     }
     return !a.ctrl->done;
@@ -271,8 +294,8 @@ __device__ __forceinline__ bool ls_launch_args(const LsPass& a, int& nsw, int& s
 
 // One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
 // one being number blk.
-template <int R, int WAVES>
-__device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int bx, int by, int blk,
+template <int R, int WAVES, bool CHEB = false>
+__device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int base, int bx, int by, int blk,
                                         int nblk) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -287,6 +310,7 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
     const bool edge_tile = x0 <= 0 || y0 <= 0 || x0 + 63 >= W - 1 || y0 + WAVES * R - 1 >= H - 1;
 
     float u[R], v[R], IIX[R], IIY[R], II[R], IXT[R], IYT[R], B11[R], B12[R], B22[R];
+    float up[R], vp[R];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
 #pragma unroll
@@ -295,6 +319,10 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
         size_t q = (size_t)cy * W + cx;
         u[r] = uin[q];
         v[r] = vin[q];
+        if (CHEB) {
+            up[r] = a.pin[src][0][q];
+            vp[r] = a.pin[src][1][q];
+        }
         IIX[r] = a.F[q];
         IIY[r] = a.F[N + q];
         II[r] = a.F[2 * N + q];
@@ -305,11 +333,11 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
         B22[r] = a.F[7 * N + q];
     }
     if (edge_tile)
-        ls_sweeps<R, WAVES, true>(a, nsw, blk, nblk, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12,
-                                  B22);
+        ls_sweeps<R, WAVES, true, CHEB>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY, II,
+                                        IXT, IYT, B11, B12, B22);
     else
-        ls_sweeps<R, WAVES, false>(a, nsw, blk, nblk, lane, w, gx, gy0, u, v, IIX, IIY, II, IXT, IYT, B11, B12,
-                                   B22);
+        ls_sweeps<R, WAVES, false, CHEB>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY, II,
+                                         IXT, IYT, B11, B12, B22);
 
     float* uo = a.out[src ^ 1][0];
     float* vo = a.out[src ^ 1][1];
@@ -321,6 +349,10 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
                 size_t q = (size_t)gy * W + gx;
                 uo[q] = u[r];
                 vo[q] = v[r];
+                if (CHEB) {
+                    a.pout[src ^ 1][0][q] = up[r];
+                    a.pout[src ^ 1][1][q] = vp[r];
+                }
             }
         }
     }
@@ -329,12 +361,12 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
                          a.check_last ? nsw - 1 : 0);
 }
 
-template <int R, int WAVES>
+template <int R, int WAVES, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
-    int nsw, src;
-    if (!ls_launch_args(a, nsw, src)) return;
-    ls_tile<R, WAVES>(a, nsw, src, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
-                      gridDim.x * gridDim.y);
+    int nsw, src, base;
+    if (!ls_launch_args(a, nsw, src, base)) return;
+    ls_tile<R, WAVES, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
+                            gridDim.x * gridDim.y);
 }
 
 // Many images in one launch (workgroup b is tile b - blk0[p] of image p,
@@ -357,10 +389,10 @@ __global__ __launch_bounds__(64 * WAVES) void ls_batch_k(const LsPass* __restric
     a.src = src0;
     a.redo = redo;
     if (redo) a.part = nullptr;
-    int nsw, src;
-    if (!ls_launch_args(a, nsw, src)) return;
+    int nsw, src, base;
+    if (!ls_launch_args(a, nsw, src, base)) return;
     const int local = b - blk0[lo];
-    ls_tile<R, WAVES>(a, nsw, src, local % a.tiles_x, local / a.tiles_x, local, a.tiles_x * a.tiles_y);
+    ls_tile<R, WAVES>(a, nsw, src, base, local % a.tiles_x, local / a.tiles_x, local, a.tiles_x * a.tiles_y);
 }
 
 
@@ -368,8 +400,9 @@ struct LsVariant {
     int R, waves;
     void (*kernel)(LsPass);
     void (*batched)(const LsPass*, const int*, int, int, int, int);
+    void (*cheb)(LsPass);
 };
-#define LS_SHAPE(R, WV) {R, WV, ls_pass_k<R, WV>, ls_batch_k<R, WV>}
+#define LS_SHAPE(R, WV) {R, WV, ls_pass_k<R, WV, false>, ls_batch_k<R, WV>, ls_pass_k<R, WV, true>}
 const LsVariant kLsVariants[] = {
     LS_SHAPE(8, 8), LS_SHAPE(8, 4), LS_SHAPE(4, 8), LS_SHAPE(12, 4),
     LS_SHAPE(2, 16), LS_SHAPE(4, 16), LS_SHAPE(6, 8),
@@ -390,12 +423,20 @@ void ls_default(size_t npx, int* variant, int* depth) {
 }  // namespace
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
-             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last) {
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
+             float cheb_hi) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
+    const bool cheb = cheb_hi > 0.0f;
+    if (cheb && !(cheb_lo > -1.0f && cheb_lo < 0.0f && cheb_hi <= 0.999f))
+        return fail(OFRI_E_ARG, "ls: Chebyshev interval [%g, %g] outside (-1, 0) x (0, 0.999]", cheb_lo, cheb_hi);
     size_t N = (size_t)H * W;
     int vi, depth;
     ls_default((size_t)H * W, &vi, &depth);
+    if (cheb) {
+        vi = 2;      // 4 rows x 8 wavefronts: room for the previous iterate in registers
+        depth = 6;
+    }
     if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
     if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
     if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
@@ -406,7 +447,8 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
     int nblk = grid.x * grid.y;
     void *fbuf, *pbuf, *mbuf;
-    OFRI_TRY(scratch(ctx, "ls_fields", 10 * N * sizeof(float), &fbuf));
+    OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float) + (cheb ? (size_t)maxnum * 4 + 64 : 0),
+                     &fbuf));
     OFRI_TRY(scratch(ctx, "ls_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
     OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
     float* F = (float*)fbuf;
@@ -429,13 +471,31 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.tol = tol;
     p.maxnum = maxnum;
     p.check_last = check_last ? 1 : 0;
+    auto kern = cheb ? var.cheb : var.kernel;
+    if (cheb) {
+        // previous-iterate buffers (the first step's previous iterate is the
+        // start) and the weights for [cheb_lo, cheb_hi]
+        float* P0 = F + 10 * N;
+        float* omega = F + 14 * N;
+        OFRI_HIP(hipMemcpyAsync(P0, u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(P0 + N, v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        const double lo = cheb_lo, hi = cheb_hi;
+        if (maxnum > 0)
+            cheb_omega_k<<<1, 256, 0, ctx->stream>>>(cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, maxnum);
+        OFRI_HIP(hipGetLastError());
+        p.pin[0][0] = P0; p.pin[0][1] = P0 + N; p.pin[1][0] = P0 + 2 * N; p.pin[1][1] = P0 + 3 * N;
+        p.pout[0][0] = P0; p.pout[0][1] = P0 + N; p.pout[1][0] = P0 + 2 * N; p.pout[1][1] = P0 + 3 * N;
+        p.omega = omega;
+        p.gamma = (float)(2.0 / (2.0 - lo - hi));
+    }
     int done = 0, src = 0;
     while (done < maxnum) {
         int ns = maxnum - done < depth ? maxnum - done : depth;
         p.nsweeps = ns;
         p.src = src;
+        p.sweep0 = done;
         p.redo = 0;
-        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+        kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_HIP(hipGetLastError());
         src ^= 1;
         done += ns;
@@ -443,7 +503,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (maxnum > 0) {
         p.redo = 1;
         p.part = nullptr;
-        var.kernel<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+        kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_TRY(select_copy(ctx, d_ctrl, u, ub, u, N));
         OFRI_TRY(select_copy(ctx, d_ctrl, v, vb, v, N));
     }

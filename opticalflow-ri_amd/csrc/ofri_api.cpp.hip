@@ -476,7 +476,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
             if (lrn)
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
-                                  ctrls + l, true));
+                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi));
             else
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
         }
@@ -922,6 +922,9 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (lp->hs_budget < 1 || lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
     if (lp->cheb_bound != 0.0f && !(lp->cheb_bound > 0.0f && lp->cheb_bound <= 0.999f))
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g", lp->cheb_bound);
+    if (lp->ls_cheb_hi != 0.0f &&
+        !(lp->ls_cheb_lo > -1.0f && lp->ls_cheb_lo < 0.0f && lp->ls_cheb_hi > 0.0f && lp->ls_cheb_hi <= 0.999f))
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen interval [%g, %g]", lp->ls_cheb_lo, lp->ls_cheb_hi);
     const int L = 2;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
@@ -935,7 +938,7 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
         key.append((const char*)&np, sizeof np);
         key.append((const char*)lp->hs_tol, 2 * sizeof(float));
         key.append((const char*)lp->ls_tol, 2 * sizeof(float));
-        key.append((const char*)&lp->hs_budget, sizeof(int) * 2 + sizeof(float) * 2);
+        key.append((const char*)&lp->hs_budget, sizeof(int) * 2 + sizeof(float) * 4);
     }
     auto enqueue = [&]() { return pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, nullptr, false, &run); };
     if (!ctx->graphs_on) {

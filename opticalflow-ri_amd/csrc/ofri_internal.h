@@ -141,8 +141,11 @@ int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw
 // ---- Liu-Shen (k_ls.hip) ----
 // check_last: the stop test runs once per fused launch, on its last sweep
 // (the solve may run up to depth - 1 sweeps past the first one that met tol)
+// cheb_hi > 0: Chebyshev-accelerated sweeps for a spectrum in
+// [cheb_lo, cheb_hi] (cheb_lo in (-1, 0)).
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
-             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false);
+             int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false,
+             float cheb_lo = 0.0f, float cheb_hi = 0.0f);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets

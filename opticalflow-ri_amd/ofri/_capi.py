@@ -52,7 +52,8 @@ class Pair(C.Structure):
 
 class LearnedParams(C.Structure):
     _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int),
-                ("ls_budget", C.c_int), ("cheb_bound", C.c_float), ("cheb_finish", C.c_float)]
+                ("ls_budget", C.c_int), ("cheb_bound", C.c_float), ("cheb_finish", C.c_float),
+                ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float)]
 
 
 class FbParams(C.Structure):

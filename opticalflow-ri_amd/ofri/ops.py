@@ -281,8 +281,8 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
 
 
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
-                     hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_maxnum=60, ls_tol_plain=1e-8,
-                     Uacc=None, Vacc=None, stats=True):
+                     hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
+                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True):
     """The learned form (ofri_pyr_lshs_learned): network warm starts, HS and
     Liu-Shen run to the per-level residual targets hs_tol / ls_tol.  net: an
     ofri.net.Net or None (zero warm starts).  Returns (Uacc, Vacc, stats,
@@ -293,7 +293,7 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     ht = np.ascontiguousarray(hs_tol, dtype=np.float32)
     lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
     lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget), float(cheb_bound),
-                             float(cheb_finish))
+                             float(cheb_finish), float(ls_cheb[0]), float(ls_cheb[1]))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None
