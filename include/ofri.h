@@ -246,18 +246,19 @@ int ofri_render_pair(ofri_ctx* ctx, int H, int W, int family, float density, uns
                      const float* h_par, int npar, const float* h_modes, int nmodes, float noise, float* d_im1,
                      float* d_im2, float* d_u, float* d_v);
 
-/* ---- Warm-start network (the learned form) ------------------------------ */
+/* ---- Correction network (the learned form) ------------------------------ */
 /* A fixed 4-level convolutional encoder-decoder (3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on MFMA, about 0.57M
- * parameters) mapping a pre-filtered frame pair to the plain LS(HS) 2-level
- * solver's per-level results: at the coarse level (H/2 x W/2) the HS flow and
- * the Liu-Shen correction on top of it, at the fine level (H x W) the HS
- * increment and its Liu-Shen correction.  No counterpart in the reference.
- * Frames are batches of B float32 H x W images (H, W multiples of 16).
- * Parameters are exchanged packed: layer by layer, weights [cout][tap][cin]
- * then bias [cout], float32. */
+ * parameters).  Its input is a batch of B planar float32 H x W stacks of
+ * `planes` planes (H, W multiples of 16): planes 0, 1 are frames (any grey
+ * range, scaled inside by (I - 128) / 64), planes 2, 3 (planes = 4) flow
+ * components in pixels.  In the learned LS(HS) form (planes = 4) it maps a
+ * level's Liu-Shen input pair and the HS flow to the correction Liu-Shen's
+ * sweeps would add (fine head, fields 0, 1).  No counterpart in the
+ * reference.  Parameters are exchanged packed: layer by layer, weights
+ * [cout][tap][cin] then bias [cout], float32. */
 typedef struct ofri_net ofri_net;
-int ofri_net_create(ofri_ctx* ctx, ofri_net** out);
+int ofri_net_create(ofri_ctx* ctx, int planes, ofri_net** out);
 int ofri_net_destroy(ofri_net* net);
 int ofri_net_num_params(ofri_net* net, size_t* n);
 int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n);
@@ -265,32 +266,34 @@ int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n)
 /* Random initial parameters (He-normal for the leaky-ReLU layers, heads
  * scaled down, zero biases) from a seed; resets the optimiser state. */
 int ofri_net_init(ofri_ctx* ctx, ofri_net* net, unsigned long long seed);
-/* Planar outputs: d_coarse [B][4][H/2][W/2] = (HS u, HS v, LS-HS u, LS-HS v),
- * d_fine [B][4][H][W] likewise for the fine level's increment. */
-int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H,
-                     int W, float* d_coarse, float* d_fine);
-/* One training step: forward, weighted Charbonnier loss against planar labels
- * of the same layout as ofri_net_predict's outputs (h_field_w: 8 weights,
- * coarse fields then fine fields), backward, and with lr > 0 an Adam step.
- * *h_loss (may be NULL) receives the loss (synchronises). */
-int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H, int W,
+/* Planar outputs of the two heads: d_coarse [B][4][H/2][W/2] and d_fine
+ * [B][4][H][W] (4 fields each). */
+int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W, float* d_coarse,
+                     float* d_fine);
+/* One training step: forward, weighted Charbonnier loss against planar
+ * labels laid out like ofri_net_predict's outputs (h_field_w: 8 weights,
+ * coarse fields then fine fields; weight 0 leaves a field untrained),
+ * backward, and with lr > 0 an Adam step.  *h_loss (may be NULL) receives
+ * the loss (synchronises). */
+int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W,
                    const float* d_lab_coarse, const float* d_lab_fine, const float* h_field_w, float lr,
                    float* h_loss);
 /* Gradients of the last ofri_net_train call, packed like the parameters. */
 int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
 
-/* The learned form of the LS(HS) 2-level pyramid: the network's predictions
- * warm-start each level's HS (its flow planes) and Liu-Shen (HS result plus
- * the predicted correction); HS runs (Chebyshev-accelerated when
- * cheb_bound > 0) until its update norm is <= hs_tol[level] or hs_budget
- * sweeps, Liu-Shen until <= ls_tol[level] or ls_budget sweeps.  The targets
- * are the plain method's residuals at its stopping points (r*).  net NULL:
- * zero warm starts (the method without the network).  Both solvers test
+/* The learned form of the LS(HS) 2-level pyramid: at each level HS runs
+ * (Chebyshev-accelerated when cheb_bound > 0) from zero until its update
+ * norm is <= hs_tol[level] or hs_budget sweeps; the correction network
+ * (4 input planes) maps the level's Liu-Shen pair and HS's flow to the
+ * correction Liu-Shen would add, and Liu-Shen runs from HS's flow plus that
+ * correction until its norm is <= ls_tol[level] or ls_budget sweeps.  The
+ * targets are the plain method's residuals at its stopping points (r*).
+ * net NULL: Liu-Shen from HS's flow alone (the method without the network).  Both solvers test
  * their targets once per fused launch (at its last sweep).  *h_converged (may be
  * NULL) is 1 when every level met its targets; otherwise the caller falls
- * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries.  H, W multiples
- * of 16.  Repeated calls replay a HIP graph.  No counterpart in the
- * reference. */
+ * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries.  Levels whose
+ * size is not a multiple of 16 skip the network.  Repeated calls replay a
+ * HIP graph.  No counterpart in the reference. */
 typedef struct ofri_learned_params {
     const float* hs_tol;   /* per level (coarse, fine) */
     const float* ls_tol;

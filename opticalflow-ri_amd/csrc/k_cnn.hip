@@ -469,28 +469,34 @@ __global__ void zero_f_k(float* p, long long n) {
     if (i < n) p[i] = 0.0f;
 }
 
-// Network input: B pairs of H x W float32 frames -> B x (H/2) x (W/2) x 8
-// bfloat16, channel = (2 * dy + dx) * 2 + frame, value (I - 128) / 64.
-__global__ void net_input_k(const float* im1, const float* im2, long long fstride, int H, int W, u16* x) {
+// Network input: B x P planar float32 H x W planes -> B x (H/2) x (W/2) x 4P
+// bfloat16, channel = (2 * dy + dx) * P + plane.  Planes 0, 1 are frames,
+// scaled to (I - 128) / 64; planes 2, 3 are flow components in pixels.
+__global__ void net_input_k(const float* in, int P, int H, int W, u16* x) {
     const int h = H / 2, w = W / 2;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int B = gridDim.y;
     const int b = blockIdx.y;
-    if (i >= (long long)h * w || b >= B) return;
+    if (i >= (long long)h * w) return;
     const int y = (int)(i / w), xx = (int)(i - (long long)y * w);
-    u16 out[8];
+    const long long HW = (long long)H * W;
+    u16 out[16];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const size_t q = (size_t)(2 * y + (s >> 1)) * W + 2 * xx + (s & 1);
-        out[2 * s] = f2bf((im1[b * fstride + q] - 128.0f) * (1.0f / 64.0f));
-        out[2 * s + 1] = f2bf((im2[b * fstride + q] - 128.0f) * (1.0f / 64.0f));
+        const long long q = (long long)(2 * y + (s >> 1)) * W + 2 * xx + (s & 1);
+        for (int pl = 0; pl < P; ++pl) {
+            const float v = in[((long long)b * P + pl) * HW + q];
+            out[s * P + pl] = f2bf(pl < 2 ? (v - 128.0f) * (1.0f / 64.0f) : v);
+        }
     }
-    uint4 r;
-    r.x = out[0] | ((unsigned)out[1] << 16);
-    r.y = out[2] | ((unsigned)out[3] << 16);
-    r.z = out[4] | ((unsigned)out[5] << 16);
-    r.w = out[6] | ((unsigned)out[7] << 16);
-    *(uint4*)(x + ((long long)b * h * w + i) * 8) = r;
+    u16* dst = x + ((long long)b * h * w + i) * 4 * P;
+    for (int c = 0; c < 4 * P; c += 8) {
+        uint4 r;
+        r.x = out[c] | ((unsigned)out[c + 1] << 16);
+        r.y = out[c + 2] | ((unsigned)out[c + 3] << 16);
+        r.z = out[c + 4] | ((unsigned)out[c + 5] << 16);
+        r.w = out[c + 6] | ((unsigned)out[c + 7] << 16);
+        *(uint4*)(dst + c) = r;
+    }
 }
 
 // Head outputs (float32 NHWC at H/2) -> planar fields: coarse [B][4][h][w]
@@ -577,6 +583,7 @@ struct LayerDef {
     int act;
     int level;                   // conv resolution
 };
+// enc0a's input channels are 4 x the network's input planes (set per network)
 const LayerDef kLayers[] = {
     {"enc0a", T_X0, 0, 0, T_T0, 0, 0, 8, 32, 1, 0},
     {"enc0b", T_T0, 0, 0, T_CAT0, 32, 0, 32, 32, 1, 0},
@@ -610,6 +617,8 @@ struct LayerParams {
 }  // namespace
 
 struct Net {
+    int planes = 2;            // input planes (2: frame pair; 4: pair + flow)
+    LayerDef def[kNumLayers];  // the layer table with enc0a's input width
     LayerParams L[kNumLayers];
     size_t n_par = 0;      // floats in the padded parameter vector
     size_t n_wb = 0;       // bf16 words of forward weights (prefix of the vector's layout)
@@ -630,11 +639,13 @@ struct Net {
 namespace {
 
 void net_layout(Net* n) {
+    for (int l = 0; l < kNumLayers; ++l) n->def[l] = kLayers[l];
+    n->def[0].cin = 4 * n->planes;
     size_t off = 0, wb = 0, wt = 0, valid = 0;
     // forward weights of every layer first (their bf16 copies mirror them), then biases
     for (int l = 0; l < kNumLayers; ++l) {
         LayerParams& P = n->L[l];
-        const LayerDef& d = kLayers[l];
+        const LayerDef& d = n->def[l];
         P.rows = rup(d.cout, 64);
         P.Kp = rup(9 * d.cin, 32);
         P.rows_t = rup(d.cin, 64);
@@ -650,7 +661,7 @@ void net_layout(Net* n) {
     n->n_wb = wb;
     for (int l = 0; l < kNumLayers; ++l) {
         n->L[l].b_off = off;
-        off += (size_t)rup(kLayers[l].cout, 64);
+        off += (size_t)rup(n->def[l].cout, 64);
     }
     n->n_par = off;
     n->n_wt = wt;
@@ -658,6 +669,9 @@ void net_layout(Net* n) {
 }
 
 inline dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// stored channels of a tensor (the input tensor's follow the input planes)
+inline int tchan(const Net* n, int t) { return t == T_X0 ? 4 * n->planes : kTensors[t].C; }
 
 int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
     if (B < 1 || H < 16 || W < 16 || H % 16 || W % 16)
@@ -667,7 +681,7 @@ int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
     for (int t = 0; t < T_N; ++t) {
         const int lh = H >> (kTensors[t].level + 1), lw = W >> (kTensors[t].level + 1);
         n->t_off[t] = off;
-        off += align256((size_t)B * lh * lw * kTensors[t].C * (kTensors[t].f32 ? 4 : 2));
+        off += align256((size_t)B * lh * lw * tchan(n, t) * (kTensors[t].f32 ? 4 : 2));
     }
     // the buffers only grow: a smaller batch reuses them with new offsets
     if (off > n->act_bytes) {
@@ -690,11 +704,12 @@ int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
 View tview(Net* n, void* base, int t, int c0, int mode) {
     const TensorDef& d = kTensors[t];
     const int lh = n->H >> (d.level + 1), lw = n->W >> (d.level + 1);
+    const int C = tchan(n, t);
     View v;
     const size_t esz = d.f32 ? 4 : 2;
     v.p = (char*)base + n->t_off[t] + (size_t)c0 * esz;
-    v.img = (long long)lh * lw * d.C;
-    v.ps = d.C;
+    v.img = (long long)lh * lw * C;
+    v.ps = C;
     v.f32 = d.f32;
     v.mode = mode;
     return v;
@@ -717,14 +732,14 @@ int launch_conv(ofri_ctx* ctx, const ConvArgs& a) {
     return OFRI_OK;
 }
 
-int net_forward(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B, int H, int W) {
+int net_forward(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W) {
     OFRI_TRY(net_shape(ctx, n, B, H, W));
     const long long hw = (long long)(H / 2) * (W / 2);
     net_input_k<<<dim3((unsigned)((hw + 255) / 256), B), 256, 0, ctx->stream>>>(
-        im1, im2, (long long)H * W, H, W, (u16*)((char*)n->act + n->t_off[T_X0]));
+        in, n->planes, H, W, (u16*)((char*)n->act + n->t_off[T_X0]));
     OFRI_HIP(hipGetLastError());
     for (int l = 0; l < kNumLayers; ++l) {
-        const LayerDef& d = kLayers[l];
+        const LayerDef& d = n->def[l];
         const LayerParams& P = n->L[l];
         ConvArgs a{};
         a.x = tview(n, n->act, d.src, d.src_c0, d.src_mode);
@@ -756,7 +771,7 @@ int net_backward(ofri_ctx* ctx, Net* n) {
         for (int l = 0; l < kNumLayers; ++l) {
             const LayerParams& P = n->L[l];
             const long long cnt = (long long)P.rows_t * P.Kp_t;
-            wflip_k<<<grid1(cnt), 256, 0, ctx->stream>>>(n->wb + P.wb_off, P.Kp, kLayers[l].cin, kLayers[l].cout,
+            wflip_k<<<grid1(cnt), 256, 0, ctx->stream>>>(n->wb + P.wb_off, P.Kp, n->def[l].cin, n->def[l].cout,
                                                          n->wt + P.wt_off, P.rows_t, P.Kp_t);
         }
         OFRI_HIP(hipGetLastError());
@@ -766,7 +781,7 @@ int net_backward(ofri_ctx* ctx, Net* n) {
     bool have[T_N] = {};
     have[T_HC] = have[T_HF] = true;
     for (int l = kNumLayers - 1; l >= 0; --l) {
-        const LayerDef& d = kLayers[l];
+        const LayerDef& d = n->def[l];
         const LayerParams& P = n->L[l];
         const int lh = H >> (d.level + 1), lw = W >> (d.level + 1);
         DwArgs g{};
@@ -815,8 +830,10 @@ int net_backward(ofri_ctx* ctx, Net* n) {
 
 // ---- interface used by the C ABI and the learned pipeline ----
 
-int net_create(ofri_ctx* ctx, Net** out) {
+int net_create(ofri_ctx* ctx, int planes, Net** out) {
+    if (planes != 2 && planes != 4) return fail(OFRI_E_ARG, "net: %d input planes (2 or 4)", planes);
     Net* n = new Net();
+    n->planes = planes;
     net_layout(n);
     hipError_t e = hipSuccess;
     auto al = [&](void** p, size_t b) {
@@ -850,6 +867,7 @@ void net_destroy(Net* n) {
 }
 
 size_t net_num_params(const Net* n) { return n->n_valid; }
+int net_planes(const Net* n) { return n->planes; }
 
 // Packed order: layer by layer, weights [cout][tap][cin] then bias [cout].
 int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count) {
@@ -857,7 +875,7 @@ int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count) {
     std::vector<float> pad(n->n_par, 0.0f);
     size_t q = 0;
     for (int l = 0; l < kNumLayers; ++l) {
-        const LayerDef& d = kLayers[l];
+        const LayerDef& d = n->def[l];
         const LayerParams& P = n->L[l];
         for (int co = 0; co < d.cout; ++co)
             for (int k = 0; k < 9 * d.cin; ++k) pad[P.w_off + (size_t)co * P.Kp + k] = h[q++];
@@ -882,7 +900,7 @@ int net_init_params(ofri_ctx* ctx, Net* n, unsigned long long seed) {
     std::vector<float> h;
     h.reserve(n->n_valid);
     for (int l = 0; l < kNumLayers; ++l) {
-        const LayerDef& d = kLayers[l];
+        const LayerDef& d = n->def[l];
         const double fan = 9.0 * d.cin;
         const double sd = d.act ? std::sqrt(2.0 / fan) : 0.1 * std::sqrt(1.0 / fan);
         for (int i = 0; i < 9 * d.cin * d.cout; ++i) h.push_back((float)(nd(gen) * sd));
@@ -898,7 +916,7 @@ static int unpack(ofri_ctx* ctx, Net* n, const float* d, float* h, size_t count)
     OFRI_HIP(hipStreamSynchronize(ctx->stream));
     size_t q = 0;
     for (int l = 0; l < kNumLayers; ++l) {
-        const LayerDef& dd = kLayers[l];
+        const LayerDef& dd = n->def[l];
         const LayerParams& P = n->L[l];
         for (int co = 0; co < dd.cout; ++co)
             for (int k = 0; k < 9 * dd.cin; ++k) h[q++] = pad[P.w_off + (size_t)co * P.Kp + k];
@@ -911,9 +929,8 @@ int net_get_params(ofri_ctx* ctx, Net* n, float* h, size_t count) { return unpac
 int net_get_grads(ofri_ctx* ctx, Net* n, float* h, size_t count) { return unpack(ctx, n, n->grad, h, count); }
 
 // Forward pass; planar outputs: coarse [B][4][H/2][W/2], fine [B][4][H][W].
-int net_predict(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B, int H, int W, float* coarse,
-                float* fine) {
-    OFRI_TRY(net_forward(ctx, n, im1, im2, B, H, W));
+int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, float* coarse, float* fine) {
+    OFRI_TRY(net_forward(ctx, n, in, B, H, W));
     const int h = H / 2, w = W / 2;
     heads_to_planes_k<<<grid1((long long)B * h * w), 256, 0, ctx->stream>>>(
         (const float*)((char*)n->act + n->t_off[T_HC]), (const float*)((char*)n->act + n->t_off[T_HF]), B, h, w,
@@ -924,9 +941,9 @@ int net_predict(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B
 
 // Forward, loss against planar labels, backward; with lr > 0 an Adam step.
 // d_loss: one float on the device, overwritten.
-int net_train(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B, int H, int W,
-              const float* lab_c, const float* lab_f, const float* h_field_w, float lr, float* d_loss) {
-    OFRI_TRY(net_forward(ctx, n, im1, im2, B, H, W));
+int net_train(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, const float* lab_c, const float* lab_f,
+              const float* h_field_w, float lr, float* d_loss) {
+    OFRI_TRY(net_forward(ctx, n, in, B, H, W));
     const int h = H / 2, w = W / 2;
     void* fwb;
     OFRI_TRY(scratch(ctx, "net_fw", 8 * sizeof(float), &fwb));

@@ -352,15 +352,18 @@ namespace ofri {
 // d_taps (may be null): per level, planar [HS u, HS v, LS u - HS u, LS v -
 // HS v] at the level's size, levels one after another.  record_res: the HS
 // sweeps also reduce their update norms (the last one lands in the stats).
-// lrn (may be null): the learned form.  The network's predictions (pred_c:
-// 4 planes at level 1, pred_f: 4 planes at level 2) warm-start every level's
-// HS (planes 0, 1) and its Liu-Shen seed (HS result + planes 2, 3); HS runs
-// accelerated to the level's residual target, Liu-Shen to its own.
+// lrn (may be null): the learned form.  Every level's HS runs accelerated
+// to the level's residual target; then the correction network maps the
+// level's Liu-Shen input pair and HS's flow (in_stack: 4 planes) to the
+// correction Liu-Shen would add, which seeds Liu-Shen (HS result plus the
+// correction, fine head fields 0, 1 in pred_f); Liu-Shen runs to its own
+// target.
 struct LearnedRun {
     const ofri_learned_params* lp;
-    Net* net;               // null: zero warm starts
-    float* pred_c;
-    float* pred_f;
+    Net* net;               // null: Liu-Shen seeded with HS's flow alone
+    float* in_stack;        // 4 planes of the finest level's size
+    float* pred_c;          // coarse head output (unused)
+    float* pred_f;          // 4 planes of the finest level's size
 };
 
 static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
@@ -391,17 +394,6 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
     float *acc_u = Ua, *acc_v = Va;
     float* tap = d_taps;
     int ph = 0, pw = 0;
-    if (lrn && lrn->net) {
-        // the network sees the FILTER-pre-filtered full-size pair
-        const float *g1 = d_im1, *g2 = d_im2;
-        if (p->filt_w && p->filt_n > 0) {
-            OFRI_TRY(gauss_sep(ctx, d_im1, f1, H, W, p->filt_w, p->filt_n));
-            OFRI_TRY(gauss_sep(ctx, d_im2, f2, H, W, p->filt_w, p->filt_n));
-            g1 = f1;
-            g2 = f2;
-        }
-        OFRI_TRY(net_predict(ctx, lrn->net, g1, g2, 1, H, W, lrn->pred_c, lrn->pred_f));
-    }
     for (int l = 0; l < L; ++l) {
         const int h = lh[l], w = lw[l];
         const size_t N = (size_t)h * w;
@@ -439,23 +431,12 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             s1 = f1;
             s2 = f2;
         }
-        const float* pred = nullptr;
-        if (lrn && lrn->net) pred = l == 0 ? lrn->pred_c : lrn->pred_f;
-        if (pred) {
-            OFRI_HIP(hipMemcpyAsync(U, pred, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-            OFRI_HIP(hipMemcpyAsync(V, pred + N, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        } else {
-            OFRI_TRY(fill_zero(ctx, U, N));
-            OFRI_TRY(fill_zero(ctx, V, N));
-        }
+        OFRI_TRY(fill_zero(ctx, U, N));
+        OFRI_TRY(fill_zero(ctx, V, N));
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, hs_errs + l, lp->hs_tol[l],
                               hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish, true));
-            if (pred) {
-                OFRI_TRY(accumulate(ctx, U, pred + 2 * N, N));
-                OFRI_TRY(accumulate(ctx, V, pred + 3 * N, N));
-            }
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
         } else {
@@ -472,6 +453,17 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
                 OFRI_TRY(gauss_sep(ctx, c2, o2, h, w, p->opt_w, p->opt_n));
                 q1 = o1;
                 q2 = o2;
+            }
+            if (lrn && lrn->net && h % 16 == 0 && w % 16 == 0) {
+                // the correction network: (Liu-Shen pair, HS flow) -> Liu-Shen's correction
+                float* in = lrn->in_stack;
+                OFRI_HIP(hipMemcpyAsync(in, q1, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_HIP(hipMemcpyAsync(in + N, q2, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_HIP(hipMemcpyAsync(in + 2 * N, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_HIP(hipMemcpyAsync(in + 3 * N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_TRY(net_predict(ctx, lrn->net, in, 1, h, w, lrn->pred_c, lrn->pred_f));
+                OFRI_TRY(accumulate(ctx, U, lrn->pred_f, N));
+                OFRI_TRY(accumulate(ctx, V, lrn->pred_f + N, N));
             }
             // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
             if (lrn)
@@ -832,11 +824,11 @@ int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lsh
 // ---- warm-start network ----
 #define NET(p) reinterpret_cast<ofri::Net*>(p)
 
-int ofri_net_create(ofri_ctx* ctx, ofri_net** out) {
+int ofri_net_create(ofri_ctx* ctx, int planes, ofri_net** out) {
     CHECK_CTX(ctx);
     if (!out) return fail(OFRI_E_ARG, "ofri_net_create: null pointer");
     Net* n = nullptr;
-    OFRI_TRY(net_create(ctx, &n));
+    OFRI_TRY(net_create(ctx, planes, &n));
     *out = reinterpret_cast<ofri_net*>(n);
     return OFRI_OK;
 }
@@ -876,22 +868,22 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n) {
     return net_get_grads(ctx, NET(net), h_grads, n);
 }
 
-int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H,
-                     int W, float* d_coarse, float* d_fine) {
+int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W, float* d_coarse,
+                     float* d_fine) {
     CHECK_CTX(ctx);
-    if (!net || !d_im1 || !d_im2 || !d_coarse || !d_fine) return fail(OFRI_E_ARG, "ofri_net_predict: null pointer");
-    return net_predict(ctx, NET(net), d_im1, d_im2, B, H, W, d_coarse, d_fine);
+    if (!net || !d_in || !d_coarse || !d_fine) return fail(OFRI_E_ARG, "ofri_net_predict: null pointer");
+    return net_predict(ctx, NET(net), d_in, B, H, W, d_coarse, d_fine);
 }
 
-int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_im1, const float* d_im2, int B, int H, int W,
+int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W,
                    const float* d_lab_coarse, const float* d_lab_fine, const float* h_field_w, float lr,
                    float* h_loss) {
     CHECK_CTX(ctx);
-    if (!net || !d_im1 || !d_im2 || !d_lab_coarse || !d_lab_fine || !h_field_w)
+    if (!net || !d_in || !d_lab_coarse || !d_lab_fine || !h_field_w)
         return fail(OFRI_E_ARG, "ofri_net_train: null pointer");
     void* l;
     OFRI_TRY(scratch(ctx, "net_loss", sizeof(float), &l));
-    OFRI_TRY(net_train(ctx, NET(net), d_im1, d_im2, B, H, W, d_lab_coarse, d_lab_fine, h_field_w, lr, (float*)l));
+    OFRI_TRY(net_train(ctx, NET(net), d_in, B, H, W, d_lab_coarse, d_lab_fine, h_field_w, lr, (float*)l));
     if (h_loss) {
         OFRI_HIP(hipMemcpyAsync(h_loss, l, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         OFRI_HIP(hipStreamSynchronize(ctx->stream));
@@ -917,8 +909,7 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: null pointer");
     if (p->levels != 2) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: 2 levels only (got %d)", p->levels);
     if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
-    if (H % 16 || W % 16 || H < 32 || W < 32)
-        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d (needs multiples of 16)", H, W);
+    if (H < 8 || W < 8) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d too small", H, W);
     if (lp->hs_budget < 1 || lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
     if (lp->cheb_bound != 0.0f && !(lp->cheb_bound > 0.0f && lp->cheb_bound <= 0.999f))
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g", lp->cheb_bound);
@@ -928,10 +919,12 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     const int L = 2;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
+    if (net && net_planes(NET(net)) != 4)
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: the correction network takes 4 input planes");
     void* pr;
-    const size_t nc = 4 * (size_t)(H / 2) * (W / 2), nf = 4 * (size_t)H * W;
-    OFRI_TRY(scratch(ctx, "lrn_pred", (nc + nf) * sizeof(float), &pr));
-    LearnedRun run{lp, NET(net), (float*)pr, (float*)pr + nc};
+    const size_t nf = 4 * (size_t)H * W, nc = nf / 4;
+    OFRI_TRY(scratch(ctx, "lrn_pred", (nf + nc + nf) * sizeof(float), &pr));
+    LearnedRun run{lp, NET(net), (float*)pr, (float*)pr + nf, (float*)pr + nf + nc};
     std::string key = "learned" + lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
     {
         const void* np = net;

@@ -176,17 +176,18 @@ int render_pair(ofri_ctx* ctx, int H, int W, int family, float density, unsigned
 
 // ---- warm-start network (k_cnn.hip) ----
 struct Net;
-int net_create(ofri_ctx* ctx, Net** out);
+int net_create(ofri_ctx* ctx, int planes, Net** out);
+int net_planes(const Net* n);
 void net_destroy(Net* n);
 size_t net_num_params(const Net* n);
 int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count);
 int net_init_params(ofri_ctx* ctx, Net* n, unsigned long long seed);
 int net_get_params(ofri_ctx* ctx, Net* n, float* h, size_t count);
 int net_get_grads(ofri_ctx* ctx, Net* n, float* h, size_t count);
-// planar outputs: coarse [B][4][H/2][W/2], fine [B][4][H][W]
-int net_predict(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B, int H, int W, float* coarse,
-                float* fine);
-int net_train(ofri_ctx* ctx, Net* n, const float* im1, const float* im2, int B, int H, int W,
-              const float* lab_c, const float* lab_f, const float* h_field_w, float lr, float* d_loss);
+// input: [B][planes][H][W] planar float32; planar outputs: coarse
+// [B][4][H/2][W/2], fine [B][4][H][W]
+int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, float* coarse, float* fine);
+int net_train(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, const float* lab_c, const float* lab_f,
+              const float* h_field_w, float lr, float* d_loss);
 
 }  // namespace ofri

@@ -111,15 +111,15 @@ SIGNATURES = {
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],
     "ofri_render_pair": [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp, C.c_int, _fp, C.c_int,
                          C.c_float, _vp, _vp, _vp, _vp],
-    "ofri_net_create": [_vp, C.POINTER(_vp)],
+    "ofri_net_create": [_vp, C.c_int, C.POINTER(_vp)],
     "ofri_net_destroy": [_vp],
     "ofri_net_num_params": [_vp, C.POINTER(C.c_size_t)],
     "ofri_net_set_params": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_init": [_vp, _vp, C.c_ulonglong],
     "ofri_net_get_params": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_get_grads": [_vp, _vp, _fp, C.c_size_t],
-    "ofri_net_predict": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp],
-    "ofri_net_train": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_float, _fp],
+    "ofri_net_predict": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp],
+    "ofri_net_train": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_float, _fp],
     "ofri_pyr_lshs_taps": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp, _vp,
                            C.POINTER(LevelStats)],
     "ofri_pyr_lshs_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), C.POINTER(LearnedParams),

@@ -1,12 +1,14 @@
-"""The learned form of LS(HS) 2-level: network warm starts, solves to the
-plain method's residual targets, plain fallback.
+"""The learned form of LS(HS) 2-level: accelerated HS and a learned
+Liu-Shen correction, solves to the plain method's residual targets, plain
+fallback.
 
     r*  per level: the plain solver's update norm at its stopping point (HS
         after its 600 sweeps, Liu-Shen at tol or 60 sweeps), from
         ofri_pyr_lshs_taps on the same pair (plain_targets), or given.
-    run per level: HS from the network's flow (Chebyshev-accelerated, plain
-        finish) until its norm <= r*_HS; Liu-Shen from HS's result plus the
-        network's correction until its norm <= r*_LS.
+    run per level: HS from zero (Chebyshev-accelerated, plain finish) until
+        its norm <= r*_HS; the correction network maps the level's Liu-Shen
+        pair and HS's flow to Liu-Shen's correction; Liu-Shen from HS's flow
+        plus that correction until its norm <= r*_LS.
     fallback: when a level misses a target within the sweep budgets, the
         plain pyramid runs instead (its result is returned).
 """

@@ -1,10 +1,10 @@
-"""Warm-start network handle (libofri ofri_net_*): the learned form's
+"""Correction network handle (libofri ofri_net_*): the learned form's
 encoder-decoder, its parameters, prediction and training steps.
 
-The network maps a batch of pre-filtered frame pairs to the plain LS(HS)
-2-level solver's per-level results (coarse HS flow and Liu-Shen correction
-at H/2 x W/2, fine HS increment and Liu-Shen correction at H x W).  All
-compute runs in libofri's kernels; this module only moves parameters.
+In the learned LS(HS) form the network (4 input planes) maps a level's
+Liu-Shen input pair and the HS flow to the correction Liu-Shen's sweeps add
+on top of HS (fine head, fields 0, 1).  All compute runs in libofri's
+kernels; this module only moves parameters and buffers.
 """
 import ctypes as C
 import os
@@ -14,16 +14,18 @@ import numpy as np
 from . import _capi
 from .device import DeviceArray, as_device, context
 
-# Loss weights per output field: coarse HS u, v, LS-HS u, v; fine likewise.
-FIELD_WEIGHTS = np.array([1.0, 1.0, 4.0, 4.0, 1.0, 1.0, 4.0, 4.0], np.float32)
-WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+# Loss weights per output field (coarse head fields 0..3, fine head fields
+# 0..3): the correction network learns the fine head's first two fields.
+FIELD_WEIGHTS = np.array([0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0], np.float32)
+WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "correction_weights.npz")
 
 
 class Net(object):
-    def __init__(self, params=None, ctx=None):
+    def __init__(self, params=None, ctx=None, planes=4):
         self.ctx = ctx or context()
+        self.planes = int(planes)
         h = C.c_void_p()
-        _capi.check(self.ctx.lib.ofri_net_create(self.ctx.handle, C.byref(h)))
+        _capi.check(self.ctx.lib.ofri_net_create(self.ctx.handle, self.planes, C.byref(h)))
         self.handle = h
         n = C.c_size_t()
         _capi.check(self.ctx.lib.ofri_net_num_params(h, C.byref(n)))
@@ -35,10 +37,11 @@ class Net(object):
     def load(cls, path=WEIGHTS_FILE, ctx=None):
         """The committed trained parameters (numpy .npz, no pickles)."""
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"].astype(np.float32), ctx)
+            planes = int(z["planes"]) if "planes" in z.files else 4
+            return cls(z["params"].astype(np.float32), ctx, planes)
 
     def save(self, path, **meta):
-        np.savez_compressed(path, params=self.get_params(), **meta)
+        np.savez_compressed(path, params=self.get_params(), planes=self.planes, **meta)
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)
@@ -60,29 +63,33 @@ class Net(object):
         _capi.check(self.ctx.lib.ofri_net_get_grads(self.ctx.handle, self.handle, _capi.fptr(out), out.size))
         return out
 
-    def predict(self, im1, im2, coarse=None, fine=None):
-        """im1, im2: DeviceArray or numpy of shape (B, H, W) or (H, W).
+    def predict(self, inp, coarse=None, fine=None):
+        """inp: DeviceArray or numpy, (B, planes, H, W) or (planes, H, W).
         Returns (coarse [B, 4, H/2, W/2], fine [B, 4, H, W]) DeviceArrays."""
-        d1, d2 = as_device(im1), as_device(im2)
-        shp = d1.shape if len(d1.shape) == 3 else (1,) + d1.shape
-        B, H, W = shp
+        d = as_device(inp)
+        B, H, W = self._dims(d)
         coarse = coarse if coarse is not None else DeviceArray((B, 4, H // 2, W // 2))
         fine = fine if fine is not None else DeviceArray((B, 4, H, W))
-        _capi.check(self.ctx.lib.ofri_net_predict(self.ctx.handle, self.handle, d1.ptr, d2.ptr, B, H, W,
-                                                  coarse.ptr, fine.ptr))
+        _capi.check(self.ctx.lib.ofri_net_predict(self.ctx.handle, self.handle, d.ptr, B, H, W, coarse.ptr,
+                                                  fine.ptr))
         return coarse, fine
 
-    def train_step(self, im1, im2, lab_c, lab_f, lr, field_w=FIELD_WEIGHTS):
+    def train_step(self, inp, lab_c, lab_f, lr, field_w=FIELD_WEIGHTS):
         """One step on a batch (lr = 0: gradients only).  Returns the loss."""
-        d1, d2 = as_device(im1), as_device(im2)
+        d = as_device(inp)
         lc, lf = as_device(lab_c), as_device(lab_f)
-        shp = d1.shape if len(d1.shape) == 3 else (1,) + d1.shape
-        B, H, W = shp
+        B, H, W = self._dims(d)
         fw = np.ascontiguousarray(field_w, dtype=np.float32)
         loss = C.c_float()
-        _capi.check(self.ctx.lib.ofri_net_train(self.ctx.handle, self.handle, d1.ptr, d2.ptr, B, H, W, lc.ptr,
-                                                lf.ptr, _capi.fptr(fw), float(lr), C.byref(loss)))
+        _capi.check(self.ctx.lib.ofri_net_train(self.ctx.handle, self.handle, d.ptr, B, H, W, lc.ptr, lf.ptr,
+                                                _capi.fptr(fw), float(lr), C.byref(loss)))
         return float(loss.value)
+
+    def _dims(self, d):
+        shp = d.shape if len(d.shape) == 4 else (1,) + tuple(d.shape)
+        if len(shp) != 4 or shp[1] != self.planes:
+            raise ValueError("input of shape %s: expected (B, %d, H, W)" % (d.shape, self.planes))
+        return shp[0], shp[2], shp[3]
 
     def __del__(self):
         try:

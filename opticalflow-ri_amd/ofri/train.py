@@ -1,18 +1,20 @@
-"""Training of the warm-start network on the GPU.
+"""Training of the correction network on the GPU.
 
 Every training pair is rendered on the device (ofri_render_pair) from its
 own seed: H, W from {512, 576, ..., 1024}, density U[0.02, 0.08], flow
 family uniform over Poiseuille, Lamb-Oseen and shear + turbulence (max 4 px).
-Its labels are the plain LS(HS) 2-level solver's own per-level results
-(ofri_pyr_lshs_taps with the workload's parameters: FILTER 3.4 / 3 taps, HS
-alpha 45 then 21 with 600 sweeps, FILTER_OPT 0.48 / 5 taps, Liu-Shen alpha 5,
-tol 1e-8, 60 sweeps).  The network sees the FILTER-pre-filtered full-size
-pair.  Forward, loss, backward and Adam all run in libofri's kernels.
+The plain LS(HS) 2-level solver runs on it with the workload's parameters
+(FILTER 3.4 / 3 taps, HS alpha 45 then 21 with 600 sweeps, FILTER_OPT 0.48 /
+5 taps, Liu-Shen alpha 5, tol 1e-8, 60 sweeps) and exports every level's HS
+flow and Liu-Shen correction (ofri_pyr_lshs_taps).  Each level gives one
+sample: input = the level's Liu-Shen pair (FILTER_OPT-filtered, resized to
+the level) and HS's flow; label = the Liu-Shen correction.  Forward, loss,
+backward and Adam all run in libofri's kernels.
 
-    python -m ofri.train --pairs 20000 --batch 8 --out ofri/warmstart_weights.npz
+    python -m ofri.train --pairs 20000 --batch 8 --out ofri/correction_weights.npz
 
 The pair set is a pure function of --seed; an epoch re-renders and
-re-labels it (labels for 20k pairs do not fit next to the images in HBM).
+re-solves it (the labels of 20k pairs do not fit next to the images in HBM).
 """
 import argparse
 import math
@@ -66,12 +68,14 @@ class Batch(object):
 
     def __init__(self, Bmax, Hmax, Wmax):
         n = Bmax * Hmax * Wmax
-        self.buf = {k: DeviceArray((n,)) for k in ("im1", "im2", "f1", "f2")}
-        self.buf["lab_c"] = DeviceArray((n,))       # B x 4 planes at H/2 x W/2
-        self.buf["lab_f"] = DeviceArray((4 * n,))   # B x 4 planes at H x W
-        self.buf["taps"] = DeviceArray((4 * (Hmax // 2) * (Wmax // 2) + 4 * Hmax * Wmax,))
-        self.buf["U"] = DeviceArray((Hmax * Wmax,))
-        self.buf["V"] = DeviceArray((Hmax * Wmax,))
+        self.buf = {"im1": DeviceArray((n,)), "im2": DeviceArray((n,)),
+                    "in_f": DeviceArray((4 * n,)), "in_c": DeviceArray((n,)),      # 4 planes, full / half size
+                    # fine-head labels: fields 0, 1 written per sample, 2, 3 stay finite (weight 0)
+                    "lab_f": DeviceArray.zeros((4 * n,)), "lab_c": DeviceArray.zeros((n,)),
+                    "zero": DeviceArray.zeros((n,)),                              # coarse-head labels (unused)
+                    "taps": DeviceArray((5 * Hmax * Wmax,)),
+                    "U": DeviceArray((Hmax * Wmax,)), "V": DeviceArray((Hmax * Wmax,)),
+                    "t1": DeviceArray((Hmax * Wmax,)), "t2": DeviceArray((Hmax * Wmax,))}
         self.cap = (Bmax, Hmax, Wmax)
 
     def set(self, B, H, W):
@@ -80,39 +84,56 @@ class Batch(object):
             raise ValueError("batch %dx%dx%d exceeds %dx%dx%d" % (B, H, W, Bm, Hm, Wm))
         self.B, self.H, self.W = B, H, W
         b = self.buf
+        h, w = H // 2, W // 2
         self.im1 = b["im1"].view(0, (B, H, W))
         self.im2 = b["im2"].view(0, (B, H, W))
-        self.f1 = b["f1"].view(0, (B, H, W))
-        self.f2 = b["f2"].view(0, (B, H, W))
-        self.lab_c = b["lab_c"].view(0, (B, 4, H // 2, W // 2))
+        self.in_f = b["in_f"].view(0, (B, 4, H, W))
+        self.in_c = b["in_c"].view(0, (B, 4, h, w))
         self.lab_f = b["lab_f"].view(0, (B, 4, H, W))
-        self.taps = b["taps"].view(0, (4 * (H // 2) * (W // 2) + 4 * H * W,))
+        self.lab_c = b["lab_c"].view(0, (B, 4, h, w))
+        self.zero_f = b["zero"].view(0, (B, 4, h, w))      # coarse-head labels of the fine sample
+        self.zero_c = b["zero"].view(0, (B, 4, h // 2, w // 2))
+        self.taps = b["taps"].view(0, (4 * h * w + 4 * H * W,))
         self.U = b["U"].view(0, (H, W))
         self.V = b["V"].view(0, (H, W))
         return self
 
 
-def fill(batch, specs, fw, ow, labels=True):
-    """Render the batch's pairs, pre-filter them and (labels=True) label them
-    with the plain solver.  Returns the per-pair level statistics."""
+def _copy(dst, src, n):
     ctx = context()
+    _ = ctx.lib.ofri_copy(ctx.handle, dst.ptr, src.ptr, 4 * n)
+
+
+def fill(batch, specs, fw, ow):
+    """Render the batch's pairs, solve them with the plain pyramid and lay out
+    the two samples per pair: fine level (in_f, lab_f) and coarse level
+    (in_c, lab_c).  Returns the per-pair level statistics."""
     B, H, W = batch.B, batch.H, batch.W
-    hw, HW = (H // 2) * (W // 2), H * W
+    h, w = H // 2, W // 2
+    hw, HW = h * w, H * W
+    t1, t2 = batch.buf["t1"], batch.buf["t2"]
     stats = []
     for i, sp in enumerate(specs):
         a = batch.im1.view(i * HW, (H, W))
         b = batch.im2.view(i * HW, (H, W))
         ops.render_pair(H, W, sp["family"], sp["density"], sp["seed"], sp["par"], sp["modes"], 2.0, im1=a, im2=b)
-        ops.gauss_sep(a, fw, out=batch.f1.view(i * HW, (H, W)))
-        ops.gauss_sep(b, fw, out=batch.f2.view(i * HW, (H, W)))
-        if labels:
-            _, _, _, st = ops.pyr_lshs_taps(a, b, filt_w=fw, opt_w=ow, Uacc=batch.U, Vacc=batch.V, taps=batch.taps,
-                                            **LSHS)
-            stats.append(st)
-            lib, h = ctx.lib, ctx.handle
-            lib.ofri_copy(h, batch.lab_c.view(i * 4 * hw, (4 * hw,)).ptr, batch.taps.ptr, 4 * hw * 4)
-            lib.ofri_copy(h, batch.lab_f.view(i * 4 * HW, (4 * HW,)).ptr,
-                          batch.taps.view(4 * hw, (4 * HW,)).ptr, 4 * HW * 4)
+        _, _, taps, st = ops.pyr_lshs_taps(a, b, filt_w=fw, opt_w=ow, Uacc=batch.U, Vacc=batch.V, taps=batch.taps,
+                                           **LSHS)
+        stats.append(st)
+        # fine level: Liu-Shen pair = FILTER_OPT of the frames
+        base = i * 4 * HW
+        ops.gauss_sep(a, ow, out=batch.in_f.view(base, (H, W)))
+        ops.gauss_sep(b, ow, out=batch.in_f.view(base + HW, (H, W)))
+        _copy(batch.in_f.view(base + 2 * HW, (2 * HW,)), taps.view(4 * hw, (2 * HW,)), 2 * HW)
+        _copy(batch.lab_f.view(base, (2 * HW,)), taps.view(4 * hw + 2 * HW, (2 * HW,)), 2 * HW)
+        # coarse level: FILTER_OPT of the bicubic half-size frames
+        basec = i * 4 * hw
+        r1 = ops.resize(a, h, w)
+        r2 = ops.resize(b, h, w)
+        ops.gauss_sep(r1, ow, out=batch.in_c.view(basec, (h, w)))
+        ops.gauss_sep(r2, ow, out=batch.in_c.view(basec + hw, (h, w)))
+        _copy(batch.in_c.view(basec + 2 * hw, (2 * hw,)), taps.view(0, (2 * hw,)), 2 * hw)
+        _copy(batch.lab_c.view(basec, (2 * hw,)), taps.view(2 * hw, (2 * hw,)), 2 * hw)
     return stats
 
 
@@ -129,7 +150,7 @@ def train(pairs=20000, batch=8, epochs=1.0, lr=1e-3, seed=0, out=None, resume=No
     passes in minibatches of `batch` same-size pairs.  Returns the Net."""
     ctx = context()
     fw, ow = prefilter_weights()
-    net = Net()
+    net = Net(planes=4)
     if resume:
         with np.load(resume, allow_pickle=False) as z:
             net.set_params(z["params"])
@@ -159,7 +180,10 @@ def train(pairs=20000, batch=8, epochs=1.0, lr=1e-3, seed=0, out=None, resume=No
         bt.set(len(mb), H, W)
         fill(bt, mb, fw, ow)
         rate = lr_at(step, steps, lr)
-        loss = net.train_step(bt.f1, bt.f2, bt.lab_c, bt.lab_f, rate)
+        # the fine level's sample, then the coarse level's (fine-head labels
+        # at each level's size; the coarse head is not trained)
+        loss = net.train_step(bt.in_f, bt.zero_f, bt.lab_f, rate)
+        loss += net.train_step(bt.in_c, bt.zero_c, bt.lab_c, rate)
         hist.append(loss)
         if step % log_every == 0 or step == steps - 1:
             el = time.time() - t0

@@ -26,6 +26,7 @@ TENSORS = {"X0": (0, 8), "T0": (0, 32), "CAT0": (0, 64), "T1": (1, 32), "CAT1": 
            "HC": (0, 8), "HF": (0, 16)}
 F32_TENSORS = ("HC", "HF")
 # name, src, src_c0, s2d, dst, dst_c0, d2s, cin, cout, act, level
+# (enc0a's input width is 4 x the input planes: see layers())
 LAYERS = [
     ("enc0a", "X0", 0, 0, "T0", 0, 0, 8, 32, 1, 0),
     ("enc0b", "T0", 0, 0, "CAT0", 32, 0, 32, 32, 1, 0),
@@ -46,14 +47,21 @@ LAYERS = [
 ]
 
 
-def num_params():
-    return sum(9 * L[7] * L[8] + L[8] for L in LAYERS)
+def layers(planes=2):
+    """The layer table for a network with `planes` input planes."""
+    out = [list(L) for L in LAYERS]
+    out[0][7] = 4 * planes
+    return [tuple(L) for L in out]
 
 
-def unpack(flat):
+def num_params(planes=2):
+    return sum(9 * L[7] * L[8] + L[8] for L in layers(planes))
+
+
+def unpack(flat, planes=2):
     """Packed vector -> list of (W [cout, 9, cin], b [cout])."""
     out, q = [], 0
-    for L in LAYERS:
+    for L in layers(planes):
         cin, cout = L[7], L[8]
         W = flat[q:q + 9 * cin * cout].reshape(cout, 9, cin)
         q += 9 * cin * cout
@@ -107,8 +115,10 @@ def _col2im(c, shape):
     return xp[:, 1:-1, 1:-1]
 
 
-def _alloc(B, H, W):
-    return {k: np.zeros((B, H >> (lv + 1), W >> (lv + 1), C)) for k, (lv, C) in TENSORS.items()}
+def _alloc(B, H, W, planes=2):
+    T = {k: np.zeros((B, H >> (lv + 1), W >> (lv + 1), C)) for k, (lv, C) in TENSORS.items()}
+    T["X0"] = np.zeros((B, H >> 1, W >> 1, 4 * planes))
+    return T
 
 
 def _read(T, L):
@@ -128,19 +138,21 @@ def _write(T, L, y, add=False):
         T[dst][..., c0:c0 + C] = v
 
 
-def forward(params, im1, im2, bf=True):
-    """im1, im2: [B, H, W].  Returns the tensor dict (all activations)."""
+def forward(params, inp, bf=True):
+    """inp: [B, P, H, W] planar input (planes 0, 1 frames, 2, 3 flow).
+    Returns the tensor dict (all activations)."""
     rnd = bf16 if bf else (lambda a: np.asarray(a, np.float64))
-    im1 = np.asarray(im1, np.float32)
-    im2 = np.asarray(im2, np.float32)
-    B, H, W = im1.shape
-    T = _alloc(B, H, W)
-    x1 = (im1 - np.float32(128)) * np.float32(1.0 / 64.0)
-    x2 = (im2 - np.float32(128)) * np.float32(1.0 / 64.0)
-    st = np.stack([x1, x2], axis=-1)                      # [B, H, W, 2]
-    T["X0"][:] = rnd(_s2d(st))                             # channel (2dy+dx)*2 + frame
-    P = unpack(np.asarray(params, np.float64))
-    for L, (Wt, b) in zip(LAYERS, P):
+    inp = np.asarray(inp, np.float32)
+    B, P_, H, W = inp.shape
+    T = _alloc(B, H, W, P_)
+    planes = []
+    for k in range(P_):
+        x = inp[:, k]
+        planes.append((x - np.float32(128)) * np.float32(1.0 / 64.0) if k < 2 else x)
+    st = np.stack(planes, axis=-1)                         # [B, H, W, P]
+    T["X0"][:] = rnd(_s2d(st))                             # channel (2dy+dx)*P + plane
+    P = unpack(np.asarray(params, np.float64), P_)
+    for L, (Wt, b) in zip(layers(P_), P):
         x = _read(T, L)
         Wm = rnd(Wt).reshape(L[8], -1)
         B_, h, w, _ = x.shape
@@ -189,7 +201,9 @@ def loss_and_head_grads(T, lab_c, lab_f, fw, eps2=1e-6):
 def backward(params, T, ghc, ghf, bf=True):
     """Packed parameter gradients, following the device's rounding points."""
     rnd = bf16 if bf else (lambda a: np.asarray(a, np.float64))
-    P = unpack(np.asarray(params, np.float64))
+    planes = T["X0"].shape[-1] // 4
+    LAY = layers(planes)
+    P = unpack(np.asarray(params, np.float64), planes)
     B = T["X0"].shape[0]
     G = {k: np.zeros_like(v) for k, v in T.items()}
     G["HC"][:] = ghc
@@ -198,9 +212,9 @@ def backward(params, T, ghc, ghf, bf=True):
         G["HC"][:] = ghc.astype(np.float32)
         G["HF"][:] = ghf.astype(np.float32)
     have = {"HC", "HF"}
-    grads = [None] * len(LAYERS)
-    for li in range(len(LAYERS) - 1, -1, -1):
-        L = LAYERS[li]
+    grads = [None] * len(LAY)
+    for li in range(len(LAY) - 1, -1, -1):
+        L = LAY[li]
         Wt, b = P[li]
         cout, cin = L[8], L[7]
         # output gradient and forward output in conv space
@@ -237,11 +251,11 @@ def backward(params, T, ghc, ghf, bf=True):
     return np.concatenate(flat)
 
 
-def init_params(seed=0, scale=1.0):
-    """He-style random initialisation (the training's starting point)."""
+def init_params(seed=0, scale=1.0, planes=2):
+    """He-style random initialisation (test parameters)."""
     rng = np.random.default_rng(seed)
     flat = []
-    for L in LAYERS:
+    for L in layers(planes):
         cin, cout = L[7], L[8]
         std = scale * np.sqrt(2.0 / (9 * cin)) if L[9] else scale * 0.1 * np.sqrt(1.0 / (9 * cin))
         flat.append(rng.standard_normal(9 * cin * cout) * std)

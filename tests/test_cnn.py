@@ -7,24 +7,26 @@ import pytest
 from oracle import cnn
 
 
-def _batch(B, H, W, seed):
+def _batch(B, H, W, seed, planes=4):
+    """Planar input (frames in [0, 255], then flow planes) and labels."""
     rng = np.random.default_rng(seed)
-    im1 = rng.uniform(0, 255, (B, H, W)).astype(np.float32)
-    im2 = rng.uniform(0, 255, (B, H, W)).astype(np.float32)
+    inp = np.concatenate([rng.uniform(0, 255, (B, 2, H, W)),
+                          rng.standard_normal((B, planes - 2, H, W)) * 2.0], axis=1).astype(np.float32)
     lab_c = rng.standard_normal((B, 4, H // 2, W // 2))
     lab_f = rng.standard_normal((B, 4, H, W)) * 0.3
-    return im1, im2, lab_c, lab_f
+    return inp, lab_c, lab_f
 
 
 FW = np.array([1.0, 1.0, 2.0, 2.0, 1.0, 1.0, 2.0, 2.0])
 
 
 def test_packing_and_size():
-    assert cnn.num_params() == sum(9 * L[7] * L[8] + L[8] for L in cnn.LAYERS)
-    assert 450_000 < cnn.num_params() < 650_000
-    p = np.arange(cnn.num_params(), dtype=np.float64)
-    W0, b0 = cnn.unpack(p)[0]
-    assert W0.shape == (32, 9, 8) and b0[0] == 9 * 8 * 32
+    for planes in (2, 4):
+        assert cnn.num_params(planes) == sum(9 * L[7] * L[8] + L[8] for L in cnn.layers(planes))
+        assert 450_000 < cnn.num_params(planes) < 650_000
+    p = np.arange(cnn.num_params(4), dtype=np.float64)
+    W0, b0 = cnn.unpack(p, 4)[0]
+    assert W0.shape == (32, 9, 16) and b0[0] == 9 * 16 * 32
 
 
 def test_space_to_depth_round_trip_and_order():
@@ -44,15 +46,15 @@ def test_im2col_adjoint():
 
 
 def test_restated_backward_matches_finite_differences():
-    im1, im2, lc, lf = _batch(1, 32, 32, 1)
-    p = cnn.init_params(3, scale=1.0).astype(np.float64)
-    T = cnn.forward(p, im1, im2, bf=False)
+    inp, lc, lf = _batch(1, 32, 32, 1)
+    p = cnn.init_params(3, scale=1.0, planes=4).astype(np.float64)
+    T = cnn.forward(p, inp, bf=False)
     loss, ghc, ghf = cnn.loss_and_head_grads(T, lc, lf, FW)
     g = cnn.backward(p, T, ghc, ghf, bf=False)
     rng = np.random.default_rng(5)
     # a spread of indices over every layer's weights and biases
     idx, q = [], 0
-    for L in cnn.LAYERS:
+    for L in cnn.layers(4):
         nw = 9 * L[7] * L[8]
         idx += list(q + rng.integers(0, nw, 2)) + [q + nw + int(rng.integers(0, L[8]))]
         q += nw + L[8]
@@ -61,8 +63,8 @@ def test_restated_backward_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        lp = cnn.loss_and_head_grads(cnn.forward(pp, im1, im2, bf=False), lc, lf, FW)[0]
-        lm = cnn.loss_and_head_grads(cnn.forward(pm, im1, im2, bf=False), lc, lf, FW)[0]
+        lp = cnn.loss_and_head_grads(cnn.forward(pp, inp, bf=False), lc, lf, FW)[0]
+        lm = cnn.loss_and_head_grads(cnn.forward(pm, inp, bf=False), lc, lf, FW)[0]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-4 * max(1e-3, abs(fd)) + 1e-9, (i, fd, g[i])
 
@@ -71,13 +73,15 @@ def test_restated_backward_matches_finite_differences():
 @pytest.fixture(scope="module")
 def net():
     from ofri.net import Net
-    return Net()
+    return Net(planes=4)
 
 
 @pytest.mark.gpu
 def test_device_parameter_count_and_round_trip(net):
-    assert net.num_params == cnn.num_params()
-    p = cnn.init_params(11)
+    from ofri.net import Net
+    assert net.num_params == cnn.num_params(4)
+    assert Net(planes=2).num_params == cnn.num_params(2)
+    p = cnn.init_params(11, planes=4)
     net.set_params(p)
     assert np.array_equal(net.get_params(), p)
 
@@ -85,11 +89,11 @@ def test_device_parameter_count_and_round_trip(net):
 @pytest.mark.gpu
 @pytest.mark.parametrize("B,H,W", [(1, 32, 32), (2, 32, 48), (1, 64, 80)])
 def test_device_forward_matches_restatement(net, B, H, W):
-    im1, im2, _, _ = _batch(B, H, W, 2)
-    p = cnn.init_params(7)
+    inp, _, _ = _batch(B, H, W, 2)
+    p = cnn.init_params(7, planes=4)
     net.set_params(p)
-    coarse, fine = net.predict(im1, im2)
-    rc, rf = cnn.heads(cnn.forward(p.astype(np.float64), im1, im2, bf=True))
+    coarse, fine = net.predict(inp)
+    rc, rf = cnn.heads(cnn.forward(p.astype(np.float64), inp, bf=True))
     gc, gf = coarse.download(), fine.download()
     assert gc.shape == rc.shape and gf.shape == rf.shape
     for g, r in ((gc, rc), (gf, rf)):
@@ -106,21 +110,21 @@ def test_device_gradients_match_restatement(net):
     bf16 and the float64 restatements): a wrong index or missing term shows
     up far above that noise, rounding differences stay within a few times it."""
     B, H, W = 2, 64, 64
-    im1, im2, lc, lf = _batch(B, H, W, 4)
+    inp, lc, lf = _batch(B, H, W, 4)
     lc, lf = lc.astype(np.float32).astype(np.float64), lf.astype(np.float32).astype(np.float64)
-    p = cnn.init_params(9)
+    p = cnn.init_params(9, planes=4)
     net.set_params(p)
-    loss = net.train_step(im1, im2, lc.astype(np.float32), lf.astype(np.float32), lr=0.0, field_w=FW)
+    loss = net.train_step(inp, lc.astype(np.float32), lf.astype(np.float32), lr=0.0, field_w=FW)
     g = net.get_grads()
-    T = cnn.forward(p.astype(np.float64), im1, im2, bf=True)
+    T = cnn.forward(p.astype(np.float64), inp, bf=True)
     rl, ghc, ghf = cnn.loss_and_head_grads(T, lc, lf, FW)
     rg = cnn.backward(p.astype(np.float64), T, ghc, ghf, bf=True)
-    T64 = cnn.forward(p.astype(np.float64), im1, im2, bf=False)
+    T64 = cnn.forward(p.astype(np.float64), inp, bf=False)
     _, h64c, h64f = cnn.loss_and_head_grads(T64, lc, lf, FW)
     rg64 = cnn.backward(p.astype(np.float64), T64, h64c, h64f, bf=False)
     assert abs(loss - rl) <= 1e-3 * abs(rl)
     q, bad = 0, []
-    for L in cnn.LAYERS:
+    for L in cnn.layers(4):
         n = 9 * L[7] * L[8] + L[8]
         a, b, c = g[q:q + n], rg[q:q + n], rg64[q:q + n]
         dev = np.linalg.norm(a - b) / np.linalg.norm(b)
@@ -136,10 +140,10 @@ def test_device_gradients_match_restatement(net):
 @pytest.mark.gpu
 def test_device_adam_steps_reduce_loss(net):
     B, H, W = 2, 32, 32
-    im1, im2, lc, lf = _batch(B, H, W, 6)
+    inp, lc, lf = _batch(B, H, W, 6)
     lc = (lc * 0.5).astype(np.float32)
     lf = (lf * 0.5).astype(np.float32)
-    net.set_params(cnn.init_params(13))
-    losses = [net.train_step(im1, im2, lc, lf, lr=2e-3) for _ in range(40)]
+    net.set_params(cnn.init_params(13, planes=4))
+    losses = [net.train_step(inp, lc, lf, lr=2e-3, field_w=FW) for _ in range(40)]
     print("[cnn adam] loss %.4f -> %.4f" % (losses[0], losses[-1]))
     assert losses[-1] < 0.8 * losses[0]
