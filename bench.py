@@ -40,10 +40,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="lshs", choices=["lshs", "batch", "fbls", "lk"],
-                    help="lshs: 1024^2 LS(HS) 2-level (default); batch: 64 uneven pairs LS(HS); "
+    ap.add_argument("--workload", default="lshs", choices=["lshs", "plain", "batch", "fbls", "lk"],
+                    help="lshs: 1024^2 LS(HS) 2-level learned vs plain (default); plain: the plain form alone; "
+                         "batch: 64 uneven pairs LS(HS); "
                          "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level")
     ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
+    ap.add_argument("--eval-pairs", type=int, default=7,
+                    help="pairs the default workload cycles over (its own pair + held-out rendered pairs)")
     return ap.parse_args()
 
 
@@ -57,16 +60,23 @@ def traffic_from_profile():
         return None
 
 
-def cpu_baseline(im1, im2, fw, ow):
+def cpu_baseline(im1, im2, fw, ow, min_s=10.0):
+    """The C oracle's LS(HS) 2-level on the same pair, repeated until at least
+    min_s seconds of CPU work have run."""
     from oracle import oracle as orc
     threads = orc.get_threads()
+    reps = 0
     t0 = time.perf_counter()
-    orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
-    dt = time.perf_counter() - t0
+    while True:
+        orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_s:
+            break
     H, W = im1.shape
-    return {"value": round(H * W / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": threads, "kind": "port",
-            "sample": "1 pair %dx%d LS(HS) 2-level, C oracle (float32 fields, float64 stencil sums), "
-                      "OpenMP %d threads, %.1f s" % (H, W, threads, dt)}
+    return {"value": round(reps * H * W / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": threads, "kind": "port",
+            "sample": "%d runs of 1 pair %dx%d LS(HS) 2-level (plain form), C oracle (float32 fields, float64 "
+                      "stencil sums), OpenMP %d threads, %.1f s" % (reps, H, W, threads, dt)}
 
 
 def _timed(step, ctx, steps, warmup):
@@ -218,16 +228,8 @@ def bench_lk(args):
                   "frac": round(achieved / 157.3, 4), "traffic": None}, cpu)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (pairs are independent, "
-              "see the sharding notes)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    if args.workload != "lshs":
-        fn = {"batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk}[args.workload]
-        print(json.dumps(fn(args)))
-        return
+def bench_plain(args):
+    """1024^2 LS(HS) 2-level, plain form only (the reference's sweep counts)."""
     from gaussian_filter import prepareGaussianKernel
     from ofri import ops, synth
     from ofri.device import DeviceArray, context
@@ -256,23 +258,7 @@ def main():
     value = mpix * args.steps / wall
 
     # dominant kernel: HS sweeps at the finest level, timed alone on the context stream
-    depth, _, hs_variant, _ = ctx.query_tuning(n, n)
-    niter = 600
-    launches = -(-niter // depth)
-    f1 = ops.gauss_sep(d1, fw)
-    f2 = ops.gauss_sep(d2, fw)
-    U, V = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
-    lib, h = ctx.lib, ctx.handle
-    lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)  # warm
-    reps = 3
-    ctx.timer_start()
-    for _ in range(reps):
-        lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)
-    hs_ms = ctx.timer_stop() / reps
-    alg_bytes = (niter * HS_BYTES_PER_PX_SWEEP + HS_SETUP_BYTES_PER_PX) * n * n
-    launch_ms = hs_ms / launches
-    achieved = alg_bytes / (hs_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile()
+    roof = hs_roofline(ctx, d1, d2, fw, n)
 
     # the headline run's per-level statistics (for the record)
     _, _, st = ops.pyr_lshs(d1, d2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
@@ -284,7 +270,7 @@ def main():
     ctx.sync()
 
     out = {
-        "metric": METRIC,
+        "metric": "Mpix-pairs/s, LS(HS) 2-level %dx%d plain form" % (n, n),
         "value": round(value, 3),
         "unit": "Mpix-pairs/s",
         "n_gpus": 1,
@@ -303,18 +289,164 @@ def main():
             "pairs_per_step": 1, "height": n, "width": n, "levels": 2,
             "ls_sweeps": [s["ls_iters"] for s in st], "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm", "kernel": "hs_pass_k (temporally blocked HS sweep, tile variant %d, %d sweeps/launch)" % (hs_variant, depth),
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "launch_ms": round(launch_ms, 5),
-            "algorithmic_bytes_per_launch": round(alg_bytes / launches),
-        },
+        "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(im1, im2, fw, ow)
-    print(json.dumps(out))
+    return out
+
+
+def hs_roofline(ctx, d1, d2, fw, n):
+    """The temporally blocked HS sweep at n x n (600 sweeps, alpha 21) timed
+    alone on the context stream with HIP events."""
+    from ofri import ops
+    from ofri.device import DeviceArray
+    depth, _, hs_variant, _ = ctx.query_tuning(n, n)
+    niter = 600
+    launches = -(-niter // depth)
+    f1 = ops.gauss_sep(d1, fw)
+    f2 = ops.gauss_sep(d2, fw)
+    U, V = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
+    lib, h = ctx.lib, ctx.handle
+    lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)  # warm
+    reps = 3
+    ctx.timer_start()
+    for _ in range(reps):
+        lib.ofri_hs(h, f1.ptr, f2.ptr, n, n, 21.0, niter, U.ptr, V.ptr, None)
+    hs_ms = ctx.timer_stop() / reps
+    alg_bytes = (niter * HS_BYTES_PER_PX_SWEEP + HS_SETUP_BYTES_PER_PX) * n * n
+    achieved = alg_bytes / (hs_ms * 1e-3) / 1e9
+    return {"bound": "hbm",
+            "kernel": "hs_pass_k (temporally blocked HS sweep, tile variant %d, %d sweeps/launch)" % (hs_variant, depth),
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+            "launch_ms": round(hs_ms / launches, 5), "algorithmic_bytes_per_launch": round(alg_bytes / launches)}
+
+
+def net_roofline(ctx, net, n):
+    """One forward pass of the correction network over a 4-plane n x n input
+    (all convolution launches plus the input and head kernels), timed with
+    HIP events; flops on the real channels."""
+    from ofri.device import DeviceArray
+    rng = np.random.default_rng(0)
+    x = DeviceArray.from_host(rng.standard_normal((1, 4, n, n)).astype(np.float32))
+    c, f = DeviceArray((1, 4, n // 2, n // 2)), DeviceArray((1, 4, n, n))
+    for _ in range(3):
+        net.predict(x, c, f)
+    reps = 20
+    ctx.timer_start()
+    for _ in range(reps):
+        net.predict(x, c, f)
+    ms = ctx.timer_stop() / reps
+    fl = net.flops(n, n)
+    achieved = fl / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": "correction network forward %dx%d (conv3_lds_k bf16 MFMA + input/head "
+                                       "kernels)" % (n, n),
+            "achieved": round(achieved, 2), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
+            "traffic": None, "ms": round(ms, 4), "flops": fl}
+
+
+def bench_lshs(args):
+    """The headline: LS(HS) 2-level at 1024^2 solved to the plain method's
+    per-level residual, learned form against plain form.  A step is one pair;
+    the steps cycle over the workload's pair (Lamb-Oseen, density 0.04) and
+    held-out rendered pairs of the three flow families."""
+    from ofri import learned, ops, synth
+    from ofri.device import DeviceArray, context
+    from ofri.net import Net
+    ctx = context()
+    n = args.size
+    im1, im2, _, _ = synth.config_pair(4, size=n)
+    fw, ow = learned.filters()
+    pairs = [(DeviceArray.from_host(im1), DeviceArray.from_host(im2))]
+    extra, specs = learned.heldout_pairs(args.eval_pairs - 1, n)
+    pairs += extra
+    net = Net.load()
+    wl = learned.WORKLOAD
+    # the plain results and residual targets of every pair (they define the workload: not timed)
+    tgt, plain, plain_iters = [], [], []
+    for d1, d2 in pairs:
+        Up, Vp, ht, lt, st = learned.plain_targets(d1, d2, fw, ow)
+        tgt.append((ht, lt))
+        plain.append((Up.download(), Vp.download()))
+        plain_iters.append([s["ls_iters"] for s in st])
+    Ua, Va = DeviceArray((n, n)), DeviceArray((n, n))
+    np_ = len(pairs)
+    cnt = {"i": 0, "fell": 0, "runs": 0}
+
+    def plain_step():
+        d1, d2 = pairs[cnt["i"] % np_]
+        cnt["i"] += 1
+        ops.pyr_lshs(d1, d2, filt_w=fw, opt_w=ow, Uacc=Ua, Vacc=Va, stats=False, **wl)
+
+    def learned_step(nt):
+        k = cnt["i"] % np_
+        cnt["i"] += 1
+        ht, lt = tgt[k]
+        _, _, info = learned.solve(pairs[k][0], pairs[k][1], nt, ht, lt, fw, ow, Uacc=Ua, Vacc=Va, fallback=True,
+                                   stats=False)
+        cnt["fell"] += int(info["fell_back"])
+        cnt["runs"] += 1
+
+    cnt["i"] = 0
+    wall_p, gpu_p = _timed(plain_step, ctx, args.steps, args.warmup)
+    cnt["i"] = 0
+    wall_l, gpu_l = _timed(lambda: learned_step(net), ctx, args.steps, args.warmup)
+    fell, runs = cnt["fell"], cnt["runs"]
+    cnt.update(i=0, fell=0, runs=0)
+    wall_z, _ = _timed(lambda: learned_step(None), ctx, args.steps, args.warmup)
+    fell_z, runs_z = cnt["fell"], cnt["runs"]
+    mpix = n * n / 1e6
+    value = mpix * args.steps / wall_l
+    plain_value = mpix * args.steps / wall_p
+    # per pair, untimed: the learned result against the plain one, and sweeps
+    rows = []
+    for k, (d1, d2) in enumerate(pairs):
+        U, V, info = learned.solve(d1, d2, net, tgt[k][0], tgt[k][1], fw, ow, fallback=True)
+        e = learned.epe((U.download(), V.download()), plain[k])
+        rows.append({"family": "lamb_oseen" if k == 0 else ("poiseuille", "lamb_oseen", "turbulence")[specs[k - 1]["family"]],
+                     "epe_mean": round(e[0], 5), "epe_max": round(e[1], 4), "fell_back": info["fell_back"],
+                     "hs_sweeps": [s["hs_iters"] for s in info["stats"]],
+                     "ls_sweeps": [s["ls_iters"] for s in info["stats"]], "plain_ls_sweeps": plain_iters[k]})
+    out = _line(METRIC, value, "Mpix-pairs/s", args, wall_l, gpu_l,
+                {"workload": "LS(HS) 2-level %dx%d to the plain method's per-level residual: FILTER 3.4/3 taps, HS "
+                             "alpha 45 / 21 (plain: 600 sweeps), FILTER_OPT 0.48/5 taps, Liu-Shen alpha 5 (plain: tol "
+                             "1e-8, max 60); learned form = accelerated HS + correction network + Liu-Shen to the "
+                             "targets, plain fallback" % (n, n),
+                 "pairs_per_step": 1, "height": n, "width": n, "levels": 2, "eval_pairs": np_,
+                 "parallelism": "single GPU"},
+                hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
+    out["learned_vs_plain"] = {
+        "learned_value": round(value, 3), "plain_value": round(plain_value, 3),
+        "no_network_value": round(mpix * args.steps / wall_z, 3),
+        "speedup_vs_plain": round(value / plain_value, 3), "plain_ms_per_pair": round(wall_p * 1e3 / args.steps, 4),
+        "learned_ms_per_pair": round(wall_l * 1e3 / args.steps, 4),
+        "fallback_rate": round(fell / max(1, runs), 4), "no_network_fallback_rate": round(fell_z / max(1, runs_z), 4),
+        "epe_to_plain_mean": round(float(np.mean([r["epe_mean"] for r in rows])), 5),
+        "epe_to_plain_max_pair_mean": round(float(np.max([r["epe_mean"] for r in rows])), 5),
+        "pairs": rows,
+    }
+    out["roofline_net"] = net_roofline(ctx, net, n)
+    # calibration dispatch for the HBM counters: one plain device-to-device
+    # copy of CALIB_BYTES (read CALIB_BYTES, write CALIB_BYTES)
+    DeviceArray((CALIB_BYTES // 4 // 1024, 1024)).copy()
+    ctx.sync()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(im1, im2, fw, ow)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (pairs are independent, "
+              "see the sharding notes)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.workload != "lshs":
+        fn = {"plain": bench_plain, "batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk}[args.workload]
+        print(json.dumps(fn(args)))
+        return
+    print(json.dumps(bench_lshs(args)))
 
 
 if __name__ == "__main__":

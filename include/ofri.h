@@ -261,6 +261,9 @@ typedef struct ofri_net ofri_net;
 int ofri_net_create(ofri_ctx* ctx, int planes, ofri_net** out);
 int ofri_net_destroy(ofri_net* net);
 int ofri_net_num_params(ofri_net* net, size_t* n);
+/* Floating-point operations (2 per multiply-add, real channels) of one
+ * forward pass over one H x W input; H, W multiples of 16. */
+int ofri_net_flops(ofri_net* net, int H, int W, double* flops);
 int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n);
 int ofri_net_get_params(ofri_ctx* ctx, ofri_net* net, float* h_params, size_t n);
 /* Random initial parameters (He-normal for the leaky-ReLU layers, heads

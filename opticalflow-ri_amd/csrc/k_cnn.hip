@@ -869,6 +869,17 @@ void net_destroy(Net* n) {
 size_t net_num_params(const Net* n) { return n->n_valid; }
 int net_planes(const Net* n) { return n->planes; }
 
+// Multiply-adds of one forward pass over one H x W input, counted on the
+// real (unpadded) channels: 9 * cin * cout per output pixel of each layer.
+double net_forward_flops(const Net* n, int H, int W) {
+    double macs = 0.0;
+    for (int l = 0; l < kNumLayers; ++l) {
+        const LayerDef& d = n->def[l];
+        macs += 9.0 * d.cin * d.cout * (double)(H >> (d.level + 1)) * (double)(W >> (d.level + 1));
+    }
+    return 2.0 * macs;
+}
+
 // Packed order: layer by layer, weights [cout][tap][cin] then bias [cout].
 int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count) {
     if (count != n->n_valid) return fail(OFRI_E_ARG, "net: %zu parameters given, %zu expected", count, n->n_valid);

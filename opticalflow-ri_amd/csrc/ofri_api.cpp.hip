@@ -844,6 +844,13 @@ int ofri_net_num_params(ofri_net* net, size_t* n) {
     return OFRI_OK;
 }
 
+int ofri_net_flops(ofri_net* net, int H, int W, double* flops) {
+    if (!net || !flops) return fail(OFRI_E_ARG, "ofri_net_flops: null pointer");
+    if (H <= 0 || W <= 0 || H % 16 || W % 16) return fail(OFRI_E_ARG, "ofri_net_flops: H, W must be positive multiples of 16");
+    *flops = net_forward_flops(NET(net), H, W);
+    return OFRI_OK;
+}
+
 int ofri_net_set_params(ofri_ctx* ctx, ofri_net* net, const float* h_params, size_t n) {
     CHECK_CTX(ctx);
     if (!net || !h_params) return fail(OFRI_E_ARG, "ofri_net_set_params: null pointer");

@@ -180,6 +180,7 @@ int net_create(ofri_ctx* ctx, int planes, Net** out);
 int net_planes(const Net* n);
 void net_destroy(Net* n);
 size_t net_num_params(const Net* n);
+double net_forward_flops(const Net* n, int H, int W);
 int net_set_params(ofri_ctx* ctx, Net* n, const float* h, size_t count);
 int net_init_params(ofri_ctx* ctx, Net* n, unsigned long long seed);
 int net_get_params(ofri_ctx* ctx, Net* n, float* h, size_t count);

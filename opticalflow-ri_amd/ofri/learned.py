@@ -86,6 +86,27 @@ def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None,
     return U, V, dict(converged=conv, fell_back=fell, stats=st)
 
 
+# Held-out evaluation pairs use seeds from here on (training uses 0 .. 19999)
+HELDOUT_SEED = 1000000
+
+
+def heldout_pairs(n, size=1024):
+    """n device-rendered pairs of size x size outside the training seeds, flow
+    families cycling Poiseuille, Lamb-Oseen, shear + turbulence.  Returns a
+    list of (im1, im2) DeviceArrays and the list of their specs."""
+    from .train import pair_spec
+    out, specs, seed = [], [], HELDOUT_SEED
+    for k in range(n):
+        while True:
+            sp = pair_spec(seed, size, size)
+            seed += 1
+            if sp["family"] == k % 3:
+                break
+        out.append(ops.render_pair(size, size, sp["family"], sp["density"], sp["seed"], sp["par"], sp["modes"]))
+        specs.append(sp)
+    return out, specs
+
+
 def epe(a, b):
     e = np.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2)
     return float(e.mean()), float(e.max())

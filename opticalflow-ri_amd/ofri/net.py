@@ -58,6 +58,13 @@ class Net(object):
         _capi.check(self.ctx.lib.ofri_net_get_params(self.ctx.handle, self.handle, _capi.fptr(out), out.size))
         return out
 
+    def flops(self, H, W):
+        """Floating-point operations of one forward pass over an H x W input
+        (2 per multiply-add on the real channels)."""
+        f = C.c_double()
+        _capi.check(self.ctx.lib.ofri_net_flops(self.handle, int(H), int(W), C.byref(f)))
+        return f.value
+
     def get_grads(self):
         out = np.empty(self.num_params, np.float32)
         _capi.check(self.ctx.lib.ofri_net_get_grads(self.ctx.handle, self.handle, _capi.fptr(out), out.size))

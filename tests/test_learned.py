@@ -1,5 +1,5 @@
-"""Learned form infrastructure: the device particle renderer and the
-pyramid's exported per-level labels."""
+"""The learned form: the device particle renderer, the pyramid's exported
+per-level labels, and the learned LS(HS) form with the committed network."""
 import numpy as np
 import pytest
 
@@ -72,3 +72,72 @@ def test_taps_run_equals_the_plain_pyramid(ops):
     t = taps1.download().reshape(4, 256, 320)
     assert np.abs(t[0] + t[2] - U1.download()).max() <= 1e-6 * max(1.0, np.abs(t[0]).max())
     assert np.abs(t[1] + t[3] - V1.download()).max() <= 1e-6 * max(1.0, np.abs(t[1]).max())
+
+
+# ---------------------------------------------------------------- learned form
+
+@pytest.fixture(scope="module")
+def trained():
+    """The committed correction network, a held-out Lamb-Oseen pair (seeds
+    outside the training set) and its plain results and residual targets."""
+    from ofri import learned
+    from ofri.net import Net
+    net = Net.load()
+    pairs, specs = learned.heldout_pairs(2, 1024)
+    assert specs[1]["family"] == 1
+    pair = pairs[1]
+    fw, ow = learned.filters()
+    Up, Vp, ht, lt, st = learned.plain_targets(pair[0], pair[1], fw, ow)
+    return dict(net=net, pair=pair, plain=(Up.download(), Vp.download()), ht=ht, lt=lt, st=st, fw=fw, ow=ow)
+
+
+def test_learned_form_reaches_the_plain_residual_with_fewer_sweeps(trained):
+    """The trained network is what lets Liu-Shen reach the plain residual:
+    with it every level converges in a fraction of the plain 60 sweeps and
+    lands within 0.01 px (mean EPE) of the plain result; the same method
+    without the network needs the full budget or misses the target."""
+    from ofri import learned
+    t = trained
+    a, b = t["pair"]
+    U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
+    assert info["converged"]
+    ls = [s["ls_iters"] for s in info["stats"]]
+    e_mean, e_max = learned.epe((U.download(), V.download()), t["plain"])
+    U0, V0, info0 = learned.solve(a, b, None, t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
+    ls0 = [s["ls_iters"] for s in info0["stats"]]
+    print("[learned] LS sweeps %s (no network %s, plain %s), EPE to plain mean %.5f max %.4f px"
+          % (ls, ls0, [s["ls_iters"] for s in t["st"]], e_mean, e_max))
+    assert all(s["hs_res"] <= tol for s, tol in zip(info["stats"], t["ht"]))
+    assert all(s["ls_err"] <= tol for s, tol in zip(info["stats"], t["lt"]))
+    assert max(ls) <= 30 and sum(ls) * 2 <= sum(ls0)
+    assert not info0["converged"] or sum(ls0) >= 2 * sum(ls)
+    assert e_mean <= 0.01
+
+
+def test_learned_fallback_returns_the_plain_result(trained):
+    """A sweep budget too small to reach the targets triggers the fallback,
+    whose result is the plain pyramid's, bit for bit."""
+    from ofri import learned, ops
+    t = trained
+    a, b = t["pair"]
+    U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=True, hs_budget=8)
+    assert info["fell_back"] and not info["converged"]
+    assert np.array_equal(U.download(), t["plain"][0]) and np.array_equal(V.download(), t["plain"][1])
+    Ur, Vr, _ = ops.pyr_lshs(a, b, filt_w=t["fw"], opt_w=t["ow"], **learned.WORKLOAD)
+    assert np.array_equal(Ur.download(), t["plain"][0])
+
+
+def test_learned_form_replays_identically(trained):
+    """Eager run, graph capture and replays of the learned form give the same bits."""
+    from ofri import learned
+    from ofri.device import DeviceArray
+    t = trained
+    a, b = t["pair"]
+    Ua, Va = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
+    outs = []
+    for _ in range(4):
+        U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], Uacc=Ua, Vacc=Va,
+                                   fallback=False)
+        outs.append((U.download(), V.download(), [s["ls_iters"] for s in info["stats"]]))
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1]) and o[2] == outs[0][2]
