@@ -113,11 +113,16 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
  * plain sweeps continue to tol (float32 Chebyshev iterates stall near the
  * rounding level, plain sweeps settle); finish <= 1: accelerated to tol.
  * maxnum bounds the sweeps of both phases together, the plain phase to at
- * most maxnum / 2 of them; d_hist, *h_err and *h_iters cover both.  Same fixed point as ofri_hs; no counterpart in the
- * reference (an accelerated form for solving to a residual). */
+ * most maxnum / 2 of them; d_hist, *h_err and *h_iters cover both.
+ * launch_test != 0: the stop test runs only at the last sweep of each launch
+ * (cheaper; d_hist must be NULL): the accelerated phase then ends with the
+ * iterate that launch produced, the plain phase with T(x_k) as before, and
+ * the sweep count is a whole number of launches per phase.  Same fixed point
+ * as ofri_hs; no counterpart in the reference (an accelerated form for
+ * solving to a residual). */
 int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
                   int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
-                  float* d_hist, float* h_err, int* h_iters);
+                  float* d_hist, float* h_err, int* h_iters, int launch_test);
 
 /* ---- Liu-Shen ------------------------------------------------------------ */
 /* Liu-Shen physics-based refinement in the solver's own orientation: d_u is

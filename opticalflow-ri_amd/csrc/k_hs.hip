@@ -66,9 +66,11 @@ struct HsPass {
     float* out[2][2];
     int src;                 // ping-pong index read (normal launch)
     // residual mode (hs_pass_k<..., true>): per-sweep norms and stop test
-    double* part;            // [K][nblocks][2] squared-update sums
+    double* part;            // two slots of [K][nblocks][2] squared-update sums
+    size_t part_stride;      // doubles per slot
+    int slot;                // launch parity within the phase (partials / state slot written)
     SolveCtrl* ctrl;
-    int redo;                // 1: re-run launch driven by ctrl->redo
+    int final_;              // 1: the phase's final launch (test, re-run, publish)
     double tol;
     int maxnum;
     size_t N;
@@ -179,22 +181,29 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         float d = a.a2 + FX[r] * FX[r];
         DEN[r] = d + FY[r] * FY[r];
         RCP[r] = 1.0f / DEN[r];   // correctly rounded reciprocal
+        // held, not re-derived: under register pressure the compiler would
+        // otherwise redo this division (a dozen instructions) every sweep
+        asm volatile("" : "+v"(RCP[r]));
     }
+    // the launch's Chebyshev weights, one per lane, broadcast per sweep
+    float om_lane = 1.0f;
+    if (CHEB && lane < nsw) om_lane = a.omega[base + lane];
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
     // per-wave squared-update sums of a sweep, double-buffered: the next
     // sweep's barrier publishes them, so the norm costs no barrier of its own
     __shared__ double red[2][WAVES][2];
     // sweep sw reduces its update norm (every sweep, or a launch's last)
-    auto normed = [&](int sw) { return NORM && !a.redo && (!a.check_last || sw == nsw - 1); };
+    auto normed = [&](int sw) { return NORM && !a.final_ && (!a.check_last || sw == nsw - 1); };
+    double* part = NORM ? a.part + (size_t)a.slot * a.part_stride : nullptr;
     auto flush = [&](int sw) {
         double tu = 0.0, tv = 0.0;
         for (int q = 0; q < WAVES; ++q) {
             tu = tu + red[sw & 1][q][0];
             tv = tv + red[sw & 1][q][1];
         }
-        a.part[((size_t)sw * nblk + blk) * 2] = tu;
-        a.part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
+        part[((size_t)sw * nblk + blk) * 2] = tu;
+        part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
     };
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
@@ -216,8 +225,9 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         float om = 1.0f;
         bool plain_step = true;
         if (CHEB) {
-            om = a.omega[base + s];
-            plain_step = a.redo && s == nsw - 1;
+            om = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(om_lane), s));
+            // the re-run of a stopping launch ends with a plain sweep
+            plain_step = a.final_ && s == nsw - 1;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -229,7 +239,18 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             float num = x * ub;
             num = num + y * vb;
             num = num + FT[r];
-            float der = quotient(num, DEN[r], RCP[r]);
+            float den;
+            if (NORM) {
+                // the residual-mode tiles carry more live state: the
+                // denominator is formed again (same operations, same bits)
+                // instead of held, which keeps the tile in registers
+                asm volatile("" : "+v"(x), "+v"(y));
+                den = a.a2 + x * x;
+                den = den + y * y;
+            } else {
+                den = DEN[r];
+            }
+            float der = quotient(num, den, RCP[r]);
             float un = ub - x * der;
             float vn = vb - y * der;
             if (norm) {
@@ -293,32 +314,15 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             }
         }
     }
-    if (NORM && !a.redo)
-        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, CHEB && !a.check_last,
-                         a.check_last ? nsw - 1 : 0);
-}
 
-// Start of a launch in residual mode: the re-run launch takes its sweep count
-// and input buffer from the control word; a finished solve does nothing.
-__device__ __forceinline__ bool hs_launch_args(const HsPass& a, int& nsw, int& src, int& base) {
-    nsw = a.nsweeps;
-    src = a.src;
-    base = a.sweep0;
-    if (a.redo) {
-        nsw = a.ctrl->redo;
-        src = a.ctrl->redo_in;
-        base = a.ctrl->sweeps - nsw;
-        return nsw > 0;
-    }
-    return !a.ctrl->done;
 }
 
 template <int R, int WAVES, bool NORM, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     int nsw = a.nsweeps, src = a.src, base = 0;
-    if (NORM && !hs_launch_args(a, nsw, src, base)) return;
-    hs_tile<R, WAVES, NORM, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y,
-                                  blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
+    if (NORM && !launch_args(a, blk, nblk, CHEB && !a.check_last, nsw, src, base)) return;
+    hs_tile<R, WAVES, NORM, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
 
 
@@ -422,7 +426,9 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* U0 = Vb + N;
     float* V0 = U0 + N;
     void* pbuf = nullptr;
-    if (d_ctrl) OFRI_TRY(scratch(ctx, "hs_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
+    // per-sweep partial sums, two launch-parity slots
+    const size_t pstride = (size_t)depth * nblk * 2;
+    if (d_ctrl) OFRI_TRY(scratch(ctx, "hs_part", 2 * pstride * sizeof(double), &pbuf));
     float *P0 = nullptr, *omega = nullptr;
     if (cheb) {
         void* cb;
@@ -460,6 +466,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.a2 = alpha * alpha;
     p.K = depth;
     p.part = (double*)pbuf;
+    p.part_stride = pstride;
     p.ctrl = d_ctrl;
     p.tol = tol;
     p.maxnum = niter;
@@ -477,21 +484,26 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     // copied back into U, V
     auto phase = [&](void (*kern)(HsPass), double t, int sweeps) -> int {
         p.tol = t;
-        int done = 0, src = 0;
+        int done = 0, src = 0, j = 0;
         while (done < sweeps) {
             int ns = sweeps - done < depth ? sweeps - done : depth;
             p.nsweeps = ns;
             p.src = src;
             p.sweep0 = done;
-            p.redo = 0;
+            p.slot = j & 1;
+            p.final_ = 0;
             kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
             OFRI_HIP(hipGetLastError());
             src ^= 1;
             done += ns;
+            ++j;
         }
         if (d_ctrl) {
             if (niter > 0) {
-                p.redo = 1;
+                // tests the last launch, re-runs a launch the test stopped
+                // inside, publishes the result
+                p.slot = j & 1;
+                p.final_ = 1;
                 kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
                 OFRI_HIP(hipGetLastError());
                 OFRI_TRY(select_copy(ctx, d_ctrl, U, Ub, U, N));

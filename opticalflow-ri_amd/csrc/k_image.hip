@@ -83,32 +83,86 @@ __global__ void resize_v(const float* __restrict__ tmp, int W2, float* __restric
     dst[(size_t)yy * W2 + xx] = (float)ss;
 }
 
-// Spline pass along x: tmp[r][x] = sum_t w[x][t] * src[r][start[x] + t]  (double)
-__global__ void spline_x(const float* __restrict__ src, int Wc, double* __restrict__ tmp, int W,
-                         int Hc, const double* __restrict__ w, const int* __restrict__ st, int width) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (x >= W || r >= Hc) return;
-    const double* ww = w + (size_t)x * width;
-    const float* row = src + (size_t)r * Wc + st[x];
-    double s = 0.0;
-    for (int t = 0; t < width; ++t) s = s + ww[t] * (double)row[t];
-    tmp[(size_t)r * W + x] = s;
+// Not-a-knot spline upsample, two passes in double over a banded weight
+// table: tmp[r][x] = sum_t w[x][t] * src[r][st[x] + t], then
+// dst[y][x] = float(sum_t w'[y][t] * tmp[st'[y] + t][x]) * scale.  Every sum
+// runs t = 0, 1, ... from 0.0.  blockIdx.z selects one of up to two planes
+// (U and V of a flow share the tables).
+struct SplinePlanes {
+    const float* src[2];
+    double* tmp[2];
+    float* dst[2];
+    float scale[2];
+};
+
+constexpr int kSplineRowsX = 8;    // coarse rows per workgroup of the x pass
+constexpr int kSplineRowsY = 8;    // output rows per thread of the y pass
+
+// x pass: kSplineRowsX coarse rows staged in LDS (nrows * Wc floats); the
+// weights are read transposed (wT[t][x]) so a wavefront's loads are
+// contiguous, and each weight serves every staged row.
+__global__ __launch_bounds__(256) void spline_x(SplinePlanes p, int Wc, int W, int Hc, const double* __restrict__ wT,
+                                                const int* __restrict__ st, int width, int nrows) {
+    extern __shared__ float rows[];
+    const int z = blockIdx.z;
+    const int r0 = blockIdx.y * nrows;
+    const int nr = min(nrows, Hc - r0);
+    const float* src = p.src[z] + (size_t)r0 * Wc;
+    for (int i = threadIdx.x; i < nr * Wc; i += 256) rows[i] = src[i];
+    __syncthreads();
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= W) return;
+    const float* b = rows + st[x];
+    double s[kSplineRowsX];
+#pragma unroll
+    for (int i = 0; i < kSplineRowsX; ++i) s[i] = 0.0;
+    for (int t = 0; t < width; ++t) {
+        const double w = wT[(size_t)t * W + x];
+#pragma unroll
+        for (int i = 0; i < kSplineRowsX; ++i)
+            if (i < nr) s[i] = s[i] + w * (double)b[i * Wc + t];
+    }
+    double* tmp = p.tmp[z] + (size_t)r0 * W + x;
+#pragma unroll
+    for (int i = 0; i < kSplineRowsX; ++i)
+        if (i < nr) tmp[(size_t)i * W] = s[i];
 }
 
-// Spline pass along y, rounding to float and scaling in float32.
-__global__ void spline_y(const double* __restrict__ tmp, int W, float* __restrict__ dst, int H,
-                         const double* __restrict__ w, const int* __restrict__ st, int width,
-                         float scale) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= W || y >= H) return;
-    const double* ww = w + (size_t)y * width;
-    const double* col = tmp + (size_t)st[y] * W + x;
-    double s = 0.0;
-    for (int t = 0; t < width; ++t) s = s + ww[t] * col[(size_t)t * W];
-    float o = (float)s;
-    dst[(size_t)y * W + x] = o * scale;
+// y pass: a thread owns column x of kSplineRowsY consecutive output rows and
+// walks the coarse rows their bands cover once, adding each row's value to
+// the outputs whose band holds it (in band order).
+__global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, const double* __restrict__ w,
+                                                const int* __restrict__ st, int width) {
+    const int z = blockIdx.z;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y0 = blockIdx.y * kSplineRowsY;
+    if (x >= W) return;
+    const int ny = min(kSplineRowsY, H - y0);
+    const int j0 = st[y0], j1 = st[y0 + ny - 1] + width;
+    const double* tmp = p.tmp[z] + x;
+    double s[kSplineRowsY];
+    int sj[kSplineRowsY];
+#pragma unroll
+    for (int i = 0; i < kSplineRowsY; ++i) {
+        s[i] = 0.0;
+        sj[i] = i < ny ? st[y0 + i] : 0x3fffffff;
+    }
+    for (int j = j0; j < j1; ++j) {
+        const double v = tmp[(size_t)j * W];
+#pragma unroll
+        for (int i = 0; i < kSplineRowsY; ++i) {
+            const int t = j - sj[i];
+            if (t >= 0 && t < width) s[i] = s[i] + w[(size_t)(y0 + i) * width + t] * v;
+        }
+    }
+    float* dst = p.dst[z] + x;
+    const float sc = p.scale[z];
+#pragma unroll
+    for (int i = 0; i < kSplineRowsY; ++i)
+        if (i < ny) {
+            float o = (float)s[i];
+            dst[(size_t)(y0 + i) * W] = o * sc;
+        }
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -399,9 +453,14 @@ int get_spline_table(ofri_ctx* ctx, int n, int N, SplineTable** t) {
     std::vector<double> w;
     std::vector<int> st;
     spline_weights(n, N, w, st, s.width);
+    std::vector<double> wt(w.size());
+    for (int k = 0; k < N; ++k)
+        for (int t = 0; t < s.width; ++t) wt[(size_t)t * N + k] = w[(size_t)k * s.width + t];
     OFRI_HIP(hipMalloc(&s.d_w, w.size() * sizeof(double)));
+    OFRI_HIP(hipMalloc(&s.d_wT, wt.size() * sizeof(double)));
     OFRI_HIP(hipMalloc(&s.d_start, st.size() * sizeof(int)));
     OFRI_HIP(hipMemcpy(s.d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    OFRI_HIP(hipMemcpy(s.d_wT, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice));
     OFRI_HIP(hipMemcpy(s.d_start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
     auto res = ctx->spline_tables.emplace(key, s);
     *t = &res.first->second;
@@ -442,21 +501,36 @@ int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, in
     return OFRI_OK;
 }
 
-int spline_upsample(ofri_ctx* ctx, const float* src, int Hc, int Wc, float* dst, int H, int W,
-                    float scale) {
+int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int Hc, int Wc, float* dst_u,
+                     float* dst_v, int H, int W, float scale_u, float scale_v) {
     if (Hc < 4 || Wc < 4) return fail(OFRI_E_ARG, "spline_upsample: coarse grid %dx%d below 4x4", Hc, Wc);
     if (H < 1 || W < 1) return fail(OFRI_E_ARG, "spline_upsample: empty output");
+    // the x pass stages whole coarse rows in LDS (at most 64 KiB)
+    const int nrows = std::min(kSplineRowsX, 16384 / Wc);
+    if (nrows < 1) return fail(OFRI_E_ARG, "spline_upsample: coarse width %d above 16384", Wc);
     SplineTable *tx, *ty;
     OFRI_TRY(get_spline_table(ctx, Wc, W, &tx));
     OFRI_TRY(get_spline_table(ctx, Hc, H, &ty));
+    const int planes = src_v ? 2 : 1;
     void* tmp;
-    OFRI_TRY(scratch(ctx, "spline_tmp", (size_t)Hc * W * sizeof(double), &tmp));
-    spline_x<<<grid_rows(W, Hc), 256, 0, ctx->stream>>>(src, Wc, (double*)tmp, W, Hc, tx->d_w,
-                                                       tx->d_start, tx->width);
-    spline_y<<<grid_rows(W, H), 256, 0, ctx->stream>>>((const double*)tmp, W, dst, H, ty->d_w,
-                                                      ty->d_start, ty->width, scale);
+    OFRI_TRY(scratch(ctx, "spline_tmp", (size_t)planes * Hc * W * sizeof(double), &tmp));
+    SplinePlanes p{};
+    p.src[0] = src_u; p.src[1] = src_v;
+    p.tmp[0] = (double*)tmp; p.tmp[1] = (double*)tmp + (size_t)Hc * W;
+    p.dst[0] = dst_u; p.dst[1] = dst_v;
+    p.scale[0] = scale_u; p.scale[1] = scale_v;
+    dim3 gx((W + 255) / 256, (Hc + nrows - 1) / nrows, planes);
+    spline_x<<<gx, 256, (size_t)nrows * Wc * sizeof(float), ctx->stream>>>(p, Wc, W, Hc, tx->d_wT, tx->d_start,
+                                                                           tx->width, nrows);
+    dim3 gy((W + 255) / 256, (H + kSplineRowsY - 1) / kSplineRowsY, planes);
+    spline_y<<<gy, 256, 0, ctx->stream>>>(p, W, H, ty->d_w, ty->d_start, ty->width);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
+}
+
+int spline_upsample(ofri_ctx* ctx, const float* src, int Hc, int Wc, float* dst, int H, int W,
+                    float scale) {
+    return spline_upsample2(ctx, src, nullptr, Hc, Wc, dst, nullptr, H, W, scale, 1.0f);
 }
 
 int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const float* U,

@@ -87,12 +87,14 @@ struct LsPass {
     int nsweeps;
     int src;                // ping-pong index read (normal launch)
     float h;
-    double* part;           // [K][nblocks][2] squared-update sums (nullptr: none)
+    double* part;           // two slots of [K][nblocks][2] squared-update sums
+    size_t part_stride;     // doubles per slot
+    int slot;               // launch parity within the solve (partials / state slot written)
     double tol;             // stopping tolerance on the update norm
     int maxnum;             // sweep budget
     float* hist;            // per-sweep norms (nullptr: not recorded)
     SolveCtrl* ctrl;
-    int redo;               // 1: re-run launch driven by ctrl->redo
+    int final_;             // 1: the final launch (test, re-run, publish)
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
     int check_last;         // 1: only a launch's last sweep is reduced and tested
     // Chebyshev mode (ls_pass_k<..., true>): previous iterate buffers, the
@@ -202,15 +204,16 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
     const bool el = gx == 0, er = gx == W - 1;
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
     const float h = a.h;
-    auto normed = [&](int sw) { return a.part != nullptr && (!a.check_last || sw == nsw - 1); };
+    auto normed = [&](int sw) { return !a.final_ && (!a.check_last || sw == nsw - 1); };
+    double* part = a.part + (size_t)a.slot * a.part_stride;
     auto flush = [&](int sw) {
         double tu = 0.0, tv = 0.0;
         for (int q = 0; q < WAVES; ++q) {
             tu = tu + red[sw & 1][q][0];
             tv = tv + red[sw & 1][q][1];
         }
-        a.part[((size_t)sw * nblk + blk) * 2] = tu;
-        a.part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
+        part[((size_t)sw * nblk + blk) * 2] = tu;
+        part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
     };
     for (int s = 0; s < nsw; ++s) {
         const int b = s & 1;
@@ -277,21 +280,6 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
     }
 }
 
-// Start of a launch: the re-run launch takes its sweep count and input buffer
-// from the control word; a finished solve does nothing.
-__device__ __forceinline__ bool ls_launch_args(const LsPass& a, int& nsw, int& src, int& base) {
-    nsw = a.nsweeps;
-    src = a.src;
-    base = a.sweep0;
-    if (a.redo) {
-        nsw = a.ctrl->redo;
-        src = a.ctrl->redo_in;
-        base = a.ctrl->sweeps - nsw;
-        return nsw > 0;
-    }
-    return !a.ctrl->done;
-}
-
 // One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
 // one being number blk.
 template <int R, int WAVES, bool CHEB = false>
@@ -356,17 +344,14 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
             }
         }
     }
-    if (a.part)
-        arrive_and_check(a.part, nblk, nsw, src ^ 1, a.tol, a.maxnum, a.N, a.ctrl, a.hist, false,
-                         a.check_last ? nsw - 1 : 0);
 }
 
 template <int R, int WAVES, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
     int nsw, src, base;
-    if (!ls_launch_args(a, nsw, src, base)) return;
-    ls_tile<R, WAVES, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x,
-                            gridDim.x * gridDim.y);
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
+    if (!launch_args(a, blk, nblk, false, nsw, src, base)) return;
+    ls_tile<R, WAVES, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
 
 // Many images in one launch (workgroup b is tile b - blk0[p] of image p,
@@ -376,7 +361,7 @@ __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
 template <int R, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ls_batch_k(const LsPass* __restrict__ tab,
                                                          const int* __restrict__ blk0, int n,
-                                                         int nsweeps, int src0, int redo) {
+                                                         int nsweeps, int src0, int slot, int final_) {
     const int b = blockIdx.x;
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -387,19 +372,19 @@ __global__ __launch_bounds__(64 * WAVES) void ls_batch_k(const LsPass* __restric
     LsPass a = tab[lo];
     a.nsweeps = nsweeps;
     a.src = src0;
-    a.redo = redo;
-    if (redo) a.part = nullptr;
+    a.slot = slot;
+    a.final_ = final_;
     int nsw, src, base;
-    if (!ls_launch_args(a, nsw, src, base)) return;
-    const int local = b - blk0[lo];
-    ls_tile<R, WAVES>(a, nsw, src, base, local % a.tiles_x, local / a.tiles_x, local, a.tiles_x * a.tiles_y);
+    const int local = b - blk0[lo], nloc = a.tiles_x * a.tiles_y;
+    if (!launch_args(a, local, nloc, false, nsw, src, base)) return;
+    ls_tile<R, WAVES>(a, nsw, src, base, local % a.tiles_x, local / a.tiles_x, local, nloc);
 }
 
 
 struct LsVariant {
     int R, waves;
     void (*kernel)(LsPass);
-    void (*batched)(const LsPass*, const int*, int, int, int, int);
+    void (*batched)(const LsPass*, const int*, int, int, int, int, int);
     void (*cheb)(LsPass);
 };
 #define LS_SHAPE(R, WV) {R, WV, ls_pass_k<R, WV, false>, ls_batch_k<R, WV>, ls_pass_k<R, WV, true>}
@@ -449,7 +434,9 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     void *fbuf, *pbuf, *mbuf;
     OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float) + (cheb ? (size_t)maxnum * 4 + 64 : 0),
                      &fbuf));
-    OFRI_TRY(scratch(ctx, "ls_part", (size_t)depth * nblk * 2 * sizeof(double), &pbuf));
+    // per-sweep partial sums, two launch-parity slots
+    const size_t pstride = (size_t)depth * nblk * 2;
+    OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
     OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
     float* F = (float*)fbuf;
     float* ub = F + 8 * N;
@@ -467,6 +454,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.out[0][0] = u; p.out[0][1] = v; p.out[1][0] = ub; p.out[1][1] = vb;
     p.F = F; p.N = N; p.H = H; p.W = W; p.K = depth; p.h = h;
     p.part = (double*)pbuf;
+    p.part_stride = pstride;
     p.ctrl = d_ctrl;
     p.tol = tol;
     p.maxnum = maxnum;
@@ -488,21 +476,25 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.omega = omega;
         p.gamma = (float)(2.0 / (2.0 - lo - hi));
     }
-    int done = 0, src = 0;
+    int done = 0, src = 0, j = 0;
     while (done < maxnum) {
         int ns = maxnum - done < depth ? maxnum - done : depth;
         p.nsweeps = ns;
         p.src = src;
         p.sweep0 = done;
-        p.redo = 0;
+        p.slot = j & 1;
+        p.final_ = 0;
         kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_HIP(hipGetLastError());
         src ^= 1;
         done += ns;
+        ++j;
     }
     if (maxnum > 0) {
-        p.redo = 1;
-        p.part = nullptr;
+        // tests the last launch, re-runs a launch the test stopped inside,
+        // publishes the result
+        p.slot = j & 1;
+        p.final_ = 1;
         kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
         OFRI_TRY(select_copy(ctx, d_ctrl, u, ub, u, N));
         OFRI_TRY(select_copy(ctx, d_ctrl, v, vb, v, N));
@@ -544,7 +536,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
     size_t nparts = 0;
     for (int i = 0; i < n; ++i) {
         const size_t tiles = (size_t)((im[i].W + outW - 1) / outW) * ((im[i].H + outH - 1) / outH);
-        nparts += (size_t)depth * tiles * 2;
+        nparts += 2 * (size_t)depth * tiles * 2;
     }
     void *fbuf, *pbuf, *mbuf;
     OFRI_TRY(scratch(ctx, "lsb_fields", 10 * total * sizeof(float), &fbuf));
@@ -579,7 +571,8 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         p.tiles_x = (W + outW - 1) / outW;
         p.tiles_y = (H + outH - 1) / outH;
         p.part = part;
-        part += (size_t)depth * p.tiles_x * p.tiles_y * 2;
+        p.part_stride = (size_t)depth * p.tiles_x * p.tiles_y * 2;
+        part += 2 * p.part_stride;
         blk0[i + 1] = blk0[i] + p.tiles_x * p.tiles_y;
     }
     LsPass* d_pass = (LsPass*)d_tab;
@@ -589,16 +582,17 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         OFRI_HIP(hipMemcpyAsync(d_blk0, blk0.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         OFRI_HIP(hipStreamSynchronize(ctx->stream));
     }
-    int done = 0, src = 0;
+    int done = 0, src = 0, j = 0;
     while (done < maxnum) {
         const int ns = maxnum - done < depth ? maxnum - done : depth;
-        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, ns, src, 0);
+        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, ns, src, j & 1, 0);
         OFRI_HIP(hipGetLastError());
         src ^= 1;
         done += ns;
+        ++j;
     }
     if (maxnum > 0) {
-        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, 0, 0, 1);
+        var.batched<<<blk0[n], 64 * var.waves, 0, ctx->stream>>>(d_pass, d_blk0, n, 0, 0, j & 1, 1);
         OFRI_HIP(hipGetLastError());
         f = (float*)fbuf;
         for (int i = 0; i < n; ++i) {

@@ -104,6 +104,7 @@ int ofri_destroy(ofri_ctx* ctx) {
     }
     for (auto& kv : ctx->spline_tables) {
         (void)hipFree(kv.second.d_w);
+        (void)hipFree(kv.second.d_wT);
         (void)hipFree(kv.second.d_start);
     }
     for (auto& kv : ctx->graphs) {
@@ -289,15 +290,17 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
 
 int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
                   int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
-                  float* d_hist, float* h_err, int* h_iters) {
+                  float* d_hist, float* h_err, int* h_iters, int launch_test) {
     CHECK_CTX(ctx);
     if (!d_im1 || !d_im2 || !d_U || !d_V) return fail(OFRI_E_ARG, "ofri_hs_accel: null pointer");
     if (!(bound > 0.0f && bound <= 0.999f))
         return fail(OFRI_E_ARG, "ofri_hs_accel: bound %g not in (0, 0.999]", bound);
+    if (launch_test && d_hist)
+        return fail(OFRI_E_ARG, "ofri_hs_accel: the per-launch stop test records no per-sweep history");
     void* c;
     OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
     OFRI_TRY(hs_solve(ctx, d_im1, d_im2, H, W, alpha, maxnum, d_U, d_V, nullptr, tol, (SolveCtrl*)c, d_hist,
-                      bound, finish));
+                      bound, finish, launch_test != 0));
     if (h_err || h_iters) {
         SolveCtrl hc;
         OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
@@ -412,8 +415,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             // updateNextPyramidalLevel with warping, biLinear and scaling
             if (h != ph || w != pw) {
                 float sx = (float)w / (float)pw, sy = (float)h / (float)ph;
-                OFRI_TRY(spline_upsample(ctx, acc_u, ph, pw, out_u, h, w, sx));
-                OFRI_TRY(spline_upsample(ctx, acc_v, ph, pw, out_v, h, w, sy));
+                OFRI_TRY(spline_upsample2(ctx, acc_u, acc_v, ph, pw, out_u, out_v, h, w, sx, sy));
             } else {
                 OFRI_HIP(hipMemcpyAsync(out_u, acc_u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
                 OFRI_HIP(hipMemcpyAsync(out_v, acc_v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
@@ -570,8 +572,7 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
             if (l > 0) {
                 if (h != q.ph || w != q.pw) {
                     float sx = (float)w / (float)q.pw, sy = (float)h / (float)q.ph;
-                    OFRI_TRY(spline_upsample(ctx, q.acc_u, q.ph, q.pw, out_u, h, w, sx));
-                    OFRI_TRY(spline_upsample(ctx, q.acc_v, q.ph, q.pw, out_v, h, w, sy));
+                    OFRI_TRY(spline_upsample2(ctx, q.acc_u, q.acc_v, q.ph, q.pw, out_u, out_v, h, w, sx, sy));
                 } else {
                     OFRI_HIP(hipMemcpyAsync(out_u, q.acc_u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
                     OFRI_HIP(hipMemcpyAsync(out_v, q.acc_v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));

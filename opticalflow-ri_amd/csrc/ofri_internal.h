@@ -33,15 +33,35 @@ void clear_error();
     } while (0)
 
 // Device-side control word of a solver run with a convergence test.
-struct SolveCtrl {
+// Stop-test state of a Jacobi solve as one launch hands it to the next: the
+// decision of every test evaluated so far, plus the launch whose per-sweep
+// partial sums are still untested (pend_nsw > 0).
+struct SolveState {
     int done;        // 1 once the stop condition held or the sweep budget ran out
     int sweeps;      // sweeps taken so far
     int final_buf;   // ping-pong index holding the result
-    int redo;        // >0: the stopping pass must be re-run with this many sweeps
-    int redo_in;     // ping-pong index the redo reads
+    int redo;        // >0: the stopping launch must be re-run with this many sweeps
+    int redo_in;     // ping-pong index the re-run reads
     float last_err;  // last update norm (float32, like the reference)
-    int arrived;     // workgroups of the current launch that have finished
     int limit;       // >0: the solve also stops at this many sweeps (a later phase's budget)
+    int pend_nsw;    // sweeps of the untested launch (0: none)
+    int pend_dst;    // its output buffer
+    int pend_s0;     // its first tested sweep (earlier ones are counted, not tested)
+};
+
+// Control word of one solve.  The leading fields are the published result
+// (written by a phase's final launch); launch j of a phase reads slot[(j+1)&1]
+// and its workgroup 0 writes slot[j&1], so no launch reads what it writes.
+struct SolveCtrl {
+    int done;
+    int sweeps;
+    int final_buf;
+    int redo;
+    int redo_in;
+    float last_err;
+    int limit;
+    int pad_;
+    SolveState slot[2];
 };
 
 // A device allocation owned by the context, grown on demand.
@@ -59,6 +79,7 @@ struct ResizeTable {
 struct SplineTable {
     int n, N, width;
     double* d_w = nullptr;  // N x width
+    double* d_wT = nullptr; // width x N (the same weights transposed)
     int* d_start = nullptr;  // N
 };
 
@@ -95,6 +116,9 @@ int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out);
 // ---- image kernels (k_image.hip) ----
 int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n);
 int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter);
+// U and V in one pair of launches (src_v == nullptr: one plane)
+int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int Hc, int Wc, float* dst_u,
+                     float* dst_v, int H, int W, float scale_u, float scale_v);
 int spline_upsample(ofri_ctx* ctx, const float* src, int Hc, int Wc, float* dst, int H, int W,
                     float scale);
 int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const float* U,

@@ -7,97 +7,134 @@
 namespace ofri {
 namespace {
 
-// Test of one launch of `nsw` fused sweeps: `while err > tol and k < maxnum`
-// (sweeps before s0 carry no partial sums: they are counted, not tested)
-// with err = (||du||_2 + ||dv||_2) / (H*W) (the reference's Liu-Shen update
-// norm, PhysicsBasedOpticalFlowLiuShen.py:79,141; the same residual is used
-// for Horn-Schunck when a tolerance is given).  Run by the last workgroup of
-// the launch to finish, with all its threads: the per-sweep squared-update
-// sums of every workgroup are added in a fixed order, the float32 norm of
-// each sweep is tested in turn (and recorded in `hist` when given), and the
-// control word records the sweep count, the last norm, the buffer holding
-// the result and, when the test held before the launch's last sweep (or at
-// any sweep, with always_redo), how many sweeps the stopping launch must be
-// re-run with.
-__device__ void solve_check(const double* part, int nblk, int nsw, int dst, double tol, int maxnum,
-                            size_t N, SolveCtrl* ctrl, float* hist, bool always_redo, int s0) {
-    // one pass: wavefront q sums sweeps q, q + nwaves, ... over the
-    // workgroups (lane-strided, then a fixed butterfly), thread 0 then tests
-    // the sweeps in order
-    __shared__ double tot[64][2];
+// Stop test at the start of a launch.  Every workgroup of launch j reads the
+// state launch j-1 published (slot `prev`) and, when that launch is pending,
+// reduces its per-sweep partial sums (part_prev: [nsw][nblk][2] doubles, the
+// squared updates of u and v per workgroup) itself, in one fixed order, so
+// that all workgroups reach the same decision with no synchronisation
+// between them.  The test is `while err > tol and k < maxnum` with
+// err = (||du||_2 + ||dv||_2) / (H*W) (the reference's Liu-Shen update norm,
+// PhysicsBasedOpticalFlowLiuShen.py:79,141; the same residual serves
+// Horn-Schunck when a tolerance is given), evaluated sweep by sweep in float32
+// (recorded in `hist` when given).  When it holds before the pending launch's
+// last sweep (or at any sweep, with always_redo) the state asks for that
+// launch to be re-run with the sweeps up to the stopping one.  Returns the
+// updated state, the same in every thread; nsw <= 32.
+__device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* part_prev, int nblk, double tol,
+                                int maxnum, size_t N, float* hist, bool always_redo) {
+    __shared__ double tot[32][2];
+    __shared__ SolveState sh;
+    SolveState s = ctrl->slot[prev];
+    const bool ev = !s.done && s.pend_nsw > 0;
     const int T = blockDim.x * blockDim.y;
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = t & 63, wq = t >> 6, nw = (T + 63) >> 6;
-    for (int s = s0 + wq; s < nsw; s += nw) {
-        double a = 0.0, b = 0.0;
-        for (int i = lane; i < nblk; i += 64) {
-            // other workgroups' sums: read at device scope, past this CU's cache
-            a = a + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            b = b + __hip_atomic_load(part + ((size_t)s * nblk + i) * 2 + 1, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            a = a + __shfl_xor(a, off);
-            b = b + __shfl_xor(b, off);
-        }
-        if (lane == 0) {
-            tot[s][0] = a;
-            tot[s][1] = b;
+    if (ev) {
+        // wavefront q sums sweeps q, q + nw, ... over the workgroups
+        // (lane-strided, then a fixed butterfly)
+        for (int k = s.pend_s0 + wq; k < s.pend_nsw; k += nw) {
+            double a = 0.0, b = 0.0;
+            for (int i = lane; i < nblk; i += 64) {
+                a = a + part_prev[((size_t)k * nblk + i) * 2];
+                b = b + part_prev[((size_t)k * nblk + i) * 2 + 1];
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                a = a + __shfl_xor(a, off);
+                b = b + __shfl_xor(b, off);
+            }
+            if (lane == 0) {
+                tot[k][0] = a;
+                tot[k][1] = b;
+            }
         }
     }
     __syncthreads();
-    if (t != 0) return;
-    ctrl->sweeps += s0;   // sweeps run but not reduced
-    for (int s = s0; s < nsw; ++s) {
-        float err = (float)((sqrt(tot[s][0]) + sqrt(tot[s][1])) / (double)N);
-        if (hist) hist[ctrl->sweeps] = err;
-        ctrl->sweeps += 1;
-        ctrl->last_err = err;
-        ctrl->final_buf = dst;
-        if (!((double)err > tol)) {
-            ctrl->done = 1;
-            if (s + 1 < nsw || always_redo) {
-                ctrl->redo = s + 1;
-                ctrl->redo_in = dst ^ 1;
+    if (t == 0) {
+        if (ev) {
+            const int nsw = s.pend_nsw, dst = s.pend_dst;
+            s.sweeps += s.pend_s0;   // sweeps run but not reduced
+            for (int k = s.pend_s0; k < nsw; ++k) {
+                float err = (float)((sqrt(tot[k][0]) + sqrt(tot[k][1])) / (double)N);
+                if (hist) hist[s.sweeps] = err;
+                s.sweeps += 1;
+                s.last_err = err;
+                s.final_buf = dst;
+                if (!((double)err > tol)) {
+                    s.done = 1;
+                    if (k + 1 < nsw || always_redo) {
+                        s.redo = k + 1;
+                        s.redo_in = dst ^ 1;
+                    }
+                    break;
+                }
+                if (s.sweeps >= maxnum || (s.limit > 0 && s.sweeps >= s.limit)) {
+                    s.done = 1;
+                    break;
+                }
             }
-            return;
         }
-        if (ctrl->sweeps >= maxnum || (ctrl->limit > 0 && ctrl->sweeps >= ctrl->limit)) {
-            ctrl->done = 1;
-            return;
-        }
+        s.pend_nsw = 0;
+        sh = s;
     }
+    __syncthreads();
+    return sh;
 }
 
-// Called by every workgroup of an image (nblk of them) at the end of a launch
-// with per-sweep partial sums: the last one to arrive runs solve_check and
-// re-arms the counter.
-__device__ __forceinline__ void arrive_and_check(const double* part, int nblk, int nsw, int dst,
-                                                 double tol, int maxnum, size_t N, SolveCtrl* ctrl,
-                                                 float* hist, bool always_redo = false, int s0 = 0) {
-    __shared__ int last;
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        __threadfence();
-        last = atomicAdd(&ctrl->arrived, 1) == nblk - 1;
+// A launch's sweep count, input buffer and first global sweep in residual
+// mode.  Pass provides ctrl, part (two slots of part_stride doubles), slot
+// (launch parity within the phase), final_, nsweeps, src, sweep0,
+// check_last, tol, maxnum, N and hist.  A regular launch runs unless the
+// solve is done, and its workgroup 0 publishes the state with this launch
+// pending; the phase's final launch (final_) runs only the re-run of a
+// launch whose test held inside it, and publishes the result.
+template <class Pass>
+__device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bool always_redo, int& nsw, int& src,
+                                            int& base) {
+    const int prev = a.slot ^ 1;
+    SolveState s = take_test(a.ctrl, prev, a.part + (size_t)prev * a.part_stride, nblk, a.tol, a.maxnum, a.N,
+                             blk == 0 ? a.hist : nullptr, always_redo);
+    const bool lead = blk == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+    if (a.final_) {
+        nsw = s.done ? s.redo : 0;
+        src = s.redo_in;
+        base = s.sweeps - nsw;
+        if (lead) {
+            SolveCtrl* c = a.ctrl;
+            c->done = s.done;
+            c->sweeps = s.sweeps;
+            c->final_buf = s.final_buf;
+            c->redo = s.redo;
+            c->redo_in = s.redo_in;
+            c->last_err = s.last_err;
+            c->limit = s.limit;
+        }
+        return nsw > 0;
     }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        solve_check(part, nblk, nsw, dst, tol, maxnum, N, ctrl, hist, always_redo, s0);
-        if (threadIdx.x == 0 && threadIdx.y == 0) ctrl->arrived = 0;
+    nsw = a.nsweeps;
+    src = a.src;
+    base = a.sweep0;
+    const bool runs = !s.done;
+    if (lead) {
+        if (runs) {
+            s.pend_nsw = nsw;
+            s.pend_dst = src ^ 1;
+            s.pend_s0 = a.check_last ? nsw - 1 : 0;
+        }
+        a.ctrl->slot[a.slot] = s;
     }
+    return runs;
 }
 
 // Re-arms a finished solve for a further phase of at most `budget` sweeps
 // on the same sweep count and history (the phase starts unless the overall
 // budget maxnum is spent).
 __global__ void ctrl_continue_k(SolveCtrl* c, int maxnum, int budget) {
-    c->done = c->sweeps >= maxnum;
-    c->limit = c->sweeps + budget;
-    c->final_buf = 0;
-    c->redo = 0;
-    c->redo_in = 0;
-    c->arrived = 0;
+    SolveState s{};
+    s.done = c->sweeps >= maxnum;
+    s.sweeps = c->sweeps;
+    s.last_err = c->last_err;
+    s.limit = c->sweeps + budget;
+    c->slot[1] = s;
 }
 
 // Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
@@ -134,8 +171,9 @@ __global__ void ctrl_init_k(SolveCtrl* c) {
     c->redo = 0;
     c->redo_in = 0;
     c->last_err = 0.0f;
-    c->arrived = 0;
     c->limit = 0;
+    c->slot[0] = SolveState{};
+    c->slot[1] = SolveState{};
 }
 
 }  // namespace

@@ -122,9 +122,10 @@ def render_pair(H, W, family, density, seed, par, modes=None, noise=2.0, im1=Non
     return (im1, im2, u, v) if truth else (im1, im2)
 
 
-def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0):
+def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0, launch_test=False):
     """Chebyshev-accelerated Horn-Schunck to the plain residual tol
-    (ofri_hs_accel; finish > 1 ends with plain sweeps from finish * tol);
+    (ofri_hs_accel; finish > 1 ends with plain sweeps from finish * tol;
+    launch_test: stop test at each launch's last sweep only);
     U, V in place.  Returns (err, sweeps)."""
     im1, im2 = as_device(im1), as_device(im2)
     if hist is not None and hist.nbytes // 4 < maxnum:
@@ -134,7 +135,7 @@ def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0):
     _capi.check(_lib().ofri_hs_accel(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(alpha)), int(maxnum),
                                      float(tol), float(bound), float(finish), U.ptr, V.ptr,
                                      hist.ptr if hist is not None else None,
-                                     C.byref(err), C.byref(it)))
+                                     C.byref(err), C.byref(it), int(bool(launch_test))))
     return np.float32(err.value), it.value
 
 

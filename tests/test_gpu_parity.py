@@ -380,6 +380,81 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     return U, V, len(hist), np.array(hist)
 
 
+def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
+    """_cheb_restated with the stop test only at the last sweep of each
+    launch of `depth` sweeps (launches counted from each phase's start): the
+    accelerated phase ends with the iterate its stopping launch produced, the
+    plain phase (at most maxnum // 2 sweeps) with T(x_k).  Returns (U, V,
+    sweeps, last tested residual)."""
+    t1 = tol * finish if finish > 1 else tol
+    lo, hi = -1.0 / 3.0, float(np.float32(bound))
+    sig = (hi - lo) / (2.0 - lo - hi)
+    g = np.float32(2.0 / (2.0 - lo - hi))
+    om, w = [], 1.0
+    for k in range(maxnum):
+        w = 1.0 if k == 0 else (2.0 / (2.0 - sig * sig) if k == 1 else 1.0 / (1.0 - 0.25 * sig * sig * w))
+        om.append(np.float32(w))
+    fx, fy, ft = orc.hs_derivs(im1, im2)
+    U = np.zeros(im1.shape, np.float32); V = np.zeros(im1.shape, np.float32)
+    UP, VP = U.copy(), V.copy()
+
+    def res(du, dv):
+        return np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) +
+                           np.sqrt((dv.astype(np.float64) ** 2).sum())) / U.size)
+    count, r, stopped = 0, None, False
+    for k in range(maxnum):
+        Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        du, dv = Tu - U, Tv - V
+        tested = k % depth == depth - 1 or k == maxnum - 1
+        if tested:
+            r = res(du, dv)
+        nu = UP + om[k] * ((U + g * du) - UP)
+        nv = VP + om[k] * ((V + g * dv) - VP)
+        UP, VP, U, V = U, V, nu, nv
+        count = k + 1
+        if tested and not r > t1:
+            stopped = True
+            break
+    if finish > 1 and stopped:
+        fin = max(1, maxnum // 2)
+        for j in range(fin):
+            if count >= maxnum:
+                break
+            Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+            tested = j % depth == depth - 1 or j == fin - 1
+            if tested:
+                r = res(Tu - U, Tv - V)
+            U, V = Tu, Tv
+            count += 1
+            if tested and not r > tol:
+                break
+    return U, V, count, r
+
+
+@pytest.mark.parametrize("depth,finish", [(4, 0.0), (3, 4.0), (10, 4.0)])
+def test_hs_accel_launch_test_matches_restated_iteration(ops, depth, finish):
+    """ofri_hs_accel with the stop test once per launch (the learned form's
+    mode) equals its restatement over the oracle's sweep bit for bit."""
+    from ofri.device import DeviceArray, context
+    H, W = 96, 120
+    im1, im2 = _piv(H, W, seed=41)
+    plain_hist, _, _ = _hs_norm_history(im1, im2, 45.0, 200)
+    tol = float(plain_hist[199])
+    Ur, Vr, kr, rr = _cheb_restated_launches(im1, im2, 45.0, 0.99, tol, 200, finish, depth)
+    assert kr < 150, kr
+    ctx = context()
+    ctx.set_tuning(hs_depth=depth)
+    try:
+        U = DeviceArray.zeros((H, W)); V = DeviceArray.zeros((H, W))
+        err, it = ops.hs_accel(im1, im2, 45.0, 200, tol, 0.99, U, V, finish=finish, launch_test=True)
+    finally:
+        ctx.set_tuning()
+    print("[hs accel per-launch test, depth %d, finish %g] %d sweeps" % (depth, finish, it))
+    assert it == kr
+    assert abs(float(err) - float(rr)) <= 1e-6 * float(rr)
+    assert np.array_equal(U.download(), Ur) and np.array_equal(V.download(), Vr)
+
+
 @pytest.mark.parametrize("depth,finish", [(0, 0.0), (4, 0.0), (3, 4.0)])
 def test_hs_accel_matches_restated_iteration(ops, depth, finish):
     """ofri_hs_accel equals the Chebyshev iteration restated over the oracle's
