@@ -299,7 +299,8 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
  * net NULL: Liu-Shen from HS's flow alone (the method without the network).  Both solvers test
  * their targets once per fused launch (at its last sweep).  *h_converged (may be
  * NULL) is 1 when every level met its targets; otherwise the caller falls
- * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries.  Levels whose
+ * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries (hs_err, the
+ * plain HS's change-from-start value, is not formed: 0).  Levels whose
  * size is not a multiple of 16 skip the network.  Repeated calls replay a
  * HIP graph.  No counterpart in the reference. */
 typedef struct ofri_learned_params {

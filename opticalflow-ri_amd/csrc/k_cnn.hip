@@ -469,26 +469,35 @@ __global__ void zero_f_k(float* p, long long n) {
     if (i < n) p[i] = 0.0f;
 }
 
-// Network input: B x P planar float32 H x W planes -> B x (H/2) x (W/2) x 4P
-// bfloat16, channel = (2 * dy + dx) * P + plane.  Planes 0, 1 are frames,
-// scaled to (I - 128) / 64; planes 2, 3 are flow components in pixels.
-__global__ void net_input_k(const float* in, int P, int H, int W, u16* x) {
+// The network's input planes: plane p of batch item b starts at p[p] + b *
+// bstride (a stacked [B][P][H][W] tensor, or separate planes when B = 1).
+struct InPlanes {
+    const float* p[4];
+    long long bstride;
+};
+
+// Input planes -> space-to-depth bf16 NHWC at H/2 x W/2 (4P channels,
+// channel = sub * P + plane, sub = 2 * dy + dx); planes 0, 1 (frames) scaled
+// to (I - 128) / 64, the others (flow) as they are.
+template <int P>
+__global__ void net_input_k(InPlanes in, int H, int W, u16* x) {
     const int h = H / 2, w = W / 2;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (i >= (long long)h * w) return;
     const int y = (int)(i / w), xx = (int)(i - (long long)y * w);
-    const long long HW = (long long)H * W;
-    u16 out[16];
+    u16 out[4 * P];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const long long q = (long long)(2 * y + (s >> 1)) * W + 2 * xx + (s & 1);
+#pragma unroll
         for (int pl = 0; pl < P; ++pl) {
-            const float v = in[((long long)b * P + pl) * HW + q];
+            const float v = in.p[pl][(long long)b * in.bstride + q];
             out[s * P + pl] = f2bf(pl < 2 ? (v - 128.0f) * (1.0f / 64.0f) : v);
         }
     }
     u16* dst = x + ((long long)b * h * w + i) * 4 * P;
+#pragma unroll
     for (int c = 0; c < 4 * P; c += 8) {
         uint4 r;
         r.x = out[c] | ((unsigned)out[c + 1] << 16);
@@ -496,6 +505,22 @@ __global__ void net_input_k(const float* in, int P, int H, int W, u16* x) {
         r.z = out[c + 4] | ((unsigned)out[c + 5] << 16);
         r.w = out[c + 6] | ((unsigned)out[c + 7] << 16);
         *(uint4*)(dst + c) = r;
+    }
+}
+
+// The fine head's fields 0, 1 (float32 NHWC at H/2, channel = sub * 4 +
+// field) added into two full-size planes: u += field 0, v += field 1.
+__global__ void fine_add_k(const float* __restrict__ hf, int h, int w, float* __restrict__ u,
+                           float* __restrict__ v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)h * w) return;
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    const int W = 2 * w;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const long long q = (long long)(2 * y + (s >> 1)) * W + 2 * x + (s & 1);
+        u[q] = u[q] + hf[i * 16 + s * 4];
+        v[q] = v[q] + hf[i * 16 + s * 4 + 1];
     }
 }
 
@@ -732,14 +757,19 @@ int launch_conv(ofri_ctx* ctx, const ConvArgs& a) {
     return OFRI_OK;
 }
 
-int net_forward(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W) {
+// coarse_head false: the coarse head (layer head_c) is skipped.
+int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, bool coarse_head = true) {
     OFRI_TRY(net_shape(ctx, n, B, H, W));
     const long long hw = (long long)(H / 2) * (W / 2);
-    net_input_k<<<dim3((unsigned)((hw + 255) / 256), B), 256, 0, ctx->stream>>>(
-        in, n->planes, H, W, (u16*)((char*)n->act + n->t_off[T_X0]));
+    const dim3 g((unsigned)((hw + 255) / 256), B);
+    u16* x0 = (u16*)((char*)n->act + n->t_off[T_X0]);
+    if (n->planes == 4) net_input_k<4><<<g, 256, 0, ctx->stream>>>(in, H, W, x0);
+    else if (n->planes == 2) net_input_k<2><<<g, 256, 0, ctx->stream>>>(in, H, W, x0);
+    else return fail(OFRI_E_ARG, "net: %d input planes not supported", n->planes);
     OFRI_HIP(hipGetLastError());
     for (int l = 0; l < kNumLayers; ++l) {
         const LayerDef& d = n->def[l];
+        if (!coarse_head && d.dst == T_HC) continue;
         const LayerParams& P = n->L[l];
         ConvArgs a{};
         a.x = tview(n, n->act, d.src, d.src_c0, d.src_mode);
@@ -939,9 +969,17 @@ static int unpack(ofri_ctx* ctx, Net* n, const float* d, float* h, size_t count)
 int net_get_params(ofri_ctx* ctx, Net* n, float* h, size_t count) { return unpack(ctx, n, n->par, h, count); }
 int net_get_grads(ofri_ctx* ctx, Net* n, float* h, size_t count) { return unpack(ctx, n, n->grad, h, count); }
 
+InPlanes stacked(const Net* n, const float* in, int H, int W) {
+    InPlanes r{};
+    const long long HW = (long long)H * W;
+    for (int i = 0; i < n->planes && i < 4; ++i) r.p[i] = in + i * HW;
+    r.bstride = n->planes * HW;
+    return r;
+}
+
 // Forward pass; planar outputs: coarse [B][4][H/2][W/2], fine [B][4][H][W].
 int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, float* coarse, float* fine) {
-    OFRI_TRY(net_forward(ctx, n, in, B, H, W));
+    OFRI_TRY(net_forward(ctx, n, stacked(n, in, H, W), B, H, W));
     const int h = H / 2, w = W / 2;
     heads_to_planes_k<<<grid1((long long)B * h * w), 256, 0, ctx->stream>>>(
         (const float*)((char*)n->act + n->t_off[T_HC]), (const float*)((char*)n->act + n->t_off[T_HF]), B, h, w,
@@ -950,11 +988,26 @@ int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, flo
     return OFRI_OK;
 }
 
+// One H x W input given as separate planes (planes[0 .. planes-1]); the fine
+// head's fields 0, 1 are added into add_u, add_v (full size).  The coarse
+// head is not computed.
+int net_correct(ofri_ctx* ctx, Net* n, const float* const* planes, int H, int W, float* add_u, float* add_v) {
+    InPlanes in{};
+    for (int i = 0; i < n->planes && i < 4; ++i) in.p[i] = planes[i];
+    in.bstride = 0;
+    OFRI_TRY(net_forward(ctx, n, in, 1, H, W, false));
+    const int h = H / 2, w = W / 2;
+    fine_add_k<<<grid1((long long)h * w), 256, 0, ctx->stream>>>((const float*)((char*)n->act + n->t_off[T_HF]), h,
+                                                                 w, add_u, add_v);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
 // Forward, loss against planar labels, backward; with lr > 0 an Adam step.
 // d_loss: one float on the device, overwritten.
 int net_train(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, const float* lab_c, const float* lab_f,
               const float* h_field_w, float lr, float* d_loss) {
-    OFRI_TRY(net_forward(ctx, n, in, B, H, W));
+    OFRI_TRY(net_forward(ctx, n, stacked(n, in, H, W), B, H, W));
     const int h = H / 2, w = W / 2;
     void* fwb;
     OFRI_TRY(scratch(ctx, "net_fw", 8 * sizeof(float), &fwb));

@@ -172,8 +172,9 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         U[r] = uin[q];
         V[r] = vin[q];
         if (CHEB) {
-            UP[r] = a.pin[src][0][q];
-            VP[r] = a.pin[src][1][q];
+            // x_{-1} = x_0 before the first sweep
+            UP[r] = base == 0 ? U[r] : a.pin[src][0][q];
+            VP[r] = base == 0 ? V[r] : a.pin[src][1][q];
         }
         FX[r] = a.fx[q];
         FY[r] = a.fy[q];
@@ -444,18 +445,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    if (d_ctrl) {
-        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
-        OFRI_HIP(hipGetLastError());
-    }
     // spectrum of one sweep's linear part lies in [-1/3, b]: the 8-neighbour
     // average has eigenvalues in [-1/3, 1] and the per-pixel data projection
-    // only shrinks them; b < 1 is the caller's bound on the slowest mode
+    // only shrinks them; b < 1 is the caller's bound on the slowest mode.
+    // (The first accelerated launch takes the previous iterate from U, V.)
     const double lo = -1.0 / 3.0, hi = cheb_b;
     if (cheb) {
-        OFRI_HIP(hipMemcpyAsync(P0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        OFRI_HIP(hipMemcpyAsync(P0 + N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        if (niter > 0) cheb_omega_k<<<1, 256, 0, ctx->stream>>>(cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, niter);
+        solve_start_k<<<1, 256, 0, ctx->stream>>>(d_ctrl, cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, niter);
+        OFRI_HIP(hipGetLastError());
+    } else if (d_ctrl) {
+        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
         OFRI_HIP(hipGetLastError());
     }
     HsPass p{};
@@ -506,8 +505,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
                 p.final_ = 1;
                 kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
                 OFRI_HIP(hipGetLastError());
-                OFRI_TRY(select_copy(ctx, d_ctrl, U, Ub, U, N));
-                OFRI_TRY(select_copy(ctx, d_ctrl, V, Vb, V, N));
+                OFRI_TRY(select_copy2(ctx, d_ctrl, U, Ub, U, V, Vb, V, N));
             }
         } else if (src != 0) {
             OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));

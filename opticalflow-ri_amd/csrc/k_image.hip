@@ -26,12 +26,18 @@ __device__ __forceinline__ int pf_index(int p, int L, int h) {
 }
 
 // Row pass: out[m] = sum_j w[j] * line(m + h - j), float32, tap order, from 0.
-__global__ void gauss_rows(const float* __restrict__ src, float* __restrict__ dst, int H, int W,
-                           Taps t) {
+// Up to two planes per launch (blockIdx.z): source s[z] -> destination d[z].
+struct Planes2 {
+    const float* s[2];
+    float* d[2];
+};
+
+__global__ void gauss_rows(Planes2 p, int H, int W, Taps t) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (x >= W) return;
-    const float* row = src + (size_t)y * W;
+    float* __restrict__ dst = p.d[blockIdx.z];
+    const float* row = p.s[blockIdx.z] + (size_t)y * W;
     int h = t.n / 2;
     float acc = 0.0f;
     for (int j = 0; j < t.n; ++j) {
@@ -42,11 +48,12 @@ __global__ void gauss_rows(const float* __restrict__ src, float* __restrict__ ds
 }
 
 // Column pass on the row-filtered image.
-__global__ void gauss_cols(const float* __restrict__ src, float* __restrict__ dst, int H, int W,
-                           Taps t) {
+__global__ void gauss_cols(Planes2 p, int H, int W, Taps t) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (x >= W) return;
+    const float* __restrict__ src = p.s[blockIdx.z];
+    float* __restrict__ dst = p.d[blockIdx.z];
     int h = t.n / 2;
     float acc = 0.0f;
     for (int j = 0; j < t.n; ++j) {
@@ -57,12 +64,13 @@ __global__ void gauss_cols(const float* __restrict__ src, float* __restrict__ ds
 }
 
 // Pillow resample, horizontal pass over the rows [first, first+Ht).
-__global__ void resize_h(const float* __restrict__ src, int W, float* __restrict__ tmp, int W2,
-                         int Ht, int first, const double* __restrict__ k, const int* __restrict__ b,
-                         int ks) {
+__global__ void resize_h(Planes2 p, int W, int W2, int Ht, int first, const double* __restrict__ k,
+                         const int* __restrict__ b, int ks) {
     int xx = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (xx >= W2 || y >= Ht) return;
+    const float* __restrict__ src = p.s[blockIdx.z];
+    float* __restrict__ tmp = p.d[blockIdx.z];
     int xmin = b[2 * xx], xn = b[2 * xx + 1];
     const double* kk = k + (size_t)xx * ks;
     const float* row = src + (size_t)(y + first) * W + xmin;
@@ -71,11 +79,13 @@ __global__ void resize_h(const float* __restrict__ src, int W, float* __restrict
     tmp[(size_t)y * W2 + xx] = (float)ss;
 }
 
-__global__ void resize_v(const float* __restrict__ tmp, int W2, float* __restrict__ dst, int H2,
-                         int first, const double* __restrict__ k, const int* __restrict__ b, int ks) {
+__global__ void resize_v(Planes2 p, int W2, int H2, int first, const double* __restrict__ k,
+                         const int* __restrict__ b, int ks) {
     int xx = blockIdx.x * blockDim.x + threadIdx.x;
     int yy = blockIdx.y;
     if (xx >= W2 || yy >= H2) return;
+    const float* __restrict__ tmp = p.s[blockIdx.z];
+    float* __restrict__ dst = p.d[blockIdx.z];
     int ymin = b[2 * yy] - first, yn = b[2 * yy + 1];
     const double* kk = k + (size_t)yy * ks;
     double ss = 0.0;
@@ -97,6 +107,8 @@ struct SplinePlanes {
 
 constexpr int kSplineRowsX = 8;    // coarse rows per workgroup of the x pass
 constexpr int kSplineRowsY = 8;    // output rows per thread of the y pass
+constexpr int kSplineChunk = 8;    // weights in flight per thread of the x pass
+constexpr int kSplineMaxWidth = 66;  // band width of the weight tables (2 * 32 + 2)
 
 // x pass: kSplineRowsX coarse rows staged in LDS (nrows * Wc floats); the
 // weights are read transposed (wT[t][x]) so a wavefront's loads are
@@ -116,11 +128,19 @@ __global__ __launch_bounds__(256) void spline_x(SplinePlanes p, int Wc, int W, i
     double s[kSplineRowsX];
 #pragma unroll
     for (int i = 0; i < kSplineRowsX; ++i) s[i] = 0.0;
-    for (int t = 0; t < width; ++t) {
-        const double w = wT[(size_t)t * W + x];
+    // weights fetched kSplineChunk at a time so their loads overlap
+    for (int t0 = 0; t0 < width; t0 += kSplineChunk) {
+        double wc[kSplineChunk];
 #pragma unroll
-        for (int i = 0; i < kSplineRowsX; ++i)
-            if (i < nr) s[i] = s[i] + w * (double)b[i * Wc + t];
+        for (int k = 0; k < kSplineChunk; ++k) wc[k] = t0 + k < width ? wT[(size_t)(t0 + k) * W + x] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kSplineChunk; ++k) {
+            if (t0 + k < width) {
+#pragma unroll
+                for (int i = 0; i < kSplineRowsX; ++i)
+                    if (i < nr) s[i] = s[i] + wc[k] * (double)b[i * Wc + t0 + k];
+            }
+        }
     }
     double* tmp = p.tmp[z] + (size_t)r0 * W + x;
 #pragma unroll
@@ -133,11 +153,15 @@ __global__ __launch_bounds__(256) void spline_x(SplinePlanes p, int Wc, int W, i
 // the outputs whose band holds it (in band order).
 __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, const double* __restrict__ w,
                                                 const int* __restrict__ st, int width) {
+    // the weights of this workgroup's output rows, staged in LDS
+    __shared__ double ws[kSplineRowsY][kSplineMaxWidth];
     const int z = blockIdx.z;
     const int x = blockIdx.x * 256 + threadIdx.x;
     const int y0 = blockIdx.y * kSplineRowsY;
-    if (x >= W) return;
     const int ny = min(kSplineRowsY, H - y0);
+    for (int i = threadIdx.x; i < ny * width; i += 256) ws[i / width][i % width] = w[(size_t)y0 * width + i];
+    __syncthreads();
+    if (x >= W) return;
     const int j0 = st[y0], j1 = st[y0 + ny - 1] + width;
     const double* tmp = p.tmp[z] + x;
     double s[kSplineRowsY];
@@ -147,12 +171,13 @@ __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, co
         s[i] = 0.0;
         sj[i] = i < ny ? st[y0 + i] : 0x3fffffff;
     }
+#pragma unroll 4
     for (int j = j0; j < j1; ++j) {
         const double v = tmp[(size_t)j * W];
 #pragma unroll
         for (int i = 0; i < kSplineRowsY; ++i) {
             const int t = j - sj[i];
-            if (t >= 0 && t < width) s[i] = s[i] + w[(size_t)(y0 + i) * width + t] * v;
+            if (t >= 0 && t < width) s[i] = s[i] + ws[i][t] * v;
         }
     }
     float* dst = p.dst[z] + x;
@@ -212,7 +237,10 @@ __global__ void warp_coords_k(const float* __restrict__ img, const float* __rest
     out[i] = warp_px(img, H, W, cy[i], cx[i]);
 }
 
-__global__ void accumulate_k(float* __restrict__ acc, const float* __restrict__ x, size_t n) {
+// d[z] += s[z]
+__global__ void accumulate_k(Planes2 p, size_t n) {
+    float* __restrict__ acc = p.d[blockIdx.z];
+    const float* __restrict__ x = p.s[blockIdx.z];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) acc[i] = acc[i] + x[i];
@@ -224,16 +252,24 @@ __global__ void sub_k(float* __restrict__ d, const float* __restrict__ a, const 
     for (; i < n; i += stride) d[i] = a[i] - b[i];
 }
 
-__global__ void fill_zero_k(float* __restrict__ p, size_t n) {
+__global__ void fill_zero_k(Planes2 q, size_t n) {
+    float* __restrict__ p = q.d[blockIdx.z];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = 0.0f;
 }
 
-// s0 may alias d (the result may already be in place).
-__global__ void select_copy_k(const SolveCtrl* c, const float* s0, const float* s1, float* d,
-                              size_t n) {
-    const float* s = c->final_buf ? s1 : s0;
+// d = c->final_buf ? s1 : s0 for plane blockIdx.z; s0 may alias d (the
+// result may already be in place).
+struct Select2 {
+    const float* s0[2];
+    const float* s1[2];
+    float* d[2];
+};
+__global__ void select_copy_k(const SolveCtrl* c, Select2 p, size_t n) {
+    const int z = blockIdx.z;
+    const float* s = c->final_buf ? p.s1[z] : p.s0[z];
+    float* d = p.d[z];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) d[i] = s[i];
@@ -243,8 +279,10 @@ constexpr int kRedBlocks = 512;
 constexpr int kRedThreads = 256;
 
 // Stage 1 of a deterministic max: per-block maxima.
-__global__ void max_stage1(const float* __restrict__ x, size_t n, float* __restrict__ part) {
+__global__ void max_stage1(Planes2 p, size_t n, float* __restrict__ part_all) {
     __shared__ float sm[kRedThreads];
+    const float* __restrict__ x = p.s[blockIdx.z];
+    float* __restrict__ part = part_all + (size_t)blockIdx.z * gridDim.x;
     float m = -INFINITY;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -258,8 +296,10 @@ __global__ void max_stage1(const float* __restrict__ x, size_t n, float* __restr
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-__global__ void max_stage2(const float* __restrict__ part, int np, float* __restrict__ out) {
+__global__ void max_stage2(const float* __restrict__ part_all, int np, float* __restrict__ out_all) {
     __shared__ float sm[kRedThreads];
+    const float* __restrict__ part = part_all + (size_t)blockIdx.x * np;
+    float* __restrict__ out = out_all + blockIdx.x;
     float m = -INFINITY;
     for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmaxf(m, part[i]);
     sm[threadIdx.x] = m;
@@ -469,36 +509,55 @@ int get_spline_table(ofri_ctx* ctx, int n, int N, SplineTable** t) {
 
 }  // namespace
 
-int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n) {
+int gauss_sep2(ofri_ctx* ctx, const float* src1, float* dst1, const float* src2, float* dst2, int H, int W,
+               const float* w, int n) {
     if (n < 1 || n > kMaxTaps || (n & 1) == 0)
         return fail(OFRI_E_ARG, "gauss_sep: tap count %d must be odd and <= %d", n, kMaxTaps);
     if (H < n || W < n) return fail(OFRI_E_ARG, "gauss_sep: image %dx%d smaller than %d taps", H, W, n);
     Taps t{};
     for (int i = 0; i < n; ++i) t.w[i] = w[i];
     t.n = n;
+    const int planes = src2 ? 2 : 1;
     void* tmp;
-    OFRI_TRY(scratch(ctx, "gauss_tmp", (size_t)H * W * sizeof(float), &tmp));
-    gauss_rows<<<grid_rows(W, H), 256, 0, ctx->stream>>>(src, (float*)tmp, H, W, t);
-    gauss_cols<<<grid_rows(W, H), 256, 0, ctx->stream>>>((const float*)tmp, dst, H, W, t);
+    OFRI_TRY(scratch(ctx, "gauss_tmp", (size_t)planes * H * W * sizeof(float), &tmp));
+    float* t0 = (float*)tmp;
+    float* t1 = t0 + (size_t)H * W;
+    dim3 g((W + 255) / 256, H, planes);
+    gauss_rows<<<g, 256, 0, ctx->stream>>>(Planes2{{src1, src2}, {t0, t1}}, H, W, t);
+    gauss_cols<<<g, 256, 0, ctx->stream>>>(Planes2{{t0, t1}, {dst1, dst2}}, H, W, t);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }
 
-int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter) {
+int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n) {
+    return gauss_sep2(ctx, src, dst, nullptr, nullptr, H, W, w, n);
+}
+
+int resize2(ofri_ctx* ctx, const float* src1, float* dst1, const float* src2, float* dst2, int H, int W, int H2,
+            int W2, int filter) {
     if (filter != 2 && filter != 3) return fail(OFRI_E_ARG, "resize: filter %d not supported", filter);
     if (H < 1 || W < 1 || H2 < 1 || W2 < 1) return fail(OFRI_E_ARG, "resize: empty image");
     ResizeTable *th, *tv;
     OFRI_TRY(get_resize_table(ctx, W, W2, filter, &th));
     OFRI_TRY(get_resize_table(ctx, H, H2, filter, &tv));
     int Ht = tv->last - tv->first;
+    const int planes = src2 ? 2 : 1;
     void* tmp;
-    OFRI_TRY(scratch(ctx, "resize_tmp", (size_t)Ht * W2 * sizeof(float), &tmp));
-    resize_h<<<grid_rows(W2, Ht), 256, 0, ctx->stream>>>(src, W, (float*)tmp, W2, Ht, tv->first,
-                                                        th->d_k, th->d_bounds, th->ksize);
-    resize_v<<<grid_rows(W2, H2), 256, 0, ctx->stream>>>((const float*)tmp, W2, dst, H2, tv->first,
-                                                        tv->d_k, tv->d_bounds, tv->ksize);
+    OFRI_TRY(scratch(ctx, "resize_tmp", (size_t)planes * Ht * W2 * sizeof(float), &tmp));
+    float* t0 = (float*)tmp;
+    float* t1 = t0 + (size_t)Ht * W2;
+    resize_h<<<dim3((W2 + 255) / 256, Ht, planes), 256, 0, ctx->stream>>>(Planes2{{src1, src2}, {t0, t1}}, W, W2,
+                                                                          Ht, tv->first, th->d_k, th->d_bounds,
+                                                                          th->ksize);
+    resize_v<<<dim3((W2 + 255) / 256, H2, planes), 256, 0, ctx->stream>>>(Planes2{{t0, t1}, {dst1, dst2}}, W2, H2,
+                                                                          tv->first, tv->d_k, tv->d_bounds,
+                                                                          tv->ksize);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
+}
+
+int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter) {
+    return resize2(ctx, src, dst, nullptr, nullptr, H, W, H2, W2, filter);
 }
 
 int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int Hc, int Wc, float* dst_u,
@@ -511,6 +570,8 @@ int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int 
     SplineTable *tx, *ty;
     OFRI_TRY(get_spline_table(ctx, Wc, W, &tx));
     OFRI_TRY(get_spline_table(ctx, Hc, H, &ty));
+    if (tx->width > kSplineMaxWidth || ty->width > kSplineMaxWidth)
+        return fail(OFRI_E_STATE, "spline_upsample: weight band wider than %d", kSplineMaxWidth);
     const int planes = src_v ? 2 : 1;
     void* tmp;
     OFRI_TRY(scratch(ctx, "spline_tmp", (size_t)planes * Hc * W * sizeof(double), &tmp));
@@ -549,11 +610,15 @@ int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* c
     return OFRI_OK;
 }
 
-int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n) {
+int accumulate2(ofri_ctx* ctx, float* acc1, const float* x1, float* acc2, const float* x2, size_t n) {
     if (n == 0) return OFRI_OK;
-    accumulate_k<<<grid1d(n), 256, 0, ctx->stream>>>(acc, x, n);
+    accumulate_k<<<dim3(grid1d(n), 1, acc2 ? 2 : 1), 256, 0, ctx->stream>>>(Planes2{{x1, x2}, {acc1, acc2}}, n);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
+}
+
+int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n) {
+    return accumulate2(ctx, acc, x, nullptr, nullptr, n);
 }
 
 int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n) {
@@ -563,30 +628,44 @@ int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n) 
     return OFRI_OK;
 }
 
-int fill_zero(ofri_ctx* ctx, float* p, size_t n) {
+int fill_zero2(ofri_ctx* ctx, float* p1, float* p2, size_t n) {
     if (n == 0) return OFRI_OK;
-    fill_zero_k<<<grid1d(n), 256, 0, ctx->stream>>>(p, n);
+    fill_zero_k<<<dim3(grid1d(n), 1, p2 ? 2 : 1), 256, 0, ctx->stream>>>(Planes2{{nullptr, nullptr}, {p1, p2}}, n);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
+int fill_zero(ofri_ctx* ctx, float* p, size_t n) { return fill_zero2(ctx, p, nullptr, n); }
+
+int select_copy2(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* a0, const float* a1, float* ad,
+                 const float* b0, const float* b1, float* bd, size_t n) {
+    select_copy_k<<<dim3(grid1d(n), 1, bd ? 2 : 1), 256, 0, ctx->stream>>>(ctrl, Select2{{a0, b0}, {a1, b1}, {ad, bd}},
+                                                                            n);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }
 
 int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const float* s1, float* dst,
                 size_t n) {
-    select_copy_k<<<grid1d(n), 256, 0, ctx->stream>>>(ctrl, s0, s1, dst, n);
+    return select_copy2(ctx, ctrl, s0, s1, dst, nullptr, nullptr, nullptr, n);
+}
+
+int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out) {
+    const int planes = x2 ? 2 : 1;
+    void* part;
+    OFRI_TRY(scratch(ctx, "max_part", 2 * kRedBlocks * sizeof(float), &part));
+    int nb = (int)((n + kRedThreads - 1) / kRedThreads);
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb < 1) nb = 1;
+    max_stage1<<<dim3(nb, 1, planes), kRedThreads, 0, ctx->stream>>>(Planes2{{x1, x2}, {nullptr, nullptr}}, n,
+                                                                       (float*)part);
+    max_stage2<<<planes, kRedThreads, 0, ctx->stream>>>((const float*)part, nb, d_out);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }
 
 int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out) {
-    void* part;
-    OFRI_TRY(scratch(ctx, "max_part", kRedBlocks * sizeof(float), &part));
-    int nb = (int)((n + kRedThreads - 1) / kRedThreads);
-    if (nb > kRedBlocks) nb = kRedBlocks;
-    if (nb < 1) nb = 1;
-    max_stage1<<<nb, kRedThreads, 0, ctx->stream>>>(x, n, (float*)part);
-    max_stage2<<<1, kRedThreads, 0, ctx->stream>>>((const float*)part, nb, d_out);
-    OFRI_HIP(hipGetLastError());
-    return OFRI_OK;
+    return reduce_max2(ctx, x, nullptr, n, d_out);
 }
 
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,

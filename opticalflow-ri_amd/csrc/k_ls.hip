@@ -36,9 +36,21 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128,138 and :47-73).
 __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restrict__ im2, int H,
                            int W, const float* __restrict__ mx, float h, float* __restrict__ F,
-                           size_t N) {
+                           size_t N, SolveCtrl* ctrl) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
+    if (ctrl && i == 0 && j == 0) {
+        // the solve's control word starts cleared
+        ctrl->done = 0;
+        ctrl->sweeps = 0;
+        ctrl->final_buf = 0;
+        ctrl->redo = 0;
+        ctrl->redo_in = 0;
+        ctrl->last_err = 0.0f;
+        ctrl->limit = 0;
+        ctrl->slot[0] = SolveState{};
+        ctrl->slot[1] = SolveState{};
+    }
     if (j >= W) return;
     const float m1 = mx[0], m2 = mx[1];
     int im = clampi(i - 1, 0, H - 1), ip = clampi(i + 1, 0, H - 1);
@@ -442,11 +454,9 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* ub = F + 8 * N;
     float* vb = F + 9 * N;
     float* mx = (float*)mbuf;
-    OFRI_TRY(reduce_max(ctx, im1, N, mx));
-    OFRI_TRY(reduce_max(ctx, im2, N, mx + 1));
+    OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx));
     dim3 g((W + 255) / 256, H);
-    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N);
-    ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
+    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N, d_ctrl);
     OFRI_HIP(hipGetLastError());
 
     LsPass p{};
@@ -496,8 +506,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.slot = j & 1;
         p.final_ = 1;
         kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
-        OFRI_TRY(select_copy(ctx, d_ctrl, u, ub, u, N));
-        OFRI_TRY(select_copy(ctx, d_ctrl, v, vb, v, N));
+        OFRI_TRY(select_copy2(ctx, d_ctrl, u, ub, u, v, vb, v, N));
     }
     return OFRI_OK;
 }
@@ -554,11 +563,9 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         float *ub = F + 8 * N, *vb = F + 9 * N;
         f += 10 * N;
         float* mx = (float*)mbuf + 2 * i;
-        OFRI_TRY(reduce_max(ctx, im[i].im1, N, mx));
-        OFRI_TRY(reduce_max(ctx, im[i].im2, N, mx + 1));
+        OFRI_TRY(reduce_max2(ctx, im[i].im1, im[i].im2, N, mx));
         dim3 g((W + 255) / 256, H);
-        ls_setup_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, mx, h, F, N);
-        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrls + i);
+        ls_setup_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, mx, h, F, N, d_ctrls + i);
         OFRI_HIP(hipGetLastError());
         LsPass& p = tab[i];
         p = LsPass{};
@@ -597,8 +604,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         f = (float*)fbuf;
         for (int i = 0; i < n; ++i) {
             const size_t N = (size_t)im[i].H * im[i].W;
-            OFRI_TRY(select_copy(ctx, d_ctrls + i, im[i].U, f + 8 * N, im[i].U, N));
-            OFRI_TRY(select_copy(ctx, d_ctrls + i, im[i].V, f + 9 * N, im[i].V, N));
+            OFRI_TRY(select_copy2(ctx, d_ctrls + i, im[i].U, f + 8 * N, im[i].U, im[i].V, f + 9 * N, im[i].V, N));
             f += 10 * N;
         }
     }

@@ -357,16 +357,12 @@ namespace ofri {
 // sweeps also reduce their update norms (the last one lands in the stats).
 // lrn (may be null): the learned form.  Every level's HS runs accelerated
 // to the level's residual target; then the correction network maps the
-// level's Liu-Shen input pair and HS's flow (in_stack: 4 planes) to the
-// correction Liu-Shen would add, which seeds Liu-Shen (HS result plus the
-// correction, fine head fields 0, 1 in pred_f); Liu-Shen runs to its own
-// target.
+// level's Liu-Shen input pair and HS's flow (4 planes) to the correction
+// Liu-Shen would add (fine head fields 0, 1), which is added to HS's flow to
+// seed Liu-Shen; Liu-Shen runs to its own target.
 struct LearnedRun {
     const ofri_learned_params* lp;
     Net* net;               // null: Liu-Shen seeded with HS's flow alone
-    float* in_stack;        // 4 planes of the finest level's size
-    float* pred_c;          // coarse head output (unused)
-    float* pred_f;          // 4 planes of the finest level's size
 };
 
 static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
@@ -403,8 +399,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         const bool last = l == L - 1;
         const float *c1 = d_im1, *c2 = d_im2;
         if (h != H || w != W) {
-            OFRI_TRY(resize(ctx, d_im1, H, W, n1, h, w, 3));
-            OFRI_TRY(resize(ctx, d_im2, H, W, n2, h, w, 3));
+            OFRI_TRY(resize2(ctx, d_im1, n1, d_im2, n2, H, W, h, w, 3));
             c1 = n1;
             c2 = n2;
         }
@@ -424,20 +419,18 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             s1 = w1;
             s2 = w2;
         } else {
-            OFRI_TRY(fill_zero(ctx, out_u, N));
-            OFRI_TRY(fill_zero(ctx, out_v, N));
+            OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
         }
         if (p->filt_w && p->filt_n > 0) {
-            OFRI_TRY(gauss_sep(ctx, s1, f1, h, w, p->filt_w, p->filt_n));
-            OFRI_TRY(gauss_sep(ctx, s2, f2, h, w, p->filt_w, p->filt_n));
+            OFRI_TRY(gauss_sep2(ctx, s1, f1, s2, f2, h, w, p->filt_w, p->filt_n));
             s1 = f1;
             s2 = f2;
         }
-        OFRI_TRY(fill_zero(ctx, U, N));
-        OFRI_TRY(fill_zero(ctx, V, N));
+        OFRI_TRY(fill_zero2(ctx, U, V, N));
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
-            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, hs_errs + l, lp->hs_tol[l],
+            // (HS's own error value, the change from the initial flow, is not formed here)
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, nullptr, lp->hs_tol[l],
                               hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish, true));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
@@ -451,21 +444,14 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         if (p->ls_alpha > 0.0f) {
             const float *q1 = c1, *q2 = c2;
             if (p->opt_w && p->opt_n > 0) {
-                OFRI_TRY(gauss_sep(ctx, c1, o1, h, w, p->opt_w, p->opt_n));
-                OFRI_TRY(gauss_sep(ctx, c2, o2, h, w, p->opt_w, p->opt_n));
+                OFRI_TRY(gauss_sep2(ctx, c1, o1, c2, o2, h, w, p->opt_w, p->opt_n));
                 q1 = o1;
                 q2 = o2;
             }
             if (lrn && lrn->net && h % 16 == 0 && w % 16 == 0) {
                 // the correction network: (Liu-Shen pair, HS flow) -> Liu-Shen's correction
-                float* in = lrn->in_stack;
-                OFRI_HIP(hipMemcpyAsync(in, q1, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-                OFRI_HIP(hipMemcpyAsync(in + N, q2, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-                OFRI_HIP(hipMemcpyAsync(in + 2 * N, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-                OFRI_HIP(hipMemcpyAsync(in + 3 * N, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-                OFRI_TRY(net_predict(ctx, lrn->net, in, 1, h, w, lrn->pred_c, lrn->pred_f));
-                OFRI_TRY(accumulate(ctx, U, lrn->pred_f, N));
-                OFRI_TRY(accumulate(ctx, V, lrn->pred_f + N, N));
+                const float* planes[4] = {q1, q2, U, V};
+                OFRI_TRY(net_correct(ctx, lrn->net, planes, h, w, U, V));
             }
             // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
             if (lrn)
@@ -479,8 +465,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
             tap += 4 * N;
         }
-        OFRI_TRY(accumulate(ctx, out_u, U, N));
-        OFRI_TRY(accumulate(ctx, out_v, V, N));
+        OFRI_TRY(accumulate2(ctx, out_u, U, out_v, V, N));
         acc_u = out_u;
         acc_v = out_v;
         ph = h;
@@ -561,8 +546,7 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
             q.c1 = pr[i].im1;
             q.c2 = pr[i].im2;
             if (h != H || w != W) {
-                OFRI_TRY(resize(ctx, pr[i].im1, H, W, q.n1, h, w, 3));
-                OFRI_TRY(resize(ctx, pr[i].im2, H, W, q.n2, h, w, 3));
+                OFRI_TRY(resize2(ctx, pr[i].im1, q.n1, pr[i].im2, q.n2, H, W, h, w, 3));
                 q.c1 = q.n1;
                 q.c2 = q.n2;
             }
@@ -581,17 +565,14 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
                 s1 = q.w1;
                 s2 = q.w2;
             } else {
-                OFRI_TRY(fill_zero(ctx, out_u, N));
-                OFRI_TRY(fill_zero(ctx, out_v, N));
+                OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
             }
             if (p->filt_w && p->filt_n > 0) {
-                OFRI_TRY(gauss_sep(ctx, s1, q.f1, h, w, p->filt_w, p->filt_n));
-                OFRI_TRY(gauss_sep(ctx, s2, q.f2, h, w, p->filt_w, p->filt_n));
+                OFRI_TRY(gauss_sep2(ctx, s1, q.f1, s2, q.f2, h, w, p->filt_w, p->filt_n));
                 s1 = q.f1;
                 s2 = q.f2;
             }
-            OFRI_TRY(fill_zero(ctx, q.U, N));
-            OFRI_TRY(fill_zero(ctx, q.V, N));
+            OFRI_TRY(fill_zero2(ctx, q.U, q.V, N));
             hsim[i] = BatchImage{s1, s2, h, w, q.U, q.V, hs_errs + (size_t)l * n + i};
             q.acc_u = out_u;  // the accumulation target of this level
             q.acc_v = out_v;
@@ -604,8 +585,7 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
                 const int h = lh[i][l], w = lw[i][l];
                 const float *q1 = q.c1, *q2 = q.c2;
                 if (p->opt_w && p->opt_n > 0) {
-                    OFRI_TRY(gauss_sep(ctx, q.c1, q.o1, h, w, p->opt_w, p->opt_n));
-                    OFRI_TRY(gauss_sep(ctx, q.c2, q.o2, h, w, p->opt_w, p->opt_n));
+                    OFRI_TRY(gauss_sep2(ctx, q.c1, q.o1, q.c2, q.o2, h, w, p->opt_w, p->opt_n));
                     q1 = q.o1;
                     q2 = q.o2;
                 }
@@ -618,8 +598,7 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
         for (int i = 0; i < n; ++i) {
             PairWs& q = P[i];
             const size_t N = (size_t)lh[i][l] * lw[i][l];
-            OFRI_TRY(accumulate(ctx, q.acc_u, q.U, N));
-            OFRI_TRY(accumulate(ctx, q.acc_v, q.V, N));
+            OFRI_TRY(accumulate2(ctx, q.acc_u, q.U, q.acc_v, q.V, N));
             q.ph = lh[i][l];
             q.pw = lw[i][l];
         }
@@ -629,7 +608,8 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
 
 // Per-level statistics of the last pyramid run (the "pyr_stats" scratch).
 static int pyr_stats_download(ofri_ctx* ctx, int L, const std::vector<int>& lh, const std::vector<int>& lw,
-                              const ofri_lshs_params* p, bool with_res, ofri_level_stats* out) {
+                              const ofri_lshs_params* p, bool with_res, ofri_level_stats* out,
+                              bool with_hs_err = true) {
     void* st;
     OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (2 * sizeof(SolveCtrl) + 16), &st));
     SolveCtrl* ctrls = (SolveCtrl*)st;
@@ -641,7 +621,7 @@ static int pyr_stats_download(ofri_ctx* ctx, int L, const std::vector<int>& lh, 
     for (int l = 0; l < L; ++l) {
         out[l].H = lh[l];
         out[l].W = lw[l];
-        out[l].hs_err = he[l];
+        out[l].hs_err = with_hs_err ? he[l] : 0.0f;
         out[l].ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
         out[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
         out[l].hs_res = with_res ? hc[L + l].last_err : 0.0f;
@@ -929,10 +909,7 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     level_sizes(H, W, L, lh, lw);
     if (net && net_planes(NET(net)) != 4)
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: the correction network takes 4 input planes");
-    void* pr;
-    const size_t nf = 4 * (size_t)H * W, nc = nf / 4;
-    OFRI_TRY(scratch(ctx, "lrn_pred", (nf + nc + nf) * sizeof(float), &pr));
-    LearnedRun run{lp, NET(net), (float*)pr, (float*)pr + nf, (float*)pr + nf + nc};
+    LearnedRun run{lp, NET(net)};
     std::string key = "learned" + lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
     {
         const void* np = net;
@@ -975,7 +952,7 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     }
     if (h_stats || h_converged) {
         ofri_level_stats st[2];
-        OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, st));
+        OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, st, false));
         int ok = 1;
         for (int l = 0; l < L; ++l) {
             if (!(st[l].hs_res <= lp->hs_tol[l])) ok = 0;

@@ -114,7 +114,13 @@ namespace ofri {
 int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out);
 
 // ---- image kernels (k_image.hip) ----
+// "2" variants: the same operation on two planes in one launch (second plane
+// null: one plane).
+int gauss_sep2(ofri_ctx* ctx, const float* src1, float* dst1, const float* src2, float* dst2, int H, int W,
+               const float* w, int n);
 int gauss_sep(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* w, int n);
+int resize2(ofri_ctx* ctx, const float* src1, float* dst1, const float* src2, float* dst2, int H, int W, int H2,
+            int W2, int filter);
 int resize(ofri_ctx* ctx, const float* src, int H, int W, float* dst, int H2, int W2, int filter);
 // U and V in one pair of launches (src_v == nullptr: one plane)
 int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int Hc, int Wc, float* dst_u,
@@ -125,16 +131,21 @@ int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const floa
                    const float* V, int H, int W, float* o1, float* o2);
 int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* cx, int H, int W,
                 float* out);
+int accumulate2(ofri_ctx* ctx, float* acc1, const float* x1, float* acc2, const float* x2, size_t n);
 int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n);
+int fill_zero2(ofri_ctx* ctx, float* p1, float* p2, size_t n);
 int fill_zero(ofri_ctx* ctx, float* p, size_t n);
 // d = a - b (float32), n elements; d may alias neither input
 int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n);
 // max over n floats -> d_out[0]
+int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out);
 int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
 // d_out[0] = (||A-A0|| + ||B-B0||) / n as float32 (differences in float32)
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
                   size_t n, float* d_out);
 // dst[i] = src[ctrl->final_buf][i] for a ping-pong pair
+int select_copy2(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* a0, const float* a1, float* ad,
+                 const float* b0, const float* b1, float* bd, size_t n);
 int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const float* s1, float* dst,
                 size_t n);
 
@@ -212,6 +223,8 @@ int net_get_grads(ofri_ctx* ctx, Net* n, float* h, size_t count);
 // input: [B][planes][H][W] planar float32; planar outputs: coarse
 // [B][4][H/2][W/2], fine [B][4][H][W]
 int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, float* coarse, float* fine);
+// one input as separate planes; fine head fields 0, 1 added into add_u, add_v
+int net_correct(ofri_ctx* ctx, Net* n, const float* const* planes, int H, int W, float* add_u, float* add_v);
 int net_train(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, const float* lab_c, const float* lab_f,
               const float* h_field_w, float lr, float* d_loss);
 

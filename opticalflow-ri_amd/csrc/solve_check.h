@@ -164,6 +164,23 @@ __global__ void cheb_omega_k(OmegaTab t, float* w, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
 }
 
+// Start of an accelerated solve: control word cleared and n weights written
+// (one launch of any block size).
+__global__ void solve_start_k(SolveCtrl* c, OmegaTab t, float* w, int n) {
+    if (threadIdx.x == 0) {
+        c->done = 0;
+        c->sweeps = 0;
+        c->final_buf = 0;
+        c->redo = 0;
+        c->redo_in = 0;
+        c->last_err = 0.0f;
+        c->limit = 0;
+        c->slot[0] = SolveState{};
+        c->slot[1] = SolveState{};
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
+}
+
 __global__ void ctrl_init_k(SolveCtrl* c) {
     c->done = 0;
     c->sweeps = 0;
