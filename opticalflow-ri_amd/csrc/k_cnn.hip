@@ -239,46 +239,8 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    const bool plain_x = !a.x.f32 && !a.gate.p;
-    for (int c0 = 0; c0 < a.Cin; c0 += 32) {
-        // stage the input patch: PS x PS pixels x 4 groups of 8 channels
-        for (int idx = t; idx < PS * PS * 4; idx += 256) {
-            const int pix = idx >> 2, part = idx & 3;
-            const int py = pix / PS, px = pix - py * PS;
-            const int gy = y0 - 1 + py, gx = x0 - 1 + px;
-            const int ch = c0 + part * 8;
-            uint4 raw = {0u, 0u, 0u, 0u};
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W && ch < a.Cin) {
-                const long long off = view_off(a.x, b, gy, gx, ch, H, W, a.Cin);
-                if (plain_x) {
-                    raw = *(const uint4*)((const u16*)a.x.p + off);
-                } else {
-                    float f[8];
-                    load8f(a.x, off, f);
-                    if (a.gate.p) {
-                        float gt[8];
-                        load8f(a.gate, view_off(a.gate, b, gy, gx, ch, H, W, a.Cin), gt);
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) f[i] = gt[i] > 0.0f ? f[i] : f[i] * kSlope;
-                    }
-                    raw.x = (unsigned)f2bf(f[0]) | ((unsigned)f2bf(f[1]) << 16);
-                    raw.y = (unsigned)f2bf(f[2]) | ((unsigned)f2bf(f[3]) << 16);
-                    raw.z = (unsigned)f2bf(f[4]) | ((unsigned)f2bf(f[5]) << 16);
-                    raw.w = (unsigned)f2bf(f[6]) | ((unsigned)f2bf(f[7]) << 16);
-                }
-            }
-            *(uint4*)(sx + pix * kPixStride + part * 8) = raw;
-        }
-        // stage the weights: NT*16 rows x 9 taps x 4 groups of 8 channels
-        for (int idx = t; idx < NT * 16 * 36; idx += 256) {
-            const int co = idx / 36, rem = idx - co * 36;
-            const int tap = rem >> 2, part = rem & 3;
-            const int ch = c0 + part * 8;
-            uint4 raw = {0u, 0u, 0u, 0u};
-            if (ch < a.Cin) raw = *(const uint4*)(a.w + (size_t)(cobase + co) * a.Kp + tap * a.Cin + ch);
-            *(uint4*)(sw + co * kCoStride + tap * 32 + part * 8) = raw;
-        }
-        __syncthreads();
+    // the MFMAs of one staged 32-channel chunk: 9 taps
+    auto chunk_mfma = [&]() {
 #pragma unroll 3
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap - 3 * (tap / 3);   // patch offsets 0..2
@@ -296,7 +258,102 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
                     acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
             }
         }
-        __syncthreads();
+    };
+
+    const bool plain_x = !a.x.f32 && !a.gate.p;
+    if (plain_x) {
+        // bfloat16 input: the next chunk's patch and weights are fetched into
+        // registers while the current chunk's MFMAs run
+        constexpr int XN = (PS * PS * 4 + 255) / 256, WN = (NT * 16 * 36 + 255) / 256;
+        uint4 rx[XN], rw[WN];
+        auto fetch = [&](int c0) {
+#pragma unroll
+            for (int k = 0; k < XN; ++k) {
+                const int idx = t + k * 256;
+                rx[k] = uint4{0u, 0u, 0u, 0u};
+                if (idx < PS * PS * 4) {
+                    const int pix = idx >> 2, part = idx & 3;
+                    const int py = pix / PS, px = pix - py * PS;
+                    const int gy = y0 - 1 + py, gx = x0 - 1 + px;
+                    const int ch = c0 + part * 8;
+                    if (gy >= 0 && gy < H && gx >= 0 && gx < W && ch < a.Cin)
+                        rx[k] = *(const uint4*)((const u16*)a.x.p + view_off(a.x, b, gy, gx, ch, H, W, a.Cin));
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < WN; ++k) {
+                const int idx = t + k * 256;
+                rw[k] = uint4{0u, 0u, 0u, 0u};
+                if (idx < NT * 16 * 36) {
+                    const int co = idx / 36, rem = idx - co * 36;
+                    const int tap = rem >> 2, part = rem & 3;
+                    const int ch = c0 + part * 8;
+                    if (ch < a.Cin) rw[k] = *(const uint4*)(a.w + (size_t)(cobase + co) * a.Kp + tap * a.Cin + ch);
+                }
+            }
+        };
+        auto put = [&]() {
+#pragma unroll
+            for (int k = 0; k < XN; ++k) {
+                const int idx = t + k * 256;
+                if (idx < PS * PS * 4) *(uint4*)(sx + (idx >> 2) * kPixStride + (idx & 3) * 8) = rx[k];
+            }
+#pragma unroll
+            for (int k = 0; k < WN; ++k) {
+                const int idx = t + k * 256;
+                if (idx < NT * 16 * 36) {
+                    const int co = idx / 36, rem = idx - co * 36;
+                    *(uint4*)(sw + co * kCoStride + (rem >> 2) * 32 + (rem & 3) * 8) = rw[k];
+                }
+            }
+        };
+        fetch(0);
+        for (int c0 = 0; c0 < a.Cin; c0 += 32) {
+            put();
+            __syncthreads();
+            if (c0 + 32 < a.Cin) fetch(c0 + 32);
+            chunk_mfma();
+            __syncthreads();
+        }
+    } else {
+        for (int c0 = 0; c0 < a.Cin; c0 += 32) {
+            // stage the input patch: PS x PS pixels x 4 groups of 8 channels
+            for (int idx = t; idx < PS * PS * 4; idx += 256) {
+                const int pix = idx >> 2, part = idx & 3;
+                const int py = pix / PS, px = pix - py * PS;
+                const int gy = y0 - 1 + py, gx = x0 - 1 + px;
+                const int ch = c0 + part * 8;
+                uint4 raw = {0u, 0u, 0u, 0u};
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W && ch < a.Cin) {
+                    const long long off = view_off(a.x, b, gy, gx, ch, H, W, a.Cin);
+                    float f[8];
+                    load8f(a.x, off, f);
+                    if (a.gate.p) {
+                        float gt[8];
+                        load8f(a.gate, view_off(a.gate, b, gy, gx, ch, H, W, a.Cin), gt);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) f[i] = gt[i] > 0.0f ? f[i] : f[i] * kSlope;
+                    }
+                    raw.x = (unsigned)f2bf(f[0]) | ((unsigned)f2bf(f[1]) << 16);
+                    raw.y = (unsigned)f2bf(f[2]) | ((unsigned)f2bf(f[3]) << 16);
+                    raw.z = (unsigned)f2bf(f[4]) | ((unsigned)f2bf(f[5]) << 16);
+                    raw.w = (unsigned)f2bf(f[6]) | ((unsigned)f2bf(f[7]) << 16);
+                }
+                *(uint4*)(sx + pix * kPixStride + part * 8) = raw;
+            }
+            // stage the weights: NT*16 rows x 9 taps x 4 groups of 8 channels
+            for (int idx = t; idx < NT * 16 * 36; idx += 256) {
+                const int co = idx / 36, rem = idx - co * 36;
+                const int tap = rem >> 2, part = rem & 3;
+                const int ch = c0 + part * 8;
+                uint4 raw = {0u, 0u, 0u, 0u};
+                if (ch < a.Cin) raw = *(const uint4*)(a.w + (size_t)(cobase + co) * a.Kp + tap * a.Cin + ch);
+                *(uint4*)(sw + co * kCoStride + tap * 32 + part * 8) = raw;
+            }
+            __syncthreads();
+            chunk_mfma();
+            __syncthreads();
+        }
     }
 
     // D layout: lane holds m-tile pixels 4g..4g+3, channel r16

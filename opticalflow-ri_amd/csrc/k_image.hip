@@ -105,9 +105,9 @@ struct SplinePlanes {
     float scale[2];
 };
 
-constexpr int kSplineRowsX = 8;    // coarse rows per workgroup of the x pass
-constexpr int kSplineRowsY = 8;    // output rows per thread of the y pass
-constexpr int kSplineChunk = 8;    // weights in flight per thread of the x pass
+constexpr int kSplineRowsX = 4;    // coarse rows per workgroup of the x pass
+constexpr int kSplineRowsY = 16;   // output rows per thread of the y pass
+constexpr int kSplineChunk = 22;   // weights in flight per thread of the x pass
 constexpr int kSplineMaxWidth = 66;  // band width of the weight tables (2 * 32 + 2)
 
 // x pass: kSplineRowsX coarse rows staged in LDS (nrows * Wc floats); the
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, co
         s[i] = 0.0;
         sj[i] = i < ny ? st[y0 + i] : 0x3fffffff;
     }
-#pragma unroll 4
+#pragma unroll 8
     for (int j = j0; j < j1; ++j) {
         const double v = tmp[(size_t)j * W];
 #pragma unroll
