@@ -106,7 +106,7 @@ struct SplinePlanes {
 };
 
 constexpr int kSplineRowsX = 4;    // coarse rows per workgroup of the x pass
-constexpr int kSplineRowsY = 16;   // output rows per thread of the y pass
+constexpr int kSplineRowsY = 8;    // output rows per thread of the y pass
 constexpr int kSplineChunk = 22;   // weights in flight per thread of the x pass
 constexpr int kSplineMaxWidth = 66;  // band width of the weight tables (2 * 32 + 2)
 

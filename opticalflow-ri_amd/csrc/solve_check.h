@@ -26,10 +26,16 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
     __shared__ SolveState sh;
     SolveState s = ctrl->slot[prev];
     const bool ev = !s.done && s.pend_nsw > 0;
+    if (!ev) {
+        // nothing to test (a finished solve, or the first launch): every
+        // thread already holds the state
+        s.pend_nsw = 0;
+        return s;
+    }
     const int T = blockDim.x * blockDim.y;
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = t & 63, wq = t >> 6, nw = (T + 63) >> 6;
-    if (ev) {
+    {
         // wavefront q sums sweeps q, q + nw, ... over the workgroups
         // (lane-strided, then a fixed butterfly)
         for (int k = s.pend_s0 + wq; k < s.pend_nsw; k += nw) {
@@ -50,7 +56,7 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
     }
     __syncthreads();
     if (t == 0) {
-        if (ev) {
+        {
             const int nsw = s.pend_nsw, dst = s.pend_dst;
             s.sweeps += s.pend_s0;   // sweeps run but not reduced
             for (int k = s.pend_s0; k < nsw; ++k) {

@@ -62,10 +62,14 @@ def plain_targets(im1, im2, fw=None, ow=None):
     return U, V, hs_tol, [float(s["ls_err"]) for s in st], st
 
 
-# Sweep budgets of the learned form: HS at most 160 per level (about twice
-# what the accelerated sweeps need from a zero start), Liu-Shen the plain 60.
-HS_BUDGET = 160
-LS_BUDGET = 60
+# Sweep budgets of the learned form per level: HS at most 120 accelerated
+# sweeps (70-100 reach the plain residual on 1024^2 PIV pairs) plus at most
+# 60 plain finishing sweeps (about 10 used), Liu-Shen 32 (5-16 used after
+# the network's correction; the plain method takes 60).  A level that misses
+# its target within them sends the pair to the plain fallback; launches past
+# a finished solve only cost their dispatch, so the budgets stay close to use.
+HS_BUDGET = 120
+LS_BUDGET = 32
 
 
 def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None, fallback=True, stats=True,
