@@ -1,16 +1,27 @@
-"""Benchmark: LS(HS) 2-level dense flow at 1024x1024 on one MI355X.
+"""Benchmark: LS(HS) 2-level dense flow at 1024x1024 on one MI355X, solved to
+the plain method's residual -- learned form against plain form.
 
-One step = one synthetic PIV pair (1024^2, Lamb-Oseen, density 0.04) solved
-coarse-to-fine: bicubic 1/2 downsample, FILTER 3.4 (3 taps) and HS with 600
-sweeps (alpha 45 at 512^2, 21 at 1024^2), FILTER_OPT 0.48 (5 taps) and
-Liu-Shen alpha 5 (tol 1e-8, at most 60 sweeps), spline upsample x2, symmetric
-bilinear warp, accumulation.  Inputs are resident in HBM before timing.
+The workload (default): LS(HS) 2-level on 1024^2 synthetic PIV pairs (the
+workload's Lamb-Oseen pair, density 0.04, plus held-out rendered pairs of
+the three flow families): bicubic 1/2 downsample, FILTER 3.4 (3 taps) and HS
+(alpha 45 at 512^2, 21 at 1024^2), FILTER_OPT 0.48 (5 taps) and Liu-Shen
+alpha 5, spline upsample x2, symmetric bilinear warp, accumulation.  The
+plain form runs the reference's sweep counts (HS 600, Liu-Shen to 1e-8 or 60
+sweeps); the learned form (accelerated HS, correction network, Liu-Shen)
+runs every level to the plain form's residual at its stopping point, with
+the plain form as fallback.  One step = one pair; inputs are resident in HBM
+before timing.
 
-Prints ONE JSON line.  `value` is Mpix-pairs/s (finest-level H*W/1e6 per pair
-per second).  `roofline` is measured live for the dominant kernel (the
-temporally blocked Horn-Schunck sweep) with HIP events on its stream;
-`cpu_baseline` times the CPU oracle (the reference dataflow restated in C) on
-this host.
+Prints ONE JSON line.  `value` is the learned form's Mpix-pairs/s
+(finest-level H*W/1e6 per pair per second, fallbacks included);
+`learned_vs_plain` holds the plain form's rate, the method without the
+network, EPE to the plain result and the fallback rate.  `roofline` is
+measured live (HIP events on the library stream) for the learned form's
+dominant kernel, the Chebyshev HS sweep; `roofline_plain_hs` for the plain
+form's HS sweep; `roofline_net` for the network's forward pass against the
+bf16 MFMA peak.  `cpu_baseline` times the CPU oracle (the reference dataflow
+restated in C) on this host.  Other workloads: --workload plain | batch |
+fbls | lk.
 """
 import argparse
 import json
@@ -29,6 +40,7 @@ import numpy as np  # noqa: E402
 METRIC = "Mpix-pairs/s to fixed residual, LiuShen-HS 1024² learned vs plain; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 HS_BYTES_PER_PX_SWEEP = 28.0     # read U, V, fx, fy, ft; write U, V
+CHEB_BYTES_PER_PX_SWEEP = 36.0   # read U, V, previous U, V, fx, fy, ft; write U, V
 HS_SETUP_BYTES_PER_PX = 20.0     # read 2 images, write fx, fy, ft
 CALIB_BYTES = 64 << 20           # size of the counter-calibration copy
 
@@ -50,9 +62,9 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile():
-    """HBM bytes per HS launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", "hs_pass_1024_traffic.json")
+def traffic_from_profile(name="hs_pass_1024_traffic.json"):
+    """HBM bytes per launch from a committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return float(json.load(f)["bytes_per_launch"])
@@ -323,6 +335,45 @@ def hs_roofline(ctx, d1, d2, fw, n):
             "launch_ms": round(hs_ms / launches, 5), "algorithmic_bytes_per_launch": round(alg_bytes / launches)}
 
 
+def cheb_roofline(ctx, d1, d2, fw, n):
+    """The Chebyshev-accelerated HS sweep at n x n (alpha 21, bound 0.99, stop
+    test once per launch, a tolerance it never meets): HIP events around 16
+    and 8 full launches, per-launch time from the difference."""
+    from ofri import ops
+    from ofri.device import DeviceArray
+    f1 = ops.gauss_sep(d1, fw)
+    f2 = ops.gauss_sep(d2, fw)
+    U, V = DeviceArray.zeros((n, n)), DeviceArray.zeros((n, n))
+    lib, h = ctx.lib, ctx.handle
+    depth = 10
+
+    def run(sweeps):
+        lib.ofri_memset(h, U.ptr, 0, 4 * n * n)
+        lib.ofri_memset(h, V.ptr, 0, 4 * n * n)
+        ops.hs_accel(f1, f2, 21.0, sweeps, 1e-30, 0.99, U, V, finish=0.0, launch_test=True)
+
+    ctx.set_tuning(hs_depth=depth, hs_variant=12)   # the residual-mode default at this size
+    try:
+        run(16 * depth)
+        t = {}
+        for k in (16, 8):
+            reps = 5
+            ctx.timer_start()
+            for _ in range(reps):
+                run(k * depth)
+            t[k] = ctx.timer_stop() / reps
+    finally:
+        ctx.set_tuning()
+    launch_ms = (t[16] - t[8]) / 8
+    alg = CHEB_BYTES_PER_PX_SWEEP * depth * n * n
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "hs_pass_k<8, 16, true, true> (Chebyshev HS sweep, %d sweeps/launch, "
+                                      "stop test at the launch's last sweep)" % depth,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("hs_cheb_1024_traffic.json"),
+            "launch_ms": round(launch_ms, 5), "algorithmic_bytes_per_launch": round(alg)}
+
+
 def net_roofline(ctx, net, n):
     """One forward pass of the correction network over a 4-plane n x n input
     (all convolution launches plus the input and head kernels), timed with
@@ -415,7 +466,8 @@ def bench_lshs(args):
                              "targets, plain fallback" % (n, n),
                  "pairs_per_step": 1, "height": n, "width": n, "levels": 2, "eval_pairs": np_,
                  "parallelism": "single GPU"},
-                hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
+                cheb_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
+    out["roofline_plain_hs"] = hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n)
     out["learned_vs_plain"] = {
         "learned_value": round(value, 3), "plain_value": round(plain_value, 3),
         "no_network_value": round(mpix * args.steps / wall_z, 3),
