@@ -283,6 +283,14 @@ int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int
  * coarse fields then fine fields; weight 0 leaves a field untrained),
  * backward, and with lr > 0 an Adam step.  *h_loss (may be NULL) receives
  * the loss (synchronises). */
+/* The correction step on one H x W input given as separate device planes
+ * (d_planes: `planes` device pointers, the layout ofri_net_predict reads as
+ * [planes][H][W]): the fine head's fields 0, 1 are added into d_u, d_v
+ * (H x W each, may be the flow planes among the inputs); the coarse head is
+ * not computed.  H, W multiples of 16.  Same values as ofri_net_predict's
+ * fine fields 0, 1 added to d_u, d_v. */
+int ofri_net_correct(ofri_ctx* ctx, ofri_net* net, const float* const* d_planes, int H, int W, float* d_u,
+                     float* d_v);
 int ofri_net_train(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W,
                    const float* d_lab_coarse, const float* d_lab_fine, const float* h_field_w, float lr,
                    float* h_loss);

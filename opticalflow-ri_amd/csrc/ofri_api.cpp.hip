@@ -856,6 +856,18 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n) {
     return net_get_grads(ctx, NET(net), h_grads, n);
 }
 
+int ofri_net_correct(ofri_ctx* ctx, ofri_net* net, const float* const* d_planes, int H, int W, float* d_u,
+                     float* d_v) {
+    CHECK_CTX(ctx);
+    if (!net || !d_planes || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_net_correct: null pointer");
+    if (H <= 0 || W <= 0 || H % 16 || W % 16)
+        return fail(OFRI_E_ARG, "ofri_net_correct: H, W must be positive multiples of 16 (got %dx%d)", H, W);
+    const int P = net_planes(NET(net));
+    for (int i = 0; i < P; ++i)
+        if (!d_planes[i]) return fail(OFRI_E_ARG, "ofri_net_correct: input plane %d is null", i);
+    return net_correct(ctx, NET(net), d_planes, H, W, d_u, d_v);
+}
+
 int ofri_net_predict(ofri_ctx* ctx, ofri_net* net, const float* d_in, int B, int H, int W, float* d_coarse,
                      float* d_fine) {
     CHECK_CTX(ctx);

@@ -115,6 +115,7 @@ SIGNATURES = {
     "ofri_net_destroy": [_vp],
     "ofri_net_num_params": [_vp, C.POINTER(C.c_size_t)],
     "ofri_net_flops": [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "ofri_net_correct": [_vp, _vp, C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp],
     "ofri_net_set_params": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_init": [_vp, _vp, C.c_ulonglong],
     "ofri_net_get_params": [_vp, _vp, _fp, C.c_size_t],

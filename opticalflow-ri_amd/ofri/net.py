@@ -81,6 +81,17 @@ class Net(object):
                                                   fine.ptr))
         return coarse, fine
 
+    def correct(self, planes, u, v):
+        """Adds the fine head's fields 0, 1 for one input (a list of `planes`
+        H x W DeviceArrays) into the H x W DeviceArrays u, v."""
+        planes = [as_device(p) for p in planes]
+        if len(planes) != self.planes:
+            raise ValueError("network takes %d input planes, got %d" % (self.planes, len(planes)))
+        H, W = planes[0].shape[-2:]
+        arr = (C.c_void_p * self.planes)(*[p.ptr.value for p in planes])
+        _capi.check(self.ctx.lib.ofri_net_correct(self.ctx.handle, self.handle, arr, H, W, u.ptr, v.ptr))
+        return u, v
+
     def train_step(self, inp, lab_c, lab_f, lr, field_w=FIELD_WEIGHTS):
         """One step on a batch (lr = 0: gradients only).  Returns the loss."""
         d = as_device(inp)

@@ -1,6 +1,7 @@
-"""Warm-start network: the CPU restatement's backward pass against finite
+"""Correction network: the CPU restatement's backward pass against finite
 differences of its own loss, and the device's forward pass, loss and
-gradients against the restatement at the device's bfloat16 rounding points."""
+gradients against the restatement at the device's bfloat16 rounding points;
+the fused correction step against the forward pass."""
 import numpy as np
 import pytest
 
@@ -147,3 +148,22 @@ def test_device_adam_steps_reduce_loss(net):
     losses = [net.train_step(inp, lc, lf, lr=2e-3, field_w=FW) for _ in range(40)]
     print("[cnn adam] loss %.4f -> %.4f" % (losses[0], losses[-1]))
     assert losses[-1] < 0.8 * losses[0]
+
+
+@pytest.mark.gpu
+def test_device_correction_step_equals_forward_plus_add(net):
+    """ofri_net_correct (separate input planes, coarse head skipped, fine
+    fields 0, 1 added into the flow in place) gives exactly the forward
+    pass's fine fields added to the flow, also when the flow planes are the
+    network's own inputs."""
+    from ofri.device import DeviceArray
+    net.init(5)
+    inp, _, _ = _batch(1, 64, 80, 31)
+    _, fine = net.predict(inp)
+    f = fine.download()[0]
+    planes = [DeviceArray.from_host(inp[0, k]) for k in range(4)]
+    u, v = planes[2], planes[3]            # the flow inputs, updated in place
+    want_u = (inp[0, 2] + f[0]).astype(np.float32)
+    want_v = (inp[0, 3] + f[1]).astype(np.float32)
+    net.correct(planes, u, v)
+    assert np.array_equal(u.download(), want_u) and np.array_equal(v.download(), want_v)
