@@ -430,23 +430,41 @@ def bench_lshs(args):
         cnt["i"] += 1
         ops.pyr_lshs(d1, d2, filt_w=fw, opt_w=ow, Uacc=Ua, Vacc=Va, stats=False, **wl)
 
-    def learned_step(nt):
-        k = cnt["i"] % np_
-        cnt["i"] += 1
-        ht, lt = tgt[k]
-        _, _, info = learned.solve(pairs[k][0], pairs[k][1], nt, ht, lt, fw, ow, Uacc=Ua, Vacc=Va, fallback=True,
-                                   stats=False)
-        cnt["fell"] += int(info["fell_back"])
-        cnt["runs"] += 1
+    # the learned form: every pair queued with its convergence flag on the
+    # device, the flags read once, the pairs that missed a target re-run by
+    # the plain pyramid -- all inside the timed region
+    flags = DeviceArray((max(args.steps, args.warmup, np_),))
+    jobs = [(pairs[k % np_][0], pairs[k % np_][1], tgt[k % np_][0], tgt[k % np_][1], Ua, Va)
+            for k in range(args.steps)]
 
+    def learned_run(nt, n):
+        return learned.solve_many(jobs[:n], nt, fw, ow, flags=flags)
+
+    def timed_many(nt):
+        if args.warmup:
+            learned_run(nt, min(args.warmup, len(jobs)))
+        ctx.sync()
+        ctx.timer_start()
+        t0 = time.perf_counter()
+        fb = learned_run(nt, args.steps)
+        g = ctx.timer_stop()
+        return time.perf_counter() - t0, g, fb
+
+    # every pair's launch sequences are captured as HIP graphs on their
+    # second call: run each pair twice in each form before any timing
+    prime = [(pairs[k][0], pairs[k][1], tgt[k][0], tgt[k][1], Ua, Va) for k in range(np_)]
+    for _ in range(2):
+        for k in range(np_):
+            plain_step()
+        learned.solve_many(prime, net, fw, ow, flags=flags)
+        learned.solve_many(prime, None, fw, ow, flags=flags)
+    ctx.sync()
     cnt["i"] = 0
     wall_p, gpu_p = _timed(plain_step, ctx, args.steps, args.warmup)
-    cnt["i"] = 0
-    wall_l, gpu_l = _timed(lambda: learned_step(net), ctx, args.steps, args.warmup)
-    fell, runs = cnt["fell"], cnt["runs"]
-    cnt.update(i=0, fell=0, runs=0)
-    wall_z, _ = _timed(lambda: learned_step(None), ctx, args.steps, args.warmup)
-    fell_z, runs_z = cnt["fell"], cnt["runs"]
+    wall_l, gpu_l, fb = timed_many(net)
+    fell, runs = sum(fb), len(fb)
+    wall_z, _, fbz = timed_many(None)
+    fell_z, runs_z = sum(fbz), len(fbz)
     mpix = n * n / 1e6
     value = mpix * args.steps / wall_l
     plain_value = mpix * args.steps / wall_p

@@ -307,7 +307,9 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
  * net NULL: Liu-Shen from HS's flow alone (the method without the network).  Both solvers test
  * their targets once per fused launch (at its last sweep).  *h_converged (may be
  * NULL) is 1 when every level met its targets; otherwise the caller falls
- * back to ofri_pyr_lshs.  h_stats (may be NULL): 2 entries (hs_err, the
+ * back to ofri_pyr_lshs.  d_converged (may be NULL): the same flag written to
+ * device memory (int) with no host synchronisation, so that many pairs can
+ * be queued and their flags read once.  h_stats (may be NULL): 2 entries (hs_err, the
  * plain HS's change-from-start value, is not formed: 0).  Levels whose
  * size is not a multiple of 16 skip the network.  Repeated calls replay a
  * HIP graph.  No counterpart in the reference. */
@@ -316,14 +318,15 @@ typedef struct ofri_learned_params {
     const float* ls_tol;
     int hs_budget;
     int ls_budget;
-    float cheb_bound;      /* HS: e.g. 0.99; 0 = plain sweeps */
+    float cheb_bound[2];   /* HS per level (coarse, fine): e.g. 0.99, 0.985; 0 = plain sweeps */
     float cheb_finish;     /* HS: e.g. 4; see ofri_hs_accel */
     float ls_cheb_lo;      /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
     float ls_cheb_hi;      /* hi = 0: plain Liu-Shen sweeps */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,
-                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged);
+                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged,
+                          int* d_converged);
 
 /* Tuning knobs of the temporally blocked sweeps.  0 = the library's
  * per-image-size default (measured on gfx950); a variant k >= 1 selects the

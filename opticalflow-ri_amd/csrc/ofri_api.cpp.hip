@@ -1,6 +1,7 @@
 // C ABI of libofri: context, errors, device memory, the per-operation entry
 // points and the whole-pyramid LS(HS) driver.
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 
@@ -431,7 +432,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, nullptr, lp->hs_tol[l],
-                              hs_ctrls + l, nullptr, lp->cheb_bound, lp->cheb_finish, true));
+                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
         } else {
@@ -604,6 +605,17 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
         }
     }
     return OFRI_OK;
+}
+
+// The learned form's convergence flag on the device: every level's HS and
+// Liu-Shen residuals (the published control words of the "pyr_stats"
+// scratch: 2 Liu-Shen, then 2 HS) against their targets, as the host test.
+__global__ void learned_check_k(const SolveCtrl* c, float hs0, float hs1, float ls0, float ls1, int with_ls,
+                                int* out) {
+    int ok = 1;
+    if (!(c[2].last_err <= hs0) || !(c[3].last_err <= hs1)) ok = 0;
+    if (with_ls && (!(c[0].last_err <= ls0) || !(c[1].last_err <= ls1))) ok = 0;
+    *out = ok;
 }
 
 // Per-level statistics of the last pyramid run (the "pyr_stats" scratch).
@@ -903,7 +915,8 @@ int ofri_render_pair(ofri_ctx* ctx, int H, int W, int family, float density, uns
 
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,
-                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged) {
+                          float* d_Uacc, float* d_Vacc, ofri_level_stats* h_stats, int* h_converged,
+                          int* d_converged) {
     CHECK_CTX(ctx);
     if (!p || !lp || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc)
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: null pointer");
@@ -911,8 +924,9 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
     if (H < 8 || W < 8) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d too small", H, W);
     if (lp->hs_budget < 1 || lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
-    if (lp->cheb_bound != 0.0f && !(lp->cheb_bound > 0.0f && lp->cheb_bound <= 0.999f))
-        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g", lp->cheb_bound);
+    for (int l = 0; l < 2; ++l)
+        if (lp->cheb_bound[l] != 0.0f && !(lp->cheb_bound[l] > 0.0f && lp->cheb_bound[l] <= 0.999f))
+            return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g at level %d", lp->cheb_bound[l], l);
     if (lp->ls_cheb_hi != 0.0f &&
         !(lp->ls_cheb_lo > -1.0f && lp->ls_cheb_lo < 0.0f && lp->ls_cheb_hi > 0.0f && lp->ls_cheb_hi <= 0.999f))
         return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen interval [%g, %g]", lp->ls_cheb_lo, lp->ls_cheb_hi);
@@ -928,7 +942,7 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
         key.append((const char*)&np, sizeof np);
         key.append((const char*)lp->hs_tol, 2 * sizeof(float));
         key.append((const char*)lp->ls_tol, 2 * sizeof(float));
-        key.append((const char*)&lp->hs_budget, sizeof(int) * 2 + sizeof(float) * 4);
+        key.append((const char*)&lp->hs_budget, sizeof(ofri_learned_params) - offsetof(ofri_learned_params, hs_budget));
     }
     auto enqueue = [&]() { return pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, nullptr, false, &run); };
     if (!ctx->graphs_on) {
@@ -961,6 +975,13 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
             g.ws_gen = ctx->ws_gen;
             OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
         }
+    }
+    if (d_converged) {
+        void* stp;
+        OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (2 * sizeof(SolveCtrl) + 16), &stp));
+        learned_check_k<<<1, 1, 0, ctx->stream>>>((const SolveCtrl*)stp, lp->hs_tol[0], lp->hs_tol[1], lp->ls_tol[0],
+                                                  lp->ls_tol[1], p->ls_alpha > 0.0f ? 1 : 0, d_converged);
+        OFRI_HIP(hipGetLastError());
     }
     if (h_stats || h_converged) {
         ofri_level_stats st[2];

@@ -52,7 +52,7 @@ class Pair(C.Structure):
 
 class LearnedParams(C.Structure):
     _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int),
-                ("ls_budget", C.c_int), ("cheb_bound", C.c_float), ("cheb_finish", C.c_float),
+                ("ls_budget", C.c_int), ("cheb_bound", C.c_float * 2), ("cheb_finish", C.c_float),
                 ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float)]
 
 
@@ -125,7 +125,7 @@ SIGNATURES = {
     "ofri_pyr_lshs_taps": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp, _vp,
                            C.POINTER(LevelStats)],
     "ofri_pyr_lshs_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), C.POINTER(LearnedParams),
-                              _vp, _vp, _vp, C.POINTER(LevelStats), _ip],
+                              _vp, _vp, _vp, C.POINTER(LevelStats), _ip, _vp],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],

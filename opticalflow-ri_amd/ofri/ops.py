@@ -283,25 +283,31 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
 
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
                      hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
-                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True):
-    """The learned form (ofri_pyr_lshs_learned): network warm starts, HS and
-    Liu-Shen run to the per-level residual targets hs_tol / ls_tol.  net: an
-    ofri.net.Net or None (zero warm starts).  Returns (Uacc, Vacc, stats,
-    converged); converged is None when stats is False (no synchronisation)."""
+                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None):
+    """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
+    correction network and Liu-Shen run to the per-level residual targets
+    hs_tol / ls_tol.  net: an ofri.net.Net or None (the method without it).
+    cheb_bound: one value or (coarse, fine).  flag: optional DeviceArray view
+    of one 4-byte word receiving the convergence flag (int 1 / 0) with no
+    synchronisation.  Returns (Uacc, Vacc, stats, converged); converged is
+    None when stats is False."""
     im1, im2 = as_device(im1), as_device(im2)
     H, W = im1.shape[-2:]
     p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol_plain)
     ht = np.ascontiguousarray(hs_tol, dtype=np.float32)
     lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
-    lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget), float(cheb_bound),
-                             float(cheb_finish), float(ls_cheb[0]), float(ls_cheb[1]))
+    cb = (cheb_bound, cheb_bound) if np.isscalar(cheb_bound) else tuple(cheb_bound)
+    lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget),
+                             (C.c_float * 2)(float(cb[0]), float(cb[1])), float(cheb_finish), float(ls_cheb[0]),
+                             float(ls_cheb[1]))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None
     conv = C.c_int(0)
     _capi.check(_lib().ofri_pyr_lshs_learned(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), C.byref(lp),
                                              net.handle if net is not None else None, Uacc.ptr, Vacc.ptr, st,
-                                             C.byref(conv) if stats else None))
+                                             C.byref(conv) if stats else None,
+                                             flag.ptr if flag is not None else None))
     return Uacc, Vacc, (_stats_dicts(st) if stats else []), (bool(conv.value) if stats else None)
 
 

@@ -141,3 +141,24 @@ def test_learned_form_replays_identically(trained):
         outs.append((U.download(), V.download(), [s["ls_iters"] for s in info["stats"]]))
     for o in outs[1:]:
         assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1]) and o[2] == outs[0][2]
+
+
+def test_queued_pairs_flag_on_the_device_and_fall_back(trained):
+    """solve_many queues pairs with their convergence flags written on the
+    device: a pair within budget keeps the learned result (the same bits as
+    a single call), a pair whose budget is too small gets the plain result."""
+    from ofri import learned
+    from ofri.device import DeviceArray
+    t = trained
+    a, b = t["pair"]
+    U1, V1 = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
+    U2, V2 = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
+    Us, Vs, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
+    assert info["converged"]
+    fell = learned.solve_many([(a, b, t["ht"], t["lt"], U1, V1)], t["net"], t["fw"], t["ow"])
+    assert fell == [False]
+    assert np.array_equal(U1.download(), Us.download()) and np.array_equal(V1.download(), Vs.download())
+    fell = learned.solve_many([(a, b, t["ht"], t["lt"], U1, V1), (a, b, t["ht"], t["lt"], U2, V2)], t["net"],
+                              t["fw"], t["ow"], hs_budget=8)
+    assert fell == [True, True]
+    assert np.array_equal(U2.download(), t["plain"][0]) and np.array_equal(V2.download(), t["plain"][1])
