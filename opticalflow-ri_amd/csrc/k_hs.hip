@@ -380,7 +380,7 @@ constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
 // sweep's shape, whose per-launch test would cost more).
 void hs_cheb_default(size_t npx, int* variant, int* depth) {
     *variant = npx >= (size_t)700000 ? 11 : 9;
-    *depth = 10;
+    *depth = npx >= (size_t)700000 ? 10 : 12;
 }
 
 void hs_default(size_t npx, int* variant, int* depth) {

@@ -148,6 +148,116 @@ __global__ __launch_bounds__(256) void spline_x(SplinePlanes p, int Wc, int W, i
         if (i < nr) tmp[(size_t)i * W] = s[i];
 }
 
+// Exact factor 2 (W = 2 Wc): output 2i sits on sample i (its weight row is
+// the unit vector, so the band sum is the sample itself) and the last output
+// is clamped onto the last sample; only the odd outputs 2i+1 < W-1 need the
+// band.  x pass: thread i writes outputs 2i and 2i+1 of its staged rows.
+__global__ __launch_bounds__(256) void spline_x2(SplinePlanes p, int Wc, int W, int Hc, const double* __restrict__ wT,
+                                                 const int* __restrict__ st, int width, int nrows) {
+    extern __shared__ float rows[];
+    const int z = blockIdx.z;
+    const int r0 = blockIdx.y * nrows;
+    const int nr = min(nrows, Hc - r0);
+    const float* src = p.src[z] + (size_t)r0 * Wc;
+    for (int i = threadIdx.x; i < nr * Wc; i += 256) rows[i] = src[i];
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Wc) return;
+    const int x = 2 * i + 1;
+    double* tmp = p.tmp[z] + (size_t)r0 * W;
+    double s[kSplineRowsX];
+#pragma unroll
+    for (int q = 0; q < kSplineRowsX; ++q) s[q] = q < nr ? (double)rows[q * Wc + i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kSplineRowsX; ++q)
+        if (q < nr) tmp[(size_t)q * W + 2 * i] = s[q];
+    if (x == W - 1) {
+#pragma unroll
+        for (int q = 0; q < kSplineRowsX; ++q)
+            if (q < nr) tmp[(size_t)q * W + x] = s[q];
+        return;
+    }
+    const float* b = rows + st[x];
+#pragma unroll
+    for (int q = 0; q < kSplineRowsX; ++q) s[q] = 0.0;
+    for (int t0 = 0; t0 < width; t0 += kSplineChunk) {
+        double wc[kSplineChunk];
+#pragma unroll
+        for (int k = 0; k < kSplineChunk; ++k) wc[k] = t0 + k < width ? wT[(size_t)(t0 + k) * W + x] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kSplineChunk; ++k) {
+            if (t0 + k < width) {
+#pragma unroll
+                for (int q = 0; q < kSplineRowsX; ++q)
+                    if (q < nr) s[q] = s[q] + wc[k] * (double)b[q * Wc + t0 + k];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kSplineRowsX; ++q)
+        if (q < nr) tmp[(size_t)q * W + x] = s[q];
+}
+
+// Exact factor 2 (H = 2 Hc), y pass: a thread owns column x of output rows
+// 2k and 2k+1 for kSplineRowsY consecutive coarse rows k; even rows are the
+// x-pass rows, odd rows walk the coarse rows their bands cover once.
+__global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, int Hc, const double* __restrict__ w,
+                                                 const int* __restrict__ st, int width) {
+    __shared__ double ws[kSplineRowsY][kSplineMaxWidth];
+    const int z = blockIdx.z;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int k0 = blockIdx.y * kSplineRowsY;
+    const int nk = min(kSplineRowsY, Hc - k0);
+    // weights of the odd rows 2(k0+q)+1 (the last output row has none)
+    for (int i = threadIdx.x; i < nk * width; i += 256) {
+        const int q = i / width, y = 2 * (k0 + q) + 1;
+        ws[q][i % width] = y < H - 1 ? w[(size_t)y * width + i % width] : 0.0;
+    }
+    __syncthreads();
+    if (x >= W) return;
+    const double* tmp = p.tmp[z] + x;
+    float* dst = p.dst[z] + x;
+    const float sc = p.scale[z];
+#pragma unroll
+    for (int q = 0; q < kSplineRowsY; ++q)
+        if (q < nk) {
+            float o = (float)tmp[(size_t)(k0 + q) * W];
+            dst[(size_t)(2 * (k0 + q)) * W] = o * sc;
+        }
+    double s[kSplineRowsY];
+    int sj[kSplineRowsY];
+    int j0 = 0x3fffffff, j1 = 0;
+#pragma unroll
+    for (int q = 0; q < kSplineRowsY; ++q) {
+        const int y = 2 * (k0 + q) + 1;
+        const bool band = q < nk && y < H - 1;
+        s[q] = 0.0;
+        sj[q] = band ? st[y] : 0x3fffffff;
+        if (band) {
+            j0 = min(j0, sj[q]);
+            j1 = max(j1, sj[q] + width);
+        }
+    }
+#pragma unroll 8
+    for (int j = j0; j < j1; ++j) {
+        const double v = tmp[(size_t)j * W];
+#pragma unroll
+        for (int q = 0; q < kSplineRowsY; ++q) {
+            const int t = j - sj[q];
+            if (t >= 0 && t < width) s[q] = s[q] + ws[q][t] * v;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kSplineRowsY; ++q) {
+        const int y = 2 * (k0 + q) + 1;
+        if (q < nk) {
+            // the last output row is clamped onto the last coarse row
+            float o = y < H - 1 ? (float)s[q] : (float)tmp[(size_t)(Hc - 1) * W];
+            dst[(size_t)y * W] = o * sc;
+        }
+    }
+}
+
 // y pass: a thread owns column x of kSplineRowsY consecutive output rows and
 // walks the coarse rows their bands cover once, adding each row's value to
 // the outputs whose band holds it (in band order).
@@ -580,11 +690,22 @@ int spline_upsample2(ofri_ctx* ctx, const float* src_u, const float* src_v, int 
     p.tmp[0] = (double*)tmp; p.tmp[1] = (double*)tmp + (size_t)Hc * W;
     p.dst[0] = dst_u; p.dst[1] = dst_v;
     p.scale[0] = scale_u; p.scale[1] = scale_v;
-    dim3 gx((W + 255) / 256, (Hc + nrows - 1) / nrows, planes);
-    spline_x<<<gx, 256, (size_t)nrows * Wc * sizeof(float), ctx->stream>>>(p, Wc, W, Hc, tx->d_wT, tx->d_start,
-                                                                           tx->width, nrows);
-    dim3 gy((W + 255) / 256, (H + kSplineRowsY - 1) / kSplineRowsY, planes);
-    spline_y<<<gy, 256, 0, ctx->stream>>>(p, W, H, ty->d_w, ty->d_start, ty->width);
+    if (W == 2 * Wc) {
+        dim3 gx((Wc + 255) / 256, (Hc + nrows - 1) / nrows, planes);
+        spline_x2<<<gx, 256, (size_t)nrows * Wc * sizeof(float), ctx->stream>>>(p, Wc, W, Hc, tx->d_wT, tx->d_start,
+                                                                                tx->width, nrows);
+    } else {
+        dim3 gx((W + 255) / 256, (Hc + nrows - 1) / nrows, planes);
+        spline_x<<<gx, 256, (size_t)nrows * Wc * sizeof(float), ctx->stream>>>(p, Wc, W, Hc, tx->d_wT, tx->d_start,
+                                                                               tx->width, nrows);
+    }
+    if (H == 2 * Hc) {
+        dim3 gy((W + 255) / 256, (Hc + kSplineRowsY - 1) / kSplineRowsY, planes);
+        spline_y2<<<gy, 256, 0, ctx->stream>>>(p, W, H, Hc, ty->d_w, ty->d_start, ty->width);
+    } else {
+        dim3 gy((W + 255) / 256, (H + kSplineRowsY - 1) / kSplineRowsY, planes);
+        spline_y<<<gy, 256, 0, ctx->stream>>>(p, W, H, ty->d_w, ty->d_start, ty->width);
+    }
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }
