@@ -145,8 +145,8 @@ __device__ __forceinline__ float quotient(float num, float den, float rcp) {
 //
 // CHEB: Chebyshev semi-iteration over the same sweep T.  With the damped step
 // T_g(x) = x + gamma (T(x) - x), whose spectrum lies in [-sigma, sigma],
-// x_{k+1} = x_{k-1} + omega_{k+1} (T_g(x_k) - x_{k-1}); the previous iterate
-// x_{k-1} rides along in registers.  The residual of sweep k is the plain
+// x_{k+1} = x_{k-1} + omega_{k+1} (T_g(x_k) - x_{k-1}), each multiply-add
+// fused; the previous iterate x_{k-1} rides along in registers.  The residual of sweep k is the plain
 // one, |T(x_k) - x_k|, so the stop test is the plain method's.  The re-run
 // of a stopping launch makes its last sweep a plain one: the result is
 // T(x_k) for the first iterate x_k that met the tolerance.
@@ -263,10 +263,11 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                 }
             }
             if (CHEB && !plain_step) {
+                // both multiply-adds fused (one rounding each)
                 const float du = un - U[r], dv = vn - V[r];
-                const float tu = U[r] + a.gamma * du, tv = V[r] + a.gamma * dv;
-                const float nu2 = UP[r] + om * (tu - UP[r]);
-                const float nv2 = VP[r] + om * (tv - VP[r]);
+                const float tu = fmaf(a.gamma, du, U[r]), tv = fmaf(a.gamma, dv, V[r]);
+                const float nu2 = fmaf(om, tu - UP[r], UP[r]);
+                const float nv2 = fmaf(om, tv - VP[r], VP[r]);
                 UP[r] = U[r];
                 VP[r] = V[r];
                 un = nu2;

@@ -337,10 +337,18 @@ def test_hs_tol_stops_inside_a_fused_launch(ops):
         ctx.set_tuning()
 
 
+def _fma(a, b, c):
+    """float32 fused multiply-add: a*b + c rounded once (the 64-bit-mantissa
+    intermediate holds the float32 product exactly; its sum then rounds to
+    float32 as the single rounding would, but for a one-in-2^40 tie)."""
+    return (np.longdouble(a) * np.asarray(b, np.longdouble) + np.asarray(c, np.longdouble)).astype(np.float32)
+
+
 def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     """Chebyshev semi-iteration over the oracle's float32 sweep T, with the
     device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
-    g (T(x_k) - x_k)) - x_{k-1}); stop at the first x_k with plain residual
+    g (T(x_k) - x_k)) - x_{k-1}), each multiply-add fused (fma(g, d, x_k),
+    then fma(w, t - x_{k-1}, x_{k-1})); stop at the first x_k with plain residual
     <= tol (finish * tol when finish > 1) and take T(x_k); with finish > 1,
     plain sweeps then continue to tol on the same count.  Returns (U, V,
     sweeps, residual history)."""
@@ -365,8 +373,8 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
         if not r > t1:
             U, V = Tu, Tv
             break
-        nu = UP + om[k] * ((U + g * du) - UP)
-        nv = VP + om[k] * ((V + g * dv) - VP)
+        nu = _fma(om[k], _fma(g, du, U) - UP, UP)
+        nv = _fma(om[k], _fma(g, dv, V) - VP, VP)
         UP, VP, U, V = U, V, nu, nv
     else:
         return U, V, maxnum, np.array(hist)
@@ -408,8 +416,8 @@ def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
         tested = k % depth == depth - 1 or k == maxnum - 1
         if tested:
             r = res(du, dv)
-        nu = UP + om[k] * ((U + g * du) - UP)
-        nv = VP + om[k] * ((V + g * dv) - VP)
+        nu = _fma(om[k], _fma(g, du, U) - UP, UP)
+        nv = _fma(om[k], _fma(g, dv, V) - VP, VP)
         UP, VP, U, V = U, V, nu, nv
         count = k + 1
         if tested and not r > t1:
