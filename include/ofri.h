@@ -322,6 +322,7 @@ typedef struct ofri_learned_params {
     float cheb_finish;     /* HS: e.g. 4; see ofri_hs_accel */
     float ls_cheb_lo;      /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
     float ls_cheb_hi;      /* hi = 0: plain Liu-Shen sweeps */
+    int hs_finish_budget;  /* HS plain finishing sweeps at most (0: hs_budget / 2), within hs_budget */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,

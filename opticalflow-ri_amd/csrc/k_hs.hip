@@ -398,7 +398,7 @@ void hs_default(size_t npx, int* variant, int* depth) {
 
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
-             float* d_hist, float cheb_b, float cheb_switch, bool check_last) {
+             float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
     const bool cheb = cheb_b > 0.0f;
@@ -521,9 +521,9 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     } else {
         // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
         // still settle: accelerate to cheb_switch * tol, then finish with at
-        // most niter / 2 plain sweeps to tol, on one sweep count, budget and
-        // history (niter sweeps in all)
-        const int fin = niter / 2 > 0 ? niter / 2 : 1;
+        // most fin_budget (default niter / 2) plain sweeps to tol, on one
+        // sweep count, budget and history (niter sweeps in all)
+        const int fin = fin_budget > 0 ? fin_budget : (niter / 2 > 0 ? niter / 2 : 1);
         OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter));
         ctrl_continue_k<<<1, 1, 0, ctx->stream>>>(d_ctrl, niter, fin);
         OFRI_HIP(hipGetLastError());

@@ -432,7 +432,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, nullptr, lp->hs_tol[l],
-                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true));
+                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true, lp->hs_finish_budget));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
         } else {
@@ -923,7 +923,8 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (p->levels != 2) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: 2 levels only (got %d)", p->levels);
     if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
     if (H < 8 || W < 8) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d too small", H, W);
-    if (lp->hs_budget < 1 || lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
+    if (lp->hs_budget < 1 || lp->ls_budget < 0 || lp->hs_finish_budget < 0)
+        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
     for (int l = 0; l < 2; ++l)
         if (lp->cheb_bound[l] != 0.0f && !(lp->cheb_bound[l] > 0.0f && lp->cheb_bound[l] <= 0.999f))
             return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g at level %d", lp->cheb_bound[l], l);

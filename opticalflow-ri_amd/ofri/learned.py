@@ -66,18 +66,19 @@ def plain_targets(im1, im2, fw=None, ow=None):
     return U, V, hs_tol, [float(s["ls_err"]) for s in st], st
 
 
-# Sweep budgets of the learned form per level: HS at most 120 accelerated
-# sweeps (70-100 reach the plain residual on 1024^2 PIV pairs) plus at most
-# 60 plain finishing sweeps (about 10 used), Liu-Shen 32 (5-16 used after
-# the network's correction; the plain method takes 60).  A level that misses
-# its target within them sends the pair to the plain fallback; launches past
-# a finished solve only cost their dispatch, so the budgets stay close to use.
+# Sweep budgets of the learned form per level: HS 120 sweeps in all (the
+# accelerated phase takes 60-80 on 1024^2 PIV pairs), of them at most 30
+# plain finishing sweeps (about 10 used), Liu-Shen 24 (5-16 used after the
+# network's correction; the plain method takes 60).  A level that misses its
+# target within them sends the pair to the plain fallback; launches past a
+# finished solve still cost their dispatch, so the budgets stay close to use.
 HS_BUDGET = 120
-LS_BUDGET = 32
+HS_FINISH_BUDGET = 30
+LS_BUDGET = 24
 
 
 def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None, fallback=True, stats=True,
-          hs_budget=HS_BUDGET, ls_budget=LS_BUDGET, ls_cheb=LS_CHEB):
+          hs_budget=HS_BUDGET, ls_budget=LS_BUDGET, ls_cheb=LS_CHEB, hs_finish_budget=HS_FINISH_BUDGET):
     """One pair through the learned form.  Returns (U, V, info) with info:
     converged, fell_back, per-level stats."""
     if fw is None:
@@ -85,6 +86,7 @@ def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None,
     U, V, st, conv = ops.pyr_lshs_learned(im1, im2, net=net, hs_tol=hs_tol, ls_tol=ls_tol, filt_w=fw, opt_w=ow,
                                           hs_budget=hs_budget, ls_budget=ls_budget, cheb_bound=CHEB_BOUND,
                                           cheb_finish=CHEB_FINISH, ls_cheb=ls_cheb, Uacc=Uacc, Vacc=Vacc,
+                                          hs_finish_budget=min(hs_finish_budget, hs_budget // 2 or 1),
                                           stats=stats or fallback,
                                           **{k: WORKLOAD[k] for k in ("levels", "alphas", "niter", "ls_alpha")})
     fell = False
@@ -115,7 +117,8 @@ def heldout_pairs(n, size=1024):
     return out, specs
 
 
-def solve_many(jobs, net, fw=None, ow=None, flags=None, hs_budget=HS_BUDGET, ls_budget=LS_BUDGET):
+def solve_many(jobs, net, fw=None, ow=None, flags=None, hs_budget=HS_BUDGET, ls_budget=LS_BUDGET,
+               hs_finish_budget=HS_FINISH_BUDGET):
     """The learned form for many pairs with one host synchronisation: every
     pair is queued with its convergence flag written on the device, the flags
     are read once, and the pairs that missed a target run the plain pyramid.
@@ -130,6 +133,7 @@ def solve_many(jobs, net, fw=None, ow=None, flags=None, hs_budget=HS_BUDGET, ls_
         ops.pyr_lshs_learned(a, b, net=net, hs_tol=ht, ls_tol=lt, filt_w=fw, opt_w=ow, hs_budget=hs_budget,
                              ls_budget=ls_budget, cheb_bound=CHEB_BOUND, cheb_finish=CHEB_FINISH, Uacc=U, Vacc=V,
                              stats=False, flag=flags.view(k, (1,)),
+                             hs_finish_budget=min(hs_finish_budget, hs_budget // 2 or 1),
                              **{key: WORKLOAD[key] for key in ("levels", "alphas", "niter", "ls_alpha")})
     ok = flags.download().reshape(-1)[:len(jobs)].view(np.int32)
     fell = []

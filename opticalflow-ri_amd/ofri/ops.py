@@ -283,7 +283,7 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
 
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
                      hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
-                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None):
+                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None, hs_finish_budget=0):
     """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
     correction network and Liu-Shen run to the per-level residual targets
     hs_tol / ls_tol.  net: an ofri.net.Net or None (the method without it).
@@ -299,7 +299,7 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     cb = (cheb_bound, cheb_bound) if np.isscalar(cheb_bound) else tuple(cheb_bound)
     lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget),
                              (C.c_float * 2)(float(cb[0]), float(cb[1])), float(cheb_finish), float(ls_cheb[0]),
-                             float(ls_cheb[1]))
+                             float(ls_cheb[1]), int(hs_finish_budget))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None
