@@ -63,6 +63,51 @@ __global__ void gauss_cols(Planes2 p, int H, int W, Taps t) {
     dst[(size_t)y * W + x] = acc;
 }
 
+// Both passes in one launch for up to 9 taps: a workgroup stages a source
+// tile (padded positions mapped through pf_index), filters its rows into LDS
+// and then its columns, with the two-pass kernels' operations and order.
+constexpr int kGTW = 64, kGTH = 16, kGMaxH = 4;
+__global__ __launch_bounds__(256) void gauss_fused(Planes2 p, int H, int W, Taps t) {
+    __shared__ float sin_[kGTH + 2 * kGMaxH][kGTW + 2 * kGMaxH];
+    __shared__ float srow[kGTH + 2 * kGMaxH][kGTW];
+    const int h = t.n / 2;
+    const int x0 = blockIdx.x * kGTW, y0 = blockIdx.y * kGTH;
+    const float* __restrict__ src = p.s[blockIdx.z];
+    float* __restrict__ dst = p.d[blockIdx.z];
+    const int R = kGTH + 2 * h, Cw = kGTW + 2 * h;
+    // source tile: positions [y0 - h, y0 + kGTH + h) x [x0 - h, x0 + kGTW + h);
+    // positions past the padding domain are never read
+    for (int i = threadIdx.x; i < R * Cw; i += 256) {
+        const int rr = i / Cw, cc = i - rr * Cw;
+        const int pr = y0 - h + rr, pc = x0 - h + cc;
+        float v = 0.0f;
+        if (pr < H + h && pc < W + h) v = src[(size_t)pf_index(pr, H, h) * W + pf_index(pc, W, h)];
+        sin_[rr][cc] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < R * kGTW; i += 256) {
+        const int rr = i / kGTW, cc = i - rr * kGTW;
+        float acc = 0.0f;
+        for (int j = 0; j < t.n; ++j) {
+            float prod = t.w[j] * sin_[rr][cc + 2 * h - j];
+            acc = acc + prod;
+        }
+        srow[rr][cc] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kGTH * kGTW; i += 256) {
+        const int ry = i / kGTW, cc = i - ry * kGTW;
+        const int y = y0 + ry, x = x0 + cc;
+        if (y >= H || x >= W) continue;
+        float acc = 0.0f;
+        for (int j = 0; j < t.n; ++j) {
+            float prod = t.w[j] * srow[ry + 2 * h - j][cc];
+            acc = acc + prod;
+        }
+        dst[(size_t)y * W + x] = acc;
+    }
+}
+
 // Pillow resample, horizontal pass over the rows [first, first+Ht).
 __global__ void resize_h(Planes2 p, int W, int W2, int Ht, int first, const double* __restrict__ k,
                          const int* __restrict__ b, int ks) {
@@ -628,6 +673,15 @@ int gauss_sep2(ofri_ctx* ctx, const float* src1, float* dst1, const float* src2,
     for (int i = 0; i < n; ++i) t.w[i] = w[i];
     t.n = n;
     const int planes = src2 ? 2 : 1;
+    // one launch unless an output is also an input (the staged tiles of other
+    // workgroups would read what it writes): then the two passes via tmp
+    const bool aliased = dst1 == src1 || (src2 && (dst1 == src2 || dst2 == src1 || dst2 == src2));
+    if (n <= 2 * kGMaxH + 1 && !aliased) {
+        dim3 g((W + kGTW - 1) / kGTW, (H + kGTH - 1) / kGTH, planes);
+        gauss_fused<<<g, 256, 0, ctx->stream>>>(Planes2{{src1, src2}, {dst1, dst2}}, H, W, t);
+        OFRI_HIP(hipGetLastError());
+        return OFRI_OK;
+    }
     void* tmp;
     OFRI_TRY(scratch(ctx, "gauss_tmp", (size_t)planes * H * W * sizeof(float), &tmp));
     float* t0 = (float*)tmp;
