@@ -71,6 +71,9 @@ struct HsPass {
     int slot;                // launch parity within the phase (partials / state slot written)
     SolveCtrl* ctrl;
     int final_;              // 1: the phase's final launch (test, re-run, publish)
+    int next_budget;         // final launch: > 0 arms a following phase of this many sweeps
+    int rel_src;             // 1: buffer indices relative to ctrl->base_buf
+    int zero_start;          // 1: the solve starts from U = V = 0 (the first sweep reads no input)
     double tol;
     int maxnum;
     size_t N;
@@ -166,11 +169,14 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     float UP[CHEB ? R : 1], VP[CHEB ? R : 1];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
+    // a zero start: the launch holding the first sweep (or its re-run) reads
+    // no flow
+    const bool zs = a.zero_start && (NORM ? base : a.sweep0) == 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
-        U[r] = uin[q];
-        V[r] = vin[q];
+        U[r] = zs ? 0.0f : uin[q];
+        V[r] = zs ? 0.0f : vin[q];
         if (CHEB) {
             // x_{-1} = x_0 before the first sweep
             UP[r] = base == 0 ? U[r] : a.pin[src][0][q];
@@ -398,9 +404,11 @@ void hs_default(size_t npx, int* variant, int* depth) {
 
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
-             float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget) {
+             float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget,
+             bool zero_start) {
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
+    if (zero_start && d_err) return fail(OFRI_E_ARG, "hs: a zero start keeps no initial flow for the error");
     const bool cheb = cheb_b > 0.0f;
     if (cheb && (!d_ctrl || !(cheb_b <= 0.999f)))
         return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 0.999], got %g", cheb_b);
@@ -482,7 +490,13 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     // one phase: up to niter sweeps (fewer when the control word's shared
     // count reaches the budget), the stopping launch re-run, the result
     // copied back into U, V
-    auto phase = [&](void (*kern)(HsPass), double t, int sweeps) -> int {
+    // one phase: up to `sweeps` sweeps (fewer when the control word's shared
+    // count reaches the budget), then the final launch (test of the last
+    // launch, re-run of a launch stopped inside, result published).  Launch
+    // parities start at slot0; next_budget > 0 makes the final launch arm a
+    // following phase instead of copying the result back into U, V.
+    // Returns the number of regular launches in *nl.
+    auto phase = [&](void (*kern)(HsPass), double t, int sweeps, int slot0, int next_budget, int* nl) -> int {
         p.tol = t;
         int done = 0, src = 0, j = 0;
         while (done < sweeps) {
@@ -490,23 +504,24 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
             p.nsweeps = ns;
             p.src = src;
             p.sweep0 = done;
-            p.slot = j & 1;
+            p.slot = (slot0 + j) & 1;
             p.final_ = 0;
+            p.next_budget = 0;
             kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
             OFRI_HIP(hipGetLastError());
             src ^= 1;
             done += ns;
             ++j;
         }
+        if (nl) *nl = j;
         if (d_ctrl) {
             if (niter > 0) {
-                // tests the last launch, re-runs a launch the test stopped
-                // inside, publishes the result
-                p.slot = j & 1;
+                p.slot = (slot0 + j) & 1;
                 p.final_ = 1;
+                p.next_budget = next_budget;
                 kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
                 OFRI_HIP(hipGetLastError());
-                OFRI_TRY(select_copy2(ctx, d_ctrl, U, Ub, U, V, Vb, V, N));
+                if (next_budget <= 0) OFRI_TRY(select_copy2(ctx, d_ctrl, U, Ub, U, V, Vb, V, N));
             }
         } else if (src != 0) {
             OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
@@ -514,20 +529,26 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         }
         return OFRI_OK;
     };
+    p.rel_src = 0;
+    p.zero_start = zero_start ? 1 : 0;
     if (!cheb) {
-        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol, niter));
+        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol, niter, 0, 0, nullptr));
     } else if (!(cheb_switch > 1.0f)) {
-        OFRI_TRY(phase(var.cheb, tol, niter));
+        OFRI_TRY(phase(var.cheb, tol, niter, 0, 0, nullptr));
     } else {
         // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
         // still settle: accelerate to cheb_switch * tol, then finish with at
         // most fin_budget (default niter / 2) plain sweeps to tol, on one
-        // sweep count, budget and history (niter sweeps in all)
+        // sweep count, budget and history (niter sweeps in all).  The
+        // finishing phase starts from the accelerated phase's result buffer
+        // (its launches take buffers relative to it) and the opposite parity.
         const int fin = fin_budget > 0 ? fin_budget : (niter / 2 > 0 ? niter / 2 : 1);
-        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter));
-        ctrl_continue_k<<<1, 1, 0, ctx->stream>>>(d_ctrl, niter, fin);
-        OFRI_HIP(hipGetLastError());
-        OFRI_TRY(phase(var.normed, tol, fin));
+        int nl = 0;
+        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter, 0, niter > 0 ? fin : 0, &nl));
+        p.rel_src = 1;
+        p.zero_start = 0;
+        OFRI_TRY(phase(var.normed, tol, fin, nl + 1, 0, nullptr));
+        p.rel_src = 0;
     }
     if (d_err) OFRI_TRY(diff_norm_err(ctx, U, U0, V, V0, N, d_err));
     return OFRI_OK;

@@ -48,6 +48,7 @@ __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restric
         ctrl->redo_in = 0;
         ctrl->last_err = 0.0f;
         ctrl->limit = 0;
+        ctrl->base_buf = 0;
         ctrl->slot[0] = SolveState{};
         ctrl->slot[1] = SolveState{};
     }
@@ -107,6 +108,8 @@ struct LsPass {
     float* hist;            // per-sweep norms (nullptr: not recorded)
     SolveCtrl* ctrl;
     int final_;             // 1: the final launch (test, re-run, publish)
+    int next_budget;        // (a following phase: unused here)
+    int rel_src;            // (relative buffers: unused here)
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
     int check_last;         // 1: only a launch's last sweep is reduced and tested
     // Chebyshev mode (ls_pass_k<..., true>): previous iterate buffers, the

@@ -427,12 +427,14 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             s1 = f1;
             s2 = f2;
         }
-        OFRI_TRY(fill_zero2(ctx, U, V, N));
+        // (the learned form's HS starts from zero without reading U, V)
+        if (!lrn) OFRI_TRY(fill_zero2(ctx, U, V, N));
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, nullptr, lp->hs_tol[l],
-                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true, lp->hs_finish_budget));
+                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true, lp->hs_finish_budget,
+                              true));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
         } else {

@@ -60,7 +60,7 @@ struct SolveCtrl {
     int redo_in;
     float last_err;
     int limit;
-    int pad_;
+    int base_buf;    // ping-pong index a following phase starts from (its launches' buffers are relative to it)
     SolveState slot[2];
 };
 
@@ -160,7 +160,7 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
-             float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0);
+             float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0, bool zero_start = false);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);

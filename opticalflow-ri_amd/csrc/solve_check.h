@@ -88,11 +88,15 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
 
 // A launch's sweep count, input buffer and first global sweep in residual
 // mode.  Pass provides ctrl, part (two slots of part_stride doubles), slot
-// (launch parity within the phase), final_, nsweeps, src, sweep0,
+// (launch parity), final_, next_budget, rel_src, nsweeps, src, sweep0,
 // check_last, tol, maxnum, N and hist.  A regular launch runs unless the
 // solve is done, and its workgroup 0 publishes the state with this launch
-// pending; the phase's final launch (final_) runs only the re-run of a
-// launch whose test held inside it, and publishes the result.
+// pending; its buffers are relative to ctrl->base_buf when rel_src.  The
+// phase's final launch (final_) runs only the re-run of a launch whose test
+// held inside it, and publishes the result; with next_budget > 0 it also
+// arms a following phase of at most that many sweeps (starting from the
+// result's buffer, reading the state slot this launch writes: its launches
+// take the opposite parity).
 template <class Pass>
 __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bool always_redo, int& nsw, int& src,
                                             int& base) {
@@ -113,11 +117,21 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
             c->redo_in = s.redo_in;
             c->last_err = s.last_err;
             c->limit = s.limit;
+            if (a.next_budget > 0) {
+                SolveState n{};
+                n.done = s.sweeps >= a.maxnum;
+                n.sweeps = s.sweeps;
+                n.final_buf = s.final_buf;
+                n.last_err = s.last_err;
+                n.limit = s.sweeps + a.next_budget;
+                c->slot[a.slot] = n;
+                c->base_buf = s.final_buf;
+            }
         }
         return nsw > 0;
     }
     nsw = a.nsweeps;
-    src = a.src;
+    src = a.src ^ (a.rel_src ? a.ctrl->base_buf : 0);
     base = a.sweep0;
     const bool runs = !s.done;
     if (lead) {
@@ -181,6 +195,8 @@ __global__ void solve_start_k(SolveCtrl* c, OmegaTab t, float* w, int n) {
         c->redo_in = 0;
         c->last_err = 0.0f;
         c->limit = 0;
+    c->base_buf = 0;
+        c->base_buf = 0;
         c->slot[0] = SolveState{};
         c->slot[1] = SolveState{};
     }
@@ -195,6 +211,7 @@ __global__ void ctrl_init_k(SolveCtrl* c) {
     c->redo_in = 0;
     c->last_err = 0.0f;
     c->limit = 0;
+    c->base_buf = 0;
     c->slot[0] = SolveState{};
     c->slot[1] = SolveState{};
 }
