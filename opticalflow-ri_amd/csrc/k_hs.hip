@@ -37,11 +37,37 @@ __device__ __forceinline__ float from_right(float x) {
 
 // Derivatives: HS(im2=I0, im1=I1) -> computeDerivatives(I1, I0)
 // (HornSchunck.py:36,72,83,106-126), 2x2 stencils with mirror padding.
+// What the first workgroup of the derivative launch also sets up for a
+// solve with a stop test: its control word cleared (ctrl non-null) and n
+// Chebyshev weights written.
+struct SolveStart {
+    SolveCtrl* ctrl;
+    float* omega;
+    int n;
+    OmegaTab tab;
+};
+
 __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restrict__ I1, int H,
                             int W, float* __restrict__ fx, float* __restrict__ fy,
-                            float* __restrict__ ft) {
+                            float* __restrict__ ft, SolveStart st) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        if (st.ctrl && threadIdx.x == 0) {
+            SolveCtrl* c = st.ctrl;
+            c->done = 0;
+            c->sweeps = 0;
+            c->final_buf = 0;
+            c->redo = 0;
+            c->redo_in = 0;
+            c->last_err = 0.0f;
+            c->limit = 0;
+            c->base_buf = 0;
+            c->slot[0] = SolveState{};
+            c->slot[1] = SolveState{};
+        }
+        for (int k = threadIdx.x; k < st.n; k += blockDim.x) st.omega[k] = st.tab.w[k < kOmegaTab ? k : kOmegaTab - 1];
+    }
     if (j >= W) return;
     int i1 = (i + 1 < H) ? i + 1 : H - 2;
     int j1 = (j + 1 < W) ? j + 1 : W - 2;
@@ -446,25 +472,26 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         P0 = (float*)cb;
         omega = P0 + 4 * N;
     }
-    dim3 g((W + 255) / 256, H);
-    // computeDerivatives(I1=im2, I0=im1)
-    hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft);
-    OFRI_HIP(hipGetLastError());
-    if (d_err) {
-        OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-    }
     // spectrum of one sweep's linear part lies in [-1/3, b]: the 8-neighbour
     // average has eigenvalues in [-1/3, 1] and the per-pixel data projection
     // only shrinks them; b < 1 is the caller's bound on the slowest mode.
     // (The first accelerated launch takes the previous iterate from U, V.)
     const double lo = -1.0 / 3.0, hi = cheb_b;
+    SolveStart start{};
+    start.ctrl = d_ctrl;
     if (cheb) {
-        solve_start_k<<<1, 256, 0, ctx->stream>>>(d_ctrl, cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, niter);
-        OFRI_HIP(hipGetLastError());
-    } else if (d_ctrl) {
-        ctrl_init_k<<<1, 1, 0, ctx->stream>>>(d_ctrl);
-        OFRI_HIP(hipGetLastError());
+        start.omega = omega;
+        start.n = niter;
+        start.tab = cheb_omegas((hi - lo) / (2.0 - lo - hi));
+    }
+    dim3 g((W + 255) / 256, H);
+    // computeDerivatives(I1=im2, I0=im1); the first workgroup also clears the
+    // control word and writes the Chebyshev weights
+    hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft, start);
+    OFRI_HIP(hipGetLastError());
+    if (d_err) {
+        OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        OFRI_HIP(hipMemcpyAsync(V0, V, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     }
     HsPass p{};
     p.in[0][0] = U; p.in[0][1] = V; p.in[1][0] = Ub; p.in[1][1] = Vb;
@@ -598,7 +625,7 @@ int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter,
         float *U0 = f + 5 * N, *V0 = f + 6 * N;
         f += 7 * N;
         dim3 g((W + 255) / 256, H);
-        hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, fx, fy, ft);
+        hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, fx, fy, ft, SolveStart{});
         OFRI_HIP(hipGetLastError());
         if (im[i].d_err) {
             OFRI_HIP(hipMemcpyAsync(U0, im[i].U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
