@@ -356,12 +356,25 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
         }
     }
 
-    // D layout: lane holds m-tile pixels 4g..4g+3, channel r16
+    // D layout: lane holds m-tile pixels 4g..4g+3, channel r16.  The
+    // channel's part of the destination offset (and, through depth-to-space,
+    // its sub-pixel) is formed once per channel; pixels add 32-bit offsets.
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int co = cobase + nt * 16 + r16;
         if (co >= a.Cout) continue;
         const float bv = a.bias ? a.bias[co] : 0.0f;
+        int cc = co, dy = 0, dx = 0, sh = 0;
+        if (a.y.mode) {
+            const int Cs = a.Cout >> 2;
+            const int sub = co / Cs;
+            cc = co - sub * Cs;
+            dy = sub >> 1;
+            dx = sub & 1;
+            sh = 1;
+        }
+        const int Wd = W << sh;
+        const long long base_off = (long long)b * a.y.img + cc;
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -371,13 +384,13 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
                 if (y >= H || x >= W) continue;
                 float v = acc[mt][nt][i] + bv;
                 if (a.act) v = v > 0.0f ? v : v * kSlope;
-                const long long off = view_off(a.y, b, y, x, co, H, W, a.Cout);
+                const int po = (((y << sh) + dy) * Wd + ((x << sh) + dx)) * a.y.ps;
                 if (a.y.f32) {
-                    float* d = (float*)a.y.p + off;
+                    float* d = (float*)a.y.p + base_off + po;
                     if (a.beta) v = *d + v;
                     *d = v;
                 } else {
-                    u16* d = (u16*)a.y.p + off;
+                    u16* d = (u16*)a.y.p + base_off + po;
                     if (a.beta) v = bf2f(*d) + v;
                     *d = f2bf(v);
                 }
