@@ -266,30 +266,56 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
         // registers while the current chunk's MFMAs run
         constexpr int XN = (PS * PS * 4 + 255) / 256, WN = (NT * 16 * 36 + 255) / 256;
         uint4 rx[XN], rw[WN];
+        // per-thread load offsets at channel 0, formed once: a direct input
+        // and the weights then only add the chunk's first channel (-1: the
+        // patch pixel lies outside the image or the slot is unused)
+        const bool direct = a.x.mode == 0;
+        long long xo[XN];
+        int xpart[XN];
+#pragma unroll
+        for (int k = 0; k < XN; ++k) {
+            const int idx = t + k * 256;
+            xo[k] = -1;
+            xpart[k] = (idx & 3) * 8;
+            if (idx < PS * PS * 4) {
+                const int pix = idx >> 2;
+                const int py = pix / PS, px = pix - py * PS;
+                const int gy = y0 - 1 + py, gx = x0 - 1 + px;
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    xo[k] = direct ? view_off(a.x, b, gy, gx, xpart[k], H, W, a.Cin)
+                                   : (((long long)gy << 32) | (unsigned)gx);
+            }
+        }
+        long long wo[WN];
+        int wpart[WN];
+#pragma unroll
+        for (int k = 0; k < WN; ++k) {
+            const int idx = t + k * 256;
+            wo[k] = -1;
+            wpart[k] = 0;
+            if (idx < NT * 16 * 36) {
+                const int co = idx / 36, rem = idx - co * 36;
+                const int tap = rem >> 2, part = rem & 3;
+                wpart[k] = part * 8;
+                wo[k] = (long long)(cobase + co) * a.Kp + tap * a.Cin + part * 8;
+            }
+        }
         auto fetch = [&](int c0) {
 #pragma unroll
             for (int k = 0; k < XN; ++k) {
-                const int idx = t + k * 256;
                 rx[k] = uint4{0u, 0u, 0u, 0u};
-                if (idx < PS * PS * 4) {
-                    const int pix = idx >> 2, part = idx & 3;
-                    const int py = pix / PS, px = pix - py * PS;
-                    const int gy = y0 - 1 + py, gx = x0 - 1 + px;
-                    const int ch = c0 + part * 8;
-                    if (gy >= 0 && gy < H && gx >= 0 && gx < W && ch < a.Cin)
-                        rx[k] = *(const uint4*)((const u16*)a.x.p + view_off(a.x, b, gy, gx, ch, H, W, a.Cin));
+                if (xo[k] >= 0 && c0 + xpart[k] < a.Cin) {
+                    long long off;
+                    if (direct) off = xo[k] + c0;
+                    else off = view_off(a.x, b, (int)(xo[k] >> 32), (int)(xo[k] & 0xffffffff), c0 + xpart[k], H, W,
+                                        a.Cin);
+                    rx[k] = *(const uint4*)((const u16*)a.x.p + off);
                 }
             }
 #pragma unroll
             for (int k = 0; k < WN; ++k) {
-                const int idx = t + k * 256;
                 rw[k] = uint4{0u, 0u, 0u, 0u};
-                if (idx < NT * 16 * 36) {
-                    const int co = idx / 36, rem = idx - co * 36;
-                    const int tap = rem >> 2, part = rem & 3;
-                    const int ch = c0 + part * 8;
-                    if (ch < a.Cin) rw[k] = *(const uint4*)(a.w + (size_t)(cobase + co) * a.Kp + tap * a.Cin + ch);
-                }
+                if (wo[k] >= 0 && c0 + wpart[k] < a.Cin) rw[k] = *(const uint4*)(a.w + wo[k] + c0);
             }
         };
         auto put = [&]() {
@@ -359,6 +385,40 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
     // D layout: lane holds m-tile pixels 4g..4g+3, channel r16.  The
     // channel's part of the destination offset (and, through depth-to-space,
     // its sub-pixel) is formed once per channel; pixels add 32-bit offsets.
+    // A bfloat16 output written (not accumulated) from a tile inside the
+    // image takes a straight-line path.
+    if (!a.y.f32 && !a.beta && y0 + TS <= H && x0 + TS <= W) {
+        const bool act = a.act;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = cobase + nt * 16 + r16;
+            if (co >= a.Cout) continue;
+            const float bv = a.bias ? a.bias[co] : 0.0f;
+            int cc = co, dy = 0, dx = 0, sh = 0;
+            if (a.y.mode) {
+                const int Cs = a.Cout >> 2;
+                const int sub = co / Cs;
+                cc = co - sub * Cs;
+                dy = sub >> 1;
+                dx = sub & 1;
+                sh = 1;
+            }
+            const int Wd = W << sh;
+            u16* dbase = (u16*)a.y.p + ((long long)b * a.y.img + cc);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int p = (wv * MT + mt) * 16 + 4 * g + i;
+                    const int y = y0 + p / TS, x = x0 + p % TS;
+                    float v = acc[mt][nt][i] + bv;
+                    v = act ? (v > 0.0f ? v : v * kSlope) : v;
+                    dbase[(((y << sh) + dy) * Wd + ((x << sh) + dx)) * a.y.ps] = f2bf(v);
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int co = cobase + nt * 16 + r16;
