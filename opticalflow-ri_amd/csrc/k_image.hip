@@ -421,10 +421,17 @@ struct Select2 {
     const float* s1[2];
     float* d[2];
 };
-__global__ void select_copy_k(const SolveCtrl* c, Select2 p, size_t n) {
+__global__ void select_copy_k(const SolveCtrl* c, Select2 p, size_t n, int add) {
     const int z = blockIdx.z;
     const float* s = c->final_buf ? p.s1[z] : p.s0[z];
     float* d = p.d[z];
+    if (add) {
+        // d += the selected plane (the same float add as accumulate_k)
+        size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        size_t stride = (size_t)gridDim.x * blockDim.x;
+        for (; i < n; i += stride) d[i] = d[i] + s[i];
+        return;
+    }
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) d[i] = s[i];
@@ -813,9 +820,9 @@ int fill_zero2(ofri_ctx* ctx, float* p1, float* p2, size_t n) {
 int fill_zero(ofri_ctx* ctx, float* p, size_t n) { return fill_zero2(ctx, p, nullptr, n); }
 
 int select_copy2(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* a0, const float* a1, float* ad,
-                 const float* b0, const float* b1, float* bd, size_t n) {
+                 const float* b0, const float* b1, float* bd, size_t n, bool add) {
     select_copy_k<<<dim3(grid1d(n), 1, bd ? 2 : 1), 256, 0, ctx->stream>>>(ctrl, Select2{{a0, b0}, {a1, b1}, {ad, bd}},
-                                                                            n);
+                                                                            n, add ? 1 : 0);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }

@@ -424,7 +424,8 @@ void ls_default(size_t npx, int* variant, int* depth) {
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
-             float cheb_hi) {
+             float cheb_hi, float* sink_u, float* sink_v, bool sink_add) {
+    if ((sink_u == nullptr) != (sink_v == nullptr)) return fail(OFRI_E_ARG, "ls: one result sink given without the other");
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
     const bool cheb = cheb_hi > 0.0f;
@@ -509,8 +510,10 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.slot = j & 1;
         p.final_ = 1;
         kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
-        OFRI_TRY(select_copy2(ctx, d_ctrl, u, ub, u, v, vb, v, N));
+        if (!sink_u) OFRI_TRY(select_copy2(ctx, d_ctrl, u, ub, u, v, vb, v, N));
     }
+    // the result (the start when no sweep ran: the control word says buffer 0)
+    if (sink_u) OFRI_TRY(select_copy2(ctx, d_ctrl, u, ub, sink_u, v, vb, sink_v, N, sink_add));
     return OFRI_OK;
 }
 

@@ -391,6 +391,9 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
     SolveCtrl* hs_ctrls = ctrls + L;
     float* hs_errs = (float*)(hs_ctrls + L);
 
+    // Liu-Shen writes (adds) its result straight into the accumulators
+    // unless the taps need it in U, V
+    const bool ls_sink = p->ls_alpha > 0.0f && !d_taps;
     float *acc_u = Ua, *acc_v = Va;
     float* tap = d_taps;
     int ph = 0, pw = 0;
@@ -419,7 +422,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             OFRI_TRY(warp_symmetric(ctx, c1, c2, out_u, out_v, h, w, w1, w2));
             s1 = w1;
             s2 = w2;
-        } else {
+        } else if (!ls_sink) {
             OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
         }
         if (p->filt_w && p->filt_n > 0) {
@@ -456,19 +459,24 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
                 const float* planes[4] = {q1, q2, U, V};
                 OFRI_TRY(net_correct(ctx, lrn->net, planes, h, w, U, V));
             }
-            // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
+            // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U.
+            // Without taps the result lands in the accumulators directly
+            // (copied at the first level, whose accumulators start empty).
+            float* su = ls_sink ? out_v : nullptr;
+            float* sv = ls_sink ? out_u : nullptr;
             if (lrn)
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
-                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi));
+                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi, su, sv, l > 0));
             else
-                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l, false,
+                                  0.0f, 0.0f, su, sv, l > 0));
         }
         if (tap) {
             OFRI_TRY(subtract(ctx, tap + 2 * N, U, tap, N));
             OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
             tap += 4 * N;
         }
-        OFRI_TRY(accumulate2(ctx, out_u, U, out_v, V, N));
+        if (!ls_sink) OFRI_TRY(accumulate2(ctx, out_u, U, out_v, V, N));
         acc_u = out_u;
         acc_v = out_v;
         ph = h;

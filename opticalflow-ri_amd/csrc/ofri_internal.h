@@ -144,8 +144,9 @@ int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
                   size_t n, float* d_out);
 // dst[i] = src[ctrl->final_buf][i] for a ping-pong pair
+// add: ad += selected, bd += selected instead of copies
 int select_copy2(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* a0, const float* a1, float* ad,
-                 const float* b0, const float* b1, float* bd, size_t n);
+                 const float* b0, const float* b1, float* bd, size_t n, bool add = false);
 int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const float* s1, float* dst,
                 size_t n);
 
@@ -178,9 +179,12 @@ int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw
 // (the solve may run up to depth - 1 sweeps past the first one that met tol)
 // cheb_hi > 0: Chebyshev-accelerated sweeps for a spectrum in
 // [cheb_lo, cheb_hi] (cheb_lo in (-1, 0)).
+// sink_u / sink_v (optional): the result goes there instead of into u, v
+// (added when sink_add, else copied); u, v then hold the start only.
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false,
-             float cheb_lo = 0.0f, float cheb_hi = 0.0f);
+             float cheb_lo = 0.0f, float cheb_hi = 0.0f, float* sink_u = nullptr, float* sink_v = nullptr,
+             bool sink_add = false);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets
