@@ -19,6 +19,12 @@
 // is grouped symmetrically (vertical pairs, corners, edges), so a tile loaded
 // as the mirror extension of the image stays an exact mirror extension under
 // the sweeps and the border needs no special case.
+//
+// The Chebyshev-accelerated phase of the learned form (hs_cheb_tile) is not
+// held to the plain method's iterates: it forms its sweep in float32 with
+// fused multiply-adds, and plain sweeps finish every solve it starts.
+#include <type_traits>
+
 #include "ofri_internal.h"
 #include "solve_check.h"
 
@@ -171,15 +177,7 @@ __device__ __forceinline__ float quotient(float num, float den, float rcp) {
 // the mirror symmetry of the extension is preserved exactly by every sweep.
 // One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
 // one being number blk; nsw sweeps from ping-pong buffer src.
-//
-// CHEB: Chebyshev semi-iteration over the same sweep T.  With the damped step
-// T_g(x) = x + gamma (T(x) - x), whose spectrum lies in [-sigma, sigma],
-// x_{k+1} = x_{k-1} + omega_{k+1} (T_g(x_k) - x_{k-1}), each multiply-add
-// fused; the previous iterate x_{k-1} rides along in registers.  The residual of sweep k is the plain
-// one, |T(x_k) - x_k|, so the stop test is the plain method's.  The re-run
-// of a stopping launch makes its last sweep a plain one: the result is
-// T(x_k) for the first iterate x_k that met the tolerance.
-template <int R, int WAVES, bool NORM, bool CHEB = false>
+template <int R, int WAVES, bool NORM>
 __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int base, int bx, int by,
                                         int blk, int nblk) {
     const int lane = threadIdx.x & 63;
@@ -192,7 +190,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
     float U[R], V[R], FX[R], FY[R], FT[R], DEN[R], RCP[R];
-    float UP[CHEB ? R : 1], VP[CHEB ? R : 1];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
     // a zero start: the launch holding the first sweep (or its re-run) reads
@@ -203,11 +200,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
         U[r] = zs ? 0.0f : uin[q];
         V[r] = zs ? 0.0f : vin[q];
-        if (CHEB) {
-            // x_{-1} = x_0 before the first sweep
-            UP[r] = base == 0 ? U[r] : a.pin[src][0][q];
-            VP[r] = base == 0 ? V[r] : a.pin[src][1][q];
-        }
         FX[r] = a.fx[q];
         FY[r] = a.fy[q];
         FT[r] = a.ft[q];
@@ -218,9 +210,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         // otherwise redo this division (a dozen instructions) every sweep
         asm volatile("" : "+v"(RCP[r]));
     }
-    // the launch's Chebyshev weights, one per lane, broadcast per sweep
-    float om_lane = 1.0f;
-    if (CHEB && lane < nsw) om_lane = a.omega[base + lane];
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
     // per-wave squared-update sums of a sweep, double-buffered: the next
@@ -255,13 +244,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         Trip up_u = trip(uup), up_v = trip(vup);
         Trip cu = trip(U[0]), cv = trip(V[0]);
         double su = 0.0, sv = 0.0;
-        float om = 1.0f;
-        bool plain_step = true;
-        if (CHEB) {
-            om = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(om_lane), s));
-            // the re-run of a stopping launch ends with a plain sweep
-            plain_step = a.final_ && s == nsw - 1;
-        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Trip dn_u = trip(r + 1 < R ? U[r + 1] : udn);
@@ -293,20 +275,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                     su = su + (double)du * (double)du;
                     sv = sv + (double)dv * (double)dv;
                 }
-            }
-            if (CHEB && !plain_step) {
-                // both multiply-adds fused (one rounding each)
-                const float du = un - U[r], dv = vn - V[r];
-                const float tu = fmaf(a.gamma, du, U[r]), tv = fmaf(a.gamma, dv, V[r]);
-                const float nu2 = fmaf(om, tu - UP[r], UP[r]);
-                const float nv2 = fmaf(om, tv - VP[r], VP[r]);
-                UP[r] = U[r];
-                VP[r] = V[r];
-                un = nu2;
-                vn = nv2;
-            } else if (CHEB) {
-                UP[r] = U[r];
-                VP[r] = V[r];
             }
             U[r] = un;
             V[r] = vn;
@@ -341,14 +309,174 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                 size_t q = (size_t)gy * W + gx;
                 uo[q] = U[r];
                 vo[q] = V[r];
-                if (CHEB) {
-                    a.pout[src ^ 1][0][q] = UP[r];
-                    a.pout[src ^ 1][1][q] = VP[r];
-                }
             }
         }
     }
+}
 
+// Lane shifts whose vacated edge lane reads 0 (bound_ctrl), so the compiler
+// can fold them into the consuming add as a DPP operand.  Only the tile's
+// outermost lanes see the 0, and they lie in the halo that is never written.
+__device__ __forceinline__ float lane_left0(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_right0(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
+}
+
+// The accelerated phase's tile: Chebyshev semi-iteration over the sweep T.
+// With the damped step T_g(x) = x + gamma (T(x) - x), whose spectrum lies in
+// [-sigma, sigma],
+//     x_{k+1} = x_{k-1} + omega_{k+1} (T_g(x_k) - x_{k-1}),
+// both multiply-adds fused; the previous iterate x_{k-1} rides along in
+// registers.  These iterates are not the plain method's, and the phase only
+// has to bring the residual |T(x_k) - x_k| to a multiple of the target before
+// plain sweeps finish the solve, so T is formed in float32 throughout: the
+// 3x3 average as vertical pair sums p, then corners p_l + p_r and edges
+// p + (x_l + x_r) weighted with one FMA (every addition still pairs
+// mirror-image operands, so the mirror-extended tile stays exact under the
+// sweeps), num = fx ub + fy vb + ft and T = ub - fx num / den with FMAs and
+// the held float32 reciprocal of den.  The residual tested is that of this
+// T.  The re-run of a stopping launch makes its last sweep a plain one: the
+// result is T(x_k) for the first iterate x_k that met the tolerance.
+template <int R, int WAVES>
+__device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, int base, int bx, int by,
+                                             int blk, int nblk) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K = a.K, H = a.H, W = a.W;
+    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
+    const int gx = bx * outW - K + lane;
+    const int gy0 = by * outH - K + w * R;
+    const int cx = mirror_idx(gx, W);
+    const bool own_x = lane >= K && lane < 64 - K && gx < W;
+
+    float U[R], V[R], UP[R], VP[R], FX[R], FY[R], FT[R], RCP[R];
+    const float* uin = a.in[src][0];
+    const float* vin = a.in[src][1];
+    // a zero start: the launch holding the first sweep (or its re-run) reads
+    // no flow
+    const bool zs = a.zero_start && base == 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
+        U[r] = zs ? 0.0f : uin[q];
+        V[r] = zs ? 0.0f : vin[q];
+        // x_{-1} = x_0 before the first sweep
+        UP[r] = base == 0 ? U[r] : a.pin[src][0][q];
+        VP[r] = base == 0 ? V[r] : a.pin[src][1][q];
+        FX[r] = a.fx[q];
+        FY[r] = a.fy[q];
+        FT[r] = a.ft[q];
+        RCP[r] = 1.0f / fmaf(FY[r], FY[r], fmaf(FX[r], FX[r], a.a2));
+    }
+    // the launch's Chebyshev weights, one per lane, broadcast per sweep
+    float om_lane = 1.0f;
+    if (lane < nsw) om_lane = a.omega[base + lane];
+
+    __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
+    __shared__ double red[2][WAVES][2];
+    auto normed = [&](int sw) { return !a.final_ && (!a.check_last || sw == nsw - 1); };
+    double* part = a.part + (size_t)a.slot * a.part_stride;
+    auto flush = [&](int sw) {
+        double tu = 0.0, tv = 0.0;
+        for (int q = 0; q < WAVES; ++q) {
+            tu = tu + red[sw & 1][q][0];
+            tv = tv + red[sw & 1][q][1];
+        }
+        part[((size_t)sw * nblk + blk) * 2] = tu;
+        part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
+    };
+    const float wc = (float)(1.0 / 12.0), we = (float)(1.0 / 6.0), g = a.gamma;
+    // one sweep; NS: reduce its residual, PL: a plain sweep (x_{k+1} = T(x_k))
+    auto sweep = [&](int s, auto ns_c, auto pl_c) {
+        constexpr bool NS = decltype(ns_c)::value;
+        constexpr bool PL = decltype(pl_c)::value;
+        const int b = s & 1;
+        edge[b][w][0][0][lane] = U[0];
+        edge[b][w][0][1][lane] = V[0];
+        edge[b][w][1][0][lane] = U[R - 1];
+        edge[b][w][1][1][lane] = V[R - 1];
+        __syncthreads();
+        if (s > 0 && normed(s - 1) && threadIdx.x == 0) flush(s - 1);
+        float up_u = w > 0 ? edge[b][w - 1][1][0][lane] : U[0];
+        float up_v = w > 0 ? edge[b][w - 1][1][1][lane] : V[0];
+        const float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : U[R - 1];
+        const float vdn = w < WAVES - 1 ? edge[b][w + 1][0][1][lane] : V[R - 1];
+        const float om = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(om_lane), s));
+        double su = 0.0, sv = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float pu = up_u + (r + 1 < R ? U[r + 1] : udn);
+            const float pv = up_v + (r + 1 < R ? V[r + 1] : vdn);
+            const float ub = fmaf(wc, lane_left0(pu) + lane_right0(pu), we * (pu + (lane_left0(U[r]) + lane_right0(U[r]))));
+            const float vb = fmaf(wc, lane_left0(pv) + lane_right0(pv), we * (pv + (lane_left0(V[r]) + lane_right0(V[r]))));
+            const float x = FX[r], y = FY[r];
+            const float der = fmaf(x, ub, fmaf(y, vb, FT[r])) * RCP[r];
+            const float du = fmaf(-x, der, ub) - U[r];
+            const float dv = fmaf(-y, der, vb) - V[r];
+            if constexpr (NS) {
+                const int ry = w * R + r, gy = gy0 + r;
+                if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
+                    su = su + (double)du * (double)du;
+                    sv = sv + (double)dv * (double)dv;
+                }
+            }
+            up_u = U[r];
+            up_v = V[r];
+            float nu, nv;
+            if constexpr (PL) {
+                nu = fmaf(-x, der, ub);
+                nv = fmaf(-y, der, vb);
+            } else {
+                nu = fmaf(om, fmaf(g, du, U[r]) - UP[r], UP[r]);
+                nv = fmaf(om, fmaf(g, dv, V[r]) - VP[r], VP[r]);
+            }
+            UP[r] = U[r];
+            VP[r] = V[r];
+            U[r] = nu;
+            V[r] = nv;
+        }
+        if constexpr (NS) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                su = su + __shfl_xor(su, off);
+                sv = sv + __shfl_xor(sv, off);
+            }
+            if (lane == 0) {
+                red[b][w][0] = su;
+                red[b][w][1] = sv;
+            }
+        }
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    for (int s = 0; s < nsw; ++s) {
+        if (normed(s)) sweep(s, T_{}, F_{});
+        else if (a.final_ && s == nsw - 1) sweep(s, F_{}, T_{});
+        else sweep(s, F_{}, F_{});
+    }
+    if (nsw > 0 && normed(nsw - 1)) {
+        __syncthreads();
+        if (threadIdx.x == 0) flush(nsw - 1);
+    }
+
+    float* uo = a.out[src ^ 1][0];
+    float* vo = a.out[src ^ 1][1];
+    if (own_x) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int ry = w * R + r;
+            int gy = gy0 + r;
+            if (ry >= K && ry < WAVES * R - K && gy < H) {
+                size_t q = (size_t)gy * W + gx;
+                uo[q] = U[r];
+                vo[q] = V[r];
+                a.pout[src ^ 1][0][q] = UP[r];
+                a.pout[src ^ 1][1][q] = VP[r];
+            }
+        }
+    }
 }
 
 template <int R, int WAVES, bool NORM, bool CHEB>
@@ -356,7 +484,8 @@ __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     int nsw = a.nsweeps, src = a.src, base = 0;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
     if (NORM && !launch_args(a, blk, nblk, CHEB && !a.check_last, nsw, src, base)) return;
-    hs_tile<R, WAVES, NORM, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
+    if constexpr (CHEB) hs_cheb_tile<R, WAVES>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
+    else hs_tile<R, WAVES, NORM>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
 
 

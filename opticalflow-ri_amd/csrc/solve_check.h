@@ -145,18 +145,6 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
     return runs;
 }
 
-// Re-arms a finished solve for a further phase of at most `budget` sweeps
-// on the same sweep count and history (the phase starts unless the overall
-// budget maxnum is spent).
-__global__ void ctrl_continue_k(SolveCtrl* c, int maxnum, int budget) {
-    SolveState s{};
-    s.done = c->sweeps >= maxnum;
-    s.sweeps = c->sweeps;
-    s.last_err = c->last_err;
-    s.limit = c->sweeps + budget;
-    c->slot[1] = s;
-}
-
 // Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
 // (omega_1 = 1, omega_2 = 2 / (2 - sigma^2), omega_{k+1} = 1 / (1 - sigma^2
 // omega_k / 4)), formed in double on the host.  The recurrence converges
@@ -182,38 +170,6 @@ OmegaTab cheb_omegas(double sigma) {
 
 __global__ void cheb_omega_k(OmegaTab t, float* w, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
-}
-
-// Start of an accelerated solve: control word cleared and n weights written
-// (one launch of any block size).
-__global__ void solve_start_k(SolveCtrl* c, OmegaTab t, float* w, int n) {
-    if (threadIdx.x == 0) {
-        c->done = 0;
-        c->sweeps = 0;
-        c->final_buf = 0;
-        c->redo = 0;
-        c->redo_in = 0;
-        c->last_err = 0.0f;
-        c->limit = 0;
-    c->base_buf = 0;
-        c->base_buf = 0;
-        c->slot[0] = SolveState{};
-        c->slot[1] = SolveState{};
-    }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
-}
-
-__global__ void ctrl_init_k(SolveCtrl* c) {
-    c->done = 0;
-    c->sweeps = 0;
-    c->final_buf = 0;
-    c->redo = 0;
-    c->redo_in = 0;
-    c->last_err = 0.0f;
-    c->limit = 0;
-    c->base_buf = 0;
-    c->slot[0] = SolveState{};
-    c->slot[1] = SolveState{};
 }
 
 }  // namespace

@@ -111,6 +111,38 @@ def hs_sweep(fx, fy, ft, alpha, U, V, prec="f32"):
     return Un, Vn
 
 
+def fma32(a, b, c):
+    """float32 fused multiply-add a*b + c, rounded once: the 64-bit-mantissa
+    intermediate holds the float32 product exactly, so only the sum rounds
+    (to float32 as a single rounding would, but for a one-in-2^40 tie)."""
+    return (np.asarray(a, np.longdouble) * np.asarray(b, np.longdouble)
+            + np.asarray(c, np.longdouble)).astype(np.float32)
+
+
+def hs_sweep_accel(fx, fy, ft, alpha, U, V):
+    """The Horn-Schunck sweep of the accelerated (Chebyshev) phase, all
+    float32: the same operator as hs_sweep (HornSchunck.py:51-66, 3x3 average
+    with scipy 'mirror' borders, then ub - fx (fx ub + fy vb + ft) / den) with
+    the device's operation order: vertical pair sums p, ub = fma(1/12, p_l +
+    p_r, (1/6) (p + (x_l + x_r))), num = fma(fx, ub, fma(fy, vb, ft)),
+    der = num * RN(1 / fma(fy, fy, fma(fx, fx, alpha^2))), T = fma(-fx, der,
+    ub).  Returns (Tu, Tv)."""
+    f32 = np.float32
+    fx, fy, ft, U, V = (np.asarray(x, f32) for x in (fx, fy, ft, U, V))
+    wc, we = f32(1.0 / 12.0), f32(1.0 / 6.0)
+    a2 = f32(alpha) * f32(alpha)
+
+    def avg(X):
+        P = np.pad(X, 1, mode="reflect")
+        pv = P[:-2, :] + P[2:, :]
+        mid = P[1:-1, :-2] + P[1:-1, 2:]
+        return fma32(wc, pv[:, :-2] + pv[:, 2:], we * (pv[:, 1:-1] + mid))
+    ub, vb = avg(U), avg(V)
+    rcp = f32(1.0) / fma32(fy, fy, fma32(fx, fx, a2))
+    der = fma32(fx, ub, fma32(fy, vb, ft)) * rcp
+    return fma32(-fx, der, ub), fma32(-fy, der, vb)
+
+
 def hs(I0, I1, alpha, niter, U=None, V=None, prec="f32"):
     """Returns (U, V, total_error)."""
     I0 = _arr(I0, prec); I1 = _arr(I1, prec)

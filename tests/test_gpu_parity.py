@@ -337,20 +337,17 @@ def test_hs_tol_stops_inside_a_fused_launch(ops):
         ctx.set_tuning()
 
 
-def _fma(a, b, c):
-    """float32 fused multiply-add: a*b + c rounded once (the 64-bit-mantissa
-    intermediate holds the float32 product exactly; its sum then rounds to
-    float32 as the single rounding would, but for a one-in-2^40 tie)."""
-    return (np.longdouble(a) * np.asarray(b, np.longdouble) + np.asarray(c, np.longdouble)).astype(np.float32)
+_fma = orc.fma32
 
 
 def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
-    """Chebyshev semi-iteration over the oracle's float32 sweep T, with the
-    device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
+    """Chebyshev semi-iteration over the accelerated phase's float32 sweep T
+    (oracle.hs_sweep_accel), with the device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
     g (T(x_k) - x_k)) - x_{k-1}), each multiply-add fused (fma(g, d, x_k),
     then fma(w, t - x_{k-1}, x_{k-1})); stop at the first x_k with plain residual
     <= tol (finish * tol when finish > 1) and take T(x_k); with finish > 1,
-    plain sweeps then continue to tol on the same count.  Returns (U, V,
+    plain sweeps (the oracle's reference-dataflow sweep) then continue to tol
+    on the same count.  Returns (U, V,
     sweeps, residual history)."""
     t1 = tol * finish if finish > 1 else tol
     lo, hi = -1.0 / 3.0, float(np.float32(bound))
@@ -365,7 +362,7 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     UP, VP = U.copy(), V.copy()
     hist = []
     for k in range(maxnum):
-        Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
         du, dv = Tu - U, Tv - V
         r = np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) + np.sqrt((dv.astype(np.float64) ** 2).sum()))
                        / U.size)
@@ -411,7 +408,7 @@ def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
                            np.sqrt((dv.astype(np.float64) ** 2).sum())) / U.size)
     count, r, stopped = 0, None, False
     for k in range(maxnum):
-        Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
         du, dv = Tu - U, Tv - V
         tested = k % depth == depth - 1 or k == maxnum - 1
         if tested:
@@ -465,8 +462,8 @@ def test_hs_accel_launch_test_matches_restated_iteration(ops, depth, finish):
 
 @pytest.mark.parametrize("depth,finish", [(0, 0.0), (4, 0.0), (3, 4.0)])
 def test_hs_accel_matches_restated_iteration(ops, depth, finish):
-    """ofri_hs_accel equals the Chebyshev iteration restated over the oracle's
-    sweep bit for bit (sweep count, residual history and the returned T(x_k)),
+    """ofri_hs_accel equals the Chebyshev iteration restated over the
+    accelerated float32 sweep bit for bit (sweep count, residual history and the returned T(x_k)),
     and stops in far fewer sweeps than plain sweeps need for the same norm."""
     from ofri.device import DeviceArray, context
     H, W = 96, 120

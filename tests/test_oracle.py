@@ -129,6 +129,33 @@ def test_hs_fixed_point_satisfies_discrete_equations():
     assert np.abs(r).max() < 1e-8 * (1 + np.abs(fy).max() ** 2)
 
 
+def test_accel_sweep_is_the_hs_sweep_to_float32_rounding():
+    """The accelerated phase's float32 sweep is the same operator as the
+    reference-dataflow sweep: on PIV-like fields both agree to a few float32
+    ulps (a dropped or mis-weighted neighbour, a wrong sign or a transposed
+    derivative moves them apart by orders of magnitude), and the float64
+    instance of the exact sweep sits between them to the same tolerance."""
+    rng = np.random.default_rng(7)
+    H, W = 37, 53
+    fx, fy, ft = (rng.standard_normal((H, W)).astype(np.float32) * s for s in (20.0, 20.0, 10.0))
+    U, V = (rng.standard_normal((H, W)).astype(np.float32) for _ in range(2))
+    for alpha in (45.0, 21.0, 3.0):
+        tu, tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
+        eu, ev = orc.hs_sweep(fx, fy, ft, alpha, U, V)
+        du, dv = orc.hs_sweep(fx, fy, ft, alpha, U, V, prec="f64")
+        scale = max(np.abs(eu).max(), np.abs(ev).max())
+        assert max(np.abs(tu - eu).max(), np.abs(tv - ev).max()) <= 8 * np.finfo(np.float32).eps * scale
+        assert max(np.abs(tu - du).max(), np.abs(tv - dv).max()) <= 8 * np.finfo(np.float32).eps * scale
+        assert not np.array_equal(tu, eu)  # it is a different rounding of the same operator
+    # a mirror-symmetric field stays mirror-symmetric (the tile's ghost cells
+    # evolve exactly like their mirror pixels)
+    S = rng.standard_normal((H, W)).astype(np.float32)
+    S = np.concatenate([S, S[:, -2::-1]], axis=1)     # mirror about column W-1
+    F = [np.concatenate([f, f[:, -2::-1]], axis=1) for f in (fx, fy, ft)]
+    tu, _ = orc.hs_sweep_accel(F[0], F[1], F[2], 21.0, S, S)
+    assert np.array_equal(tu[:, W - 1:], tu[:, W - 1::-1])
+
+
 def test_ls_fixed_point_solves_linear_system():
     """LS Jacobi fixed point: A [u; v] = -b(u, v) per pixel, where the sweep is
     u' = -(B bu): at convergence u = u'."""
