@@ -57,7 +57,7 @@ def parse():
                          "batch: 64 uneven pairs LS(HS); "
                          "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level")
     ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
-    ap.add_argument("--eval-pairs", type=int, default=7,
+    ap.add_argument("--eval-pairs", type=int, default=13,
                     help="pairs the default workload cycles over (its own pair + held-out rendered pairs)")
     return ap.parse_args()
 

@@ -102,24 +102,25 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
                 int maxnum, double tol, float* d_U, float* d_V, float* d_hist, float* h_err,
                 int* h_iters);
 
-/* ofri_hs_tol with Chebyshev semi-iteration over the same sweep: each step
- * combines the damped plain update with the previous iterate, for a
- * spectrum of the sweep's linear part in [-1/3, bound] (0 < bound <= 0.999; the
- * 8-neighbour average bounds it below by -1/3, and a bound below the true
- * slowest mode only slows that mode down).  The stop test is the plain one:
- * sweep k records |T(x_k) - x_k| for the plain sweep T, and the accelerated
- * phase returns T(x_k) for the first iterate x_k with that norm <= its
- * tolerance.  finish > 1: the accelerated phase runs to finish * tol and
- * plain sweeps continue to tol (float32 Chebyshev iterates stall near the
- * rounding level, plain sweeps settle); finish <= 1: accelerated to tol.
- * maxnum bounds the sweeps of both phases together, the plain phase to at
- * most maxnum / 2 of them; d_hist, *h_err and *h_iters cover both.
+/* ofri_hs_tol with Chebyshev semi-iteration: each step combines the damped
+ * sweep with the previous iterate, for a spectrum of the sweep's linear part
+ * in [-1/3, bound] (0 < bound <= 0.999; the 8-neighbour average bounds it
+ * below by -1/3, and a bound below the true slowest mode only slows that
+ * mode down).  The accelerated sweep T is the HS update formed in float32
+ * with fused multiply-adds (the same operator as ofri_hs's to float32
+ * rounding).  Sweep k records |T(x_k) - x_k|, and the accelerated phase
+ * returns T(x_k) for the first iterate x_k with that norm <= its tolerance.
+ * finish > 1: the accelerated phase runs to finish * tol and the reference
+ * sweeps of ofri_hs_tol continue to tol (float32 Chebyshev iterates stall
+ * near the rounding level, plain sweeps settle); finish <= 1: accelerated to
+ * tol.  maxnum bounds the sweeps of both phases together: the plain phase
+ * gets at most maxnum / 2 of them, the accelerated phase the rest; d_hist,
+ * *h_err and *h_iters cover both.
  * launch_test != 0: the stop test runs only at the last sweep of each launch
  * (cheaper; d_hist must be NULL): the accelerated phase then ends with the
  * iterate that launch produced, the plain phase with T(x_k) as before, and
- * the sweep count is a whole number of launches per phase.  Same fixed point
- * as ofri_hs; no counterpart in the reference (an accelerated form for
- * solving to a residual). */
+ * the sweep count is a whole number of launches per phase.  No counterpart
+ * in the reference (an accelerated form for solving to a residual). */
 int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
                   int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
                   float* d_hist, float* h_err, int* h_iters, int launch_test);
@@ -134,6 +135,14 @@ int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
  * helper, physicsBasedOpticalFlowLiuShen); the adapter at :37-39 swaps U, V. */
 int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
             int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
+/* Liu-Shen as the learned form runs it: the update norm is tested once per
+ * fused launch, at its last sweep, which is the reference update; the
+ * sweeps before it in the launch are the same update in float32 with fused
+ * multiply-adds.  Stops after the first launch whose tested norm is <= tol
+ * (or maxnum sweeps); *h_err = that norm, *h_iters = sweeps run.  No
+ * counterpart in the reference (the learned form's Liu-Shen step). */
+int ofri_ls_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+                    int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
 
 /* ---- Farneback ----------------------------------------------------------- */
 typedef struct ofri_fb_params {
@@ -299,7 +308,7 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
 
 /* The learned form of the LS(HS) 2-level pyramid: at each level HS runs
  * (Chebyshev-accelerated when cheb_bound > 0) from zero until its update
- * norm is <= hs_tol[level] or hs_budget sweeps; the correction network
+ * norm is <= hs_tol[level] or hs_budget[level] sweeps; the correction network
  * (4 input planes) maps the level's Liu-Shen pair and HS's flow to the
  * correction Liu-Shen would add, and Liu-Shen runs from HS's flow plus that
  * correction until its norm is <= ls_tol[level] or ls_budget sweeps.  The
@@ -314,15 +323,15 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
  * size is not a multiple of 16 skip the network.  Repeated calls replay a
  * HIP graph.  No counterpart in the reference. */
 typedef struct ofri_learned_params {
-    const float* hs_tol;   /* per level (coarse, fine) */
+    const float* hs_tol;      /* per level (coarse, fine) */
     const float* ls_tol;
-    int hs_budget;
+    int hs_budget[2];         /* HS sweeps per level, both phases */
     int ls_budget;
-    float cheb_bound[2];   /* HS per level (coarse, fine): e.g. 0.99, 0.985; 0 = plain sweeps */
-    float cheb_finish;     /* HS: e.g. 4; see ofri_hs_accel */
-    float ls_cheb_lo;      /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
-    float ls_cheb_hi;      /* hi = 0: plain Liu-Shen sweeps */
-    int hs_finish_budget;  /* HS plain finishing sweeps at most (0: hs_budget / 2), within hs_budget */
+    float cheb_bound[2];      /* HS per level (coarse, fine): e.g. 0.99, 0.985; 0 = plain sweeps */
+    float cheb_finish[2];     /* HS per level: e.g. 4, 2; see ofri_hs_accel */
+    float ls_cheb_lo;         /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
+    float ls_cheb_hi;         /* hi = 0: plain Liu-Shen sweeps */
+    int hs_finish_budget[2];  /* HS plain finishing sweeps per level at most (0: hs_budget / 2), within hs_budget */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,

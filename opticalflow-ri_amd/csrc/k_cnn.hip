@@ -97,6 +97,11 @@ struct ConvArgs {
     int B, H, W, Cin, Cout;
     int act;           // 1: leaky ReLU
     int beta;          // 1: add into the destination
+    // non-null (16 output channels, channel = sub * 4 + field): instead of
+    // storing y, fields 0 and 1 are added depth-to-space into these two
+    // float32 planes of size 2H x 2W (fields 2, 3 are dropped)
+    float* add_u;
+    float* add_v;
 };
 
 template <int MT, int NT>
@@ -385,8 +390,34 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
     // D layout: lane holds m-tile pixels 4g..4g+3, channel r16.  The
     // channel's part of the destination offset (and, through depth-to-space,
     // its sub-pixel) is formed once per channel; pixels add 32-bit offsets.
-    // A bfloat16 output written (not accumulated) from a tile inside the
-    // image takes a straight-line path.
+    // A correction head (add_u set) adds its fields 0, 1 into the flow
+    // planes; a bfloat16 output written (not accumulated) from a tile inside
+    // the image takes a straight-line path.
+    if (a.add_u) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = cobase + nt * 16 + r16;
+            if (co >= a.Cout || (co & 3) >= 2) continue;
+            const float bv = a.bias ? a.bias[co] : 0.0f;
+            const int sub = co >> 2, dy = sub >> 1, dx = sub & 1;
+            float* pl = (co & 1) ? a.add_v : a.add_u;
+            const int Wd = W << 1;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int p = (wv * MT + mt) * 16 + 4 * g + i;
+                    const int y = y0 + p / TS, x = x0 + p % TS;
+                    if (y >= H || x >= W) continue;
+                    float v = acc[mt][nt][i] + bv;
+                    if (a.act) v = v > 0.0f ? v : v * kSlope;
+                    float* d = pl + (long long)b * 4 * H * W + ((y << 1) + dy) * Wd + ((x << 1) + dx);
+                    *d = *d + v;
+                }
+            }
+        }
+        return;
+    }
     if (!a.y.f32 && !a.beta && y0 + TS <= H && x0 + TS <= W) {
         const bool act = a.act;
 #pragma unroll
@@ -638,22 +669,6 @@ __global__ void net_input_k(InPlanes in, int H, int W, u16* x) {
     }
 }
 
-// The fine head's fields 0, 1 (float32 NHWC at H/2, channel = sub * 4 +
-// field) added into two full-size planes: u += field 0, v += field 1.
-__global__ void fine_add_k(const float* __restrict__ hf, int h, int w, float* __restrict__ u,
-                           float* __restrict__ v) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)h * w) return;
-    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
-    const int W = 2 * w;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const long long q = (long long)(2 * y + (s >> 1)) * W + 2 * x + (s & 1);
-        u[q] = u[q] + hf[i * 16 + s * 4];
-        v[q] = v[q] + hf[i * 16 + s * 4 + 1];
-    }
-}
-
 // Head outputs (float32 NHWC at H/2) -> planar fields: coarse [B][4][h][w]
 // from the first 4 of the 8-channel head, fine [B][4][H][W] from the
 // 16-channel head
@@ -887,8 +902,11 @@ int launch_conv(ofri_ctx* ctx, const ConvArgs& a) {
     return OFRI_OK;
 }
 
-// coarse_head false: the coarse head (layer head_c) is skipped.
-int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, bool coarse_head = true) {
+// coarse_head false: the coarse head (layer head_c) is skipped.  add_u,
+// add_v non-null (B = 1): the fine head's fields 0, 1 are added into these
+// full-size planes instead of being stored.
+int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, bool coarse_head = true,
+                float* add_u = nullptr, float* add_v = nullptr) {
     OFRI_TRY(net_shape(ctx, n, B, H, W));
     const long long hw = (long long)(H / 2) * (W / 2);
     const dim3 g((unsigned)((hw + 255) / 256), B);
@@ -915,6 +933,10 @@ int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, 
         a.Cout = d.cout;
         a.act = d.act;
         a.beta = 0;
+        if (add_u && d.dst == T_HF) {
+            a.add_u = add_u;
+            a.add_v = add_v;
+        }
         OFRI_TRY(launch_conv(ctx, a));
     }
     return OFRI_OK;
@@ -1125,12 +1147,9 @@ int net_correct(ofri_ctx* ctx, Net* n, const float* const* planes, int H, int W,
     InPlanes in{};
     for (int i = 0; i < n->planes && i < 4; ++i) in.p[i] = planes[i];
     in.bstride = 0;
-    OFRI_TRY(net_forward(ctx, n, in, 1, H, W, false));
-    const int h = H / 2, w = W / 2;
-    fine_add_k<<<grid1((long long)h * w), 256, 0, ctx->stream>>>((const float*)((char*)n->act + n->t_off[T_HF]), h,
-                                                                 w, add_u, add_v);
-    OFRI_HIP(hipGetLastError());
-    return OFRI_OK;
+    if (!add_u || !add_v) return fail(OFRI_E_ARG, "net_correct: null flow planes");
+    // the head's last kernel adds the correction into the flow
+    return net_forward(ctx, n, in, 1, H, W, false, add_u, add_v);
 }
 
 // Forward, loss against planar labels, backward; with lr > 0 an Adam step.

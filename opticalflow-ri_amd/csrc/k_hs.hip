@@ -227,7 +227,10 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         part[((size_t)sw * nblk + blk) * 2] = tu;
         part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
     };
-    for (int s = 0; s < nsw; ++s) {
+    // one sweep; NS: reduce its update norm (compiled only into the sweeps
+    // that test)
+    auto sweep = [&](int s, auto ns_c) {
+        constexpr bool norm = decltype(ns_c)::value;
         const int b = s & 1;
         edge[b][w][0][0][lane] = U[0];
         edge[b][w][0][1][lane] = V[0];
@@ -235,7 +238,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
         edge[b][w][1][1][lane] = V[R - 1];
         __syncthreads();
         if (s > 0 && normed(s - 1) && threadIdx.x == 0) flush(s - 1);
-        const bool norm = normed(s);
         float uup = w > 0 ? edge[b][w - 1][1][0][lane] : U[0];
         float vup = w > 0 ? edge[b][w - 1][1][1][lane] : V[0];
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : U[R - 1];
@@ -268,7 +270,7 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             float der = quotient(num, den, RCP[r]);
             float un = ub - x * der;
             float vn = vb - y * der;
-            if (norm) {
+            if constexpr (norm) {
                 const int ry = w * R + r, gy = gy0 + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
                     float du = un - U[r], dv = vn - V[r];
@@ -281,7 +283,7 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             up_u = cu; up_v = cv;
             cu = dn_u; cv = dn_v;
         }
-        if (norm) {
+        if constexpr (norm) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 su = su + __shfl_xor(su, off);
@@ -292,6 +294,10 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
                 red[b][w][1] = sv;
             }
         }
+    };
+    for (int s = 0; s < nsw; ++s) {
+        if (NORM && normed(s)) sweep(s, std::true_type{});
+        else sweep(s, std::false_type{});
     }
     if (nsw > 0 && normed(nsw - 1)) {
         __syncthreads();
@@ -643,9 +649,6 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         p.omega = omega;
         p.gamma = (float)(2.0 / (2.0 - lo - hi));
     }
-    // one phase: up to niter sweeps (fewer when the control word's shared
-    // count reaches the budget), the stopping launch re-run, the result
-    // copied back into U, V
     // one phase: up to `sweeps` sweeps (fewer when the control word's shared
     // count reaches the budget), then the final launch (test of the last
     // launch, re-run of a launch stopped inside, result published).  Launch
@@ -693,14 +696,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         OFRI_TRY(phase(var.cheb, tol, niter, 0, 0, nullptr));
     } else {
         // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
-        // still settle: accelerate to cheb_switch * tol, then finish with at
-        // most fin_budget (default niter / 2) plain sweeps to tol, on one
-        // sweep count, budget and history (niter sweeps in all).  The
-        // finishing phase starts from the accelerated phase's result buffer
-        // (its launches take buffers relative to it) and the opposite parity.
-        const int fin = fin_budget > 0 ? fin_budget : (niter / 2 > 0 ? niter / 2 : 1);
+        // still settle: accelerate to cheb_switch * tol in at most niter - fin
+        // sweeps, then finish with at most fin (fin_budget, default niter / 2)
+        // plain sweeps to tol, on one sweep count and history (niter sweeps
+        // in all).  The finishing phase starts from the accelerated phase's
+        // result buffer (its launches take buffers relative to it) and the
+        // opposite parity.
+        int fin = fin_budget > 0 ? fin_budget : (niter / 2 > 0 ? niter / 2 : 1);
+        if (fin > niter) fin = niter;
         int nl = 0;
-        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter, 0, niter > 0 ? fin : 0, &nl));
+        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter - fin, 0, niter > 0 ? fin : 0, &nl));
         p.rel_src = 1;
         p.zero_start = 0;
         OFRI_TRY(phase(var.normed, tol, fin, nl + 1, 0, nullptr));

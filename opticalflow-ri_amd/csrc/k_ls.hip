@@ -17,6 +17,8 @@
 // graph.
 #include <cmath>
 
+#include <type_traits>
+
 #include "ofri_internal.h"
 #include "solve_check.h"
 
@@ -200,11 +202,66 @@ __device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Ra
     vn = -(t3 + t4);
 }
 
+// The same update in float32 with fused multiply-adds, for the sweeps the
+// learned form runs between its tests: full differences with the halves
+// folded into the coefficients (0.5 iix, 0.5 iiy), the corner term as a
+// difference of row differences, the 8-neighbour sum in pairs.  Same
+// boundary handling as ls_update.
+template <bool EDGE>
+__device__ __forceinline__ void ls_update_fast(const Raw& pu, const Raw& cu, const Raw& nu, const Raw& pv,
+                                               const Raw& cv, const Raw& nv, bool top, bool bot, bool el,
+                                               bool er, float h, float iix, float iiy, float ii2, float ixt,
+                                               float iyt, float b11, float b12, float b22, float& un,
+                                               float& vn) {
+    const Raw& uu = (EDGE && top) ? cu : pu;
+    const Raw& ud = (EDGE && bot) ? cu : nu;
+    const Raw& vu = (EDGE && top) ? cv : pv;
+    const Raw& vd = (EDGE && bot) ? cv : nv;
+    const float uu_l = (EDGE && el) ? uu.c : uu.l, uu_r = (EDGE && er) ? uu.c : uu.r;
+    const float ud_l = (EDGE && el) ? ud.c : ud.l, ud_r = (EDGE && er) ? ud.c : ud.r;
+    const float vu_l = (EDGE && el) ? vu.c : vu.l, vu_r = (EDGE && er) ? vu.c : vu.r;
+    const float vd_l = (EDGE && el) ? vd.c : vd.l, vd_r = (EDGE && er) ? vd.c : vd.r;
+    const float cu_l = (EDGE && el) ? cu.c : cu.l, cu_r = (EDGE && er) ? cu.c : cu.r;
+    const float cv_l = (EDGE && el) ? cv.c : cv.l, cv_r = (EDGE && er) ? cv.c : cv.r;
+    const float dru = ud.c - uu.c, dcu = cu_r - cu_l;   // twice Dr, Dc
+    const float drv = vd.c - vu.c, dcv = cv_r - cv_l;
+    const float fru = uu.c + ud.c, fcv = cv_l + cv_r;
+    const float mu = 0.25f * ((uu_l - uu_r) - (ud_l - ud_r));
+    const float mv = 0.25f * ((vu_l - vu_r) - (vd_l - vd_r));
+    const float zl_pu = (EDGE && (el || top)) ? 0.0f : pu.l, zc_pu = (EDGE && top) ? 0.0f : pu.c;
+    const float zr_pu = (EDGE && (er || top)) ? 0.0f : pu.r;
+    const float zl_nu = (EDGE && (el || bot)) ? 0.0f : nu.l, zc_nu = (EDGE && bot) ? 0.0f : nu.c;
+    const float zr_nu = (EDGE && (er || bot)) ? 0.0f : nu.r;
+    const float zl_pv = (EDGE && (el || top)) ? 0.0f : pv.l, zc_pv = (EDGE && top) ? 0.0f : pv.c;
+    const float zr_pv = (EDGE && (er || top)) ? 0.0f : pv.r;
+    const float zl_nv = (EDGE && (el || bot)) ? 0.0f : nv.l, zc_nv = (EDGE && bot) ? 0.0f : nv.c;
+    const float zr_nv = (EDGE && (er || bot)) ? 0.0f : nv.r;
+    const float zl_cu = (EDGE && el) ? 0.0f : cu.l, zr_cu = (EDGE && er) ? 0.0f : cu.r;
+    const float zl_cv = (EDGE && el) ? 0.0f : cv.l, zr_cv = (EDGE && er) ? 0.0f : cv.r;
+    const float h0u = ((zl_pu + zr_pu) + (zl_nu + zr_nu)) + ((zc_pu + zc_nu) + (zl_cu + zr_cu));
+    const float h0v = ((zl_pv + zr_pv) + (zl_nv + zr_nv)) + ((zc_pv + zc_nv) + (zl_cv + zr_cv));
+    const float hix = 0.5f * iix, hiy = 0.5f * iiy;
+    float bu = fmaf(iix, dru, ixt);
+    bu = fmaf(hix, dcv, bu);
+    bu = fmaf(hiy, drv, bu);
+    bu = fmaf(ii2, fru + mv, bu);
+    bu = fmaf(h, h0u, bu);
+    float bv = fmaf(hiy, dru, iyt);
+    bv = fmaf(hix, dcu, bv);
+    bv = fmaf(iiy, dcv, bv);
+    bv = fmaf(ii2, mu + fcv, bv);
+    bv = fmaf(h, h0v, bv);
+    un = -fmaf(b11, bu, b12 * bv);
+    vn = -fmaf(b12, bu, b22 * bv);
+}
+
 // CHEB: Chebyshev semi-iteration over the same sweep T (as in the
 // Horn-Schunck kernel): x_{k+1} = x_{k-1} + omega_{k+1} ((x_k + gamma
 // (T(x_k) - x_k)) - x_{k-1}), the previous iterate in registers; the
-// reduced norm is the plain one, |T(x_k) - x_k|.
-template <int R, int WAVES, bool EDGE, bool CHEB>
+// reduced norm is the plain one, |T(x_k) - x_k|.  FAST: every sweep but a
+// launch's last runs ls_update_fast; the last, the one a launch tests, is
+// the reference update.
+template <int R, int WAVES, bool EDGE, bool CHEB, bool FAST>
 __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, int blk, int nblk, int lane, int w,
                                           int gx, int gy0, float (&u)[R], float (&v)[R], float (&up)[R],
                                           float (&vp)[R], const float (&IIX)[R],
@@ -230,7 +287,9 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
         part[((size_t)sw * nblk + blk) * 2] = tu;
         part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
     };
-    for (int s = 0; s < nsw; ++s) {
+    // one sweep; F: the float32 update (no norm reduced)
+    auto sweep = [&](int s, auto f_c) {
+        constexpr bool F = decltype(f_c)::value;
         const int b = s & 1;
         edge[b][w][0][0][lane] = u[0];
         edge[b][w][0][1][lane] = v[0];
@@ -238,7 +297,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
         edge[b][w][1][1][lane] = v[R - 1];
         __syncthreads();
         if (s > 0 && normed(s - 1) && threadIdx.x == 0) flush(s - 1);
-        const bool norm = normed(s);
+        const bool norm = !F && normed(s);
         float uup = w > 0 ? edge[b][w - 1][1][0][lane] : u[0];
         float vup = w > 0 ? edge[b][w - 1][1][1][lane] : v[0];
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : u[R - 1];
@@ -253,8 +312,12 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
             Raw nv = raw(r + 1 < R ? v[r + 1] : vdn);
             const int gy = gy0 + r;
             float un, vn;
-            ls_update<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r], II[r],
-                            IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
+            if constexpr (F)
+                ls_update_fast<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r],
+                                     II[r], IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
+            else
+                ls_update<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r], II[r],
+                                IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
             if (norm) {
                 const int ry = w * R + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
@@ -288,6 +351,10 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
                 red[b][w][1] = sv;
             }
         }
+    };
+    for (int s = 0; s < nsw; ++s) {
+        if (FAST && s < nsw - 1) sweep(s, std::true_type{});
+        else sweep(s, std::false_type{});
     }
     if (nsw > 0 && normed(nsw - 1)) {
         __syncthreads();
@@ -297,7 +364,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
 
 // One workgroup's tile (bx, by) of an image whose launch has nblk tiles, this
 // one being number blk.
-template <int R, int WAVES, bool CHEB = false>
+template <int R, int WAVES, bool CHEB = false, bool FAST = false>
 __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int base, int bx, int by, int blk,
                                         int nblk) {
     const int lane = threadIdx.x & 63;
@@ -336,11 +403,11 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
         B22[r] = a.F[7 * N + q];
     }
     if (edge_tile)
-        ls_sweeps<R, WAVES, true, CHEB>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY, II,
-                                        IXT, IYT, B11, B12, B22);
+        ls_sweeps<R, WAVES, true, CHEB, FAST>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY,
+                                              II, IXT, IYT, B11, B12, B22);
     else
-        ls_sweeps<R, WAVES, false, CHEB>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY, II,
-                                         IXT, IYT, B11, B12, B22);
+        ls_sweeps<R, WAVES, false, CHEB, FAST>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY,
+                                               II, IXT, IYT, B11, B12, B22);
 
     float* uo = a.out[src ^ 1][0];
     float* vo = a.out[src ^ 1][1];
@@ -361,12 +428,12 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
     }
 }
 
-template <int R, int WAVES, bool CHEB>
+template <int R, int WAVES, bool CHEB, bool FAST = false>
 __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
     int nsw, src, base;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
     if (!launch_args(a, blk, nblk, false, nsw, src, base)) return;
-    ls_tile<R, WAVES, CHEB>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
+    ls_tile<R, WAVES, CHEB, FAST>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
 
 // Many images in one launch (workgroup b is tile b - blk0[p] of image p,
@@ -401,8 +468,10 @@ struct LsVariant {
     void (*kernel)(LsPass);
     void (*batched)(const LsPass*, const int*, int, int, int, int, int);
     void (*cheb)(LsPass);
+    void (*fast)(LsPass);   // float32 sweeps between the tests
 };
-#define LS_SHAPE(R, WV) {R, WV, ls_pass_k<R, WV, false>, ls_batch_k<R, WV>, ls_pass_k<R, WV, true>}
+#define LS_SHAPE(R, WV) \
+    {R, WV, ls_pass_k<R, WV, false>, ls_batch_k<R, WV>, ls_pass_k<R, WV, true>, ls_pass_k<R, WV, false, true>}
 const LsVariant kLsVariants[] = {
     LS_SHAPE(8, 8), LS_SHAPE(8, 4), LS_SHAPE(4, 8), LS_SHAPE(12, 4),
     LS_SHAPE(2, 16), LS_SHAPE(4, 16), LS_SHAPE(6, 8),
@@ -424,7 +493,7 @@ void ls_default(size_t npx, int* variant, int* depth) {
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
-             float cheb_hi, float* sink_u, float* sink_v, bool sink_add) {
+             float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast) {
     if ((sink_u == nullptr) != (sink_v == nullptr)) return fail(OFRI_E_ARG, "ls: one result sink given without the other");
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
@@ -473,7 +542,8 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.tol = tol;
     p.maxnum = maxnum;
     p.check_last = check_last ? 1 : 0;
-    auto kern = cheb ? var.cheb : var.kernel;
+    if (fast && (cheb || !check_last)) return fail(OFRI_E_ARG, "ls: float32 sweeps need the per-launch test alone");
+    auto kern = cheb ? var.cheb : (fast ? var.fast : var.kernel);
     if (cheb) {
         // previous-iterate buffers (the first step's previous iterate is the
         // start) and the weights for [cheb_lo, cheb_hi]

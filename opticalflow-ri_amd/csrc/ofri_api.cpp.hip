@@ -329,6 +329,24 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
     return OFRI_OK;
 }
 
+int ofri_ls_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+                    int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_ls_learned: null pointer");
+    void* c;
+    OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
+    OFRI_TRY(ls_solve(ctx, d_im1, d_im2, H, W, h, maxnum, tol, d_u, d_v, (SolveCtrl*)c, true, 0.0f, 0.0f, nullptr,
+                      nullptr, false, true));
+    if (h_err || h_iters) {
+        SolveCtrl hc;
+        OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (h_err) *h_err = hc.last_err;
+        if (h_iters) *h_iters = hc.sweeps;
+    }
+    return OFRI_OK;
+}
+
 int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
                   float* d_v, int half_win, int niter, unsigned char* d_status) {
     CHECK_CTX(ctx);
@@ -435,9 +453,9 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
-            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget, U, V, nullptr, lp->hs_tol[l],
-                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish, true, lp->hs_finish_budget,
-                              true));
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget[l], U, V, nullptr, lp->hs_tol[l],
+                              hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish[l], true,
+                              lp->hs_finish_budget[l], true));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
         } else {
@@ -466,7 +484,8 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             float* sv = ls_sink ? out_u : nullptr;
             if (lrn)
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
-                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi, su, sv, l > 0));
+                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi, su, sv, l > 0,
+                                  lrn->lp->ls_cheb_hi <= 0.0f));
             else
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l, false,
                                   0.0f, 0.0f, su, sv, l > 0));
@@ -933,8 +952,10 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (p->levels != 2) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: 2 levels only (got %d)", p->levels);
     if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
     if (H < 8 || W < 8) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d too small", H, W);
-    if (lp->hs_budget < 1 || lp->ls_budget < 0 || lp->hs_finish_budget < 0)
-        return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: budgets");
+    for (int l = 0; l < 2; ++l)
+        if (lp->hs_budget[l] < 1 || lp->hs_finish_budget[l] < 0 || !(lp->cheb_finish[l] >= 0.0f))
+            return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: HS budget / finish at level %d", l);
+    if (lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen budget");
     for (int l = 0; l < 2; ++l)
         if (lp->cheb_bound[l] != 0.0f && !(lp->cheb_bound[l] > 0.0f && lp->cheb_bound[l] <= 0.999f))
             return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g at level %d", lp->cheb_bound[l], l);

@@ -184,7 +184,7 @@ int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false,
              float cheb_lo = 0.0f, float cheb_hi = 0.0f, float* sink_u = nullptr, float* sink_v = nullptr,
-             bool sink_add = false);
+             bool sink_add = false, bool fast = false);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets

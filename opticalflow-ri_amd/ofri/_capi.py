@@ -51,9 +51,9 @@ class Pair(C.Structure):
 
 
 class LearnedParams(C.Structure):
-    _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int),
-                ("ls_budget", C.c_int), ("cheb_bound", C.c_float * 2), ("cheb_finish", C.c_float),
-                ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float), ("hs_finish_budget", C.c_int)]
+    _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int * 2),
+                ("ls_budget", C.c_int), ("cheb_bound", C.c_float * 2), ("cheb_finish", C.c_float * 2),
+                ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float), ("hs_finish_budget", C.c_int * 2)]
 
 
 class FbParams(C.Structure):
@@ -106,6 +106,7 @@ SIGNATURES = {
     "ofri_hs_accel": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, C.c_float, C.c_float, _vp,
                       _vp, _vp, _fp, _ip, C.c_int],
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
+    "ofri_ls_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],

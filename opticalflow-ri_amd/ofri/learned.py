@@ -22,11 +22,14 @@ from .device import DeviceArray, context
 WORKLOAD = dict(levels=2, alphas=[45.0, 21.0], niter=600, ls_alpha=5.0, ls_maxnum=60, ls_tol=1e-8)
 # Chebyshev spectrum bounds of the accelerated HS per level (coarse: alpha
 # 45, fine: alpha 21 -- a larger alpha leaves the slowest mode closer to 1)
-# and its plain-finish factor; measured on 1024^2 PIV pairs: 0.99 / 0.985
-# take 70-110 / 80 sweeps, a tighter bound runs out of the budget at the
-# coarse level, a looser one needs more sweeps at both.
+# and the factor over the target at which plain finishing sweeps take over.
+# Measured on 60 held-out 1024^2 PIV pairs: with 0.99 / 0.985 and factors
+# 4 / 2 the coarse level takes 60-96 sweeps and the fine one 80-90; a
+# factor 4 at the fine level leaves a fifth of the pairs needing 110-160
+# (plain sweeps from 4x the target are slow there), a bound of 0.99 there
+# takes 90-100.
 CHEB_BOUND = (0.99, 0.985)
-CHEB_FINISH = 4.0
+CHEB_FINISH = (4.0, 2.0)
 # Liu-Shen runs plain sweeps.  Its sweep's spectrum lies in about
 # [-0.53, 0.999] on PIV pairs (alpha 5; power iteration on the oracle's
 # sweep), and Chebyshev sweeps over [-0.55, 0.99] reach the plain residual
@@ -66,15 +69,25 @@ def plain_targets(im1, im2, fw=None, ow=None):
     return U, V, hs_tol, [float(s["ls_err"]) for s in st], st
 
 
-# Sweep budgets of the learned form per level: HS 120 sweeps in all (the
-# accelerated phase takes 60-80 on 1024^2 PIV pairs), of them at most 30
-# plain finishing sweeps (about 10 used), Liu-Shen 24 (5-16 used after the
-# network's correction; the plain method takes 60).  A level that misses its
-# target within them sends the pair to the plain fallback; launches past a
-# finished solve still cost their dispatch, so the budgets stay close to use.
-HS_BUDGET = 120
-HS_FINISH_BUDGET = 30
-LS_BUDGET = 24
+# Sweep budgets of the learned form per level (coarse, fine): HS 132 / 110
+# sweeps in all, of them at most 36 / 20 plain finishing sweeps and the rest
+# accelerated (on 60 held-out 1024^2 PIV pairs: 48-84 / 70-80 accelerated,
+# one finishing launch of 12 / 10 on nearly every pair, three at most),
+# Liu-Shen 20 (5-16 used after the network's correction; the plain method
+# takes 60).  A level that misses its target within them sends the pair to
+# the plain fallback; launches past a finished solve still cost their
+# dispatch, so the budgets stay close to use.
+HS_BUDGET = (132, 110)
+HS_FINISH_BUDGET = (36, 20)
+LS_BUDGET = 20
+
+
+def _levels(x):
+    return (x, x) if np.isscalar(x) else tuple(x)
+
+
+def _finish_budgets(hs_budget, hs_finish_budget):
+    return tuple(min(f, b // 2 or 1) for f, b in zip(_levels(hs_finish_budget), _levels(hs_budget)))
 
 
 def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None, fallback=True, stats=True,
@@ -86,7 +99,7 @@ def solve(im1, im2, net, hs_tol, ls_tol, fw=None, ow=None, Uacc=None, Vacc=None,
     U, V, st, conv = ops.pyr_lshs_learned(im1, im2, net=net, hs_tol=hs_tol, ls_tol=ls_tol, filt_w=fw, opt_w=ow,
                                           hs_budget=hs_budget, ls_budget=ls_budget, cheb_bound=CHEB_BOUND,
                                           cheb_finish=CHEB_FINISH, ls_cheb=ls_cheb, Uacc=Uacc, Vacc=Vacc,
-                                          hs_finish_budget=min(hs_finish_budget, hs_budget // 2 or 1),
+                                          hs_finish_budget=_finish_budgets(hs_budget, hs_finish_budget),
                                           stats=stats or fallback,
                                           **{k: WORKLOAD[k] for k in ("levels", "alphas", "niter", "ls_alpha")})
     fell = False
@@ -133,7 +146,7 @@ def solve_many(jobs, net, fw=None, ow=None, flags=None, hs_budget=HS_BUDGET, ls_
         ops.pyr_lshs_learned(a, b, net=net, hs_tol=ht, ls_tol=lt, filt_w=fw, opt_w=ow, hs_budget=hs_budget,
                              ls_budget=ls_budget, cheb_bound=CHEB_BOUND, cheb_finish=CHEB_FINISH, Uacc=U, Vacc=V,
                              stats=False, flag=flags.view(k, (1,)),
-                             hs_finish_budget=min(hs_finish_budget, hs_budget // 2 or 1),
+                             hs_finish_budget=_finish_budgets(hs_budget, hs_finish_budget),
                              **{key: WORKLOAD[key] for key in ("levels", "alphas", "niter", "ls_alpha")})
     ok = flags.download().reshape(-1)[:len(jobs)].view(np.int32)
     fell = []

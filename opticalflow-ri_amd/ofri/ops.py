@@ -150,6 +150,18 @@ def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
     return np.float32(err.value), it.value
 
 
+def ls_learned(im1, im2, h, u, v, maxnum=24, tol=1e-8):
+    """ofri_ls_learned: Liu-Shen tested once per fused launch (its last sweep,
+    the reference update) with float32 sweeps in between; u, v in place.
+    Returns (err, sweeps)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    err = C.c_float()
+    it = C.c_int()
+    _capi.check(_lib().ofri_ls_learned(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(h)), int(maxnum),
+                                       float(tol), u.ptr, v.ptr, C.byref(err), C.byref(it)))
+    return np.float32(err.value), it.value
+
+
 def farneback(im1, im2, flowx, flowy, consts):
     """One Farneback level (Farneback_PyCL.py:461-603); flowx / flowy device
     arrays updated in place.  consts: dict of float32 arrays g, xg, xxg (polyN+1
@@ -287,7 +299,8 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
     correction network and Liu-Shen run to the per-level residual targets
     hs_tol / ls_tol.  net: an ofri.net.Net or None (the method without it).
-    cheb_bound: one value or (coarse, fine).  flag: optional DeviceArray view
+    hs_budget, cheb_bound, cheb_finish, hs_finish_budget: one value or
+    (coarse, fine).  flag: optional DeviceArray view
     of one 4-byte word receiving the convergence flag (int 1 / 0) with no
     synchronisation.  Returns (Uacc, Vacc, stats, converged); converged is
     None when stats is False."""
@@ -296,10 +309,12 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol_plain)
     ht = np.ascontiguousarray(hs_tol, dtype=np.float32)
     lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
-    cb = (cheb_bound, cheb_bound) if np.isscalar(cheb_bound) else tuple(cheb_bound)
-    lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), int(hs_budget), int(ls_budget),
-                             (C.c_float * 2)(float(cb[0]), float(cb[1])), float(cheb_finish), float(ls_cheb[0]),
-                             float(ls_cheb[1]), int(hs_finish_budget))
+    def pair(x):
+        return (x, x) if np.isscalar(x) else tuple(x)
+    hb, cb, cf, fb = pair(hs_budget), pair(cheb_bound), pair(cheb_finish), pair(hs_finish_budget)
+    lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), (C.c_int * 2)(int(hb[0]), int(hb[1])), int(ls_budget),
+                             (C.c_float * 2)(float(cb[0]), float(cb[1])), (C.c_float * 2)(float(cf[0]), float(cf[1])),
+                             float(ls_cheb[0]), float(ls_cheb[1]), (C.c_int * 2)(int(fb[0]), int(fb[1])))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None

@@ -143,6 +143,47 @@ def hs_sweep_accel(fx, fy, ft, alpha, U, V):
     return fma32(-fx, der, ub), fma32(-fy, der, vb)
 
 
+def ls_sweep_fast(fields, h, u, v):
+    """The Liu-Shen sweep the learned form runs between its tests, all
+    float32: the same operator as ls_sweep (PhysicsBasedOpticalFlowLiuShen.py
+    :142-150; derivatives with 'nearest' borders, the 8-neighbour sum H0 with
+    'constant' zero borders) with the device's operation order: full
+    differences dr = u_down - u_up, dc = u_right - u_left, M = 0.25 ((ul - ur)
+    - (dl - dr)), H0 in pairs, then
+    bu = fma(h, H0u, fma(II, Fr(u) + M(v), fma(Iy/2, dr(v), fma(Ix/2, dc(v), fma(Ix, dr(u), Ixt)))))
+    bv = fma(h, H0v, fma(II, M(u) + Fc(v), fma(Iy, dc(v), fma(Ix/2, dc(u), fma(Iy/2, dr(u), Iyt)))))
+    u' = -fma(B11, bu, B12 bv), v' = -fma(B12, bu, B22 bv) (Ix = IIx, Iy = IIy).
+    Returns (u', v')."""
+    f32 = np.float32
+    u, v = np.asarray(u, f32), np.asarray(v, f32)
+    iix, iiy, ii2, ixt, iyt, b11, b12, b22 = (np.asarray(fields[k], f32) for k in
+                                              ("IIx", "IIy", "II", "Ixt", "Iyt", "B11", "B12", "B22"))
+    h = f32(h)
+
+    def terms(X):
+        E = np.pad(X, 1, mode="edge")
+        Z = np.pad(X, 1, mode="constant")
+        up, dn, lf, rt = E[:-2, 1:-1], E[2:, 1:-1], E[1:-1, :-2], E[1:-1, 2:]
+        m = f32(0.25) * ((E[:-2, :-2] - E[:-2, 2:]) - (E[2:, :-2] - E[2:, 2:]))
+        h0 = ((Z[:-2, :-2] + Z[:-2, 2:]) + (Z[2:, :-2] + Z[2:, 2:])) + \
+             ((Z[:-2, 1:-1] + Z[2:, 1:-1]) + (Z[1:-1, :-2] + Z[1:-1, 2:]))
+        return dn - up, rt - lf, up + dn, lf + rt, m, h0
+    dru, dcu, fru, _, mu, h0u = terms(u)
+    drv, dcv, _, fcv, mv, h0v = terms(v)
+    hix, hiy = f32(0.5) * iix, f32(0.5) * iiy
+    bu = fma32(iix, dru, ixt)
+    bu = fma32(hix, dcv, bu)
+    bu = fma32(hiy, drv, bu)
+    bu = fma32(ii2, fru + mv, bu)
+    bu = fma32(h, h0u, bu)
+    bv = fma32(hiy, dru, iyt)
+    bv = fma32(hix, dcu, bv)
+    bv = fma32(iiy, dcv, bv)
+    bv = fma32(ii2, mu + fcv, bv)
+    bv = fma32(h, h0v, bv)
+    return -fma32(b11, bu, b12 * bv), -fma32(b12, bu, b22 * bv)
+
+
 def hs(I0, I1, alpha, niter, U=None, V=None, prec="f32"):
     """Returns (U, V, total_error)."""
     I0 = _arr(I0, prec); I1 = _arr(I1, prec)

@@ -345,9 +345,10 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     (oracle.hs_sweep_accel), with the device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
     g (T(x_k) - x_k)) - x_{k-1}), each multiply-add fused (fma(g, d, x_k),
     then fma(w, t - x_{k-1}, x_{k-1})); stop at the first x_k with plain residual
-    <= tol (finish * tol when finish > 1) and take T(x_k); with finish > 1,
-    plain sweeps (the oracle's reference-dataflow sweep) then continue to tol
-    on the same count.  Returns (U, V,
+    <= tol (finish * tol when finish > 1) and take T(x_k); with finish > 1 the
+    accelerated phase has at most maxnum - maxnum // 2 sweeps and plain
+    sweeps (the oracle's reference-dataflow sweep) then continue to tol on
+    the same count.  Returns (U, V,
     sweeps, residual history)."""
     t1 = tol * finish if finish > 1 else tol
     lo, hi = -1.0 / 3.0, float(np.float32(bound))
@@ -361,7 +362,8 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     U = np.zeros(im1.shape, np.float32); V = np.zeros(im1.shape, np.float32)
     UP, VP = U.copy(), V.copy()
     hist = []
-    for k in range(maxnum):
+    n_acc = maxnum - max(1, maxnum // 2) if finish > 1 else maxnum
+    for k in range(n_acc):
         Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
         du, dv = Tu - U, Tv - V
         r = np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) + np.sqrt((dv.astype(np.float64) ** 2).sum()))
@@ -374,8 +376,11 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
         nv = _fma(om[k], _fma(g, dv, V) - VP, VP)
         UP, VP, U, V = U, V, nu, nv
     else:
-        return U, V, maxnum, np.array(hist)
+        if not finish > 1:
+            return U, V, n_acc, np.array(hist)
     if finish > 1:
+        # the finishing phase follows whether or not the accelerated one met
+        # its tolerance
         while len(hist) < maxnum and hist[-1] > tol:
             Tu, Tv = orc.hs_sweep(fx, fy, ft, alpha, U, V)
             du, dv = Tu - U, Tv - V
@@ -388,8 +393,9 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
 def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
     """_cheb_restated with the stop test only at the last sweep of each
     launch of `depth` sweeps (launches counted from each phase's start): the
-    accelerated phase ends with the iterate its stopping launch produced, the
-    plain phase (at most maxnum // 2 sweeps) with T(x_k).  Returns (U, V,
+    accelerated phase (at most maxnum - maxnum // 2 sweeps when finishing)
+    ends with the iterate its stopping launch produced, the plain phase (at
+    most maxnum // 2 sweeps) with T(x_k).  Returns (U, V,
     sweeps, last tested residual)."""
     t1 = tol * finish if finish > 1 else tol
     lo, hi = -1.0 / 3.0, float(np.float32(bound))
@@ -406,11 +412,12 @@ def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
     def res(du, dv):
         return np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) +
                            np.sqrt((dv.astype(np.float64) ** 2).sum())) / U.size)
-    count, r, stopped = 0, None, False
-    for k in range(maxnum):
+    count, r = 0, None
+    n_acc = maxnum - max(1, maxnum // 2) if finish > 1 else maxnum
+    for k in range(n_acc):
         Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
         du, dv = Tu - U, Tv - V
-        tested = k % depth == depth - 1 or k == maxnum - 1
+        tested = k % depth == depth - 1 or k == n_acc - 1
         if tested:
             r = res(du, dv)
         nu = _fma(om[k], _fma(g, du, U) - UP, UP)
@@ -418,9 +425,10 @@ def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
         UP, VP, U, V = U, V, nu, nv
         count = k + 1
         if tested and not r > t1:
-            stopped = True
             break
-    if finish > 1 and stopped:
+    if finish > 1:
+        # the finishing phase follows whether or not the accelerated one met
+        # its tolerance
         fin = max(1, maxnum // 2)
         for j in range(fin):
             if count >= maxnum:
@@ -484,6 +492,57 @@ def test_hs_accel_matches_restated_iteration(ops, depth, finish):
     assert it == kr
     assert np.allclose(hist.download()[:it], hr, rtol=1e-6, atol=0)
     assert np.array_equal(U.download(), Ur) and np.array_equal(V.download(), Vr)
+
+
+def _ls_learned_restated(f, h, u, v, tol, maxnum, depth):
+    """ofri_ls_learned restated over the oracle: launches of `depth` sweeps,
+    every sweep but a launch's last the float32 sweep (oracle.ls_sweep_fast),
+    the last the reference sweep, whose update norm is tested; stop after the
+    first launch whose norm is <= tol.  Returns (u, v, sweeps, norms)."""
+    count, norms = 0, []
+    while count < maxnum:
+        ns = min(depth, maxnum - count)
+        for _ in range(ns - 1):
+            u, v = orc.ls_sweep_fast(f, h, u, v)
+        un, vn, _ = orc.ls_sweep(f, h, u, v)
+        du, dv = un - u, vn - v
+        norms.append(np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) +
+                                 np.sqrt((dv.astype(np.float64) ** 2).sum())) / u.size))
+        u, v = un, vn
+        count += ns
+        if not norms[-1] > tol:
+            break
+    return u, v, count, norms
+
+
+@pytest.mark.parametrize("shape,variant,depth", [((96, 120), 2, 4), ((70, 90), 1, 3), ((130, 64), 4, 5)])
+def test_ls_learned_matches_restated_iteration(ops, shape, variant, depth):
+    """ofri_ls_learned (the learned form's Liu-Shen: float32 sweeps between
+    the per-launch tests, the tested sweep the reference update) equals its
+    restatement over the oracle bit for bit: iterate, sweep count and tested
+    norm, with tiles on and off the image border."""
+    from ofri.device import DeviceArray, context
+    H, W = shape
+    im1, im2 = _piv(H, W, seed=53)
+    rng = np.random.default_rng(54)
+    u0 = (rng.standard_normal((H, W)) * 0.3).astype(np.float32)
+    v0 = (rng.standard_normal((H, W)) * 0.3).astype(np.float32)
+    f = orc.ls_setup(im1, im2, 5.0)
+    _, _, _, norms = _ls_learned_restated(f, 5.0, u0, v0, -1.0, 6 * depth, depth)
+    tol = float(norms[2])                     # met at the end of the third launch
+    ur, vr, kr, nr = _ls_learned_restated(f, 5.0, u0, v0, tol, 6 * depth, depth)
+    assert kr == 3 * depth
+    ctx = context()
+    ctx.set_tuning(ls_depth=depth, ls_variant=variant)
+    try:
+        du, dv = DeviceArray.from_host(u0), DeviceArray.from_host(v0)
+        err, it = ops.ls_learned(im1, im2, 5.0, du, dv, maxnum=6 * depth, tol=tol)
+    finally:
+        ctx.set_tuning()
+    print("[ls learned %dx%d depth %d] %d sweeps, norm %.4g" % (H, W, depth, it, err))
+    assert it == kr
+    assert abs(float(err) - float(nr[-1])) <= 1e-6 * float(nr[-1])
+    assert np.array_equal(du.download(), ur) and np.array_equal(dv.download(), vr)
 
 
 def test_ls_small_golden(ops, golden):

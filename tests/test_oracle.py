@@ -156,6 +156,32 @@ def test_accel_sweep_is_the_hs_sweep_to_float32_rounding():
     assert np.array_equal(tu[:, W - 1:], tu[:, W - 1::-1])
 
 
+def test_fast_ls_sweep_is_the_ls_sweep_to_float32_rounding():
+    """The learned form's float32 Liu-Shen sweep is the same operator as the
+    reference-dataflow sweep on a real setup (borders included): both agree
+    to a few float32 ulps of the update's scale, while a dropped neighbour, a
+    sign flip of the corner term or swapped derivative fields move them
+    apart by orders of magnitude."""
+    rng = np.random.default_rng(11)
+    H, W = 29, 41
+    im1 = (rng.random((H, W)) * 200 + 20).astype(np.float32)
+    im2 = np.roll(im1, 1, axis=1) + rng.standard_normal((H, W)).astype(np.float32)
+    f = orc.ls_setup(im1, im2, 5.0)
+    u, v = (rng.standard_normal((H, W)).astype(np.float32) * 0.5 for _ in range(2))
+    eu, ev, _ = orc.ls_sweep(f, 5.0, u, v)
+    tu, tv = orc.ls_sweep_fast(f, 5.0, u, v)
+    scale = max(np.abs(eu).max(), np.abs(ev).max())
+    tol = 16 * np.finfo(np.float32).eps * scale
+    assert max(np.abs(tu - eu).max(), np.abs(tv - ev).max()) <= tol
+    assert not np.array_equal(tu, eu)
+    g = dict(f)
+    g["IIx"], g["IIy"] = f["IIy"], f["IIx"]
+    su, sv = orc.ls_sweep_fast(g, 5.0, u, v)
+    assert max(np.abs(su - eu).max(), np.abs(sv - ev).max()) > 1e3 * tol
+    su, sv = orc.ls_sweep_fast(f, 5.0, u, -v)
+    assert max(np.abs(su - eu).max(), np.abs(sv - ev).max()) > 1e3 * tol
+
+
 def test_ls_fixed_point_solves_linear_system():
     """LS Jacobi fixed point: A [u; v] = -b(u, v) per pixel, where the sweep is
     u' = -(B bu): at convergence u = u'."""
