@@ -491,6 +491,18 @@ void ls_default(size_t npx, int* variant, int* depth) {
 
 }  // namespace
 
+// Liu-Shen's coefficient fields for (im1, im2) into F (8 fields of H * W),
+// the images' maxima into mx (2 floats); the control word starts cleared.
+static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h, float* F,
+                      float* mx, SolveCtrl* d_ctrl) {
+    const size_t N = (size_t)H * W;
+    OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx));
+    dim3 g((W + 255) / 256, H);
+    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N, d_ctrl);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
              float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast) {
@@ -522,15 +534,11 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     // per-sweep partial sums, two launch-parity slots
     const size_t pstride = (size_t)depth * nblk * 2;
     OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
-    OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
     float* F = (float*)fbuf;
     float* ub = F + 8 * N;
     float* vb = F + 9 * N;
-    float* mx = (float*)mbuf;
-    OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx));
-    dim3 g((W + 255) / 256, H);
-    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N, d_ctrl);
-    OFRI_HIP(hipGetLastError());
+    OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
+    OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, h, F, (float*)mbuf, d_ctrl));
 
     LsPass p{};
     p.in[0][0] = u; p.in[0][1] = v; p.in[1][0] = ub; p.in[1][1] = vb;
