@@ -52,10 +52,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="lshs", choices=["lshs", "plain", "batch", "fbls", "lk"],
+    ap.add_argument("--workload", default="lshs", choices=["lshs", "plain", "batch", "fbls", "lk", "net"],
                     help="lshs: 1024^2 LS(HS) 2-level learned vs plain (default); plain: the plain form alone; "
                          "batch: 64 uneven pairs LS(HS); "
-                         "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level")
+                         "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level; "
+                         "net: the correction network's forward pass over a 4-plane 1024^2 input")
     ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
     ap.add_argument("--eval-pairs", type=int, default=13,
                     help="pairs the default workload cycles over (its own pair + held-out rendered pairs)")
@@ -397,6 +398,28 @@ def net_roofline(ctx, net, n):
             "traffic": None, "ms": round(ms, 4), "flops": fl}
 
 
+def bench_net(args):
+    """The correction network's forward pass (both heads) over a 4-plane
+    n x n input, alone: TFLOP/s on the real channels."""
+    from ofri.device import DeviceArray, context
+    from ofri.net import Net
+    ctx = context()
+    n = args.size
+    net = Net.load()
+    rng = np.random.default_rng(0)
+    x = DeviceArray.from_host(rng.standard_normal((1, 4, n, n)).astype(np.float32))
+    c, f = DeviceArray((1, 4, n // 2, n // 2)), DeviceArray((1, 4, n, n))
+    wall, gpu_ms = _timed(lambda: net.predict(x, c, f), ctx, args.steps, args.warmup)
+    fl = net.flops(n, n)
+    value = fl * args.steps / wall / 1e12
+    ach = fl / (gpu_ms / args.steps * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "conv3_lds_k (bf16 MFMA) + input/head kernels", "achieved": round(ach, 2),
+            "peak": 2500.0, "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4), "traffic": None}
+    return _line("TFLOP/s, correction network forward %dx%d" % (n, n), value, "TFLOP/s", args, wall, gpu_ms,
+                 {"workload": "correction network forward, 4 input planes %dx%d, 16 layers (both heads)" % (n, n),
+                  "height": n, "width": n, "flops_per_pass": fl, "parallelism": "single GPU"}, roof, dtype="bf16")
+
+
 def bench_lshs(args):
     """The headline: LS(HS) 2-level at 1024^2 solved to the plain method's
     per-level residual, learned form against plain form.  A step is one pair;
@@ -513,7 +536,8 @@ def main():
               "see the sharding notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload != "lshs":
-        fn = {"plain": bench_plain, "batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk}[args.workload]
+        fn = {"plain": bench_plain, "batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk,
+              "net": bench_net}[args.workload]
         print(json.dumps(fn(args)))
         return
     print(json.dumps(bench_lshs(args)))

@@ -111,6 +111,7 @@ struct HsPass {
     size_t N;
     float* hist;             // per-sweep residuals (nullptr: not recorded)
     int check_last;          // 1: only a launch's last sweep is reduced and tested
+    int copy_back;           // final launch with no re-run: the result, if in buffer 1, copied into buffer 0
     // Chebyshev mode (hs_pass_k<..., true, true>): previous iterate buffers,
     // the weight of every global sweep and the damping of the plain step
     const float* pin[2][2];
@@ -485,11 +486,36 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
     }
 }
 
+// The owned pixels of tile (bx, by) copied from buffer 1 into buffer 0.
+template <int R, int WAVES>
+__device__ __forceinline__ void hs_copy_owned(const HsPass& a, int bx, int by) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K = a.K;
+    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
+    const int gx = bx * outW - K + lane;
+    const int gy0 = by * outH - K + w * R;
+    if (!(lane >= K && lane < 64 - K && gx < a.W)) return;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ry = w * R + r, gy = gy0 + r;
+        if (ry >= K && ry < WAVES * R - K && gy < a.H) {
+            const size_t q = (size_t)gy * a.W + gx;
+            a.out[0][0][q] = a.in[1][0][q];
+            a.out[0][1][q] = a.in[1][1][q];
+        }
+    }
+}
+
 template <int R, int WAVES, bool NORM, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
-    int nsw = a.nsweeps, src = a.src, base = 0;
+    int nsw = a.nsweeps, src = a.src, base = 0, fin = 0;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
-    if (NORM && !launch_args(a, blk, nblk, CHEB && !a.check_last, nsw, src, base)) return;
+    if (NORM && !launch_args(a, blk, nblk, CHEB && !a.check_last, nsw, src, base, &fin)) {
+        // a final launch with nothing to re-run: it delivers the result
+        if (a.final_ && a.copy_back && fin == 1) hs_copy_owned<R, WAVES>(a, blockIdx.x, blockIdx.y);
+        return;
+    }
     if constexpr (CHEB) hs_cheb_tile<R, WAVES>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
     else hs_tile<R, WAVES, NORM>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
@@ -678,9 +704,13 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
                 p.slot = (slot0 + j) & 1;
                 p.final_ = 1;
                 p.next_budget = next_budget;
+                // with the test at launch ends only, no launch is ever re-run:
+                // the final launch itself copies the result into U, V
+                p.copy_back = next_budget <= 0 && check_last ? 1 : 0;
                 kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
                 OFRI_HIP(hipGetLastError());
-                if (next_budget <= 0) OFRI_TRY(select_copy2(ctx, d_ctrl, U, Ub, U, V, Vb, V, N));
+                p.copy_back = 0;
+                if (next_budget <= 0 && !check_last) OFRI_TRY(select_copy2(ctx, d_ctrl, U, Ub, U, V, Vb, V, N));
             }
         } else if (src != 0) {
             OFRI_HIP(hipMemcpyAsync(U, Ub, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));

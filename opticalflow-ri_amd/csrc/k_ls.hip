@@ -114,6 +114,10 @@ struct LsPass {
     int rel_src;            // (relative buffers: unused here)
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
     int check_last;         // 1: only a launch's last sweep is reduced and tested
+    // final launch with no re-run: the result (u, v of the buffer holding it)
+    // written into dst (dst_add: added); dst[0] null: left where it is
+    float* dst[2];
+    int dst_add;
     // Chebyshev mode (ls_pass_k<..., true>): previous iterate buffers, the
     // weight of every global sweep and the damping of the plain step
     const float* pin[2][2];
@@ -428,11 +432,41 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
     }
 }
 
+// The owned pixels of tile (bx, by) of buffer `from` written (or added)
+// into dst.
+template <int R, int WAVES>
+__device__ __forceinline__ void ls_deliver(const LsPass& a, int from, int bx, int by) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K = a.K;
+    const int outW = 64 - 2 * K, outH = WAVES * R - 2 * K;
+    const int gx = bx * outW - K + lane;
+    const int gy0 = by * outH - K + w * R;
+    if (!(lane >= K && lane < 64 - K && gx < a.W)) return;
+    const float* su = a.in[from][0];
+    const float* sv = a.in[from][1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ry = w * R + r, gy = gy0 + r;
+        if (ry >= K && ry < WAVES * R - K && gy < a.H) {
+            const size_t q = (size_t)gy * a.W + gx;
+            const float u = su[q], v = sv[q];
+            a.dst[0][q] = a.dst_add ? a.dst[0][q] + u : u;
+            a.dst[1][q] = a.dst_add ? a.dst[1][q] + v : v;
+        }
+    }
+}
+
 template <int R, int WAVES, bool CHEB, bool FAST = false>
 __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
-    int nsw, src, base;
+    int nsw, src, base, fin = 0;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
-    if (!launch_args(a, blk, nblk, false, nsw, src, base)) return;
+    if (!launch_args(a, blk, nblk, false, nsw, src, base, &fin)) {
+        // a final launch with nothing to re-run: it delivers the result
+        if (a.final_ && a.dst[0] && (a.dst[0] != a.in[fin][0] || a.dst_add))
+            ls_deliver<R, WAVES>(a, fin, blockIdx.x, blockIdx.y);
+        return;
+    }
     ls_tile<R, WAVES, CHEB, FAST>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
 }
 
@@ -584,10 +618,18 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     }
     if (maxnum > 0) {
         // tests the last launch, re-runs a launch the test stopped inside,
-        // publishes the result
+        // publishes the result; with the test at launch ends only no launch
+        // is ever re-run, and the final launch itself delivers the result
         p.slot = j & 1;
         p.final_ = 1;
+        if (check_last) {
+            p.dst[0] = sink_u ? sink_u : u;
+            p.dst[1] = sink_u ? sink_v : v;
+            p.dst_add = sink_u && sink_add ? 1 : 0;
+        }
         kern<<<grid, 64 * var.waves, 0, ctx->stream>>>(p);
+        OFRI_HIP(hipGetLastError());
+        if (check_last) return OFRI_OK;
         if (!sink_u) OFRI_TRY(select_copy2(ctx, d_ctrl, u, ub, u, v, vb, v, N));
     }
     // the result (the start when no sweep ran: the control word says buffer 0)

@@ -96,14 +96,16 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
 // held inside it, and publishes the result; with next_budget > 0 it also
 // arms a following phase of at most that many sweeps (starting from the
 // result's buffer, reading the state slot this launch writes: its launches
-// take the opposite parity).
+// take the opposite parity).  fin (optional) receives the buffer holding the
+// solve's result as this launch found it.
 template <class Pass>
 __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bool always_redo, int& nsw, int& src,
-                                            int& base) {
+                                            int& base, int* fin = nullptr) {
     const int prev = a.slot ^ 1;
     SolveState s = take_test(a.ctrl, prev, a.part + (size_t)prev * a.part_stride, nblk, a.tol, a.maxnum, a.N,
                              blk == 0 ? a.hist : nullptr, always_redo);
     const bool lead = blk == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+    if (fin) *fin = s.final_buf;
     if (a.final_) {
         nsw = s.done ? s.redo : 0;
         src = s.redo_in;
