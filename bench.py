@@ -58,6 +58,9 @@ def parse():
                          "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level; "
                          "net: the correction network's forward pass over a 4-plane 1024^2 input")
     ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel eagerly instead of replaying HIP graphs (for tools that trace "
+                         "kernels; the kernels are the same)")
     ap.add_argument("--eval-pairs", type=int, default=13,
                     help="pairs the default workload cycles over (its own pair + held-out rendered pairs)")
     return ap.parse_args()
@@ -505,7 +508,7 @@ def bench_lshs(args):
                              "alpha 45 / 21 (plain: 600 sweeps), FILTER_OPT 0.48/5 taps, Liu-Shen alpha 5 (plain: tol "
                              "1e-8, max 60); learned form = accelerated HS + correction network + Liu-Shen to the "
                              "targets, plain fallback" % (n, n),
-                 "pairs_per_step": 1, "height": n, "width": n, "levels": 2, "eval_pairs": np_,
+                 "pairs_per_step": 1, "height": n, "width": n, "levels": 2, "eval_pairs": np_, "hip_graphs": not args.no_graphs,
                  "parallelism": "single GPU"},
                 cheb_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
     out["roofline_plain_hs"] = hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n)
@@ -535,6 +538,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (pairs are independent, "
               "see the sharding notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.no_graphs:
+        from ofri.device import context
+        context().set_graphs(False)
     if args.workload != "lshs":
         fn = {"plain": bench_plain, "batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk,
               "net": bench_net}[args.workload]
