@@ -258,10 +258,10 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
             num = num + y * vb;
             num = num + FT[r];
             float den;
-            if (NORM) {
-                // the residual-mode tiles carry more live state: the
-                // denominator is formed again (same operations, same bits)
-                // instead of held, which keeps the tile in registers
+            if (norm) {
+                // a sweep that reduces its update norm carries more live
+                // state: the denominator is formed again (same operations,
+                // same bits) instead of read, which keeps the tile in registers
                 asm volatile("" : "+v"(x), "+v"(y));
                 den = a.a2 + x * x;
                 den = den + y * y;
