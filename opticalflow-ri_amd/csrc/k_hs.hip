@@ -93,6 +93,103 @@ __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restric
     ft[q] = ct0 + (-ct1);
 }
 
+// The pre-filter and the derivatives in one pass over a 64 x 16 output
+// tile.  Both frames are filtered into LDS over the tile plus one row and
+// column on each side (the derivatives' 2 x 2 stencil and its mirror at the
+// last row and column) exactly as gauss_fused filters them (row pass, then
+// column pass, float32 in tap order, the reference's padding), then
+// hs_derivs_k's arithmetic runs on the filtered values.  Taps: at most
+// 2 * kDMaxH + 1.  512 threads: each half filters one frame.
+constexpr int kDTW = 64, kDTH = 16, kDMaxH = 4;
+
+__global__ __launch_bounds__(512) void hs_prefilter_derivs_k(const float* __restrict__ I0,
+                                                             const float* __restrict__ I1, int H, int W, Taps t,
+                                                             float* __restrict__ fx, float* __restrict__ fy,
+                                                             float* __restrict__ ft, SolveStart st) {
+    constexpr int FR = kDTH + 2, FC = kDTW + 2;   // filtered rows y0-1 .. y0+kDTH, columns x0-1 .. x0+kDTW
+    __shared__ float sin_[2][FR + 2 * kDMaxH][FC + 2 * kDMaxH];
+    __shared__ float srow[2][FR + 2 * kDMaxH][FC];
+    __shared__ float filt[2][FR][FC];
+    const int tid = threadIdx.x;
+    const int img = tid >> 8, th = tid & 255;
+    const int h = t.n / 2;
+    const int x0 = blockIdx.x * kDTW, y0 = blockIdx.y * kDTH;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        if (st.ctrl && tid == 0) {
+            SolveCtrl* c = st.ctrl;
+            c->done = 0;
+            c->sweeps = 0;
+            c->final_buf = 0;
+            c->redo = 0;
+            c->redo_in = 0;
+            c->last_err = 0.0f;
+            c->limit = 0;
+            c->base_buf = 0;
+            c->slot[0] = SolveState{};
+            c->slot[1] = SolveState{};
+        }
+        for (int k = tid; k < st.n; k += blockDim.x) st.omega[k] = st.tab.w[k < kOmegaTab ? k : kOmegaTab - 1];
+    }
+    const int R = FR + 2 * h, Cw = FC + 2 * h;
+    {
+        const float* __restrict__ src = img ? I1 : I0;
+        // source positions [y0-1-h, y0-1+FR+h) x [x0-1-h, x0-1+FC+h); positions
+        // outside the padding domain [-h, L+h) only feed filtered rows and
+        // columns outside the image, which no derivative reads
+        for (int i = th; i < R * Cw; i += 256) {
+            const int rr = i / Cw, cc = i - rr * Cw;
+            const int pr = y0 - 1 - h + rr, pc = x0 - 1 - h + cc;
+            float v = 0.0f;
+            if (pr >= -h && pr < H + h && pc >= -h && pc < W + h)
+                v = src[(size_t)pf_index(pr, H, h) * W + pf_index(pc, W, h)];
+            sin_[img][rr][cc] = v;
+        }
+        __syncthreads();
+        for (int i = th; i < R * FC; i += 256) {
+            const int rr = i / FC, cc = i - rr * FC;
+            float acc = 0.0f;
+            for (int j = 0; j < t.n; ++j) {
+                float prod = t.w[j] * sin_[img][rr][cc + 2 * h - j];
+                acc = acc + prod;
+            }
+            srow[img][rr][cc] = acc;
+        }
+        __syncthreads();
+        for (int i = th; i < FR * FC; i += 256) {
+            const int ry = i / FC, cc = i - ry * FC;
+            float acc = 0.0f;
+            for (int j = 0; j < t.n; ++j) {
+                float prod = t.w[j] * srow[img][ry + 2 * h - j][cc];
+                acc = acc + prod;
+            }
+            filt[img][ry][cc] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < kDTH * kDTW; i += 512) {
+        const int ry = i / kDTW, cx = i - ry * kDTW;
+        const int y = y0 + ry, x = x0 + cx;
+        if (y >= H || x >= W) continue;
+        const int yi = y - y0 + 1, xi = x - x0 + 1;
+        const int yi1 = ((y + 1 < H) ? y + 1 : H - 2) - y0 + 1;
+        const int xi1 = ((x + 1 < W) ? x + 1 : W - 2) - x0 + 1;
+        double a1 = filt[1][yi][xi], b1 = filt[1][yi][xi1];
+        double c1 = filt[1][yi1][xi], d1 = filt[1][yi1][xi1];
+        double a0 = filt[0][yi][xi], b0 = filt[0][yi][xi1];
+        double c0 = filt[0][yi1][xi], d0 = filt[0][yi1][xi1];
+        float cx1 = (float)(0.25 * a1 - 0.25 * b1 + 0.25 * c1 - 0.25 * d1);
+        float cx0 = (float)(0.25 * a0 - 0.25 * b0 + 0.25 * c0 - 0.25 * d0);
+        float cy1 = (float)(0.25 * a1 + 0.25 * b1 - 0.25 * c1 - 0.25 * d1);
+        float cy0 = (float)(0.25 * a0 + 0.25 * b0 - 0.25 * c0 - 0.25 * d0);
+        float ct1 = (float)(0.25 * a1 + 0.25 * b1 + 0.25 * c1 + 0.25 * d1);
+        float ct0 = (float)(0.25 * a0 + 0.25 * b0 + 0.25 * c0 + 0.25 * d0);
+        size_t q = (size_t)y * W + x;
+        fx[q] = cx1 + cx0;
+        fy[q] = cy1 + cy0;
+        ft[q] = ct0 + (-ct1);
+    }
+}
+
 struct HsPass {
     const float* in[2][2];   // [ping-pong][U, V]
     float* out[2][2];
@@ -592,7 +689,9 @@ void hs_default(size_t npx, int* variant, int* depth) {
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
              float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget,
-             bool zero_start) {
+             bool zero_start, const float* pre_w, int pre_n) {
+    if (pre_w && pre_n > 0 && (pre_n > kMaxTaps || (pre_n & 1) == 0 || H < pre_n || W < pre_n))
+        return fail(OFRI_E_ARG, "hs: pre-filter of %d taps on a %dx%d image", pre_n, H, W);
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
     if (niter < 0) return fail(OFRI_E_ARG, "hs: negative iteration count");
     if (zero_start && d_err) return fail(OFRI_E_ARG, "hs: a zero start keeps no initial flow for the error");
@@ -645,10 +744,28 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         start.n = niter;
         start.tab = cheb_omegas((hi - lo) / (2.0 - lo - hi));
     }
-    dim3 g((W + 255) / 256, H);
-    // computeDerivatives(I1=im2, I0=im1); the first workgroup also clears the
-    // control word and writes the Chebyshev weights
-    hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft, start);
+    // computeDerivatives(I1=im2, I0=im1) of the pre-filtered frames (pre_w:
+    // the filter, fused into the same pass when it has at most 9 taps); the
+    // first workgroup also clears the control word and writes the Chebyshev
+    // weights
+    const bool pre = pre_w && pre_n > 0;
+    if (pre && pre_n <= 2 * kDMaxH + 1) {
+        Taps t{};
+        for (int i = 0; i < pre_n; ++i) t.w[i] = pre_w[i];
+        t.n = pre_n;
+        dim3 gd((W + kDTW - 1) / kDTW, (H + kDTH - 1) / kDTH);
+        hs_prefilter_derivs_k<<<gd, 512, 0, ctx->stream>>>(im1, im2, H, W, t, fx, fy, ft, start);
+    } else {
+        if (pre) {
+            void* fb;
+            OFRI_TRY(scratch(ctx, "hs_prefiltered", 2 * N * sizeof(float), &fb));
+            OFRI_TRY(gauss_sep2(ctx, im1, (float*)fb, im2, (float*)fb + N, H, W, pre_w, pre_n));
+            im1 = (const float*)fb;
+            im2 = (const float*)fb + N;
+        }
+        dim3 g((W + 255) / 256, H);
+        hs_derivs_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, fx, fy, ft, start);
+    }
     OFRI_HIP(hipGetLastError());
     if (d_err) {
         OFRI_HIP(hipMemcpyAsync(U0, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));

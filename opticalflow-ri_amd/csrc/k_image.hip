@@ -12,19 +12,6 @@
 namespace ofri {
 namespace {
 
-constexpr int kMaxTaps = 127;
-
-struct Taps {
-    float w[kMaxTaps];
-    int n;
-};
-
-// Padding of a line of length L for a filter of half width h, as the
-// reference pads (gaussian_filter.py:63-66): p < 0 -> -1-p, p >= L -> p-h.
-__device__ __forceinline__ int pf_index(int p, int L, int h) {
-    return p < 0 ? -1 - p : (p >= L ? p - h : p);
-}
-
 // Row pass: out[m] = sum_j w[j] * line(m + h - j), float32, tap order, from 0.
 // Up to two planes per launch (blockIdx.z): source s[z] -> destination d[z].
 struct Planes2 {

@@ -549,6 +549,10 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     size_t N = (size_t)H * W;
     int vi, depth;
     ls_default((size_t)H * W, &vi, &depth);
+    // the float32 sweeps' registers leave room for 8 wavefronts of 4 rows
+    // below about 0.7 Mpx (measured: 8 % less per sweep than 8 rows x 4
+    // wavefronts at 512^2; the depth, and so the tested sweeps, unchanged)
+    if (fast && (size_t)H * W < (size_t)700000) vi = 2;
     if (cheb) {
         vi = 2;      // 4 rows x 8 wavefronts: room for the previous iterate in registers
         depth = 6;

@@ -394,13 +394,14 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         if (lh[l] < 4 || lw[l] < 4)
             return fail(OFRI_E_ARG, "ofri_pyr_lshs: level %d size %dx%d below 4x4", l + 1, lh[l], lw[l]);
     const size_t NF = (size_t)H * W;
-    // workspace: n1 n2 (resized) w1 w2 (warped) f1 f2 (filtered) o1 o2 (opt-filtered)
+    // workspace: n1 n2 (resized) w1 w2 (warped) (2 planes unused: the HS
+    // pre-filter is fused into its derivative pass) o1 o2 (opt-filtered)
     //            U V (solver) A B (accumulated, ping-pong with upsampled)
     void* ws;
     OFRI_TRY(scratch(ctx, "pyr_ws", 14 * NF * sizeof(float), &ws));
     float* base = (float*)ws;
     float *n1 = base, *n2 = base + NF, *w1 = base + 2 * NF, *w2 = base + 3 * NF;
-    float *f1 = base + 4 * NF, *f2 = base + 5 * NF, *o1 = base + 6 * NF, *o2 = base + 7 * NF;
+    float *o1 = base + 6 * NF, *o2 = base + 7 * NF;
     float *U = base + 8 * NF, *V = base + 9 * NF;
     float *Ua = base + 10 * NF, *Va = base + 11 * NF, *Ub = base + 12 * NF, *Vb = base + 13 * NF;
     void* st;
@@ -443,23 +444,23 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         } else if (!ls_sink) {
             OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
         }
-        if (p->filt_w && p->filt_n > 0) {
-            OFRI_TRY(gauss_sep2(ctx, s1, f1, s2, f2, h, w, p->filt_w, p->filt_n));
-            s1 = f1;
-            s2 = f2;
-        }
-        // (the learned form's HS starts from zero without reading U, V)
+        // (the pre-filter runs inside HS's derivative pass; the learned
+        // form's HS starts from zero without reading U, V)
+        const float* hpw = p->filt_n > 0 ? p->filt_w : nullptr;
+        const int hpn = hpw ? p->filt_n : 0;
         if (!lrn) OFRI_TRY(fill_zero2(ctx, U, V, N));
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget[l], U, V, nullptr, lp->hs_tol[l],
                               hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish[l], true,
-                              lp->hs_finish_budget[l], true));
+                              lp->hs_finish_budget[l], true, hpw, hpn));
         } else if (record_res) {
-            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l));
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l,
+                              nullptr, 0.0f, 0.0f, false, 0, false, hpw, hpn));
         } else {
-            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l));
+            OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, nullptr, nullptr,
+                              0.0f, 0.0f, false, 0, false, hpw, hpn));
         }
         if (tap) {
             OFRI_HIP(hipMemcpyAsync(tap, U, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));

@@ -110,6 +110,19 @@ struct ofri_ctx {
 
 namespace ofri {
 
+// Taps of the separable pre-filter (odd count, float32 weights).
+constexpr int kMaxTaps = 127;
+struct Taps {
+    float w[kMaxTaps];
+    int n;
+};
+
+// Padding of a line of length L for a filter of half width h, as the
+// reference pads (gaussian_filter.py:63-66): p < 0 -> -1-p, p >= L -> p-h.
+__device__ __forceinline__ int pf_index(int p, int L, int h) {
+    return p < 0 ? -1 - p : (p >= L ? p - h : p);
+}
+
 // Named scratch buffer of at least `bytes` bytes (grows, never shrinks).
 int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out);
 
@@ -161,7 +174,8 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
-             float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0, bool zero_start = false);
+             float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0, bool zero_start = false,
+             const float* pre_w = nullptr, int pre_n = 0);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);

@@ -545,6 +545,29 @@ def test_ls_learned_matches_restated_iteration(ops, shape, variant, depth):
     assert np.array_equal(du.download(), ur) and np.array_equal(dv.download(), vr)
 
 
+@pytest.mark.parametrize("H,W,n", [(130, 98, 3), (66, 161, 5), (256, 256, 9), (40, 34, 11)])
+def test_pyramid_hs_prefilter_fused_equals_separate_passes(ops, H, W, n):
+    """The pyramid runs its HS pre-filter inside the derivative pass (one
+    kernel for up to 9 taps, separate passes above): the coarse level's HS
+    flow equals HS on the separately pre-filtered resized pair bit for bit,
+    with tiles cut by the image border on both sides."""
+    from gaussian_filter import prepareGaussianKernel
+    from ofri.device import DeviceArray
+    rng = np.random.default_rng(n)
+    im1 = (rng.random((H, W)) * 255).astype(np.float32)
+    im2 = np.roll(im1, 2, axis=1) + rng.standard_normal((H, W)).astype(np.float32)
+    fw, ow = prepareGaussianKernel(1.5, n), prepareGaussianKernel(0.48, 5)
+    _, _, taps, st = ops.pyr_lshs_taps(im1, im2, 2, [45.0, 21.0], 50, fw, ow, 5.0)
+    h, w = st[0]["H"], st[0]["W"]
+    t = taps.download()
+    f1 = ops.gauss_sep(ops.resize(im1, h, w), fw)
+    f2 = ops.gauss_sep(ops.resize(im2, h, w), fw)
+    U, V = DeviceArray.zeros((h, w)), DeviceArray.zeros((h, w))
+    ops.hs(f1, f2, 45.0, 50, U, V)
+    assert np.array_equal(t[:h * w].reshape(h, w), U.download())
+    assert np.array_equal(t[h * w:2 * h * w].reshape(h, w), V.download())
+
+
 def test_ls_small_golden(ops, golden):
     from ofri.device import DeviceArray
     g = golden("ls_small")
