@@ -176,6 +176,20 @@ int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H,
  * window left the image.  Replaces denseLucasKanade_PyCl.compute
  * (denseLucasKanade_PyCL.py:113-169) and pyrlkDenseLargeW.cl lkDense (:304-669)
  * with the symmetric window (vorticity enhancement off). */
+/* The Liu-Shen warping mode of updateNextPyramidalLevel
+ * (GenericPyramidalOpticalFlow.py:188-220, biLinear=False): d_out = d_im1 with
+ * every pixel moved by its flow (d_u column, d_v row component) rounded half
+ * up, as numpy's fancy assignment moves it (a negative target wraps once, of
+ * several sources the last in row-major order wins), then the fractional
+ * flow smoothed by the symmetric Gaussian h_w[0..radius] (h_w[0] the centre:
+ * scipy.ndimage.gaussian_filter's float64 weights, radius <= 48, 'reflect'
+ * borders, float64 sums) and the optical-flow-equation intensity correction
+ * applied to all but the last row and column.  *h_status = 1 when a target
+ * lies past the image (the reference raises IndexError; d_out is then
+ * undefined), else 0.  The second frame is not warped in this mode. */
+int ofri_warp_liushen(ofri_ctx* ctx, const float* d_im1, int H, int W, const float* d_u, const float* d_v,
+                      const double* h_w, int radius, float* d_out, int* h_status);
+
 int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
                   float* d_v, int half_win, int niter, unsigned char* d_status);
 

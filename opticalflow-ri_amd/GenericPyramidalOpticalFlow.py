@@ -11,7 +11,8 @@ examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py -- the whole pyramid runs as one
 device-resident libofri call (ofri_pyr_lshs): the adapter's alpha list is
 consumed exactly as the reference consumes it and the same numbers come out.
 Every other combination (any adapter object following the reference's
-adapter protocol, kLevels > 1, no warping) goes through the level loop below,
+adapter protocol, kLevels > 1, no warping, Liu-Shen warping) goes through the
+level loop below,
 which calls adapter.compute() with numpy arrays as the reference does and
 runs resize / upsample / warp / pre-filter on the GPU.
 """
@@ -58,16 +59,22 @@ def _upsample_flow(Uaccum, Vaccum, shape_prev, shape, scale):
 def updateNextPyramidalLevel(im1IterNext, im1IterPrev, im2IterNext, Uaccum, Vaccum, U, V, warping=True,
                              biLinear=True, scale=False):
     """GenericPyramidalOpticalFlow.py:117-234.  Returns (im1 warped, im2 warped,
-    final Uaccum, final Vaccum, initial U, initial V) as numpy float32."""
+    final Uaccum, final Vaccum, initial U, initial V) as numpy float32.  The
+    Liu-Shen warping mode (biLinear False) warps im1IterNext in place, as the
+    reference does, and returns it; it leaves the second frame as it is."""
     shape = np.shape(im1IterNext)
     us, vs = _upsample_flow(Uaccum, Vaccum, np.shape(im1IterPrev), shape, scale)
     if warping:
+        z = np.zeros(shape, dtype='float32')
         if not biLinear:
-            raise NotImplementedError(
-                "Liu-Shen (integer shift + intensity correction) warping is not implemented; use biLinear=True")
+            ops.log('Warping: Liu-Shen')
+            w1 = ops.warp_liushen(im1IterNext, us, vs).download()
+            if isinstance(im1IterNext, np.ndarray) and im1IterNext.flags.writeable:
+                im1IterNext[...] = w1
+                w1 = im1IterNext
+            return w1, np.asarray(im2IterNext), us.download(), vs.download(), z, z.copy()
         ops.log('Warping: BiLinear')
         w1, w2 = ops.warp_symmetric(im1IterNext, im2IterNext, us, vs)
-        z = np.zeros(shape, dtype='float32')
         return w1.download(), w2.download(), us.download(), vs.download(), z, z.copy()
     z = np.zeros(shape, dtype='float32')
     return (np.asarray(im1IterNext), np.asarray(im2IterNext), z, z.copy(), us.download(), vs.download())
@@ -205,10 +212,16 @@ def _generic_levels(im1, im2, FILTER, main, levels, kLevels, FILTER_OPT, optiona
         else:
             new1, new2 = d1, d2
         if level > 1:
+            n1 = new1.download()
             w1, w2, Uaccum, Vaccum, U, V = updateNextPyramidalLevel(
-                new1.download(), np.empty(prev_shape, np.float32), new2.download(), Uaccum, Vaccum, U, V,
+                n1, np.empty(prev_shape, np.float32), new2.download(), Uaccum, Vaccum, U, V,
                 warping, biLinear, localScaling)
             warp1, warp2 = as_device(w1), as_device(w2)
+            if warping and not biLinear:
+                # the reference's Liu-Shen warp moves im1IterNew's pixels in
+                # place: the optional pre-filter and the re-warp iterations
+                # below see the warped frame (GenericPyramidalOpticalFlow.py:206,353,381,393)
+                new1 = as_device(n1)
         else:
             warp1, warp2 = new1, new2
             U = np.zeros((h, w), dtype='float32')

@@ -771,6 +771,132 @@ int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const floa
     return OFRI_OK;
 }
 
+// ---- Liu-Shen warping mode (GenericPyramidalOpticalFlow.py:188-220) ----
+
+// Scatter targets and fractional flow.  Source pixel i = (y, x) goes to
+// (y + floor(v + 0.5), x + floor(u + 0.5)) as numpy's fancy assignment
+// takes it: a negative target wraps once, a target still outside is an error
+// (*err set), and of several sources the last in row-major order wins (the
+// 1-based source index is max-reduced into owner, which starts at 0).
+__global__ void lsw_target_k(const float* __restrict__ u, const float* __restrict__ v, int H, int W,
+                             int* __restrict__ owner, int* __restrict__ err, float* __restrict__ du,
+                             float* __restrict__ dv) {
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        const float fu = floorf(u[i] + 0.5f), fv = floorf(v[i] + 0.5f);
+        du[i] = u[i] - fu;
+        dv[i] = v[i] - fv;
+        if (!(fabsf(fu) < 1e9f) || !(fabsf(fv) < 1e9f)) {
+            atomicOr(err, 1);
+            continue;
+        }
+        long long tx = x + (long long)fu, ty = y + (long long)fv;
+        if (tx < 0) tx += W;
+        if (ty < 0) ty += H;
+        if (tx < 0 || tx >= W || ty < 0 || ty >= H) {
+            atomicOr(err, 1);
+            continue;
+        }
+        atomicMax(&owner[(size_t)ty * W + tx], (int)i + 1);
+    }
+}
+
+// The scattered image: each target takes its winning source, other pixels keep theirs.
+__global__ void lsw_apply_k(const float* __restrict__ im, const int* __restrict__ owner, size_t n,
+                            float* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int o = owner[i];
+        out[i] = o ? im[o - 1] : im[i];
+    }
+}
+
+// scipy's 'reflect' (half-sample symmetric) index of any position.
+__device__ __forceinline__ int reflect_idx(int p, int L) {
+    p %= 2 * L;
+    if (p < 0) p += 2 * L;
+    return p >= L ? 2 * L - 1 - p : p;
+}
+
+constexpr int kLswMaxR = 48;
+struct LswTaps {
+    double w[kLswMaxR + 1];   // w[0] centre, w[j] at offsets +-j
+    int r;
+};
+
+// One axis of scipy's gaussian_filter on float32 data: float64 sum of the
+// centre tap, then the symmetric pairs from the outermost inwards
+// (NI_Correlate1D's order for a symmetric filter), rounded to float32.
+__global__ void lsw_gauss_k(Planes2 p, int H, int W, int axis, LswTaps t) {
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float* __restrict__ src = p.s[blockIdx.z];
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        double acc = (double)src[i] * t.w[0];
+        for (int j = t.r; j > 0; --j) {
+            double a, b;
+            if (axis == 0) {
+                a = src[(size_t)reflect_idx(y - j, H) * W + x];
+                b = src[(size_t)reflect_idx(y + j, H) * W + x];
+            } else {
+                a = src[(size_t)y * W + reflect_idx(x - j, W)];
+                b = src[(size_t)y * W + reflect_idx(x + j, W)];
+            }
+            acc = acc + (a + b) * t.w[j];
+        }
+        p.d[blockIdx.z][i] = (float)acc;
+    }
+}
+
+// Optical-flow-equation intensity correction of all but the last row and
+// column, float32 without contraction: im - ((im_r du_r - im du) + (im_d dv_d - im dv)).
+__global__ void lsw_correct_k(const float* __restrict__ im, const float* __restrict__ du,
+                              const float* __restrict__ dv, int H, int W, float* __restrict__ out) {
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        if (y < H - 1 && x < W - 1) {
+            float a = im[i + 1] * du[i + 1];
+            float b = im[i] * du[i];
+            const float tdx = a - b;
+            float c = im[i + W] * dv[i + W];
+            float d = im[i] * dv[i];
+            const float tdy = c - d;
+            out[i] = im[i] - (tdx + tdy);
+        } else {
+            out[i] = im[i];
+        }
+    }
+}
+
+int warp_liushen(ofri_ctx* ctx, const float* im1, int H, int W, const float* u, const float* v,
+                 const double* w, int r, float* out, int* d_err) {
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "warp_liushen: empty image");
+    if (r < 0 || r > kLswMaxR) return fail(OFRI_E_ARG, "warp_liushen: Gaussian radius %d outside [0, %d]", r, kLswMaxR);
+    const size_t n = (size_t)H * W;
+    if (n >= (size_t)0x7fffffff) return fail(OFRI_E_ARG, "warp_liushen: %dx%d image too large", H, W);
+    void* ws;
+    OFRI_TRY(scratch(ctx, "lsw", 6 * n * sizeof(float), &ws));
+    int* owner = (int*)ws;
+    float* du = (float*)ws + n;
+    float* dv = du + n;
+    float* su = dv + n;
+    float* sv = su + n;
+    float* sc = sv + n;
+    LswTaps t{};
+    for (int j = 0; j <= r; ++j) t.w[j] = w[j];
+    t.r = r;
+    OFRI_TRY(fill_zero(ctx, (float*)owner, n));
+    lsw_target_k<<<grid1d(n), 256, 0, ctx->stream>>>(u, v, H, W, owner, d_err, du, dv);
+    lsw_apply_k<<<grid1d(n), 256, 0, ctx->stream>>>(im1, owner, n, sc);
+    // gaussian_filter: axis 0 into (su, sv), then axis 1 back into (du, dv)
+    lsw_gauss_k<<<dim3(grid1d(n), 1, 2), 256, 0, ctx->stream>>>(Planes2{{du, dv}, {su, sv}}, H, W, 0, t);
+    lsw_gauss_k<<<dim3(grid1d(n), 1, 2), 256, 0, ctx->stream>>>(Planes2{{su, sv}, {du, dv}}, H, W, 1, t);
+    lsw_correct_k<<<grid1d(n), 256, 0, ctx->stream>>>(sc, du, dv, H, W, out);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
 int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* cx, int H, int W,
                 float* out) {
     if (H < 1 || W < 1) return fail(OFRI_E_ARG, "warp: empty image");

@@ -329,6 +329,21 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
     return OFRI_OK;
 }
 
+int ofri_warp_liushen(ofri_ctx* ctx, const float* d_im1, int H, int W, const float* d_u, const float* d_v,
+                      const double* h_w, int radius, float* d_out, int* h_status) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_u || !d_v || !h_w || !d_out) return fail(OFRI_E_ARG, "ofri_warp_liushen: null pointer");
+    void* e;
+    OFRI_TRY(scratch(ctx, "lsw_err", sizeof(int), &e));
+    OFRI_TRY(fill_zero(ctx, (float*)e, 1));
+    OFRI_TRY(warp_liushen(ctx, d_im1, H, W, d_u, d_v, h_w, radius, d_out, (int*)e));
+    int st = 0;
+    OFRI_HIP(hipMemcpyAsync(&st, e, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    if (h_status) *h_status = st ? 1 : 0;
+    return OFRI_OK;
+}
+
 int ofri_ls_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
                     int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters) {
     CHECK_CTX(ctx);

@@ -144,6 +144,12 @@ int warp_symmetric(ofri_ctx* ctx, const float* im1, const float* im2, const floa
                    const float* V, int H, int W, float* o1, float* o2);
 int warp_coords(ofri_ctx* ctx, const float* img, const float* cy, const float* cx, int H, int W,
                 float* out);
+// The Liu-Shen warping mode (integer scatter, smoothed fractional flow,
+// intensity correction) of im1 by (u, v) into out; w[0..r] the symmetric
+// Gaussian's half taps (w[0] the centre); *d_err (device int, cleared by the
+// caller) set when a target lies outside the image.
+int warp_liushen(ofri_ctx* ctx, const float* im1, int H, int W, const float* u, const float* v,
+                 const double* w, int r, float* out, int* d_err);
 int accumulate2(ofri_ctx* ctx, float* acc1, const float* x1, float* acc2, const float* x2, size_t n);
 int accumulate(ofri_ctx* ctx, float* acc, const float* x, size_t n);
 int fill_zero2(ofri_ctx* ctx, float* p1, float* p2, size_t n);

@@ -150,6 +150,33 @@ def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8):
     return np.float32(err.value), it.value
 
 
+def scipy_gaussian_weights(sigma, truncate):
+    """scipy.ndimage's float64 Gaussian weights (radius int(truncate * sigma +
+    0.5), normalised), formed with numpy as scipy forms them."""
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1)
+    phi = np.exp(-0.5 / (float(sigma) * float(sigma)) * x ** 2)
+    return phi / phi.sum()
+
+
+def warp_liushen(im1, u, v, sigma=1.8, truncate=20.0):
+    """ofri_warp_liushen: the Liu-Shen warping mode of the first frame by
+    (u, v) (GenericPyramidalOpticalFlow.py:188-220).  Raises IndexError where
+    the reference's pixel scatter would."""
+    im1, u, v = as_device(im1), as_device(u), as_device(v)
+    H, W = im1.shape[-2:]
+    w = scipy_gaussian_weights(sigma, truncate)
+    r = (len(w) - 1) // 2
+    half = np.ascontiguousarray(w[r:], dtype=np.float64)
+    out = DeviceArray((H, W))
+    st = C.c_int(0)
+    _capi.check(_lib().ofri_warp_liushen(_h(), im1.ptr, H, W, u.ptr, v.ptr,
+                                         half.ctypes.data_as(C.POINTER(C.c_double)), r, out.ptr, C.byref(st)))
+    if st.value:
+        raise IndexError("Liu-Shen warping: a pixel's displaced position lies outside the %dx%d image" % (H, W))
+    return out
+
+
 def ls_learned(im1, im2, h, u, v, maxnum=24, tol=1e-8):
     """ofri_ls_learned: Liu-Shen tested once per fused launch (its last sweep,
     the reference update) with float32 sweeps in between; u, v in place.

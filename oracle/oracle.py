@@ -184,6 +184,71 @@ def ls_sweep_fast(fields, h, u, v):
     return -fma32(b11, bu, b12 * bv), -fma32(b12, bu, b22 * bv)
 
 
+def gauss_kernel_scipy(sigma, truncate):
+    """scipy.ndimage's Gaussian weights (float64, normalised; radius
+    int(truncate * sigma + 0.5)), as gaussian_filter1d forms them."""
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1)
+    phi = np.exp(-0.5 / (float(sigma) * float(sigma)) * x ** 2)
+    return phi / phi.sum()
+
+
+def _reflect(p, L):
+    """scipy 'reflect' (half-sample symmetric) index of any position."""
+    p = np.mod(p, 2 * L)
+    return np.where(p >= L, 2 * L - 1 - p, p)
+
+
+def gauss_filter_scipy(x, sigma, truncate):
+    """scipy.ndimage.gaussian_filter of a float32 image, restated: axis 0
+    then axis 1, each a symmetric float64 correlation summed as scipy's
+    NI_Correlate1D sums a symmetric filter (centre tap first, then the pairs
+    x[i-j] + x[i+j] from the outermost inwards), rounded to float32 between
+    the axes, 'reflect' borders.  Used by the Liu-Shen warping mode
+    (GenericPyramidalOpticalFlow.py:203-204)."""
+    w = gauss_kernel_scipy(sigma, truncate)
+    r = (len(w) - 1) // 2
+    out = np.asarray(x, np.float32)
+    for axis in (0, 1):
+        a = np.moveaxis(out, axis, 0).astype(np.float64)
+        L = a.shape[0]
+        idx = np.arange(L)
+        acc = a * w[r]
+        for j in range(r, 0, -1):
+            acc = acc + (a[_reflect(idx - j, L)] + a[_reflect(idx + j, L)]) * w[r - j]
+        out = np.moveaxis(acc.astype(np.float32), 0, axis)
+    return np.ascontiguousarray(out)
+
+
+def ls_warp(im1, us, vs):
+    """The Liu-Shen warping mode of updateNextPyramidalLevel
+    (GenericPyramidalOpticalFlow.py:188-220) on float32 arrays: every pixel
+    moved by its flow rounded half up (numpy fancy assignment: negative
+    targets wrap, the last source in row-major order wins, targets past the
+    end raise IndexError), the fractional flow smoothed by
+    gaussian_filter(sigma 1.8, truncate 20: 73 taps), then the
+    optical-flow-equation intensity correction of all but the last row and
+    column.  Returns the warped first image."""
+    im = np.array(im1, dtype=np.float32)
+    us = np.asarray(us, np.float32)
+    vs = np.asarray(vs, np.float32)
+    H, W = im.shape
+    ys, xs = np.meshgrid(np.arange(H, dtype=np.int32), np.arange(W, dtype=np.int32), indexing="ij")
+    fu, fv = np.floor(us + np.float32(0.5)), np.floor(vs + np.float32(0.5))
+    uswap = np.int32(xs + fu)
+    vswap = np.int32(ys + fv)
+    du = us - fu
+    dv = vs - fv
+    im[vswap, uswap] = im[ys, xs]
+    mask = 3
+    du = gauss_filter_scipy(du, 0.6 * mask, 4.0 / 0.6 * mask)
+    dv = gauss_filter_scipy(dv, 0.6 * mask, 4.0 / 0.6 * mask)
+    tdx = im[0:-1, 1:] * du[0:-1, 1:] - im[0:-1, 0:-1] * du[0:-1, 0:-1]
+    tdy = im[1:, 0:-1] * dv[1:, 0:-1] - im[0:-1, 0:-1] * dv[0:-1, 0:-1]
+    im[0:-1, 0:-1] = im[0:-1, 0:-1] - (tdx + tdy)
+    return im
+
+
 def hs(I0, I1, alpha, niter, U=None, V=None, prec="f32"):
     """Returns (U, V, total_error)."""
     I0 = _arr(I0, prec); I1 = _arr(I1, prec)

@@ -568,6 +568,52 @@ def test_pyramid_hs_prefilter_fused_equals_separate_passes(ops, H, W, n):
     assert np.array_equal(t[h * w:2 * h * w].reshape(h, w), V.download())
 
 
+@pytest.mark.parametrize("H,W,seed", [(64, 80, 1), (37, 53, 2), (130, 257, 3)])
+def test_liu_shen_warp_matches_oracle(ops, H, W, seed):
+    """ofri_warp_liushen equals the oracle's restatement of the reference's
+    Liu-Shen warping bit for bit: integer scatter with wrapping negative
+    targets and colliding sources, the 73-tap smoothing of the fractional
+    flow, the intensity correction; a target past the end raises IndexError."""
+    rng = np.random.default_rng(seed)
+    im = (rng.random((H, W)) * 200).astype(np.float32)
+    # flows up to +-3 px, kept inside the image at the far edges (the
+    # reference raises there); negative targets near the near edges wrap
+    ys, xs = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    u = (rng.random((H, W)) * 6 - 3).astype(np.float32)
+    v = (rng.random((H, W)) * 6 - 3).astype(np.float32)
+    u = np.where(xs + np.floor(u + 0.5) >= W, np.float32(-1.25), u).astype(np.float32)
+    v = np.where(ys + np.floor(v + 0.5) >= H, np.float32(-2.75), v).astype(np.float32)
+    ref = orc.ls_warp(im, u, v)
+    out = ops.warp_liushen(im, u, v).download()
+    assert np.array_equal(out, ref)
+    bad = u.copy()
+    bad[H // 2, W - 1] = 0.5
+    with pytest.raises(IndexError):
+        ops.warp_liushen(im, bad, v)
+
+
+def test_pyramid_liu_shen_warping_dropin(ops):
+    """The drop-in's updateNextPyramidalLevel with biLinear=False: the first
+    frame is warped in place (spline-upsampled, scaled flow) as the oracle
+    restates it, the second frame is left alone, the flow becomes the
+    accumulated one and the solver restarts from zero."""
+    import GenericPyramidalOpticalFlow as G
+    rng = np.random.default_rng(9)
+    im1 = (rng.random((64, 96)) * 200).astype(np.float32)
+    im2 = (rng.random((64, 96)) * 200).astype(np.float32)
+    # flows pointing up and left (negative targets wrap; none past the end)
+    Uc = (-rng.random((32, 48))).astype(np.float32)
+    Vc = (-rng.random((32, 48))).astype(np.float32)
+    prev = np.empty((32, 48), np.float32)
+    a = im1.copy()
+    w1, w2, Ua, Va, U, V = G.updateNextPyramidalLevel(a, prev, im2, Uc, Vc, None, None, True, False, True)
+    us = orc.spline_upsample(Uc, 64, 96, np.float32(2.0))
+    vs = orc.spline_upsample(Vc, 64, 96, np.float32(2.0))
+    assert np.array_equal(Ua, us) and np.array_equal(Va, vs)
+    assert np.array_equal(w1, orc.ls_warp(im1, us, vs)) and w1 is a
+    assert np.array_equal(w2, im2) and not U.any() and not V.any()
+
+
 def test_ls_small_golden(ops, golden):
     from ofri.device import DeviceArray
     g = golden("ls_small")
@@ -655,6 +701,40 @@ def test_generic_path_equals_device_path(ops):
     Ub, Vb = genericPyramidalOpticalFlow(im1, im2, 3.4, Wrapped([21, 45], 200), 2, 1, 0.48,
                                          LiuShenOpticalFlowAlgoAdapter(5))
     assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
+
+
+def test_generic_pyramid_liu_shen_warping_runs_the_reference_loop(ops):
+    """genericPyramidalOpticalFlow with Liu-Shen warping (biLinear=False),
+    through the level loop: with one iteration per level its flow is close
+    to (not the same as) bilinear warping's; with a re-warp iteration
+    (kLevels=2) it runs to finite flows.  (At levels after the first the
+    reference re-warps the frame its first warp moved in place, so kLevels=2
+    drifts from the shift; the drop-in keeps that.)"""
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from HornSchunck import HSOpticalFlowAlgoAdapter
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+
+    class NoDefaults(HSOpticalFlowAlgoAdapter):
+        def hasGenericPyramidalDefaults(self):
+            return False
+
+    im1, im2 = _piv(96, 128, seed=61, shift=(-1.3, -0.6))
+    out = {}
+    for bil in (False, True):
+        out[bil] = genericPyramidalOpticalFlow(im1, im2, 3.4, NoDefaults([21, 45], 60), 2, 1, 0.48,
+                                               LiuShenOpticalFlowAlgoAdapter(5), True, bil, True, True)
+    (U, V), (Ub, Vb) = out[False], out[True]
+    assert U.shape == (96, 128) and np.isfinite(U).all() and np.isfinite(V).all()
+    dU = np.median(np.abs(U - Ub)[16:-16, 16:-16])
+    dV = np.median(np.abs(V - Vb)[16:-16, 16:-16])
+    print("[liu-shen warping] median U %.3f V %.3f, median |difference to bilinear| %.4f %.4f px"
+          % (np.median(U), np.median(V), dU, dV))
+    assert dU < 0.3 and dV < 0.3
+    assert not np.array_equal(U, Ub)
+    # the adapter pops one alpha per compute call: two per level with kLevels=2
+    U2, V2 = genericPyramidalOpticalFlow(im1, im2, 3.4, NoDefaults([21, 21, 45, 45], 60), 2, 2, 0.48,
+                                         LiuShenOpticalFlowAlgoAdapter(5), True, False, True, True)
+    assert np.isfinite(U2).all() and np.isfinite(V2).all()
 
 
 def test_full_size_pipeline_properties(ops):

@@ -182,6 +182,58 @@ def test_fast_ls_sweep_is_the_ls_sweep_to_float32_rounding():
     assert max(np.abs(su - eu).max(), np.abs(sv - ev).max()) > 1e3 * tol
 
 
+def test_scipy_gaussian_restatement_is_bit_exact():
+    """The Liu-Shen warp's 73-tap smoothing restated (float64 symmetric sums
+    in scipy's order, float32 between the axes, 'reflect' borders, repeated
+    on lines shorter than the filter) equals scipy.ndimage.gaussian_filter
+    bit for bit."""
+    import scipy.ndimage as nd
+    rng = np.random.default_rng(3)
+    for shape in [(50, 61), (37, 20), (12, 9), (1, 40)]:
+        x = (rng.random(shape) - 0.5).astype(np.float32)
+        assert np.array_equal(orc.gauss_filter_scipy(x, 1.8, 20.0), nd.gaussian_filter(x, 1.8, truncate=20.0))
+    assert len(orc.gauss_kernel_scipy(1.8, 20.0)) == 73
+
+
+def test_liu_shen_warp_semantics():
+    """Liu-Shen warping: a zero flow leaves the image as it is; a whole-pixel
+    flow moves pixels (rounded half up, negative targets wrapping, the last
+    source in row-major order winning) with no intensity correction; a
+    target past the end raises IndexError as the reference's assignment
+    does; a fractional flow changes all but the last row and column."""
+    rng = np.random.default_rng(4)
+    H, W = 9, 11
+    im = (rng.random((H, W)) * 100).astype(np.float32)
+    z = np.zeros((H, W), np.float32)
+    assert np.array_equal(orc.ls_warp(im, z, z), im)
+    u = np.full((H, W), -1.0, np.float32)         # every pixel one column left, column 0 wrapping
+    out = orc.ls_warp(im, u, z)
+    assert np.array_equal(out[:, :-1], im[:, 1:]) and np.array_equal(out[:, -1], im[:, 0])
+    u = np.zeros((H, W), np.float32)
+    u[4, 2] = 1.0                                  # (4, 2) and (4, 3) both land on (4, 3): the later source wins
+    u[4, 3] = 0.0
+    out = orc.ls_warp(im, u, z)
+    assert out[4, 3] == im[4, 3] and out[4, 2] == im[4, 2]
+    u[4, 3] = -1.0                                 # now (4, 3) goes to (4, 2) and (4, 2) to (4, 3): a swap
+    out = orc.ls_warp(im, u, z)
+    assert out[4, 3] == im[4, 2] and out[4, 2] == im[4, 3]
+    v = np.zeros((H, W), np.float32)
+    v[0, 3] = -1.0                                 # row 0 -> row -1 = the last row,
+    v[H - 1, 3] = -1.0                             # whose own pixel moves up
+    w = orc.ls_warp(im, z, v)
+    assert w[H - 1, 3] == im[0, 3] and w[H - 2, 3] == im[H - 1, 3]
+    u2 = z.copy()
+    u2[2, W - 1] = 0.5                             # rounds half up to column W: out of range
+    try:
+        orc.ls_warp(im, u2, z)
+        raise AssertionError("expected IndexError")
+    except IndexError:
+        pass
+    f = orc.ls_warp(im, np.full((H, W), 0.25, np.float32), z)
+    assert not np.array_equal(f[:-1, :-1], im[:-1, :-1])
+    assert np.array_equal(f[-1, :], im[-1, :]) and np.array_equal(f[:, -1], im[:, -1])
+
+
 def test_ls_fixed_point_solves_linear_system():
     """LS Jacobi fixed point: A [u; v] = -b(u, v) per pixel, where the sweep is
     u' = -(B bu): at convergence u = u'."""
