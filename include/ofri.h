@@ -156,15 +156,28 @@ typedef struct ofri_fb_params {
     const float* win_kernel;  /* win_kh+1 half taps of the window blur */
     int win_kh;
     int iters;                /* flow iterations */
+    float flow_scale;         /* the initial flow is multiplied by this first (float32); 1 = as given */
 } ofri_fb_params;
 
-/* One Farneback level (the reference's compute() with pyramidalLevels = 1):
- * reflect-101 pre-blur, polynomial expansion of both frames, then `iters`
- * rounds of window blur, flow update and matrices update.  d_flowx / d_flowy:
- * initial flow in, result out.  Replaces Farneback_PyCL.compute
- * (Farneback_PyCL.py:461-603) and the kernels of optical_flow_farneback.cl. */
+/* One Farneback level (the reference's compute() at one internal pyramid
+ * level): flow *= flow_scale, reflect-101 pre-blur, polynomial expansion of
+ * both frames, then `iters` rounds of window blur, flow update and matrices
+ * update.  d_flowx / d_flowy: initial flow in, result out.  With pre_kh = 0
+ * and pre_kernel = {1} the frames are taken as already pre-filtered (the
+ * coarser levels, whose blur runs at full resolution before the resize).
+ * Replaces the per-level body of Farneback_PyCL.compute
+ * (Farneback_PyCL.py:515-596) and the kernels of optical_flow_farneback.cl;
+ * the level loop (Farneback_PyCL.py:475-482,507-520) stays on the host. */
 int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                    float* d_flowx, float* d_flowy, const ofri_fb_params* p);
+
+/* Reflect-101 separable Gaussian blur of one float plane with a half kernel
+ * (kh+1 taps, centre first; kh <= 192), vertical pass then horizontal.
+ * Replaces Farneback_PyCL.gaussianBlur (Farneback_PyCL.py:210-240) with the
+ * .cl gaussianBlur kernel; the level loop blurs each full-resolution frame
+ * with the level's kernel before resizing it (Farneback_PyCL.py:560-580). */
+int ofri_fb_blur(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, const float* half_kernel,
+                 int kh);
 
 /* ---- dense Lucas-Kanade --------------------------------------------------- */
 /* One level of dense pyramidal LK with a (2*half_win+1)^2 window (half_win

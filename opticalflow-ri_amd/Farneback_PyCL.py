@@ -7,7 +7,11 @@ accepted and ignored: the context's GPU is used).  The expansion constants
 (FarnebackPrepareGaussian) and the blur kernels (getGaussianKernelBitExact)
 are formed on the host with the reference's dtype sequence; the device runs
 pre-blur, polynomial expansion and the flow iterations with all buffers
-resident, without the reference's per-kernel host copies.
+resident, without the reference's per-kernel host copies.  The internal
+pyramid (pyramidalLevels > 1, Farneback_PyCL.py:475-596) keeps the
+reference's level loop on the host; per level the frames are blurred at full
+resolution and Pillow-BILINEAR resized on the device (ofri_fb_blur,
+ofri_resize), and the flow is resized and scaled there too.
 """
 import numpy as np
 
@@ -107,18 +111,41 @@ class Farneback_PyCL(object):
         if not self.useGaussianFilter:
             raise NotImplementedError("box-filter window (useGaussian=False) is not implemented")
         size = np.shape(im1)
+        min_size = 32
         levels, scale = 0, 1.0
         while levels < self.pyramidalLevels:
             scale *= self.pyrScale
-            if size[1] * scale < 32 or size[0] * scale < 32:
+            if size[1] * scale < min_size or size[0] * scale < min_size:
                 break
             levels += 1
-        if levels > 0:
-            raise NotImplementedError("Farneback internal pyramid levels > 1 are not implemented")
+        c = self._consts()
+        a1 = np.ascontiguousarray(im1, dtype=np.float32)
+        a2 = np.ascontiguousarray(im2, dtype=np.float32)
         fx = DeviceArray.from_host(np.ascontiguousarray(U, dtype=np.float32))
         fy = DeviceArray.from_host(np.ascontiguousarray(V, dtype=np.float32))
-        ops.farneback(np.ascontiguousarray(im1, dtype=np.float32), np.ascontiguousarray(im2, dtype=np.float32),
-                      fx, fy, self._consts())
+        if levels == 0:
+            # one level: the resizes are identities, the pre-blur (3, 0) runs on the device
+            ops.farneback(a1, a2, fx, fy, c)
+            return fx.download(), fy.download(), 'Unknown'
+        d1, d2 = DeviceArray.from_host(a1), DeviceArray.from_host(a2)
+        blurred = DeviceArray(d1.shape)
+        ident = dict(c, pre_kernel=np.ones(1, np.float32))
+        first = True
+        for k in range(levels, -1, -1):                    # Farneback_PyCL.py:507-596
+            scale = 1.0
+            for _ in range(k):
+                scale *= self.pyrScale
+            sigma = (1.0 / scale - 1.0) * 0.5
+            smoothSize = max(int(round(sigma * 5)) | 1, 3)
+            width, height = int(round(size[1] * scale)), int(round(size[0] * scale))
+            fx = ops.resize(fx, height, width, ops.BILINEAR)
+            fy = ops.resize(fy, height, width, ops.BILINEAR)
+            self.setGaussianBlurKernel(smoothSize, sigma)
+            pk = self.matrixGKernel[0]
+            f1 = ops.resize(ops.fb_blur(d1, pk, out=blurred), height, width, ops.BILINEAR)
+            f2 = ops.resize(ops.fb_blur(d2, pk, out=blurred), height, width, ops.BILINEAR)
+            ops.farneback(f1, f2, fx, fy, ident, flow_scale=scale if first else 1.0 / self.pyrScale)
+            first = False
         return fx.download(), fy.download(), 'Unknown'
 
     def getAlgoName(self):

@@ -18,7 +18,7 @@
 namespace ofri {
 namespace {
 
-constexpr int kMaxHalf = 64;
+constexpr int kMaxHalf = 192;
 constexpr int kMaxPoly = 8;
 
 struct HalfKernel {
@@ -201,6 +201,14 @@ __global__ void fb_flow_k(const float* __restrict__ Mb, float* __restrict__ flx,
     if (matrices) fb_matrices_at(x, y, H, W, dx, dy, R0, R1, M);
 }
 
+// curFlow *= scale of the level loop, float32 products.
+__global__ void fb_scale_k(float* __restrict__ fx, float* __restrict__ fy, size_t n, float s) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        fx[i] = fx[i] * s;
+        fy[i] = fy[i] * s;
+    }
+}
+
 int blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, int planes, const HalfKernel& k) {
     void* tmp;
     OFRI_TRY(scratch(ctx, "fb_blur_tmp", (size_t)planes * H * W * sizeof(float), &tmp));
@@ -212,14 +220,24 @@ int blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, int planes, 
 
 }  // namespace
 
+// The reflect-101 index maps reduce modulo the side, so any half size up to
+// kMaxHalf stays in bounds, also when it exceeds the image (the reference's
+// index helpers behave the same way).
+int fb_blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* half, int kh) {
+    if (kh < 0 || kh > kMaxHalf) return fail(OFRI_E_ARG, "fb_blur: kernel half size %d outside 0..%d", kh, kMaxHalf);
+    if (H < 1 || W < 1) return fail(OFRI_E_ARG, "fb_blur: image %dx%d", H, W);
+    HalfKernel k{};
+    k.kh = kh;
+    for (int i = 0; i <= kh; ++i) k.g[i] = half[i];
+    return blur(ctx, src, dst, H, W, 1, k);
+}
+
 int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* flx, float* fly,
               const ofri_fb_params* p) {
     if (p->poly_n < 1 || p->poly_n > kMaxPoly) return fail(OFRI_E_ARG, "farneback: polyN %d", p->poly_n);
     if (p->pre_kh < 0 || p->pre_kh > kMaxHalf || p->win_kh < 0 || p->win_kh > kMaxHalf)
         return fail(OFRI_E_ARG, "farneback: kernel half sizes %d/%d above %d", p->pre_kh, p->win_kh, kMaxHalf);
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "farneback: image %dx%d too small", H, W);
-    if (p->pre_kh >= H || p->pre_kh >= W || p->win_kh >= H || p->win_kh >= W)
-        return fail(OFRI_E_ARG, "farneback: image %dx%d smaller than the blur window", H, W);
     const size_t N = (size_t)H * W;
     void* ws;
     OFRI_TRY(scratch(ctx, "fb_ws", 21 * N * sizeof(float), &ws));
@@ -242,6 +260,8 @@ int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, f
     }
     for (int i = 0; i < 4; ++i) c.ig[i] = p->ig[i];
     dim3 g2((W + 255) / 256, H);
+    if (p->flow_scale != 1.0f)
+        fb_scale_k<<<(unsigned)((N + 255) / 256 < 4096 ? (N + 255) / 256 : 4096), 256, 0, ctx->stream>>>(flx, fly, N, p->flow_scale);
     OFRI_TRY(blur(ctx, im1, bl, H, W, 1, pk));
     fb_polyexp_k<<<g2, 256, 0, ctx->stream>>>(bl, H, W, c, RA);
     OFRI_TRY(blur(ctx, im2, bl, H, W, 1, pk));

@@ -379,6 +379,13 @@ int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H,
     return farneback(ctx, d_im1, d_im2, H, W, d_flowx, d_flowy, p);
 }
 
+int ofri_fb_blur(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, const float* half_kernel,
+                 int kh) {
+    CHECK_CTX(ctx);
+    if (!d_src || !d_dst || !half_kernel) return fail(OFRI_E_ARG, "ofri_fb_blur: null pointer");
+    return fb_blur(ctx, d_src, d_dst, H, W, half_kernel, kh);
+}
+
 }  // extern "C"
 
 namespace ofri {

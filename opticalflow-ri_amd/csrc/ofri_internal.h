@@ -189,6 +189,7 @@ void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 // ---- Farneback (k_fb.hip) ----
 int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* flx, float* fly,
               const ofri_fb_params* p);
+int fb_blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const float* half, int kh);
 
 // ---- dense Lucas-Kanade (k_lk.hip) ----
 int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw, int niter, float* u,

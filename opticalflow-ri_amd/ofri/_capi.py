@@ -68,6 +68,7 @@ class FbParams(C.Structure):
         ("win_kernel", C.POINTER(C.c_float)),
         ("win_kh", C.c_int),
         ("iters", C.c_int),
+        ("flow_scale", C.c_float),
     ]
 
 
@@ -110,6 +111,7 @@ SIGNATURES = {
     "ofri_ls_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
+    "ofri_fb_blur": [_vp, _vp, C.c_int, C.c_int, _vp, _fp, C.c_int],
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],
     "ofri_render_pair": [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp, C.c_int, _fp, C.c_int,
                          C.c_float, _vp, _vp, _vp, _vp],

@@ -189,11 +189,21 @@ def ls_learned(im1, im2, h, u, v, maxnum=24, tol=1e-8):
     return np.float32(err.value), it.value
 
 
-def farneback(im1, im2, flowx, flowy, consts):
-    """One Farneback level (Farneback_PyCL.py:461-603); flowx / flowy device
-    arrays updated in place.  consts: dict of float32 arrays g, xg, xxg (polyN+1
-    taps), ig (4), pre_kernel, win_kernel (half kernels, centre first), and
-    the int iters."""
+def fb_blur(img, half_kernel, out=None):
+    """Reflect-101 separable Gaussian blur (Farneback_PyCL.py:210-240) with a
+    half kernel (centre first); returns a new device array unless out given."""
+    img = as_device(img)
+    k = np.ascontiguousarray(half_kernel, dtype=np.float32)
+    out = out if out is not None else DeviceArray((img.H, img.W))
+    _capi.check(_lib().ofri_fb_blur(_h(), img.ptr, img.H, img.W, out.ptr, _capi.fptr(k), len(k) - 1))
+    return out
+
+
+def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
+    """One Farneback level (Farneback_PyCL.py:515-596); flowx / flowy device
+    arrays updated in place, multiplied by float32(flow_scale) first.
+    consts: dict of float32 arrays g, xg, xxg (polyN+1 taps), ig (4),
+    pre_kernel, win_kernel (half kernels, centre first), and the int iters."""
     im1, im2 = as_device(im1), as_device(im2)
     arrs = {k: np.ascontiguousarray(consts[k], dtype=np.float32)
             for k in ("g", "xg", "xxg", "ig", "pre_kernel", "win_kernel")}
@@ -205,6 +215,7 @@ def farneback(im1, im2, flowx, flowy, consts):
     p.win_kernel = _capi.fptr(arrs["win_kernel"])
     p.win_kh = len(arrs["win_kernel"]) - 1
     p.iters = int(consts["iters"])
+    p.flow_scale = float(np.float32(flow_scale))
     _capi.check(_lib().ofri_farneback(_h(), im1.ptr, im2.ptr, im1.H, im1.W, flowx.ptr, flowy.ptr, C.byref(p)))
     return flowx, flowy
 

@@ -16,6 +16,7 @@
  *   src/PhysicsBasedOpticalFlowLiuShen.py:47-158
  *                                           orc_ls_setup_*, orc_ls_sweep_*, orc_ls_*
  *   src/GenericPyramidalOpticalFlow.py:66-67   orc_resize_bicubic_* (+ orc_pil_coeffs)
+ *   src/Farneback_PyCL.py:61-62             orc_resize_* (BILINEAR, filter 2)
  *   src/GenericPyramidalOpticalFlow.py:151-171 orc_spline_upsample_* (+ orc_spline_matrix)
  *   src/GenericPyramidalOpticalFlow.py:69-115,197-200  orc_warp_symmetric_*
  *   src/GenericPyramidalOpticalFlow.py:237-415 orc_pyr_lshs_*

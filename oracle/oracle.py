@@ -54,6 +54,7 @@ def _declare(L):
         f.argtypes = [P, P, C.c_int, C.c_int, R, C.c_int, C.c_double, P, P, _ip]
         f.restype = C.c_double
         getattr(L, "orc_resize_bicubic_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int]
+        getattr(L, "orc_resize_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int]
         getattr(L, "orc_spline_upsample_" + suf).argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, R, _dp]
         getattr(L, "orc_warp_symmetric_" + suf).argtypes = [P, P, P, P, C.c_int, C.c_int, P, P]
         getattr(L, "orc_pyr_lshs_" + suf).argtypes = [
@@ -290,6 +291,16 @@ def ls(im1, im2, h, u, v, maxnum=60, tol=1e-8, prec="f32"):
     return u, v, e, it.value
 
 
+def resize(src, H2, W2, filt, prec="f32"):
+    """Pillow resize, filt 2 = BILINEAR, 3 = BICUBIC (Farneback_PyCL.py:61-62,
+    GenericPyramidalOpticalFlow.py:66-67)."""
+    src = _arr(src, prec)
+    H, W = src.shape
+    dst = np.empty((H2, W2), _dt(prec))
+    getattr(lib(), "orc_resize_" + prec)(_p(src), H, W, _p(dst), H2, W2, int(filt))
+    return dst
+
+
 def resize_bicubic(src, H2, W2, prec="f32"):
     src = _arr(src, prec)
     H, W = src.shape
@@ -418,6 +429,50 @@ def farneback(im1, im2, U, V, consts, prec="f32"):
 
 
 # ----------------------------------------------------------------- dense LK
+def farneback_levels(H, W, pyramidal_levels, pyr_scale, min_size=32):
+    """Coarser levels kept by Farneback_PyCL.compute (Farneback_PyCL.py:475-482):
+    halve by pyr_scale while both sides stay >= min_size, at most
+    pyramidal_levels - 1 times."""
+    n, scale = 0, 1.0
+    while n < pyramidal_levels - 1:
+        scale *= pyr_scale
+        if W * scale < min_size or H * scale < min_size:
+            break
+        n += 1
+    return n
+
+
+def farneback_pyramid(im1, im2, U, V, consts, n_levels, pyr_scale, level_kernel, prec="f32"):
+    """Farneback compute() over n_levels coarser levels plus the full one
+    (Farneback_PyCL.py:507-596, fastPyramids False).  Level k, coarsest
+    first: scale = pyr_scale**k (a running product), sigma = (1/scale - 1)/2,
+    smoothSize = max(round(5 sigma) | 1, 3); both full-resolution frames are
+    blurred with level_kernel(smoothSize, sigma) (a half kernel, centre first)
+    and Pillow-BILINEAR resized to (round(H scale), round(W scale)); the
+    initial flow is the resized U, V times scale at the first level, else the
+    resized previous level times 1/pyr_scale (float32 products); then one
+    level runs on the resized frames with no further pre-blur."""
+    im1, im2 = _arr(im1, prec), _arr(im2, prec)
+    H, W = im1.shape
+    dt = _dt(prec)
+    flow, first = (_arr(U, prec), _arr(V, prec)), True
+    c = dict(consts, pre_kernel=np.ones(1))
+    for k in range(n_levels, -1, -1):
+        scale = 1.0
+        for _ in range(k):
+            scale *= pyr_scale
+        sigma = (1.0 / scale - 1.0) * 0.5
+        smooth = max(int(round(sigma * 5)) | 1, 3)
+        h, w = int(round(H * scale)), int(round(W * scale))
+        mult = dt(scale if first else 1.0 / pyr_scale)
+        fx, fy = (resize(f, h, w, 2, prec) * mult for f in flow)
+        pk = level_kernel(smooth, sigma)
+        f1 = resize(fb_blur(im1, pk, prec=prec), h, w, 2, prec)
+        f2 = resize(fb_blur(im2, pk, prec=prec), h, w, 2, prec)
+        flow, first = farneback(f1, f2, fx, fy, c, prec), False
+    return flow
+
+
 def lk_dense(I, J, u, v, half_win=13, niter=5, prec="f32"):
     """One dense-LK level.  Returns (u, v, status uint8)."""
     L = lib()

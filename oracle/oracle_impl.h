@@ -366,17 +366,19 @@ double NAME(orc_ls)(const REAL* im1, const REAL* im2, int H, int W, REAL h, int 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Pillow BICUBIC resize of a float image (GenericPyramidalOpticalFlow.py:  */
-/* 66-67).  Pillow's separable antialiased resampler: horizontal pass into  */
-/* a REAL intermediate covering the rows the vertical pass reads, then the  */
-/* vertical pass; coefficients and accumulation in double.                  */
+/* Pillow BICUBIC / BILINEAR resize of a float image (GenericPyramidal-    */
+/* OpticalFlow.py:66-67 and Farneback_PyCL.py:61-62).  Pillow's separable   */
+/* antialiased resampler: horizontal pass into a REAL intermediate covering */
+/* the rows the vertical pass reads, then the vertical pass; coefficients   */
+/* and accumulation in double.  An unchanged axis has unit coefficients, so */
+/* skipping its pass (as Pillow does) gives the same values.                */
 /* ------------------------------------------------------------------------ */
-void NAME(orc_resize_bicubic)(const REAL* src, int H, int W, REAL* dst, int H2, int W2) {
+void NAME(orc_resize)(const REAL* src, int H, int W, REAL* dst, int H2, int W2, int filter) {
     int ksh, ksv;
     int* bh = (int*)malloc(sizeof(int) * 2 * (size_t)W2);
     int* bv = (int*)malloc(sizeof(int) * 2 * (size_t)H2);
-    double* kh = orc_pil_coeffs(W, W2, ORC_FILTER_BICUBIC, &ksh, bh);
-    double* kv = orc_pil_coeffs(H, H2, ORC_FILTER_BICUBIC, &ksv, bv);
+    double* kh = orc_pil_coeffs(W, W2, filter, &ksh, bh);
+    double* kv = orc_pil_coeffs(H, H2, filter, &ksv, bv);
     int yfirst = bv[0];
     int ylast = bv[2 * (H2 - 1)] + bv[2 * (H2 - 1) + 1];
     int Ht = ylast - yfirst;
@@ -402,6 +404,10 @@ void NAME(orc_resize_bicubic)(const REAL* src, int H, int W, REAL* dst, int H2, 
         }
     }
     free(tmp); free(kh); free(kv); free(bh); free(bv);
+}
+
+void NAME(orc_resize_bicubic)(const REAL* src, int H, int W, REAL* dst, int H2, int W2) {
+    NAME(orc_resize)(src, H, W, dst, H2, W2, ORC_FILTER_BICUBIC);
 }
 
 /* ------------------------------------------------------------------------ */

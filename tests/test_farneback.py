@@ -79,6 +79,57 @@ def test_farneback_oracle_recovers_known_shift():
     assert np.abs(fx - fx64).max() < 1e-3 and np.abs(fy - fy64).max() < 1e-3
 
 
+def level_kernel(n, sigma):
+    """Half kernel of the level pre-blur: setGaussianBlurKernel(n, sigma)
+    (Farneback_PyCL.py:200-208) on the bit-exact Gaussian."""
+    from GaussianKernelBitExact import getGaussianKernelBitExact
+    _, k = getGaussianKernelBitExact(n, sigma)
+    return np.float32(k)[n // 2:]
+
+
+def test_bilinear_resize_matches_pillow(golden):
+    from PIL import Image
+    g = golden("pil")
+    for t, (H, W, H2, W2) in enumerate(g["cases"]):
+        assert np.array_equal(orc.resize(g["in%d" % t], int(H2), int(W2), 2), g["bilinear%d" % t])
+    rng = np.random.default_rng(5)
+    for (H, W, H2, W2) in [(97, 70, 49, 35), (192, 224, 48, 56), (40, 52, 80, 104), (33, 65, 33, 65)]:
+        img = (rng.random((H, W)) * 255).astype(np.float32)
+        want = np.array(Image.fromarray(img).resize((W2, H2), Image.BILINEAR))
+        assert np.array_equal(orc.resize(img, H2, W2, 2), want), (H, W, H2, W2)
+
+
+def test_farneback_level_count_rule():
+    # both sides must stay >= 32 after each pyr_scale step, at most levels-1 steps
+    assert orc.farneback_levels(192, 224, 3, 0.5) == 2
+    assert orc.farneback_levels(1024, 1024, 6, 0.5) == 5
+    assert orc.farneback_levels(1024, 1024, 9, 0.5) == 5
+    assert orc.farneback_levels(63, 100, 3, 0.5) == 0
+    assert orc.farneback_levels(64, 64, 3, 0.5) == 1
+    assert orc.farneback_levels(500, 700, 1, 0.5) == 0
+
+
+def test_farneback_pyramid_without_coarser_levels_is_one_level():
+    c = consts()
+    im1, im2 = _smooth_pair(64, 80, (0.9, 0.4), seed=4)
+    z = np.zeros_like(im1)
+    a = orc.farneback_pyramid(im1, im2, z, z, c, 0, 0.5, level_kernel)
+    b = orc.farneback(im1, im2, z, z, c)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_farneback_pyramid_recovers_a_large_shift():
+    # 14 px is far outside one level's reach; two halvings bring it to 3.5 px
+    c = consts()
+    im1, im2 = _smooth_pair(192, 224, (14.0, -9.0))
+    z = np.zeros_like(im1)
+    inner = (slice(24, -24), slice(24, -24))
+    fx, fy = orc.farneback_pyramid(im1, im2, z, z, c, 0, 0.5, level_kernel)
+    assert abs(np.median(fx[inner]) - 14.0) > 5
+    fx, fy = orc.farneback_pyramid(im1, im2, z, z, c, 2, 0.5, level_kernel)
+    assert abs(np.median(fx[inner]) - 14.0) < 0.05 and abs(np.median(fy[inner]) + 9.0) < 0.05
+
+
 # ------------------------------------------------------------------ GPU
 @pytest.fixture(scope="module")
 def ops():
@@ -122,3 +173,40 @@ def test_farneback_dropin_adapter(ops):
     assert fb.getGenericPyramidalDefaults() == {'warping': False, 'scaling': True}
     with pytest.raises(Exception):
         Farneback_PyCL(windowSize=32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H,W,n,sigma", [(192, 224, 3, 0.0), (192, 224, 39, 7.5), (97, 70, 19, 3.5),
+                                         (40, 36, 159, 31.5)])
+def test_fb_level_blur_matches_oracle(ops, H, W, n, sigma):
+    # the last case has a half kernel (79) wider than the image: reflect-101 indexes wrap
+    rng = np.random.default_rng(n)
+    img = (rng.random((H, W)) * 255).astype(np.float32)
+    k = level_kernel(n, sigma)
+    assert np.array_equal(ops.fb_blur(img, k).download(), orc.fb_blur(img, k))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H,W,levels,shift", [(192, 224, 3, (14.0, -9.0)), (150, 134, 4, (2.5, 1.5)),
+                                              (256, 256, 5, (6.0, -3.0))])
+def test_farneback_pyramid_dropin_matches_oracle(ops, H, W, levels, shift):
+    from Farneback_PyCL import Farneback_PyCL
+    im1, im2 = _smooth_pair(H, W, shift, seed=H + W)
+    rng = np.random.default_rng(W)
+    U0 = (rng.standard_normal((H, W)) * 0.2).astype(np.float32)
+    V0 = (rng.standard_normal((H, W)) * 0.2).astype(np.float32)
+    fb = Farneback_PyCL(pyramidalLevels=levels)
+    U, V, err = fb.compute(im1, im2, U0, V0)
+    n = orc.farneback_levels(H, W, levels, 0.5)
+    c = consts()
+    ox, oy = orc.farneback_pyramid(im1, im2, U0, V0, c, n, 0.5, level_kernel)
+    dx, dy = orc.farneback_pyramid(im1, im2, U0, V0, c, n, 0.5, level_kernel, prec="f64")
+    d32 = max(np.abs(U - ox).max(), np.abs(V - oy).max())
+    d64 = max(np.abs(U - dx).max(), np.abs(V - dy).max())
+    inner = (slice(24, -24), slice(24, -24))
+    print("[farneback pyramid %dx%d, %d coarser levels] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px, "
+          "median flow %.3f %.3f" % (H, W, n, d32, d64, np.median(U[inner]), np.median(V[inner])))
+    assert U.shape == (H, W) and err == 'Unknown'
+    assert d32 <= 1e-4
+    assert d64 <= 1e-3
+    assert abs(np.median(U[inner]) - shift[0]) < 0.05 and abs(np.median(V[inner]) - shift[1]) < 0.05
