@@ -157,12 +157,15 @@ typedef struct ofri_fb_params {
     int win_kh;
     int iters;                /* flow iterations */
     float flow_scale;         /* the initial flow is multiplied by this first (float32); 1 = as given */
+    int box_filter;           /* 1: box window of half size win_kh (useGaussian=False, replicate
+                                 borders; win_kernel unused); 0: Gaussian window win_kernel */
 } ofri_fb_params;
 
 /* One Farneback level (the reference's compute() at one internal pyramid
  * level): flow *= flow_scale, reflect-101 pre-blur, polynomial expansion of
- * both frames, then `iters` rounds of window blur, flow update and matrices
- * update.  d_flowx / d_flowy: initial flow in, result out.  With pre_kh = 0
+ * both frames, then `iters` rounds of window blur (Gaussian, or box for
+ * useGaussian=False: updateFlowBoxFilter, Farneback_PyCL.py:424-439 with
+ * the .cl boxFilter5), flow update and matrices update.  d_flowx / d_flowy: initial flow in, result out.  With pre_kh = 0
  * and pre_kernel = {1} the frames are taken as already pre-filtered (the
  * coarser levels, whose blur runs at full resolution before the resize).
  * Replaces the per-level body of Farneback_PyCL.compute

@@ -98,18 +98,20 @@ class Farneback_PyCL(object):
         self.setPolynomialExpansionConsts()
         self.setGaussianBlurKernel(3, 0.0)                 # smoothSize 3, sigma 0 at scale 1
         pre = self.matrixGKernel[0].copy()
-        self.setGaussianBlurKernel(self.windowSize, self.windowSize / 2 * 0.3)
-        win = self.matrixGKernel[0].copy()
-        return {"g": self.matrixG[0], "xg": self.matrixXG[0], "xxg": self.matrixXXG[0], "ig": self.matrixIG,
-                "pre_kernel": pre, "win_kernel": win, "iters": int(self.numIters)}
+        c = {"g": self.matrixG[0], "xg": self.matrixXG[0], "xxg": self.matrixXXG[0], "ig": self.matrixIG,
+             "pre_kernel": pre, "iters": int(self.numIters)}
+        if self.useGaussianFilter:
+            self.setGaussianBlurKernel(self.windowSize, self.windowSize / 2 * 0.3)
+            c["win_kernel"] = self.matrixGKernel[0].copy()
+        else:
+            c["box_half"] = self.windowSize // 2            # boxFilter5(M, blockSize/2) truncated to int
+        return c
 
     # ---- adapter protocol -------------------------------------------------
     def compute(self, im1, im2, U, V):
         assert self.polyN == 5 or self.polyN == 7
         assert np.shape(im1) == np.shape(im2) and self.pyrScale < 1
         assert np.shape(U) == np.shape(im1) and np.shape(V) == np.shape(im1)
-        if not self.useGaussianFilter:
-            raise NotImplementedError("box-filter window (useGaussian=False) is not implemented")
         size = np.shape(im1)
         min_size = 32
         levels, scale = 0, 1.0

@@ -3,7 +3,8 @@
 //
 //   pre-blur (reflect-101 separable Gaussian) -> polynomial expansion of both
 //   frames (5 coefficient planes each) -> matrices from the initial flow ->
-//   iters x [window blur of the 5 planes -> flow update -> matrices update].
+//   iters x [window blur (Gaussian or box) of the 5 planes -> flow update ->
+//   matrices update].
 //
 // The flow update and the matrices update touch the same pixel only, so they
 // run as one fused kernel (the blurred planes are read once, the flow and the
@@ -70,6 +71,44 @@ __global__ __launch_bounds__(256) void fb_blur_h(const float* __restrict__ tmp, 
     float res = r[0] * k.g[0];
     for (int i = 1; i <= k.kh; ++i) res = fmaf(r[-i] + r[i], k.g[i], res);
     dst[(size_t)p * H * W + (size_t)y * W + x] = res;
+}
+
+// Box window (updateFlowBoxFilter): replicate borders, running sums
+// v + (a + b) in float32 outward from the centre, vertical then horizontal,
+// times 1 / (2 kh + 1)^2.  The vertical sums of clamped columns equal those of
+// the edge columns, so the horizontal pass clamps its reads into the
+// vertical results.
+__global__ void fb_box_v(const float* __restrict__ src, float* __restrict__ tmp, int H, int W, int kh) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int p = blockIdx.z;
+    if (x >= W) return;
+    const float* s = src + (size_t)p * H * W;
+    float v = s[(size_t)y * W + x];
+    for (int j = 1; j <= kh; ++j) {
+        int ylo = y - j < 0 ? 0 : y - j, yhi = y + j > H - 1 ? H - 1 : y + j;
+        v = v + (s[(size_t)ylo * W + x] + s[(size_t)yhi * W + x]);
+    }
+    tmp[(size_t)p * H * W + (size_t)y * W + x] = v;
+}
+
+__global__ __launch_bounds__(256) void fb_box_h(const float* __restrict__ tmp, float* __restrict__ dst, int H,
+                                                int W, int kh, float inv) {
+    __shared__ float row[256 + 2 * kMaxHalf];
+    const int y = blockIdx.y, p = blockIdx.z;
+    const int x0 = blockIdx.x * 256;
+    const float* t = tmp + (size_t)p * H * W + (size_t)y * W;
+    for (int i = threadIdx.x; i < 256 + 2 * kh; i += 256) {
+        int xe = x0 + i - kh;
+        row[i] = t[xe < 0 ? 0 : (xe > W - 1 ? W - 1 : xe)];
+    }
+    __syncthreads();
+    int x = x0 + threadIdx.x;
+    if (x >= W) return;
+    const float* r = row + threadIdx.x + kh;
+    float res = r[0];
+    for (int i = 1; i <= kh; ++i) res = res + (r[-i] + r[i]);
+    dst[(size_t)p * H * W + (size_t)y * W + x] = res * inv;
 }
 
 // Polynomial expansion: vertical 3-channel pass into LDS (clamped rows and
@@ -218,6 +257,16 @@ int blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, int planes, 
     return OFRI_OK;
 }
 
+int box(ofri_ctx* ctx, const float* src, float* dst, int H, int W, int planes, int kh) {
+    void* tmp;
+    OFRI_TRY(scratch(ctx, "fb_blur_tmp", (size_t)planes * H * W * sizeof(float), &tmp));
+    const float inv = 1.0f / (float)((1 + 2 * kh) * (1 + 2 * kh));
+    fb_box_v<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>(src, (float*)tmp, H, W, kh);
+    fb_box_h<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>((const float*)tmp, dst, H, W, kh, inv);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
+}
+
 }  // namespace
 
 // The reflect-101 index maps reduce modulo the side, so any half size up to
@@ -250,7 +299,8 @@ int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, f
     pk.kh = p->pre_kh;
     wk.kh = p->win_kh;
     for (int i = 0; i <= pk.kh; ++i) pk.g[i] = p->pre_kernel[i];
-    for (int i = 0; i <= wk.kh; ++i) wk.g[i] = p->win_kernel[i];
+    if (!p->box_filter)
+        for (int i = 0; i <= wk.kh; ++i) wk.g[i] = p->win_kernel[i];
     PolyConsts c{};
     c.n = p->poly_n;
     for (int i = 0; i <= c.n; ++i) {
@@ -269,7 +319,10 @@ int farneback(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, f
     fb_matrices_k<<<g2, 256, 0, ctx->stream>>>(flx, fly, H, W, RA, RB, M);
     OFRI_HIP(hipGetLastError());
     for (int i = 0; i < p->iters; ++i) {
-        OFRI_TRY(blur(ctx, M, Mb, H, W, 5, wk));
+        if (p->box_filter)
+            OFRI_TRY(box(ctx, M, Mb, H, W, 5, p->win_kh));
+        else
+            OFRI_TRY(blur(ctx, M, Mb, H, W, 5, wk));
         fb_flow_k<<<g2, 256, 0, ctx->stream>>>(Mb, flx, fly, H, W, RA, RB, M, i < p->iters - 1);
         OFRI_HIP(hipGetLastError());
     }

@@ -373,7 +373,7 @@ int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H,
                    float* d_flowx, float* d_flowy, const ofri_fb_params* p) {
     CHECK_CTX(ctx);
     if (!d_im1 || !d_im2 || !d_flowx || !d_flowy || !p || !p->g || !p->xg || !p->xxg || !p->ig ||
-        !p->pre_kernel || !p->win_kernel)
+        !p->pre_kernel || (!p->win_kernel && !p->box_filter))
         return fail(OFRI_E_ARG, "ofri_farneback: null pointer");
     if (p->iters < 0) return fail(OFRI_E_ARG, "ofri_farneback: negative iteration count");
     return farneback(ctx, d_im1, d_im2, H, W, d_flowx, d_flowy, p);

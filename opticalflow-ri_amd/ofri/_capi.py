@@ -69,6 +69,7 @@ class FbParams(C.Structure):
         ("win_kh", C.c_int),
         ("iters", C.c_int),
         ("flow_scale", C.c_float),
+        ("box_filter", C.c_int),
     ]
 
 

@@ -203,17 +203,23 @@ def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
     """One Farneback level (Farneback_PyCL.py:515-596); flowx / flowy device
     arrays updated in place, multiplied by float32(flow_scale) first.
     consts: dict of float32 arrays g, xg, xxg (polyN+1 taps), ig (4),
-    pre_kernel, win_kernel (half kernels, centre first), and the int iters."""
+    pre_kernel, win_kernel (half kernels, centre first), and the int iters;
+    an int box_half instead of win_kernel selects the box window
+    (Farneback_PyCL.py:424-439)."""
     im1, im2 = as_device(im1), as_device(im2)
+    box = consts.get("box_half")
     arrs = {k: np.ascontiguousarray(consts[k], dtype=np.float32)
-            for k in ("g", "xg", "xxg", "ig", "pre_kernel", "win_kernel")}
+            for k in ("g", "xg", "xxg", "ig", "pre_kernel", "win_kernel") if k in consts}
     p = _capi.FbParams()
     p.poly_n = len(arrs["g"]) - 1
     p.g, p.xg, p.xxg, p.ig = (_capi.fptr(arrs[k]) for k in ("g", "xg", "xxg", "ig"))
     p.pre_kernel = _capi.fptr(arrs["pre_kernel"])
     p.pre_kh = len(arrs["pre_kernel"]) - 1
-    p.win_kernel = _capi.fptr(arrs["win_kernel"])
-    p.win_kh = len(arrs["win_kernel"]) - 1
+    if box is None:
+        p.win_kernel = _capi.fptr(arrs["win_kernel"])
+        p.win_kh = len(arrs["win_kernel"]) - 1
+    else:
+        p.box_filter, p.win_kh = 1, int(box)
     p.iters = int(consts["iters"])
     p.flow_scale = float(np.float32(flow_scale))
     _capi.check(_lib().ofri_farneback(_h(), im1.ptr, im2.ptr, im1.H, im1.W, flowx.ptr, flowy.ptr, C.byref(p)))

@@ -370,6 +370,7 @@ def _decl_fb(L):
         getattr(L, "orc_fb_update_flow_" + suf).argtypes = [P, C.c_int, C.c_int, P, P]
         getattr(L, "orc_farneback_" + suf).argtypes = [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int, P, P,
                                                        C.c_int, C.c_int, P, P]
+        getattr(L, "orc_fb_box_" + suf).argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int]
 
 
 def _fblib():
@@ -415,15 +416,28 @@ def fb_update_flow(M, prec="f32"):
     return fx, fy
 
 
+def fb_box(src, kh, planes=1, prec="f32"):
+    src = _arr(src, prec)
+    dst = np.empty_like(src)
+    getattr(_fblib(), "orc_fb_box_" + prec)(_p(src), _p(dst), src.shape[0] // planes, src.shape[1], planes, int(kh))
+    return dst
+
+
 def farneback(im1, im2, U, V, consts, prec="f32"):
     """One Farneback compute() at a single internal level.  consts: dict with
-    pre_kernel, g, xg, xxg, ig, win_kernel (half kernels) and iters."""
+    pre_kernel, g, xg, xxg, ig, win_kernel (half kernels) and iters; an int
+    box_half in place of win_kernel selects the box window."""
     im1, im2 = _arr(im1, prec), _arr(im2, prec)
     H, W = im1.shape
     fx, fy = _arr(U, prec).copy(), _arr(V, prec).copy()
-    pk, g, xg, xxg, ig, wk = (_arr(consts[k], prec) for k in ("pre_kernel", "g", "xg", "xxg", "ig", "win_kernel"))
+    pk, g, xg, xxg, ig = (_arr(consts[k], prec) for k in ("pre_kernel", "g", "xg", "xxg", "ig"))
+    if consts.get("box_half") is None:
+        wk = _arr(consts["win_kernel"], prec)
+        wkp, wkh = _p(wk), len(wk) - 1
+    else:
+        wkp, wkh = None, int(consts["box_half"])
     getattr(_fblib(), "orc_farneback_" + prec)(_p(im1), _p(im2), H, W, _p(pk), len(pk) - 1, _p(g), _p(xg), _p(xxg),
-                                                len(g) - 1, _p(ig), _p(wk), len(wk) - 1, int(consts["iters"]),
+                                                len(g) - 1, _p(ig), wkp, wkh, int(consts["iters"]),
                                                 _p(fx), _p(fy))
     return fx, fy
 

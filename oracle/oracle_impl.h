@@ -739,11 +739,42 @@ void NAME(orc_fb_update_flow)(const REAL* M, int H, int W, REAL* fxw, REAL* fyw)
     }
 }
 
+/* boxFilter5 over `planes` stacked planes (Farneback_PyCL.py:273-308 with  */
+/* .cl:350-406): replicate borders; vertical v = s[y], v += (s[y-j] +       */
+/* s[y+j]) for j = 1..kh; the same horizontally on the vertical results;    */
+/* times 1 / (2 kh + 1)^2 formed in REAL.                                  */
+void NAME(orc_fb_box)(const REAL* src, REAL* dst, int H, int W, int planes, int kh) {
+    REAL* t = (REAL*)malloc(sizeof(REAL) * (size_t)H * W);
+    const REAL inv = (REAL)1 / (REAL)((1 + 2 * kh) * (1 + 2 * kh));
+    for (int k = 0; k < planes; ++k) {
+        const REAL* s = src + (size_t)k * H * W;
+        REAL* d = dst + (size_t)k * H * W;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                REAL v = s[IDX(y, x)];
+                for (int j = 1; j <= kh; ++j)
+                    v = v + (s[IDX(NAME(clp)(y - j, H), x)] + s[IDX(NAME(clp)(y + j, H), x)]);
+                t[IDX(y, x)] = v;
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                REAL r = t[IDX(y, x)];
+                for (int i = 1; i <= kh; ++i)
+                    r = r + (t[IDX(y, NAME(clp)(x - i, W))] + t[IDX(y, NAME(clp)(x + i, W))]);
+                d[IDX(y, x)] = r * inv;
+            }
+    }
+    free(t);
+}
+
 /* One Farneback compute() with pyramidalLevels = 1 (Farneback_PyCL.py:461-603):
  * pre-blur both frames with pk (pkh+1 half taps), polynomial expansion,
  * updateMatrices from the initial flow, then `iters` x [window blur (wk,
- * wkh+1 half taps) -> updateFlow -> updateMatrices (all but the last)].
- * fxw / fyw: initial flow in, result out. */
+ * wkh+1 half taps; wk NULL: box window of half size wkh, the useGaussian =
+ * False path of Farneback_PyCL.py:424-439) -> updateFlow -> updateMatrices
+ * (all but the last)].  fxw / fyw: initial flow in, result out. */
 void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const REAL* pk, int pkh,
                          const REAL* g, const REAL* xg, const REAL* xxg, int n, const REAL* ig,
                          const REAL* wk, int wkh, int iters, REAL* fxw, REAL* fyw) {
@@ -756,7 +787,10 @@ void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const R
     NAME(orc_fb_polyexp)(bl, H, W, g, xg, xxg, n, ig, RB);
     NAME(orc_fb_update_matrices)(fxw, fyw, RA, RB, H, W, M);
     for (int i = 0; i < iters; ++i) {
-        NAME(orc_fb_blur)(M, Mb, H, W, 5, wk, wkh);
+        if (wk)
+            NAME(orc_fb_blur)(M, Mb, H, W, 5, wk, wkh);
+        else
+            NAME(orc_fb_box)(M, Mb, H, W, 5, wkh);
         REAL* t = M; M = Mb; Mb = t;
         NAME(orc_fb_update_flow)(M, H, W, fxw, fyw);
         if (i < iters - 1) NAME(orc_fb_update_matrices)(fxw, fyw, RA, RB, H, W, M);

@@ -7,9 +7,9 @@ from scipy.ndimage import correlate1d
 from oracle import oracle as orc
 
 
-def consts(windowSize=33, Niters=5, polyN=7, polySigma=1.5):
+def consts(windowSize=33, Niters=5, polyN=7, polySigma=1.5, useGaussian=True):
     from Farneback_PyCL import Farneback_PyCL
-    fb = Farneback_PyCL(windowSize, Niters, polyN, polySigma)
+    fb = Farneback_PyCL(windowSize, Niters, polyN, polySigma, useGaussian)
     return fb._consts()
 
 
@@ -130,6 +130,29 @@ def test_farneback_pyramid_recovers_a_large_shift():
     assert abs(np.median(fx[inner]) - 14.0) < 0.05 and abs(np.median(fy[inner]) + 9.0) < 0.05
 
 
+@pytest.mark.parametrize("kh", [1, 16, 40])
+def test_box_window_is_replicate_uniform_filter(kh):
+    from scipy.ndimage import uniform_filter
+    rng = np.random.default_rng(kh)
+    H, W = 37, 61
+    planes = rng.random((5 * H, W)) * 100
+    out = orc.fb_box(planes, kh, planes=5, prec="f64").reshape(5, H, W)
+    for k, p in enumerate(planes.reshape(5, H, W)):
+        np.testing.assert_allclose(out[k], uniform_filter(p, size=2 * kh + 1, mode="nearest"), rtol=1e-12)
+
+
+def test_farneback_box_window_recovers_known_shift():
+    c = consts(useGaussian=False)
+    assert c["box_half"] == 16 and "win_kernel" not in c
+    im1, im2 = _smooth_pair(96, 112, (1.3, -0.7))
+    z = np.zeros_like(im1)
+    fx, fy = orc.farneback(im1, im2, z, z, c)
+    inner = (slice(24, -24), slice(24, -24))
+    assert abs(np.median(fx[inner]) - 1.3) < 0.05 and abs(np.median(fy[inner]) + 0.7) < 0.05
+    fg, _ = orc.farneback(im1, im2, z, z, consts())
+    assert np.abs(fx - fg).max() > 1e-3                    # a different window, not the Gaussian path
+
+
 # ------------------------------------------------------------------ GPU
 @pytest.fixture(scope="module")
 def ops():
@@ -210,3 +233,24 @@ def test_farneback_pyramid_dropin_matches_oracle(ops, H, W, levels, shift):
     assert d32 <= 1e-4
     assert d64 <= 1e-3
     assert abs(np.median(U[inner]) - shift[0]) < 0.05 and abs(np.median(V[inner]) - shift[1]) < 0.05
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H,W,levels,win", [(96, 112, 1, 33), (192, 224, 3, 33), (97, 70, 2, 21)])
+def test_farneback_box_window_dropin_matches_oracle(ops, H, W, levels, win):
+    from Farneback_PyCL import Farneback_PyCL
+    shift = (1.3, -0.7) if levels == 1 else (6.0, 2.5)
+    im1, im2 = _smooth_pair(H, W, shift, seed=H)
+    z = np.zeros_like(im1)
+    fb = Farneback_PyCL(windowSize=win, useGaussian=False, pyramidalLevels=levels)
+    U, V, _ = fb.compute(im1, im2, z, z)
+    n = orc.farneback_levels(H, W, levels, 0.5)
+    c = consts(windowSize=win, useGaussian=False)
+    ox, oy = orc.farneback_pyramid(im1, im2, z, z, c, n, 0.5, level_kernel)
+    dx, dy = orc.farneback_pyramid(im1, im2, z, z, c, n, 0.5, level_kernel, prec="f64")
+    d32 = max(np.abs(U - ox).max(), np.abs(V - oy).max())
+    d64 = max(np.abs(U - dx).max(), np.abs(V - dy).max())
+    print("[farneback box %dx%d, window %d, %d coarser levels] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px"
+          % (H, W, win, n, d32, d64))
+    assert d32 <= 1e-4
+    assert d64 <= 1e-3
