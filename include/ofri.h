@@ -183,15 +183,6 @@ int ofri_fb_blur(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, 
                  int kh);
 
 /* ---- dense Lucas-Kanade --------------------------------------------------- */
-/* One level of dense pyramidal LK with a (2*half_win+1)^2 window (half_win
- * 1..16): Scharr structure tensor of I over the window, up to niter
- * Gauss-Newton steps with J sampled bilinearly at the displaced window,
- * step 32*A^-1*b, early exit when both step components are below 0.01 px.
- * d_u / d_v: initial flow in, result out (left unchanged where the tensor is
- * singular).  d_status (may be NULL): 1 = tracked, 0 = singular tensor or the
- * window left the image.  Replaces denseLucasKanade_PyCl.compute
- * (denseLucasKanade_PyCL.py:113-169) and pyrlkDenseLargeW.cl lkDense (:304-669)
- * with the symmetric window (vorticity enhancement off). */
 /* The Liu-Shen warping mode of updateNextPyramidalLevel
  * (GenericPyramidalOpticalFlow.py:188-220, biLinear=False): d_out = d_im1 with
  * every pixel moved by its flow (d_u column, d_v row component) rounded half
@@ -206,8 +197,24 @@ int ofri_fb_blur(ofri_ctx* ctx, const float* d_src, int H, int W, float* d_dst, 
 int ofri_warp_liushen(ofri_ctx* ctx, const float* d_im1, int H, int W, const float* d_u, const float* d_v,
                       const double* h_w, int radius, float* d_out, int* h_status);
 
+
+/* One level of dense pyramidal LK over the reference's window (half_win
+ * 1..16; the 2*half_win+1 columns laid on a 32-wide grid of four 8-column
+ * tiles, with the reference's tile rules: tile 0 always counts, so windows
+ * under 16 columns cover 9, and columns past 32 do not count): Scharr
+ * structure tensor of I over the window, up to niter Gauss-Newton steps
+ * with J sampled bilinearly at the displaced window, step 32*A^-1*b, early
+ * exit when both step components are below 0.01 px.  d_u / d_v: initial flow
+ * in, result out (left unchanged where the tensor is singular).  d_status
+ * (may be NULL): 1 = tracked, 0 = singular tensor or the window left the
+ * image.  h_asym (may be NULL = {0,0,0,0}): the asymmetric window flags
+ * {left, right, top, bottom}, each 0 or 1, of the vorticity enhancement
+ * (denseLucasKanade_PyCL.py:75-92): left/top drop window column/row 8 (the
+ * first of tile 1), right/bottom drop the last column/row.  Replaces
+ * denseLucasKanade_PyCl.compute (denseLucasKanade_PyCL.py:113-169) and
+ * pyrlkDenseLargeW.cl lkDense (:304-669). */
 int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
-                  float* d_v, int half_win, int niter, unsigned char* d_status);
+                  float* d_v, int half_win, int niter, unsigned char* d_status, const int* h_asym);
 
 /* ---- whole LS(HS) pyramid ------------------------------------------------ */
 typedef struct ofri_lshs_params {

@@ -10,6 +10,12 @@
 // the per-iteration right-hand side b are summed in double; J is sampled
 // bilinearly (the OpenCL LINEAR sampler at pixel centres, clamp-to-edge)
 // with the pixel's fractional weights formed once per iteration.
+//
+// The window is the reference's 32x32 grid of 4 tiles of 8 per axis with
+// 0/1 weights (pyrlkDenseLargeW.cl:319-371): it is given here as one bit
+// mask per axis (bit c: window column c, at offset c - hw from the pixel),
+// so the asymmetric (vorticity) windows and the reference's rounding of the
+// window to its tiles are the same code.
 #include "ofri_internal.h"
 
 #pragma clang fp contract(off)
@@ -19,34 +25,34 @@ namespace {
 
 constexpr int kTile = 16;
 constexpr int kMaxHalf = 16;
-constexpr int kReg = kTile + 2 * kMaxHalf;       // window region edge
+constexpr int kReg = kTile + 32;                 // window region edge (window columns 0..31)
 constexpr int kRegI = kReg + 2;                  // with the Scharr ring
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restrict__ I,
                                                             const float* __restrict__ J, int H, int W,
-                                                            int hw, int niter, float* __restrict__ u,
-                                                            float* __restrict__ v,
+                                                            int hw, unsigned mx, unsigned my, int niter,
+                                                            float* __restrict__ u, float* __restrict__ v,
                                                             unsigned char* __restrict__ status) {
     __shared__ float sI[kRegI * kRegI];
     __shared__ float sDx[kReg * kReg];
     __shared__ float sDy[kReg * kReg];
-    const int win = 2 * hw + 1;
-    const int reg = kTile + 2 * hw;       // region edge actually used
-    const int regI = reg + 2;
+    const int wx = 32 - __builtin_clz(mx), wy = 32 - __builtin_clz(my);   // extent: last set bit + 1
+    const int regx = kTile + wx - 1, regy = kTile + wy - 1;                // region actually used
+    const int regIx = regx + 2, regIy = regy + 2;
     const int bx0 = blockIdx.x * kTile, by0 = blockIdx.y * kTile;
     const int tid = threadIdx.y * kTile + threadIdx.x;
     // stage I over the region with its ring: image rows by0-hw-1 .. , clamped
-    for (int k = tid; k < regI * regI; k += kTile * kTile) {
-        int ry = k / regI, rx = k % regI;
+    for (int k = tid; k < regIx * regIy; k += kTile * kTile) {
+        int ry = k / regIx, rx = k % regIx;
         int y = clampi(by0 - hw - 1 + ry, 0, H - 1), x = clampi(bx0 - hw - 1 + rx, 0, W - 1);
         sI[ry * kRegI + rx] = I[(size_t)y * W + x];
     }
     __syncthreads();
     // Scharr derivatives at every region position from the staged (clamped) I
-    for (int k = tid; k < reg * reg; k += kTile * kTile) {
-        int ry = k / reg + 1, rx = k % reg + 1;
+    for (int k = tid; k < regx * regy; k += kTile * kTile) {
+        int ry = k / regx + 1, rx = k % regx + 1;
         const float* c = sI + ry * kRegI + rx;
         float dx = 3.0f * (c[-kRegI + 1] + c[kRegI + 1] - c[-kRegI - 1] - c[kRegI - 1]) +
                    10.0f * (c[1] - c[-1]);
@@ -64,10 +70,12 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
     const int o = threadIdx.y * kReg + threadIdx.x;
     const int oI = (threadIdx.y + 1) * kRegI + threadIdx.x + 1;
     double a11 = 0.0, a12 = 0.0, a22 = 0.0;
-    for (int r = 0; r < win; ++r) {
+    for (int r = 0; r < wy; ++r) {
+        if (!((my >> r) & 1u)) continue;
         const float* dxr = sDx + o + r * kReg;
         const float* dyr = sDy + o + r * kReg;
-        for (int c = 0; c < win; ++c) {
+        for (int c = 0; c < wx; ++c) {
+            if (!((mx >> c) & 1u)) continue;
             double dx = (double)dxr[c], dy = (double)dyr[c];
             a11 = a11 + dx * dx;
             a12 = a12 + dx * dy;
@@ -95,9 +103,10 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
         const int x0 = (int)fx, y0 = (int)fy;
         const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
         const float w01 = (1.0f - a) * b, w11 = a * b;
-        const bool inside = x0 >= 0 && y0 >= 0 && x0 + win < W && y0 + win < H;
+        const bool inside = x0 >= 0 && y0 >= 0 && x0 + wx < W && y0 + wy < H;
         double b1 = 0.0, b2 = 0.0;
-        for (int r = 0; r < win; ++r) {
+        for (int r = 0; r < wy; ++r) {
+            if (!((my >> r) & 1u)) continue;
             const float* pr = sI + oI + r * kRegI;
             const float* dxr = sDx + o + r * kReg;
             const float* dyr = sDy + o + r * kReg;
@@ -108,7 +117,8 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
             }
             const float* Ja = J + (size_t)ya * W;
             const float* Jb = J + (size_t)yb * W;
-            for (int c = 0; c < win; ++c) {
+            for (int c = 0; c < wx; ++c) {
+                if (!((mx >> c) & 1u)) continue;
                 int xa = x0 + c, xb = xa + 1;
                 if (!inside) {
                     xa = clampi(xa, 0, W - 1);
@@ -137,13 +147,39 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
 
 }  // namespace
 
+// Window mask of one axis (pyrlkDenseLargeW.cl:338-371): win = 2 hw + 1
+// columns laid on 4 tiles of 8; tile 0 always counts, the first column of
+// tile 1 counts unless `lo` (the asymmetric left/top flag), the rest of
+// tiles 1-3 count below win - `hi` (right/bottom flag).  A window under 16
+// columns uses tiles 0-1 only, and there the first column of tile 1 follows
+// the `lo` flag alone.
+unsigned lk_window_mask(int hw, int lo, int hi) {
+    const int win = 2 * hw + 1;
+    unsigned m = 0xFFu;
+    for (int c = 8; c < 32; ++c) {
+        bool on;
+        if (win >= 16)
+            on = c < 16 ? (c == 8 ? lo == 0 : true) : c < win - hi;
+        else
+            on = c < 16 ? (c == 8 ? lo == 0 : c < win - hi) : false;
+        if (on) m |= 1u << c;
+    }
+    return m;
+}
+
 int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw, int niter, float* u,
-             float* v, unsigned char* status) {
+             float* v, unsigned char* status, const int* asym) {
     if (hw < 1 || hw > kMaxHalf) return fail(OFRI_E_ARG, "lk: half window %d outside 1..%d", hw, kMaxHalf);
+    int L = 0, R = 0, T = 0, B = 0;
+    if (asym) {
+        L = asym[0], R = asym[1], T = asym[2], B = asym[3];
+        if ((L | R | T | B) & ~1) return fail(OFRI_E_ARG, "lk: asymmetric window flags must be 0 or 1");
+    }
+    const unsigned mx = lk_window_mask(hw, L, R), my = lk_window_mask(hw, T, B);
     if (H < 1 || W < 1) return fail(OFRI_E_ARG, "lk: empty image");
     if (niter < 0) return fail(OFRI_E_ARG, "lk: negative iteration count");
     dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
-    lk_dense_k<<<grid, dim3(kTile, kTile), 0, ctx->stream>>>(I, J, H, W, hw, niter, u, v, status);
+    lk_dense_k<<<grid, dim3(kTile, kTile), 0, ctx->stream>>>(I, J, H, W, hw, mx, my, niter, u, v, status);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }

@@ -363,10 +363,10 @@ int ofri_ls_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H
 }
 
 int ofri_lk_dense(ofri_ctx* ctx, const float* d_I, const float* d_J, int H, int W, float* d_u,
-                  float* d_v, int half_win, int niter, unsigned char* d_status) {
+                  float* d_v, int half_win, int niter, unsigned char* d_status, const int* h_asym) {
     CHECK_CTX(ctx);
     if (!d_I || !d_J || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_lk_dense: null pointer");
-    return lk_dense(ctx, d_I, d_J, H, W, half_win, niter, d_u, d_v, d_status);
+    return lk_dense(ctx, d_I, d_J, H, W, half_win, niter, d_u, d_v, d_status, h_asym);
 }
 
 int ofri_farneback(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,

@@ -193,7 +193,8 @@ int fb_blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, const flo
 
 // ---- dense Lucas-Kanade (k_lk.hip) ----
 int lk_dense(ofri_ctx* ctx, const float* I, const float* J, int H, int W, int hw, int niter, float* u,
-             float* v, unsigned char* status);
+             float* v, unsigned char* status, const int* asym = nullptr);
+unsigned lk_window_mask(int hw, int lo, int hi);
 
 // ---- Liu-Shen (k_ls.hip) ----
 // check_last: the stop test runs once per fused launch, on its last sweep

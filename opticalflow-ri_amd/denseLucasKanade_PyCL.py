@@ -4,9 +4,12 @@ Lucas-Kanade on the GPU (libofri ofri_lk_dense) instead of PyOpenCL.
 Reference: denseLucasKanade_PyCL.py:33-182 and pyrlkDenseLargeW.cl.  The
 constructor keeps the reference's signature (platformID / deviceID accepted
 and ignored).  compute() returns (U, V, True) like the reference; the
-per-pixel tracking status is kept in self.status (1 tracked, 0 lost).
+per-pixel tracking status is kept in self.status (1 tracked, 0 lost).  With
+enableVorticityEnhancement the window flags come from the mean vorticity of
+the incoming flow, computed on the host as the reference computes it.
 """
 import numpy as np
+from scipy.ndimage import convolve as filter2
 
 from ofri import ops
 from ofri.device import DeviceArray
@@ -28,11 +31,21 @@ class denseLucasKanade_PyCl(object):
         ops.log('Runnning on device: libofri (gfx950)')
 
     def evaluateVorticityEnhancement(self, U, V):
-        """Asymmetric window selection (denseLucasKanade_PyCL.py:75-92); only
-        the symmetric window is implemented."""
+        """Asymmetric window flags [left, right, top, bottom] from the mean of
+        omega = convolve(V, D.T) - convolve(U, D) with the half central
+        difference D, 'reflect' borders, beyond +-2e-3
+        (denseLucasKanade_PyCL.py:75-92).  Convolution flips D, so omega is
+        minus the vorticity dV/dx - dU/dy."""
         if not self.enableVorticityEnhancement:
             return [0, 0, 0, 0]
-        raise NotImplementedError("vorticity-driven asymmetric windows are not implemented")
+        D = np.array([[0, -1, 0], [0, 0, 0], [0, 1, 0]], dtype=np.float32) * np.float32(0.5)
+        omega = filter2(V, D.T, mode='reflect') - filter2(U, D, mode='reflect')
+        m = np.mean(omega)
+        if m < -2e-3:
+            return [0, 1, 0, 1]
+        if m > 2e-3:
+            return [1, 0, 0, 1]
+        return [0, 0, 0, 0]
 
     def calcPatchSize(self, windowWidth, windowHeight):
         """The reference's patch-tiling helper (denseLucasKanade_PyCL.py:94-111)."""
@@ -43,13 +56,13 @@ class denseLucasKanade_PyCl(object):
         return np.int32((windowWidth + blockX - 1) / blockX), np.int32((windowHeight + blockY - 1) / blockY)
 
     def compute(self, im1, im2, U, V):
-        self.evaluateVorticityEnhancement(U, V)
+        asym = self.evaluateVorticityEnhancement(U, V)
         I = np.ascontiguousarray(im1, dtype=np.float32)
         J = np.ascontiguousarray(im2, dtype=np.float32)
         u = DeviceArray.from_host(np.ascontiguousarray(U, dtype=np.float32))
         v = DeviceArray.from_host(np.ascontiguousarray(V, dtype=np.float32))
         st = ops.ByteArray(I.shape)
-        ops.lk_dense(I, J, u, v, self.windowHalfWidth, self.Niter, st)
+        ops.lk_dense(I, J, u, v, self.windowHalfWidth, self.Niter, st, asym)
         self.status = st.download()
         return u.download(), v.download(), True
 

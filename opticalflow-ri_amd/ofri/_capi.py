@@ -110,7 +110,7 @@ SIGNATURES = {
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_warp_liushen": [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp, _ip],
     "ofri_ls_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
-    "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp],
+    "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)],
     "ofri_farneback": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(FbParams)],
     "ofri_fb_blur": [_vp, _vp, C.c_int, C.c_int, _vp, _fp, C.c_int],
     "ofri_batch_lshs": [_vp, C.POINTER(Pair), C.c_int, C.POINTER(LshsParams), C.POINTER(LevelStats)],

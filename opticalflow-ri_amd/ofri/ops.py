@@ -226,12 +226,14 @@ def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
     return flowx, flowy
 
 
-def lk_dense(I, J, u, v, half_win=13, niter=5, status=None):
+def lk_dense(I, J, u, v, half_win=13, niter=5, status=None, asym=None):
     """One dense-LK level (denseLucasKanade_PyCL.py:113-169); u, v device
-    arrays updated in place; status: optional device buffer of H*W bytes."""
+    arrays updated in place; status: optional device buffer of H*W bytes;
+    asym: optional [left, right, top, bottom] window flags (0/1)."""
     I, J = as_device(I), as_device(J)
+    a = (C.c_int * 4)(*[int(x) for x in asym]) if asym is not None else None
     _capi.check(_lib().ofri_lk_dense(_h(), I.ptr, J.ptr, I.H, I.W, u.ptr, v.ptr, int(half_win), int(niter),
-                                     status.ptr if status is not None else None))
+                                     status.ptr if status is not None else None, a))
     return u, v
 
 

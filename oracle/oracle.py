@@ -487,15 +487,43 @@ def farneback_pyramid(im1, im2, U, V, consts, n_levels, pyr_scale, level_kernel,
     return flow
 
 
-def lk_dense(I, J, u, v, half_win=13, niter=5, prec="f32"):
-    """One dense-LK level.  Returns (u, v, status uint8)."""
+def lk_window_weights(hw, lo=0, hi=0):
+    """0/1 weight of each of the 32 window-grid columns (or rows), taken from
+    the per-work-item weights of pyrlkDenseLargeW.cl:338-371: work-item xid
+    holds columns xid + 8 * tile with tile weights (1, wx0, wx1, wx2); the
+    program is built with WSX = 1 when the window has 16 columns or more.
+    lo / hi: the asymmetric left (top) and right (bottom) flags."""
+    win = 2 * hw + 1
+    w = np.zeros(32, np.uint8)
+    for xid in range(8):
+        if win >= 16:
+            wx0 = 1
+            wx1 = 1 if 2 * 8 + xid < win - hi else 0
+            wx2 = 1 if 3 * 8 + xid < win - hi else 0
+        else:
+            wx0 = 1 if 1 * 8 + xid < win - hi else 0
+            wx1 = wx2 = 0
+        if xid == 0:
+            wx0 = 1 - lo
+        for tile, wt in enumerate((1, wx0, wx1, wx2)):
+            w[tile * 8 + xid] = wt
+    return w
+
+
+def lk_dense(I, J, u, v, half_win=13, niter=5, prec="f32", asym=(0, 0, 0, 0)):
+    """One dense-LK level.  asym: the (left, right, top, bottom) window flags.
+    Returns (u, v, status uint8)."""
     L = lib()
     f = getattr(L, "orc_lk_dense_" + prec)
     P = _fp if prec == "f32" else _dp
-    f.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.POINTER(C.c_ubyte)]
+    B = C.POINTER(C.c_ubyte)
+    f.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, B, B, B]
     I, J = _arr(I, prec), _arr(J, prec)
     H, W = I.shape
     u, v = _arr(u, prec).copy(), _arr(v, prec).copy()
     st = np.zeros((H, W), np.uint8)
-    f(_p(I), _p(J), H, W, int(half_win), int(niter), _p(u), _p(v), st.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    wx = lk_window_weights(half_win, asym[0], asym[1])
+    wy = lk_window_weights(half_win, asym[2], asym[3])
+    f(_p(I), _p(J), H, W, int(half_win), int(niter), _p(u), _p(v), st.ctypes.data_as(B), wx.ctypes.data_as(B),
+      wy.ctypes.data_as(B))
     return u, v, st

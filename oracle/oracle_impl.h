@@ -800,7 +800,9 @@ void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const R
 
 /* ------------------------------------------------------------------------ */
 /* Dense pyramidal-LK, one level (denseLucasKanade_PyCL.py:113-169 with     */
-/* pyrlkDenseLargeW.cl:304-669), symmetric window of 2*hw+1 pixels.         */
+/* pyrlkDenseLargeW.cl:304-669).  The window is the kernel's 32 x 32 grid  */
+/* (4 tiles of 8 work-items per axis) with 0/1 weights wym[r] * wxm[c]; a   */
+/* zero weight removes the sample from A and b alike (.cl:247-248,260).     */
 /* For output pixel (i, j): window pixels (i-hw+r, j-hw+c); the patch I and */
 /* its Scharr derivatives come from clamp-to-edge reads (the LINEAR sampler */
 /* at pixel centres), Dx = 3(I[-1,+1]+I[+1,+1]-I[-1,-1]-I[+1,-1]) +         */
@@ -811,8 +813,8 @@ void NAME(orc_farneback)(const REAL* im1, const REAL* im2, int H, int W, const R
 /* when |delta| < 0.01 in both components.  Window sums in double.          */
 /* ------------------------------------------------------------------------ */
 void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int niter, REAL* u, REAL* v,
-                        unsigned char* status) {
-    const int win = 2 * hw + 1;
+                        unsigned char* status, const unsigned char* wxm, const unsigned char* wym) {
+    const int win = 32;   /* the window grid; wxm / wym: 0/1 weight of each grid column / row */
 #pragma omp parallel
     {
         REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)win * win * 3);
@@ -824,6 +826,7 @@ void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int 
                 double a11 = 0, a12 = 0, a22 = 0;
                 for (int r = 0; r < win; ++r)
                     for (int c = 0; c < win; ++c) {
+                        if (!wxm[c] || !wym[r]) continue;
                         int y = i - hw + r, x = j - hw + c;
                         int ym = NAME(clp)(y - 1, H), y0 = NAME(clp)(y, H), yp = NAME(clp)(y + 1, H);
                         int xm = NAME(clp)(x - 1, W), x0 = NAME(clp)(x, W), xp = NAME(clp)(x + 1, W);
@@ -861,8 +864,10 @@ void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int 
                     REAL w00 = (1 - fa) * (1 - fb), w10 = fa * (1 - fb), w01 = (1 - fa) * fb, w11 = fa * fb;
                     double b1 = 0, b2 = 0;
                     for (int r = 0; r < win; ++r) {
+                        if (!wym[r]) continue;
                         int ya = NAME(clp)(y0 + r, H), yb = NAME(clp)(y0 + r + 1, H);
                         for (int c = 0; c < win; ++c) {
+                            if (!wxm[c]) continue;
                             size_t q = (size_t)r * win + c;
                             int xa = NAME(clp)(x0 + c, W), xb = NAME(clp)(x0 + c + 1, W);
                             REAL t = w00 * J[IDX(ya, xa)];

@@ -42,11 +42,14 @@ def _sampler(T, x, y):
     return np.float32(t)
 
 
-def _lk_literal(I, J, hw, niter):
+def _lk_literal(I, J, hw, niter, wx, wy):
     """The lkDense kernel restated per output pixel with the sampler read at
-    loc = prevPt + (c + .5, r + .5) (pyrlkDenseLargeW.cl:231,272-301,505-615)."""
+    loc = prevPt + (c + .5, r + .5) (pyrlkDenseLargeW.cl:231,272-301,505-615);
+    wx / wy: 0/1 weights of the window grid's columns / rows, applied to the
+    derivatives and to the differences as the kernel applies them."""
     H, W = I.shape
-    win = 2 * hw + 1
+    win = int(max(np.nonzero(wx)[0].max(), np.nonzero(wy)[0].max())) + 1
+    wgt = np.outer(wy[:win], wx[:win]).astype(np.float32)
     U = np.zeros((H, W), np.float32)
     V = np.zeros((H, W), np.float32)
     for i in range(H):
@@ -59,6 +62,7 @@ def _lk_literal(I, J, hw, niter):
                     P[r, c] = _sampler(I, Px + c - 1 + 0.5, Py + r - 1 + 0.5)
             Dx = 3 * (P[:-2, 2:] + P[2:, 2:] - P[:-2, :-2] - P[2:, :-2]) + 10 * (P[1:-1, 2:] - P[1:-1, :-2])
             Dy = 3 * (P[2:, :-2] + P[2:, 2:] - P[:-2, :-2] - P[:-2, 2:]) + 10 * (P[2:, 1:-1] - P[:-2, 1:-1])
+            Dx, Dy = Dx * wgt, Dy * wgt
             A11 = np.float32((Dx.astype(np.float64) ** 2).sum())
             A12 = np.float32((Dx.astype(np.float64) * Dy).sum())
             A22 = np.float32((Dy.astype(np.float64) ** 2).sum())
@@ -73,7 +77,7 @@ def _lk_literal(I, J, hw, niter):
                 b1 = b2 = 0.0
                 for r in range(win):
                     for c in range(win):
-                        d = _sampler(J, px + c + 0.5, py + r + 0.5) - P[r + 1, c + 1]
+                        d = (_sampler(J, px + c + 0.5, py + r + 0.5) - P[r + 1, c + 1]) * wgt[r, c]
                         b1 += float(d) * float(Dx[r, c])
                         b2 += float(d) * float(Dy[r, c])
                 B1, B2 = np.float32(b1), np.float32(b2)
@@ -87,10 +91,37 @@ def _lk_literal(I, J, hw, niter):
     return U, V
 
 
-def test_oracle_matches_sampler_formulation():
+def _cols(w):
+    return [int(c) for c in np.nonzero(w)[0]]
+
+
+def test_window_grid_weights():
+    """Window columns counted by lkDense (pyrlkDenseLargeW.cl:338-371, 4 tiles
+    of 8 per axis, WSX = window >= 16): a window of 16 or more columns counts
+    exactly its 2hw+1 columns up to the grid's 32; a smaller one counts tile 0
+    whole plus column 8; the left flag drops column 8, the right flag the
+    last counted column past tile 0."""
+    w = orc.lk_window_weights
+    assert _cols(w(13)) == list(range(27))
+    assert _cols(w(8)) == list(range(17))
+    assert _cols(w(16)) == list(range(32))                 # 33 columns asked, the grid holds 32
+    assert _cols(w(2)) == list(range(9))
+    assert _cols(w(5)) == list(range(11))
+    assert _cols(w(13, 1, 0)) == [c for c in range(27) if c != 8]
+    assert _cols(w(13, 0, 1)) == list(range(26))
+    assert _cols(w(13, 1, 1)) == [c for c in range(26) if c != 8]
+    assert _cols(w(5, 0, 1)) == list(range(10))
+    assert _cols(w(2, 0, 1)) == list(range(9))             # column 8 follows the left flag alone
+    assert _cols(w(2, 1, 0)) == list(range(8))
+
+
+@pytest.mark.parametrize("hw,asym", [(2, (0, 0, 0, 0)), (3, (1, 0, 0, 1)), (4, (0, 1, 1, 0))])
+def test_oracle_matches_sampler_formulation(hw, asym):
     I, J = _pair(10, 11, (0.6, -0.3), seed=5)
-    u, v, st = orc.lk_dense(I, J, np.zeros_like(I), np.zeros_like(I), half_win=2, niter=3)
-    U, V = _lk_literal(I, J, 2, 3)
+    u, v, st = orc.lk_dense(I, J, np.zeros_like(I), np.zeros_like(I), half_win=hw, niter=3, asym=asym)
+    wx = orc.lk_window_weights(hw, asym[0], asym[1])
+    wy = orc.lk_window_weights(hw, asym[2], asym[3])
+    U, V = _lk_literal(I, J, hw, 3, wx, wy)
     assert np.abs(u - U).max() < 2e-5 and np.abs(v - V).max() < 2e-5
 
 
@@ -124,8 +155,11 @@ def ops():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("H,W,hw", [(64, 80, 13), (131, 97, 13), (40, 50, 5), (33, 29, 16)])
-def test_lk_gpu_matches_oracle(ops, H, W, hw):
+@pytest.mark.parametrize("H,W,hw,asym", [(64, 80, 13, (0, 0, 0, 0)), (131, 97, 13, (0, 0, 0, 0)),
+                                         (40, 50, 5, (0, 0, 0, 0)), (33, 29, 16, (0, 0, 0, 0)),
+                                         (64, 80, 13, (1, 0, 0, 1)), (64, 80, 13, (0, 1, 0, 1)),
+                                         (40, 50, 3, (1, 0, 1, 0)), (37, 45, 16, (0, 1, 1, 1))])
+def test_lk_gpu_matches_oracle(ops, H, W, hw, asym):
     from ofri.device import DeviceArray
     I, J = _pair(H, W, (1.1, 0.5), seed=H + W)
     rng = np.random.default_rng(hw)
@@ -133,13 +167,13 @@ def test_lk_gpu_matches_oracle(ops, H, W, hw):
     v0 = (rng.standard_normal((H, W)) * 0.3).astype(np.float32)
     u, v = DeviceArray.from_host(u0), DeviceArray.from_host(v0)
     st = ops.ByteArray((H, W))
-    ops.lk_dense(I, J, u, v, hw, 5, st)
+    ops.lk_dense(I, J, u, v, hw, 5, st, asym)
     gu, gv, gs = u.download(), v.download(), st.download()
-    ou, ov, os_ = orc.lk_dense(I, J, u0, v0, hw, 5)
-    du, dv, _ = orc.lk_dense(I, J, u0, v0, hw, 5, prec="f64")
+    ou, ov, os_ = orc.lk_dense(I, J, u0, v0, hw, 5, asym=asym)
+    du, dv, _ = orc.lk_dense(I, J, u0, v0, hw, 5, prec="f64", asym=asym)
     d32 = max(np.abs(gu - ou).max(), np.abs(gv - ov).max())
     d64 = max(np.abs(gu - du).max(), np.abs(gv - dv).max())
-    print("[lk %dx%d hw%d] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px" % (H, W, hw, d32, d64))
+    print("[lk %dx%d hw%d asym %s] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px" % (H, W, hw, asym, d32, d64))
     assert d32 <= 1e-4                 # parity bar: against the reference dataflow
     assert d64 <= 1e-2                 # FP32-vs-FP64 error: reported, sanity-bounded only
     assert np.array_equal(gs, os_)
@@ -157,3 +191,39 @@ def test_lk_dropin_adapter(ops):
     assert np.abs(U - ou).max() <= 1e-4 and np.abs(V - ov).max() <= 1e-4
     assert np.array_equal(lk.status, os_)
     assert lk.getGenericPyramidalDefaults() == {'warping': False, 'intermediateScaling': True, 'scaling': False}
+
+
+def _rotation_flow(H, W, omega):
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    cy, cx = (H - 1) / 2, (W - 1) / 2
+    # solid rotation u = -w/2 (y - cy), v = w/2 (x - cx): vorticity dv/dx - du/dy = w
+    return (np.float32(-omega / 2) * (yy - cy)).astype(np.float32), (np.float32(omega / 2) * (xx - cx)).astype(np.float32)
+
+
+def test_vorticity_window_selection():
+    """scipy's convolve flips the difference kernel, so the reference's omega
+    is minus the vorticity: a counter-clockwise (positive) rotation selects
+    the right/bottom window, a clockwise one the left/bottom window."""
+    from denseLucasKanade_PyCL import denseLucasKanade_PyCl
+    lk = denseLucasKanade_PyCl(enableVorticityEnhancement=True)
+    assert lk.evaluateVorticityEnhancement(*_rotation_flow(40, 50, 0.01)) == [0, 1, 0, 1]
+    assert lk.evaluateVorticityEnhancement(*_rotation_flow(40, 50, -0.01)) == [1, 0, 0, 1]
+    assert lk.evaluateVorticityEnhancement(*_rotation_flow(40, 50, 0.001)) == [0, 0, 0, 0]
+    assert denseLucasKanade_PyCl().evaluateVorticityEnhancement(*_rotation_flow(40, 50, 0.01)) == [0, 0, 0, 0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("omega", [0.02, -0.02])
+def test_lk_dropin_vorticity_enhancement(ops, omega):
+    from denseLucasKanade_PyCL import denseLucasKanade_PyCl
+    I, J = _pair(56, 72, (0.7, 0.2), seed=11)
+    U0, V0 = _rotation_flow(56, 72, omega)
+    lk = denseLucasKanade_PyCl(Niter=5, halfWindow=13, enableVorticityEnhancement=True)
+    asym = lk.evaluateVorticityEnhancement(U0, V0)
+    assert asym == ([0, 1, 0, 1] if omega > 0 else [1, 0, 0, 1])
+    U, V, flag = lk.compute(I, J, U0, V0)
+    ou, ov, os_ = orc.lk_dense(I, J, U0, V0, 13, 5, asym=asym)
+    assert np.abs(U - ou).max() <= 1e-4 and np.abs(V - ov).max() <= 1e-4
+    assert np.array_equal(lk.status, os_)
+    su, sv, _ = orc.lk_dense(I, J, U0, V0, 13, 5)
+    assert np.abs(U - su).max() > 1e-4                     # the asymmetric window changed the result
