@@ -137,8 +137,8 @@ struct SplinePlanes {
     float scale[2];
 };
 
-constexpr int kSplineRowsX = 4;    // coarse rows per workgroup of the x pass
-constexpr int kSplineRowsY = 8;    // output rows per thread of the y pass
+constexpr int kSplineRowsX = 2;    // coarse rows per workgroup of the x pass
+constexpr int kSplineRowsY = 4;    // output rows per thread of the y pass
 constexpr int kSplineChunk = 22;   // weights in flight per thread of the x pass
 constexpr int kSplineMaxWidth = 66;  // band width of the weight tables (2 * 32 + 2)
 
