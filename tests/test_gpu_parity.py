@@ -664,8 +664,9 @@ def test_pyramid_lshs_matches_oracle(ops, size, levels):
 
 
 def test_dropin_driver_on_reference_piv_pair(ops, golden):
-    """The north-star harness (examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py)
-    on the reference's own TIFF pair, through the drop-in API."""
+    """The reference's two-level LS(HS) example script
+    (examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py) on the reference's own TIFF
+    pair, through the drop-in API."""
     from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
     from HornSchunck import HSOpticalFlowAlgoAdapter
     from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
