@@ -1,27 +1,31 @@
-"""Benchmark: LS(HS) 2-level dense flow at 1024x1024 on one MI355X, solved to
-the plain method's residual -- learned form against plain form.
+"""Benchmark: LS(HS) 2-level dense flow at 1024x1024 on one MI355X -- the
+learned form against the plain form.
 
 The workload (default): LS(HS) 2-level on 1024^2 synthetic PIV pairs (the
 workload's Lamb-Oseen pair, density 0.04, plus held-out rendered pairs of
 the three flow families): bicubic 1/2 downsample, FILTER 3.4 (3 taps) and HS
 (alpha 45 at 512^2, 21 at 1024^2), FILTER_OPT 0.48 (5 taps) and Liu-Shen
 alpha 5, spline upsample x2, symmetric bilinear warp, accumulation.  The
-plain form runs the reference's sweep counts (HS 600, Liu-Shen to 1e-8 or 60
-sweeps); the learned form (accelerated HS, correction network, Liu-Shen)
-runs every level to the plain form's residual at its stopping point, with
-the plain form as fallback.  One step = one pair; inputs are resident in HBM
+plain form runs the reference's stopping rules (HS 600 sweeps, Liu-Shen to
+1e-8 or 60 sweeps).  The learned form runs pair-independent rules (HS to a
+relative residual of 2^-22, the correction network, a fixed number of
+Liu-Shen sweeps with a relative-residual acceptance test), falls back to
+the plain form when a rule fails, and lets the plain form judge every 16th
+pair (EPE to plain <= 0.01 px, else the plain result).  One step = one
+batch of 16 pairs, judge and fallbacks included; inputs are resident in HBM
 before timing.
 
 Prints ONE JSON line.  `value` is the learned form's Mpix-pairs/s
-(finest-level H*W/1e6 per pair per second, fallbacks included);
-`learned_vs_plain` holds the plain form's rate, the method without the
-network, EPE to the plain result and the fallback rate.  `roofline` is
-measured live (HIP events on the library stream) for the learned form's
-dominant kernel, the Chebyshev HS sweep; `roofline_plain_hs` for the plain
-form's HS sweep; `roofline_net` for the network's forward pass against the
-bf16 MFMA peak.  `cpu_baseline` times the CPU oracle (the reference dataflow
-restated in C) on this host.  Other workloads: --workload plain | batch |
-fbls | lk.
+(finest-level H*W/1e6 per pair per second); `learned_vs_plain` holds the
+plain form's rate, the method without the network, the EPE to the plain
+result per pair and per flow family, the judge's record and the fallback
+rate.  `roofline` is measured live (HIP events on the library stream) for
+the learned form's dominant kernel, the Chebyshev HS sweep;
+`roofline_plain_hs` for the plain form's HS sweep; `roofline_net` for the
+network's forward pass against the bf16 MFMA peak.  `cpu_baseline` times the
+CPU oracle (the reference dataflow restated in C) on this host at one
+thread, at all threads and in float64.  Other workloads: --workload plain |
+batch | fbls | lk | net.
 """
 import argparse
 import json
@@ -61,7 +65,7 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel eagerly instead of replaying HIP graphs (for tools that trace "
                          "kernels; the kernels are the same)")
-    ap.add_argument("--eval-pairs", type=int, default=13,
+    ap.add_argument("--eval-pairs", type=int, default=16,
                     help="pairs the default workload cycles over (its own pair + held-out rendered pairs)")
     return ap.parse_args()
 
@@ -76,23 +80,58 @@ def traffic_from_profile(name="hs_pass_1024_traffic.json"):
         return None
 
 
-def cpu_baseline(im1, im2, fw, ow, min_s=10.0):
-    """The C oracle's LS(HS) 2-level on the same pair, repeated until at least
-    min_s seconds of CPU work have run."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(im1, im2, fw, ow, min_s=6.0):
+    """The C oracle's LS(HS) 2-level (the reference dataflow: float32 fields,
+    float64 stencil sums, HS 600 sweeps, Liu-Shen tol 1e-8 / 60 sweeps) on
+    the same pair: at one thread (the reference is single-threaded: numba
+    parallel=False, HornSchunck.py:51), at all of the host's threads, and the
+    float64 instance at all threads.  Each leg repeats the pair until min_s
+    seconds have run (one run at least)."""
     from oracle import oracle as orc
-    threads = orc.get_threads()
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
-        reps += 1
-        dt = time.perf_counter() - t0
-        if dt >= min_s:
-            break
     H, W = im1.shape
-    return {"value": round(reps * H * W / 1e6 / dt, 5), "unit": "Mpix-pairs/s", "cores": threads, "kind": "port",
-            "sample": "%d runs of 1 pair %dx%d LS(HS) 2-level (plain form), C oracle (float32 fields, float64 "
-                      "stencil sums), OpenMP %d threads, %.1f s" % (reps, H, W, threads, dt)}
+    all_threads = orc.get_threads()
+
+    def leg(threads, prec):
+        orc.set_threads(threads)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0, prec=prec)
+            reps += 1
+            dt = time.perf_counter() - t0
+            if dt >= min_s:
+                return reps, dt
+
+    try:
+        r1, t1 = leg(1, "f32")
+        ra, ta = leg(all_threads, "f32")
+        r64, t64 = leg(all_threads, "f64")
+    finally:
+        orc.set_threads(all_threads)
+    mpx = H * W / 1e6
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"value": round(r1 * mpx / t1, 5), "unit": "Mpix-pairs/s", "cores": 1, "kind": "port",
+            "sample": "%d run(s) of 1 pair %dx%d LS(HS) 2-level (plain form), C oracle (float32 fields, float64 "
+                      "stencil sums), 1 thread, %.1f s" % (r1, H, W, t1),
+            "all_cores": {"value": round(ra * mpx / ta, 5), "cores": all_threads,
+                          "sample": "%d runs, OpenMP %d threads, %.1f s" % (ra, all_threads, ta)},
+            "f64": {"value": round(r64 * mpx / t64, 5), "cores": all_threads,
+                    "sample": "%d runs of the float64 instance, OpenMP %d threads, %.1f s" % (r64, all_threads, t64)},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": affinity,
+            "opencl_cpu": "unavailable (no CPU OpenCL device in the image)"}
 
 
 def _timed(step, ctx, steps, warmup):
@@ -424,11 +463,15 @@ def bench_net(args):
 
 
 def bench_lshs(args):
-    """The headline: LS(HS) 2-level at 1024^2 solved to the plain method's
-    per-level residual, learned form against plain form.  A step is one pair;
-    the steps cycle over the workload's pair (Lamb-Oseen, density 0.04) and
-    held-out rendered pairs of the three flow families."""
-    from ofri import learned, ops, synth
+    """The headline: LS(HS) 2-level at 1024^2, learned form against plain
+    form.  A step is one judged batch of CHECK_EVERY (16) pairs: the learned
+    form with its pair-independent rules, the judge's plain run and EPE check
+    on one pair of the batch, and the plain fallback of every pair that
+    missed a rule or failed its check -- all inside the timed region.  The
+    steps cycle over the workload's pair (Lamb-Oseen, density 0.04) and
+    held-out rendered pairs of the three flow families; the batch's first
+    (checked) pair rotates through them."""
+    from ofri import learned, synth
     from ofri.device import DeviceArray, context
     from ofri.net import Net
     ctx = context()
@@ -438,88 +481,94 @@ def bench_lshs(args):
     pairs = [(DeviceArray.from_host(im1), DeviceArray.from_host(im2))]
     extra, specs = learned.heldout_pairs(args.eval_pairs - 1, n)
     pairs += extra
-    net = Net.load()
-    wl = learned.WORKLOAD
-    # the plain results and residual targets of every pair (they define the workload: not timed)
-    tgt, plain, plain_iters = [], [], []
-    for d1, d2 in pairs:
-        Up, Vp, ht, lt, st = learned.plain_targets(d1, d2, fw, ow)
-        tgt.append((ht, lt))
-        plain.append((Up.download(), Vp.download()))
-        plain_iters.append([s["ls_iters"] for s in st])
-    Ua, Va = DeviceArray((n, n)), DeviceArray((n, n))
+    fams = ["lamb_oseen"] + [("poiseuille", "lamb_oseen", "turbulence")[sp["family"]] for sp in specs]
     np_ = len(pairs)
-    cnt = {"i": 0, "fell": 0, "runs": 0}
+    B = learned.CHECK_EVERY
+    net = Net.load()
+    outs = [(DeviceArray((n, n)), DeviceArray((n, n))) for _ in range(np_)]
+    cnt = {"step": 0}
+
+    def batch_jobs(k):
+        return [(pairs[(k + j) % np_][0], pairs[(k + j) % np_][1], outs[(k + j) % np_][0], outs[(k + j) % np_][1])
+                for j in range(B)]
 
     def plain_step():
-        d1, d2 = pairs[cnt["i"] % np_]
-        cnt["i"] += 1
-        ops.pyr_lshs(d1, d2, filt_w=fw, opt_w=ow, Uacc=Ua, Vacc=Va, stats=False, **wl)
+        for a, b, U, V in batch_jobs(cnt["step"]):
+            learned.plain(a, b, fw, ow, Uacc=U, Vacc=V)
+        cnt["step"] += 1
 
-    # the learned form: every pair queued with its convergence flag on the
-    # device, the flags read once, the pairs that missed a target re-run by
-    # the plain pyramid -- all inside the timed region
-    flags = DeviceArray((max(args.steps, args.warmup, np_),))
-    jobs = [(pairs[k % np_][0], pairs[k % np_][1], tgt[k % np_][0], tgt[k % np_][1], Ua, Va)
-            for k in range(args.steps)]
+    runs = {}
 
-    def learned_run(nt, n):
-        return learned.solve_many(jobs[:n], nt, fw, ow, flags=flags)
+    def learned_step(nt, judge, key):
+        res = learned.solve_many(batch_jobs(cnt["step"]), nt, judge, fw, ow)
+        cnt["step"] += 1
+        runs.setdefault(key, []).extend(res)
 
-    def timed_many(nt):
-        if args.warmup:
-            learned_run(nt, min(args.warmup, len(jobs)))
-        ctx.sync()
-        ctx.timer_start()
-        t0 = time.perf_counter()
-        fb = learned_run(nt, args.steps)
-        g = ctx.timer_stop()
-        return time.perf_counter() - t0, g, fb
-
-    # every pair's launch sequences are captured as HIP graphs on their
-    # second call: run each pair twice in each form before any timing
-    prime = [(pairs[k][0], pairs[k][1], tgt[k][0], tgt[k][1], Ua, Va) for k in range(np_)]
+    # graph priming: every pair's learned, plain and judge launch sequences
+    # are captured on their second call
     for _ in range(2):
         for k in range(np_):
-            plain_step()
-        learned.solve_many(prime, net, fw, ow, flags=flags)
-        learned.solve_many(prime, None, fw, ow, flags=flags)
+            a, b = pairs[k]
+            learned.plain(a, b, fw, ow, Uacc=outs[k][0], Vacc=outs[k][1])
+            for nt in (net, None):
+                learned.solve_many([(a, b, outs[k][0], outs[k][1])], nt, learned.Judge(every=1), fw, ow)
     ctx.sync()
-    cnt["i"] = 0
-    wall_p, gpu_p = _timed(plain_step, ctx, args.steps, args.warmup)
-    wall_l, gpu_l, fb = timed_many(net)
-    fell, runs = sum(fb), len(fb)
-    wall_z, _, fbz = timed_many(None)
-    fell_z, runs_z = sum(fbz), len(fbz)
+
+    def timed(step):
+        cnt["step"] = 0
+        return _timed(step, ctx, args.steps, args.warmup)
+
+    wall_p, gpu_p = timed(plain_step)
+    judge = learned.Judge()
+    runs.clear()
+    wall_l, gpu_l = timed(lambda: learned_step(net, judge, "net"))
+    judge_z = learned.Judge()
+    wall_z, _ = timed(lambda: learned_step(None, judge_z, "nonet"))
+    timed_runs = runs["net"][args.warmup * B:]
+    fell = sum(r["fell_back"] for r in timed_runs)
+    fell_z = sum(r["fell_back"] for r in runs["nonet"][args.warmup * B:])
     mpix = n * n / 1e6
-    value = mpix * args.steps / wall_l
-    plain_value = mpix * args.steps / wall_p
-    # per pair, untimed: the learned result against the plain one, and sweeps
+    total = args.steps * B
+    value = mpix * total / wall_l
+    plain_value = mpix * total / wall_p
+    # per pair, untimed: the learned result against the plain one
     rows = []
-    for k, (d1, d2) in enumerate(pairs):
-        U, V, info = learned.solve(d1, d2, net, tgt[k][0], tgt[k][1], fw, ow, fallback=True)
-        e = learned.epe((U.download(), V.download()), plain[k])
-        rows.append({"family": "lamb_oseen" if k == 0 else ("poiseuille", "lamb_oseen", "turbulence")[specs[k - 1]["family"]],
-                     "epe_mean": round(e[0], 5), "epe_max": round(e[1], 4), "fell_back": info["fell_back"],
-                     "hs_sweeps": [s["hs_iters"] for s in info["stats"]],
-                     "ls_sweeps": [s["ls_iters"] for s in info["stats"]], "plain_ls_sweeps": plain_iters[k]})
+    for k, (a, b) in enumerate(pairs):
+        Up, Vp, _ = learned.plain(a, b, fw, ow)
+        P = (Up.download(), Vp.download())
+        U, V, info = learned.solve(a, b, net, fw, ow, fallback=True)
+        e = learned.epe((U.download(), V.download()), P)
+        rows.append({"family": fams[k], "epe_mean": round(e[0], 5), "epe_max": round(e[1], 4),
+                     "fell_back": info["fell_back"], "hs_sweeps": [s["hs_iters"] for s in info["stats"]],
+                     "ls_sweeps": [s["ls_iters"] for s in info["stats"]],
+                     "ls_r_over_n": [round(x[1], 5) for x in learned.residual_ratios(info["stats"])]})
+    fam_epe = {f: round(float(np.mean([r["epe_mean"] for r in rows if r["family"] == f])), 5)
+               for f in ("poiseuille", "lamb_oseen", "turbulence") if any(r["family"] == f for r in rows)}
+    rules = learned.RULES
     out = _line(METRIC, value, "Mpix-pairs/s", args, wall_l, gpu_l,
-                {"workload": "LS(HS) 2-level %dx%d to the plain method's per-level residual: FILTER 3.4/3 taps, HS "
-                             "alpha 45 / 21 (plain: 600 sweeps), FILTER_OPT 0.48/5 taps, Liu-Shen alpha 5 (plain: tol "
-                             "1e-8, max 60); learned form = accelerated HS + correction network + Liu-Shen to the "
-                             "targets, plain fallback" % (n, n),
-                 "pairs_per_step": 1, "height": n, "width": n, "levels": 2, "eval_pairs": np_, "hip_graphs": not args.no_graphs,
-                 "parallelism": "single GPU"},
+                {"workload": "LS(HS) 2-level %dx%d: FILTER 3.4/3 taps, HS alpha 45 / 21, FILTER_OPT 0.48/5 taps, "
+                             "Liu-Shen alpha 5; plain form = the reference's stopping rules (HS 600 sweeps, "
+                             "Liu-Shen tol 1e-8 / max 60); learned form = Chebyshev HS to the relative residual "
+                             "2^-22 (pair-independent), correction network, Liu-Shen %d / %d sweeps with a "
+                             "relative-residual acceptance test, plain fallback, EPE-to-plain judge on every "
+                             "%dth pair (0.01 px), all timed" % (n, n, rules.ls_sweeps[0], rules.ls_sweeps[1], B),
+                 "pairs_per_step": B, "height": n, "width": n, "levels": 2, "eval_pairs": np_,
+                 "hip_graphs": not args.no_graphs, "parallelism": "single GPU"},
                 cheb_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
     out["roofline_plain_hs"] = hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n)
     out["learned_vs_plain"] = {
         "learned_value": round(value, 3), "plain_value": round(plain_value, 3),
-        "no_network_value": round(mpix * args.steps / wall_z, 3),
-        "speedup_vs_plain": round(value / plain_value, 3), "plain_ms_per_pair": round(wall_p * 1e3 / args.steps, 4),
-        "learned_ms_per_pair": round(wall_l * 1e3 / args.steps, 4),
-        "fallback_rate": round(fell / max(1, runs), 4), "no_network_fallback_rate": round(fell_z / max(1, runs_z), 4),
+        "no_network_value": round(mpix * total / wall_z, 3),
+        "speedup_vs_plain": round(value / plain_value, 3), "plain_ms_per_pair": round(wall_p * 1e3 / total, 4),
+        "learned_ms_per_pair": round(wall_l * 1e3 / total, 4),
+        "fallback_rate": round(fell / max(1, len(timed_runs)), 4),
+        "no_network_fallback_rate": round(fell_z / max(1, total), 4),
+        "judge": judge.summary(), "no_network_judge": judge_z.summary(),
         "epe_to_plain_mean": round(float(np.mean([r["epe_mean"] for r in rows])), 5),
         "epe_to_plain_max_pair_mean": round(float(np.max([r["epe_mean"] for r in rows])), 5),
+        "epe_to_plain_by_family": fam_epe,
+        "rules": {"hs_rel": rules.hs_rel, "ls_sweeps": rules.ls_sweeps, "ls_accept": rules.ls_accept,
+                  "hs_budget": rules.hs_budget},
         "pairs": rows,
     }
     out["roofline_net"] = net_roofline(ctx, net, n)

@@ -79,6 +79,12 @@ int ofri_warp_coords(ofri_ctx* ctx, const float* d_img, const float* d_cy, const
                      int W, float* d_out);
 /* d_acc += d_x (float32), n elements.  Replaces GenericPyramidalOpticalFlow.py:412-413. */
 int ofri_accumulate(ofri_ctx* ctx, float* d_acc, const float* d_x, size_t n);
+/* Mean endpoint error of two flows, mean over n of sqrt((U1-U2)^2 +
+ * (V1-V2)^2), written to device memory (*d_out, float) without host
+ * synchronisation: the learned form's EPE-to-plain judge on its check
+ * sample.  No counterpart in the reference. */
+int ofri_flow_epe(ofri_ctx* ctx, const float* d_U1, const float* d_V1, const float* d_U2, const float* d_V2,
+                  size_t n, float* d_out);
 
 /* ---- Horn-Schunck -------------------------------------------------------- */
 /* niter Jacobi sweeps from the flow in d_U, d_V (in/out).  d_im1 is the frame
@@ -119,8 +125,11 @@ int ofri_hs_tol(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, in
  * launch_test != 0: the stop test runs only at the last sweep of each launch
  * (cheaper; d_hist must be NULL): the accelerated phase then ends with the
  * iterate that launch produced, the plain phase with T(x_k) as before, and
- * the sweep count is a whole number of launches per phase.  No counterpart
- * in the reference (an accelerated form for solving to a residual). */
+ * the sweep count is a whole number of launches per phase.  launch_test is
+ * a flag word: bit 0 the per-launch test, bit 1 (with bit 0) a relative
+ * tolerance -- a launch's norm is compared with tol * (||U||_2 + ||V||_2) /
+ * (H*W) of that launch's result.  No counterpart in the reference (an
+ * accelerated form for solving to a residual). */
 int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
                   int maxnum, double tol, float bound, float finish, float* d_U, float* d_V,
                   float* d_hist, float* h_err, int* h_iters, int launch_test);
@@ -238,6 +247,8 @@ typedef struct ofri_level_stats {
     int ls_iters;    /* Liu-Shen sweeps */
     float hs_res;    /* HS's last update norm (0 unless recorded) */
     int hs_iters;    /* HS sweeps */
+    float hs_nrm;    /* relative tests: (||U||_2 + ||V||_2)/(H*W) of HS's tested result (0 otherwise) */
+    float ls_nrm;    /* relative tests: the same norm of Liu-Shen's tested result (0 otherwise) */
 } ofri_level_stats;
 
 /* Coarse-to-fine LS(HS): bicubic downsample, spline upsample + x2 scaling,
@@ -255,11 +266,15 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
  * (device) receives per level, levels in order, the planes HS u, HS v,
  * LS u - HS u, LS v - HS v at the level's size (4 * sum of level pixels
  * floats), and the HS sweeps record their update norms (hs_res in h_stats).
- * The warm-start network's training labels and the residual targets of the
- * learned form.  No counterpart in the reference. */
+ * h_ls_tap_back (may be NULL): per level, k > 0 takes the Liu-Shen planes
+ * k sweeps before Liu-Shen's stop (at sweep ls_maxnum - k; Liu-Shen then
+ * runs its remaining k sweeps from there, so the pyramid's result is
+ * unchanged unless the tolerance stops it earlier; the level's ls_iters then
+ * counts those k sweeps).  The warm-start network's training labels.  No
+ * counterpart in the reference. */
 int ofri_pyr_lshs_taps(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                        const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps,
-                       ofri_level_stats* h_stats);
+                       const int* h_ls_tap_back, ofri_level_stats* h_stats);
 
 /* One image pair of a batch: device frames at t and t+1 (H x W) and the
  * device outputs U, V (H x W) the pair's accumulated flow is written to. */
@@ -345,11 +360,14 @@ int ofri_net_get_grads(ofri_ctx* ctx, ofri_net* net, float* h_grads, size_t n);
 
 /* The learned form of the LS(HS) 2-level pyramid: at each level HS runs
  * (Chebyshev-accelerated when cheb_bound > 0) from zero until its update
- * norm is <= hs_tol[level] or hs_budget[level] sweeps; the correction network
- * (4 input planes) maps the level's Liu-Shen pair and HS's flow to the
- * correction Liu-Shen would add, and Liu-Shen runs from HS's flow plus that
- * correction until its norm is <= ls_tol[level] or ls_budget sweeps.  The
- * targets are the plain method's residuals at its stopping points (r*).
+ * norm is <= hs_tol[level] (relative != 0: <= hs_tol[level] * (||U||_2 +
+ * ||V||_2)/(H*W) of the tested iterate) or hs_budget[level] sweeps; the
+ * correction network (4 input planes) maps the level's Liu-Shen pair and
+ * HS's flow to a correction of HS's flow, and Liu-Shen runs from HS's flow
+ * plus that correction.  ls_sweeps[level] > 0: exactly that many Liu-Shen
+ * sweeps, after which the level is accepted when the last update norm is
+ * <= ls_tol[level] (relative as above); ls_sweeps[level] = 0: Liu-Shen runs
+ * until its norm is <= ls_tol[level] or ls_budget sweeps.
  * net NULL: Liu-Shen from HS's flow alone (the method without the network).  Both solvers test
  * their targets once per fused launch (at its last sweep).  *h_converged (may be
  * NULL) is 1 when every level met its targets; otherwise the caller falls
@@ -369,6 +387,8 @@ typedef struct ofri_learned_params {
     float ls_cheb_lo;         /* Liu-Shen Chebyshev interval [lo, hi] (lo in (-1, 0)); */
     float ls_cheb_hi;         /* hi = 0: plain Liu-Shen sweeps */
     int hs_finish_budget[2];  /* HS plain finishing sweeps per level at most (0: hs_budget / 2), within hs_budget */
+    int relative;             /* 1: hs_tol / ls_tol are relative to the tested iterate's norm */
+    int ls_sweeps[2];         /* per level: > 0 = exactly this many Liu-Shen sweeps, ls_tol accepts */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,

@@ -27,6 +27,7 @@
 
 #include "ofri_internal.h"
 #include "solve_check.h"
+#include <cstring>
 
 #pragma clang fp contract(off)
 
@@ -48,9 +49,6 @@ __device__ __forceinline__ float from_right(float x) {
 // Chebyshev weights written.
 struct SolveStart {
     SolveCtrl* ctrl;
-    float* omega;
-    int n;
-    OmegaTab tab;
 };
 
 __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restrict__ I1, int H,
@@ -69,10 +67,10 @@ __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restric
             c->last_err = 0.0f;
             c->limit = 0;
             c->base_buf = 0;
+            c->last_nrm = 0.0f;
             c->slot[0] = SolveState{};
             c->slot[1] = SolveState{};
         }
-        for (int k = threadIdx.x; k < st.n; k += blockDim.x) st.omega[k] = st.tab.w[k < kOmegaTab ? k : kOmegaTab - 1];
     }
     if (j >= W) return;
     int i1 = (i + 1 < H) ? i + 1 : H - 2;
@@ -125,10 +123,10 @@ __global__ __launch_bounds__(512) void hs_prefilter_derivs_k(const float* __rest
             c->last_err = 0.0f;
             c->limit = 0;
             c->base_buf = 0;
+            c->last_nrm = 0.0f;
             c->slot[0] = SolveState{};
             c->slot[1] = SolveState{};
         }
-        for (int k = tid; k < st.n; k += blockDim.x) st.omega[k] = st.tab.w[k < kOmegaTab ? k : kOmegaTab - 1];
     }
     const int R = FR + 2 * h, Cw = FC + 2 * h;
     {
@@ -208,6 +206,7 @@ struct HsPass {
     size_t N;
     float* hist;             // per-sweep residuals (nullptr: not recorded)
     int check_last;          // 1: only a launch's last sweep is reduced and tested
+    int rel;                 // 1: relative tolerance (result norms in row K of the partials)
     int copy_back;           // final launch with no re-run: the result, if in buffer 1, copied into buffer 0
     // Chebyshev mode (hs_pass_k<..., true, true>): previous iterate buffers,
     // the weight of every global sweep and the damping of the plain step
@@ -400,6 +399,7 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     if (nsw > 0 && normed(nsw - 1)) {
         __syncthreads();
         if (threadIdx.x == 0) flush(nsw - 1);
+        if (a.rel) store_norm<R, WAVES>(U, V, own_x, w, gy0, K, H, part + ((size_t)K * nblk + blk) * 2);
     }
 
     float* uo = a.out[src ^ 1][0];
@@ -476,7 +476,7 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
     }
     // the launch's Chebyshev weights, one per lane, broadcast per sweep
     float om_lane = 1.0f;
-    if (lane < nsw) om_lane = a.omega[base + lane];
+    if (lane < nsw) om_lane = a.omega[omega_index(base + lane)];
 
     __shared__ float edge[2][WAVES][2][2][64];  // [buf][wave][top,bottom][U,V][lane]
     __shared__ double red[2][WAVES][2];
@@ -563,6 +563,7 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
     if (nsw > 0 && normed(nsw - 1)) {
         __syncthreads();
         if (threadIdx.x == 0) flush(nsw - 1);
+        if (a.rel) store_norm<R, WAVES>(U, V, own_x, w, gy0, K, H, part + ((size_t)K * nblk + blk) * 2);
     }
 
     float* uo = a.out[src ^ 1][0];
@@ -686,10 +687,42 @@ void hs_default(size_t npx, int* variant, int* depth) {
 
 }  // namespace
 
+int omega_table(ofri_ctx* ctx, double sigma, const float** out) {
+    uint64_t key;
+    static_assert(sizeof key == sizeof sigma, "sigma key");
+    memcpy(&key, &sigma, sizeof key);
+    DevBuf& b = ctx->omega_tabs[key];
+    if (!b.p) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        OFRI_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(OFRI_E_STATE, "hs: Chebyshev weights for a new bound requested while capturing a graph");
+        float t[kOmegaTab];
+        const double s2 = sigma * sigma;
+        double om = 1.0;
+        for (int k = 0; k < kOmegaTab; ++k) {
+            if (k == 1) om = 2.0 / (2.0 - s2);
+            else if (k > 1) om = 1.0 / (1.0 - 0.25 * s2 * om);
+            t[k] = (float)om;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof t) != hipSuccess) return fail(OFRI_E_NOMEM, "hs: Chebyshev weight table");
+        hipError_t e = hipMemcpy(p, t, sizeof t, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(OFRI_E_HIP, "hs: Chebyshev weight upload: %s", hipGetErrorString(e));
+        }
+        b.p = p;
+        b.bytes = sizeof t;
+    }
+    *out = (const float*)b.p;
+    return OFRI_OK;
+}
+
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
              float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget,
-             bool zero_start, const float* pre_w, int pre_n) {
+             bool zero_start, const float* pre_w, int pre_n, bool rel) {
     if (pre_w && pre_n > 0 && (pre_n > kMaxTaps || (pre_n & 1) == 0 || H < pre_n || W < pre_n))
         return fail(OFRI_E_ARG, "hs: pre-filter of %d taps on a %dx%d image", pre_n, H, W);
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
@@ -698,6 +731,8 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     const bool cheb = cheb_b > 0.0f;
     if (cheb && (!d_ctrl || !(cheb_b <= 0.999f)))
         return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 0.999], got %g", cheb_b);
+    if (rel && (!d_ctrl || !check_last || d_hist))
+        return fail(OFRI_E_ARG, "hs: a relative tolerance needs the per-launch stop test (and no history)");
     size_t N = (size_t)H * W;
     int vi, depth;
     if (cheb) hs_cheb_default((size_t)H * W, &vi, &depth);
@@ -722,15 +757,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* U0 = Vb + N;
     float* V0 = U0 + N;
     void* pbuf = nullptr;
-    // per-sweep partial sums, two launch-parity slots
-    const size_t pstride = (size_t)depth * nblk * 2;
+    // per-sweep partial sums, two launch-parity slots (row `depth`: the
+    // result norms of a relative test)
+    const size_t pstride = (size_t)(depth + 1) * nblk * 2;
     if (d_ctrl) OFRI_TRY(scratch(ctx, "hs_part", 2 * pstride * sizeof(double), &pbuf));
-    float *P0 = nullptr, *omega = nullptr;
+    float* P0 = nullptr;
+    const float* omega = nullptr;
     if (cheb) {
         void* cb;
-        OFRI_TRY(scratch(ctx, "hs_cheb", (4 * N + (size_t)niter + 64) * sizeof(float), &cb));
+        OFRI_TRY(scratch(ctx, "hs_cheb", 4 * N * sizeof(float), &cb));
         P0 = (float*)cb;
-        omega = P0 + 4 * N;
     }
     // spectrum of one sweep's linear part lies in [-1/3, b]: the 8-neighbour
     // average has eigenvalues in [-1/3, 1] and the per-pixel data projection
@@ -739,15 +775,10 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     const double lo = -1.0 / 3.0, hi = cheb_b;
     SolveStart start{};
     start.ctrl = d_ctrl;
-    if (cheb) {
-        start.omega = omega;
-        start.n = niter;
-        start.tab = cheb_omegas((hi - lo) / (2.0 - lo - hi));
-    }
+    if (cheb) OFRI_TRY(omega_table(ctx, (hi - lo) / (2.0 - lo - hi), &omega));
     // computeDerivatives(I1=im2, I0=im1) of the pre-filtered frames (pre_w:
     // the filter, fused into the same pass when it has at most 9 taps); the
-    // first workgroup also clears the control word and writes the Chebyshev
-    // weights
+    // first workgroup also clears the control word
     const bool pre = pre_w && pre_n > 0;
     if (pre && pre_n <= 2 * kDMaxH + 1) {
         Taps t{};
@@ -786,6 +817,7 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.N = N;
     p.hist = d_hist;
     p.check_last = check_last ? 1 : 0;
+    p.rel = rel ? 1 : 0;
     if (cheb) {
         p.pin[0][0] = P0; p.pin[0][1] = P0 + N; p.pin[1][0] = P0 + 2 * N; p.pin[1][1] = P0 + 3 * N;
         p.pout[0][0] = P0; p.pout[0][1] = P0 + N; p.pout[1][0] = P0 + 2 * N; p.pout[1][1] = P0 + 3 * N;

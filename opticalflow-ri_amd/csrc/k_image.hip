@@ -488,6 +488,40 @@ __global__ void dnorm_stage1(const float* __restrict__ A, const float* __restric
     }
 }
 
+// Stage 1 of the mean endpoint error of two flows: per-block double sums
+// of sqrt(du^2 + dv^2) (the differences and the root in float32).
+__global__ void epe_stage1(const float* __restrict__ U1, const float* __restrict__ V1,
+                           const float* __restrict__ U2, const float* __restrict__ V2, size_t n,
+                           double* __restrict__ part) {
+    __shared__ double sa[kRedThreads];
+    double a = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float du = U1[i] - U2[i], dv = V1[i] - V2[i];
+        a = a + (double)sqrtf(du * du + dv * dv);
+    }
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sa[0];
+}
+
+__global__ void epe_stage2(const double* __restrict__ part, int np, size_t n, float* __restrict__ out) {
+    __shared__ double sa[kRedThreads];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) a = a + part[i];
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)(sa[0] / (double)n);
+}
+
 __global__ void dnorm_stage2(const double* __restrict__ part, int np, size_t n,
                              float* __restrict__ out) {
     __shared__ double sa[kRedThreads], sb[kRedThreads];
@@ -961,6 +995,19 @@ int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float
 
 int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out) {
     return reduce_max2(ctx, x, nullptr, n, d_out);
+}
+
+int flow_epe(ofri_ctx* ctx, const float* U1, const float* V1, const float* U2, const float* V2, size_t n,
+             float* d_out) {
+    void* part;
+    OFRI_TRY(scratch(ctx, "epe_part", kRedBlocks * sizeof(double), &part));
+    int nb = (int)((n + kRedThreads - 1) / kRedThreads);
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb < 1) nb = 1;
+    epe_stage1<<<nb, kRedThreads, 0, ctx->stream>>>(U1, V1, U2, V2, n, (double*)part);
+    epe_stage2<<<1, kRedThreads, 0, ctx->stream>>>((const double*)part, nb, n, d_out);
+    OFRI_HIP(hipGetLastError());
+    return OFRI_OK;
 }
 
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,

@@ -51,6 +51,7 @@ __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restric
         ctrl->last_err = 0.0f;
         ctrl->limit = 0;
         ctrl->base_buf = 0;
+        ctrl->last_nrm = 0.0f;
         ctrl->slot[0] = SolveState{};
         ctrl->slot[1] = SolveState{};
     }
@@ -114,6 +115,7 @@ struct LsPass {
     int rel_src;            // (relative buffers: unused here)
     int tiles_x, tiles_y;   // tile grid of this image (batched launches)
     int check_last;         // 1: only a launch's last sweep is reduced and tested
+    int rel;                // 1: relative tolerance (result norms in row K of the partials)
     // final launch with no re-run: the result (u, v of the buffer holding it)
     // written into dst (dst_add: added); dst[0] null: left where it is
     float* dst[2];
@@ -309,7 +311,7 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
         Raw pu = raw(uup), pv = raw(vup);
         Raw cu = raw(u[0]), cv = raw(v[0]);
         double su = 0.0, sv = 0.0;
-        const float om = CHEB ? a.omega[base + s] : 1.0f;
+        const float om = CHEB ? a.omega[omega_index(base + s)] : 1.0f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Raw nu = raw(r + 1 < R ? u[r + 1] : udn);
@@ -412,6 +414,9 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
     else
         ls_sweeps<R, WAVES, false, CHEB, FAST>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY,
                                                II, IXT, IYT, B11, B12, B22);
+    if (a.rel && nsw > 0 && !a.final_)
+        store_norm<R, WAVES>(u, v, lane >= K && lane < 64 - K && gx < W, w, gy0, K, H,
+                             a.part + (size_t)a.slot * a.part_stride + ((size_t)K * nblk + blk) * 2);
 
     float* uo = a.out[src ^ 1][0];
     float* vo = a.out[src ^ 1][1];
@@ -539,10 +544,11 @@ static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, 
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
-             float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast) {
+             float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast, bool rel) {
     if ((sink_u == nullptr) != (sink_v == nullptr)) return fail(OFRI_E_ARG, "ls: one result sink given without the other");
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
+    if (rel && (!d_ctrl || !check_last)) return fail(OFRI_E_ARG, "ls: a relative tolerance needs the per-launch test");
     const bool cheb = cheb_hi > 0.0f;
     if (cheb && !(cheb_lo > -1.0f && cheb_lo < 0.0f && cheb_hi <= 0.999f))
         return fail(OFRI_E_ARG, "ls: Chebyshev interval [%g, %g] outside (-1, 0) x (0, 0.999]", cheb_lo, cheb_hi);
@@ -567,10 +573,10 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
     int nblk = grid.x * grid.y;
     void *fbuf, *pbuf, *mbuf;
-    OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float) + (cheb ? (size_t)maxnum * 4 + 64 : 0),
-                     &fbuf));
-    // per-sweep partial sums, two launch-parity slots
-    const size_t pstride = (size_t)depth * nblk * 2;
+    OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float), &fbuf));
+    // per-sweep partial sums, two launch-parity slots (row `depth`: the result
+    // norms of a relative test)
+    const size_t pstride = (size_t)(depth + 1) * nblk * 2;
     OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
     float* F = (float*)fbuf;
     float* ub = F + 8 * N;
@@ -588,19 +594,18 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.tol = tol;
     p.maxnum = maxnum;
     p.check_last = check_last ? 1 : 0;
+    p.rel = rel ? 1 : 0;
     if (fast && (cheb || !check_last)) return fail(OFRI_E_ARG, "ls: float32 sweeps need the per-launch test alone");
     auto kern = cheb ? var.cheb : (fast ? var.fast : var.kernel);
     if (cheb) {
         // previous-iterate buffers (the first step's previous iterate is the
         // start) and the weights for [cheb_lo, cheb_hi]
         float* P0 = F + 10 * N;
-        float* omega = F + 14 * N;
         OFRI_HIP(hipMemcpyAsync(P0, u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(P0 + N, v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         const double lo = cheb_lo, hi = cheb_hi;
-        if (maxnum > 0)
-            cheb_omega_k<<<1, 256, 0, ctx->stream>>>(cheb_omegas((hi - lo) / (2.0 - lo - hi)), omega, maxnum);
-        OFRI_HIP(hipGetLastError());
+        const float* omega = nullptr;
+        OFRI_TRY(omega_table(ctx, (hi - lo) / (2.0 - lo - hi), &omega));
         p.pin[0][0] = P0; p.pin[0][1] = P0 + N; p.pin[1][0] = P0 + 2 * N; p.pin[1][1] = P0 + 3 * N;
         p.pout[0][0] = P0; p.pout[0][1] = P0 + N; p.pout[1][0] = P0 + 2 * N; p.pout[1][1] = P0 + 3 * N;
         p.omega = omega;

@@ -112,6 +112,8 @@ int ofri_destroy(ofri_ctx* ctx) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.tab.p) (void)hipFree(kv.second.tab.p);
     }
+    for (auto& kv : ctx->omega_tabs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -256,6 +258,14 @@ int ofri_accumulate(ofri_ctx* ctx, float* d_acc, const float* d_x, size_t n) {
     return accumulate(ctx, d_acc, d_x, n);
 }
 
+int ofri_flow_epe(ofri_ctx* ctx, const float* d_U1, const float* d_V1, const float* d_U2, const float* d_V2,
+                  size_t n, float* d_out) {
+    CHECK_CTX(ctx);
+    if (!d_U1 || !d_V1 || !d_U2 || !d_V2 || !d_out) return fail(OFRI_E_ARG, "ofri_flow_epe: null pointer");
+    if (n == 0) return fail(OFRI_E_ARG, "ofri_flow_epe: empty flow");
+    return flow_epe(ctx, d_U1, d_V1, d_U2, d_V2, n, d_out);
+}
+
 int ofri_hs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float alpha,
             int niter, float* d_U, float* d_V, float* h_err) {
     CHECK_CTX(ctx);
@@ -296,12 +306,15 @@ int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
     if (!d_im1 || !d_im2 || !d_U || !d_V) return fail(OFRI_E_ARG, "ofri_hs_accel: null pointer");
     if (!(bound > 0.0f && bound <= 0.999f))
         return fail(OFRI_E_ARG, "ofri_hs_accel: bound %g not in (0, 0.999]", bound);
-    if (launch_test && d_hist)
+    if (launch_test & ~3) return fail(OFRI_E_ARG, "ofri_hs_accel: unknown flags 0x%x", launch_test);
+    if ((launch_test & 1) && d_hist)
         return fail(OFRI_E_ARG, "ofri_hs_accel: the per-launch stop test records no per-sweep history");
+    if ((launch_test & 2) && !(launch_test & 1))
+        return fail(OFRI_E_ARG, "ofri_hs_accel: a relative tolerance needs the per-launch stop test");
     void* c;
     OFRI_TRY(scratch(ctx, "api_ctrl", sizeof(SolveCtrl), &c));
     OFRI_TRY(hs_solve(ctx, d_im1, d_im2, H, W, alpha, maxnum, d_U, d_V, nullptr, tol, (SolveCtrl*)c, d_hist,
-                      bound, finish, launch_test != 0));
+                      bound, finish, (launch_test & 1) != 0, 0, false, nullptr, 0, (launch_test & 2) != 0));
     if (h_err || h_iters) {
         SolveCtrl hc;
         OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
@@ -408,7 +421,8 @@ struct LearnedRun {
 
 static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                             const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps = nullptr,
-                            bool record_res = false, const LearnedRun* lrn = nullptr) {
+                            bool record_res = false, const LearnedRun* lrn = nullptr,
+                            const int* ls_tap_back = nullptr) {
     const int L = p->levels;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
@@ -476,7 +490,7 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             // (HS's own error value, the change from the initial flow, is not formed here)
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget[l], U, V, nullptr, lp->hs_tol[l],
                               hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish[l], true,
-                              lp->hs_finish_budget[l], true, hpw, hpn));
+                              lp->hs_finish_budget[l], true, hpw, hpn, lp->relative != 0));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l,
                               nullptr, 0.0f, 0.0f, false, 0, false, hpw, hpn));
@@ -505,17 +519,36 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
             // (copied at the first level, whose accumulators start empty).
             float* su = ls_sink ? out_v : nullptr;
             float* sv = ls_sink ? out_u : nullptr;
-            if (lrn)
-                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, lrn->lp->ls_budget, lrn->lp->ls_tol[l], V, U,
-                                  ctrls + l, true, lrn->lp->ls_cheb_lo, lrn->lp->ls_cheb_hi, su, sv, l > 0,
-                                  lrn->lp->ls_cheb_hi <= 0.0f));
-            else
+            const int back = ls_tap_back && tap ? ls_tap_back[l] : 0;
+            if (lrn) {
+                // fixed sweep count: the test never stops it (tolerance 0),
+                // the check after the solve accepts or rejects the level
+                const ofri_learned_params* lp = lrn->lp;
+                const bool fixed = lp->ls_sweeps[l] > 0;
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, fixed ? lp->ls_sweeps[l] : lp->ls_budget,
+                                  fixed ? 0.0 : (double)lp->ls_tol[l], V, U, ctrls + l, true, lp->ls_cheb_lo,
+                                  lp->ls_cheb_hi, su, sv, l > 0, lp->ls_cheb_hi <= 0.0f, lp->relative != 0));
+            } else if (back > 0 && back < p->ls_maxnum) {
+                // the tap k sweeps before the stop: ls_maxnum - k sweeps, the
+                // Liu-Shen planes exported, then the remaining k sweeps from there
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum - back, p->ls_tol, V, U, ctrls + l,
+                                  false, 0.0f, 0.0f, nullptr, nullptr, false));
+                OFRI_TRY(subtract(ctx, tap + 2 * N, U, tap, N));
+                OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
+                OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, back, p->ls_tol, V, U, ctrls + l, false, 0.0f,
+                                  0.0f, nullptr, nullptr, false));
+            } else {
                 OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l, false,
                                   0.0f, 0.0f, su, sv, l > 0));
+            }
         }
         if (tap) {
-            OFRI_TRY(subtract(ctx, tap + 2 * N, U, tap, N));
-            OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
+            const bool tapped = p->ls_alpha > 0.0f && ls_tap_back && ls_tap_back[l] > 0 &&
+                                ls_tap_back[l] < p->ls_maxnum;
+            if (!tapped) {
+                OFRI_TRY(subtract(ctx, tap + 2 * N, U, tap, N));
+                OFRI_TRY(subtract(ctx, tap + 3 * N, V, tap + N, N));
+            }
             tap += 4 * N;
         }
         if (!ls_sink) OFRI_TRY(accumulate2(ctx, out_u, U, out_v, V, N));
@@ -662,11 +695,19 @@ static int batch_lshs_enqueue(ofri_ctx* ctx, const ofri_pair* pr, int n, const o
 // The learned form's convergence flag on the device: every level's HS and
 // Liu-Shen residuals (the published control words of the "pyr_stats"
 // scratch: 2 Liu-Shen, then 2 HS) against their targets, as the host test.
+// Relative targets scale with the norm the test published (the same double
+// comparison as the stop test).
+__device__ __host__ inline bool meets(float err, float tol, float nrm, int rel) {
+    const double t = rel ? (double)tol * (double)nrm : (double)tol;
+    return !((double)err > t);
+}
+
 __global__ void learned_check_k(const SolveCtrl* c, float hs0, float hs1, float ls0, float ls1, int with_ls,
-                                int* out) {
+                                int rel, int* out) {
     int ok = 1;
-    if (!(c[2].last_err <= hs0) || !(c[3].last_err <= hs1)) ok = 0;
-    if (with_ls && (!(c[0].last_err <= ls0) || !(c[1].last_err <= ls1))) ok = 0;
+    if (!meets(c[2].last_err, hs0, c[2].last_nrm, rel) || !meets(c[3].last_err, hs1, c[3].last_nrm, rel)) ok = 0;
+    if (with_ls && (!meets(c[0].last_err, ls0, c[0].last_nrm, rel) || !meets(c[1].last_err, ls1, c[1].last_nrm, rel)))
+        ok = 0;
     *out = ok;
 }
 
@@ -690,7 +731,58 @@ static int pyr_stats_download(ofri_ctx* ctx, int L, const std::vector<int>& lh, 
         out[l].ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
         out[l].hs_res = with_res ? hc[L + l].last_err : 0.0f;
         out[l].hs_iters = with_res ? hc[L + l].sweeps : p->niter;
+        out[l].hs_nrm = with_res ? hc[L + l].last_nrm : 0.0f;
+        out[l].ls_nrm = p->ls_alpha > 0.0f ? hc[l].last_nrm : 0.0f;
     }
+    return OFRI_OK;
+}
+
+// Runs a launch sequence through the context's graph cache: the first call
+// with an entry (and any call after a scratch buffer moved) runs eagerly,
+// which also sizes every scratch buffer; the next captures the sequence into
+// an executable graph; later calls replay it.  enqueue(capturing) enqueues
+// the sequence.  stable_only: an eager run that itself moved a scratch
+// buffer does not count towards capture (work it wrote before the move,
+// such as job tables, is stale).  An executable graph is replaced only
+// after the stream has drained: its last launch may still be in flight, and
+// destroying it would free the argument and node storage that launch (and
+// any tracing tool following it) still uses.
+template <class F>
+static int run_cached(ofri_ctx* ctx, GraphEntry& g, F&& enqueue, bool stable_only = false) {
+    if (g.exec && g.ws_gen == ctx->ws_gen) {
+        OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        return OFRI_OK;
+    }
+    if (g.exec) {
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.eager_runs = 0;
+    }
+    if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
+        const uint64_t gen0 = ctx->ws_gen;
+        OFRI_TRY(enqueue(false));
+        g.eager_runs = stable_only && ctx->ws_gen != gen0 ? 0 : g.eager_runs + 1;
+        g.ws_gen = ctx->ws_gen;
+        return OFRI_OK;
+    }
+    hipGraph_t graph = nullptr;
+    OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue(true);
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != OFRI_OK || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (st != OFRI_OK) return st;
+        return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        g.exec = nullptr;
+        return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    g.ws_gen = ctx->ws_gen;
+    OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
     return OFRI_OK;
 }
 
@@ -727,35 +819,8 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
     if (!ctx->graphs_on) {
         OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc));
     } else {
-        std::string key = lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
-        GraphEntry& g = ctx->graphs[key];
-        if (g.exec && g.ws_gen == ctx->ws_gen) {
-            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
-        } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
-            // first sight (or buffers moved): run eagerly, which also sizes every scratch buffer
-            if (g.exec) {
-                (void)hipGraphExecDestroy(g.exec);
-                g.exec = nullptr;
-            }
-            OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc));
-            g.eager_runs += 1;
-            g.ws_gen = ctx->ws_gen;
-        } else {
-            hipGraph_t graph;
-            OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int st = pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc);
-            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-            if (st != OFRI_OK) return st;
-            if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-            hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) {
-                g.exec = nullptr;
-                return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
-            }
-            g.ws_gen = ctx->ws_gen;
-            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
-        }
+        GraphEntry& g = ctx->graphs[lshs_key(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc)];
+        OFRI_TRY(run_cached(ctx, g, [&](bool) { return pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc); }));
     }
     if (h_stats) OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, false, h_stats));
     return OFRI_OK;
@@ -763,7 +828,7 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
 
 int ofri_pyr_lshs_taps(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                        const ofri_lshs_params* p, float* d_Uacc, float* d_Vacc, float* d_taps,
-                       ofri_level_stats* h_stats) {
+                       const int* h_ls_tap_back, ofri_level_stats* h_stats) {
     CHECK_CTX(ctx);
     if (!p || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: null pointer");
     if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: levels %d", p->levels);
@@ -771,7 +836,10 @@ int ofri_pyr_lshs_taps(ofri_ctx* ctx, const float* d_im1, const float* d_im2, in
     const int L = p->levels;
     std::vector<int> lh, lw;
     level_sizes(H, W, L, lh, lw);
-    OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, d_taps, true));
+    if (h_ls_tap_back)
+        for (int l = 0; l < L; ++l)
+            if (h_ls_tap_back[l] < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_taps: negative tap offset at level %d", l);
+    OFRI_TRY(pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc, d_taps, true, nullptr, h_ls_tap_back));
     if (h_stats) OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, h_stats));
     else OFRI_HIP(hipStreamSynchronize(ctx->stream));
     return OFRI_OK;
@@ -808,35 +876,13 @@ int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lsh
     }
     if (!ctx->graphs_on) {
         OFRI_TRY(batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, true));
-    } else if (g.exec && g.ws_gen == ctx->ws_gen) {
-        OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
-    } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
-        if (g.exec) {
-            (void)hipGraphExecDestroy(g.exec);
-            g.exec = nullptr;
-        }
-        // The job tables hold scratch addresses; when a scratch buffer grew during
-        // this run, tables written before the growth are stale, so the next call
-        // runs eagerly again instead of capturing.
-        const uint64_t gen0 = ctx->ws_gen;
-        OFRI_TRY(batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, true));
-        g.eager_runs = ctx->ws_gen == gen0 ? g.eager_runs + 1 : 0;
-        g.ws_gen = ctx->ws_gen;
     } else {
-        hipGraph_t graph;
-        OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int st = batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, false);
-        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-        if (st != OFRI_OK) return st;
-        if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) {
-            g.exec = nullptr;
-            return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
-        }
-        g.ws_gen = ctx->ws_gen;
-        OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        // the job tables hold scratch addresses: they are written by eager
+        // runs only (never while capturing), and an eager run that moved a
+        // scratch buffer is run again before any capture
+        OFRI_TRY(run_cached(ctx, g, [&](bool capturing) {
+            return batch_lshs_enqueue(ctx, pairs, n, p, g.tab.p, !capturing);
+        }, true));
     }
     if (h_stats) {
         void* st;
@@ -980,6 +1026,8 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
             return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: HS budget / finish at level %d", l);
     if (lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen budget");
     for (int l = 0; l < 2; ++l)
+        if (lp->ls_sweeps[l] < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen sweeps at level %d", l);
+    for (int l = 0; l < 2; ++l)
         if (lp->cheb_bound[l] != 0.0f && !(lp->cheb_bound[l] > 0.0f && lp->cheb_bound[l] <= 0.999f))
             return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Chebyshev bound %g at level %d", lp->cheb_bound[l], l);
     if (lp->ls_cheb_hi != 0.0f &&
@@ -1003,48 +1051,24 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (!ctx->graphs_on) {
         OFRI_TRY(enqueue());
     } else {
-        GraphEntry& g = ctx->graphs[key];
-        if (g.exec && g.ws_gen == ctx->ws_gen) {
-            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
-        } else if (g.eager_runs == 0 || g.ws_gen != ctx->ws_gen) {
-            if (g.exec) {
-                (void)hipGraphExecDestroy(g.exec);
-                g.exec = nullptr;
-            }
-            OFRI_TRY(enqueue());
-            g.eager_runs += 1;
-            g.ws_gen = ctx->ws_gen;
-        } else {
-            hipGraph_t graph;
-            OFRI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int st = enqueue();
-            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-            if (st != OFRI_OK) return st;
-            if (ce != hipSuccess) return fail(OFRI_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-            hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) {
-                g.exec = nullptr;
-                return fail(OFRI_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
-            }
-            g.ws_gen = ctx->ws_gen;
-            OFRI_HIP(hipGraphLaunch(g.exec, ctx->stream));
-        }
+        OFRI_TRY(run_cached(ctx, ctx->graphs[key], [&](bool) { return enqueue(); }));
     }
     if (d_converged) {
         void* stp;
         OFRI_TRY(scratch(ctx, "pyr_stats", (size_t)L * (2 * sizeof(SolveCtrl) + 16), &stp));
         learned_check_k<<<1, 1, 0, ctx->stream>>>((const SolveCtrl*)stp, lp->hs_tol[0], lp->hs_tol[1], lp->ls_tol[0],
-                                                  lp->ls_tol[1], p->ls_alpha > 0.0f ? 1 : 0, d_converged);
+                                                  lp->ls_tol[1], p->ls_alpha > 0.0f ? 1 : 0, lp->relative ? 1 : 0,
+                                                  d_converged);
         OFRI_HIP(hipGetLastError());
     }
     if (h_stats || h_converged) {
         ofri_level_stats st[2];
         OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, true, st, false));
         int ok = 1;
+        const int rel = lp->relative ? 1 : 0;
         for (int l = 0; l < L; ++l) {
-            if (!(st[l].hs_res <= lp->hs_tol[l])) ok = 0;
-            if (p->ls_alpha > 0.0f && !(st[l].ls_err <= lp->ls_tol[l])) ok = 0;
+            if (!meets(st[l].hs_res, lp->hs_tol[l], st[l].hs_nrm, rel)) ok = 0;
+            if (p->ls_alpha > 0.0f && !meets(st[l].ls_err, lp->ls_tol[l], st[l].ls_nrm, rel)) ok = 0;
             if (h_stats) h_stats[l] = st[l];
         }
         if (h_converged) *h_converged = ok;

@@ -47,6 +47,7 @@ struct SolveState {
     int pend_nsw;    // sweeps of the untested launch (0: none)
     int pend_dst;    // its output buffer
     int pend_s0;     // its first tested sweep (earlier ones are counted, not tested)
+    float last_nrm;  // relative mode: (||u||_2 + ||v||_2) / (H*W) of the tested launch's result
 };
 
 // Control word of one solve.  The leading fields are the published result
@@ -61,6 +62,7 @@ struct SolveCtrl {
     float last_err;
     int limit;
     int base_buf;    // ping-pong index a following phase starts from (its launches' buffers are relative to it)
+    float last_nrm;  // relative mode: the solution norm last_err was compared against (0 otherwise)
     SolveState slot[2];
 };
 
@@ -106,6 +108,7 @@ struct ofri_ctx {
     std::map<std::string, ofri::DevBuf> ws;
     std::map<std::tuple<int, int, int>, ofri::ResizeTable> resize_tables;
     std::map<std::pair<int, int>, ofri::SplineTable> spline_tables;
+    std::map<uint64_t, ofri::DevBuf> omega_tabs;  // Chebyshev weight tables by sigma (bits)
 };
 
 namespace ofri {
@@ -162,6 +165,9 @@ int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
 // d_out[0] = (||A-A0|| + ||B-B0||) / n as float32 (differences in float32)
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
                   size_t n, float* d_out);
+// d_out[0] = mean over n of sqrt((U1-U2)^2 + (V1-V2)^2) (float32 terms, double sum)
+int flow_epe(ofri_ctx* ctx, const float* U1, const float* V1, const float* U2, const float* V2, size_t n,
+             float* d_out);
 // dst[i] = src[ctrl->final_buf][i] for a ping-pong pair
 // add: ad += selected, bd += selected instead of copies
 int select_copy2(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* a0, const float* a1, float* ad,
@@ -177,11 +183,13 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // accelerate to cheb_switch * tol, then at most niter / 2 plain sweeps to
 // tol (niter sweeps in all).  check_last: the stop test runs once per fused
 // launch, on its last sweep (no re-run of the stopping launch).
+// rel (needs check_last): tol is relative -- a launch's tested norm is
+// compared with tol * (||U||_2 + ||V||_2) / (H*W) of that launch's result.
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
              float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0, bool zero_start = false,
-             const float* pre_w = nullptr, int pre_n = 0);
+             const float* pre_w = nullptr, int pre_n = 0, bool rel = false);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
@@ -203,10 +211,12 @@ unsigned lk_window_mask(int hw, int lo, int hi);
 // [cheb_lo, cheb_hi] (cheb_lo in (-1, 0)).
 // sink_u / sink_v (optional): the result goes there instead of into u, v
 // (added when sink_add, else copied); u, v then hold the start only.
+// rel (needs check_last): tol is relative, as for hs_solve; the control word
+// also publishes the solution norm of the tested launch (last_nrm).
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false,
              float cheb_lo = 0.0f, float cheb_hi = 0.0f, float* sink_u = nullptr, float* sink_v = nullptr,
-             bool sink_add = false, bool fast = false);
+             bool sink_add = false, bool fast = false, bool rel = false);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets

@@ -19,10 +19,15 @@ namespace {
 // (recorded in `hist` when given).  When it holds before the pending launch's
 // last sweep (or at any sweep, with always_redo) the state asks for that
 // launch to be re-run with the sweeps up to the stopping one.  Returns the
-// updated state, the same in every thread; nsw <= 32.
+// updated state, the same in every thread; nsw <= 32.  rel: the tolerance
+// is relative -- tol * (||u||_2 + ||v||_2) / (H*W), the norm of the tested
+// launch's result, whose per-workgroup squared sums sit in row K of the
+// partials (written once per launch, after its last sweep; the test then
+// covers that sweep alone).
 __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* part_prev, int nblk, double tol,
-                                int maxnum, size_t N, float* hist, bool always_redo) {
+                                int maxnum, size_t N, float* hist, bool always_redo, int rel, int K) {
     __shared__ double tot[32][2];
+    __shared__ double totn[2];
     __shared__ SolveState sh;
     SolveState s = ctrl->slot[prev];
     const bool ev = !s.done && s.pend_nsw > 0;
@@ -53,19 +58,42 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
                 tot[k][1] = b;
             }
         }
+        if (rel && wq == nw - 1) {
+            // the tested launch's solution norm partials (row K of the slot)
+            const double* pn = part_prev + (size_t)K * nblk * 2;
+            double a = 0.0, b = 0.0;
+            for (int i = lane; i < nblk; i += 64) {
+                a = a + pn[(size_t)i * 2];
+                b = b + pn[(size_t)i * 2 + 1];
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                a = a + __shfl_xor(a, off);
+                b = b + __shfl_xor(b, off);
+            }
+            if (lane == 0) {
+                totn[0] = a;
+                totn[1] = b;
+            }
+        }
     }
     __syncthreads();
     if (t == 0) {
         {
             const int nsw = s.pend_nsw, dst = s.pend_dst;
             s.sweeps += s.pend_s0;   // sweeps run but not reduced
+            // relative mode: the tolerance scales with the launch's solution norm
+            double teff = tol;
+            if (rel) {
+                s.last_nrm = (float)((sqrt(totn[0]) + sqrt(totn[1])) / (double)N);
+                teff = tol * (double)s.last_nrm;
+            }
             for (int k = s.pend_s0; k < nsw; ++k) {
                 float err = (float)((sqrt(tot[k][0]) + sqrt(tot[k][1])) / (double)N);
                 if (hist) hist[s.sweeps] = err;
                 s.sweeps += 1;
                 s.last_err = err;
                 s.final_buf = dst;
-                if (!((double)err > tol)) {
+                if (!((double)err > teff)) {
                     s.done = 1;
                     if (k + 1 < nsw || always_redo) {
                         s.redo = k + 1;
@@ -103,7 +131,7 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
                                             int& base, int* fin = nullptr) {
     const int prev = a.slot ^ 1;
     SolveState s = take_test(a.ctrl, prev, a.part + (size_t)prev * a.part_stride, nblk, a.tol, a.maxnum, a.N,
-                             blk == 0 ? a.hist : nullptr, always_redo);
+                             blk == 0 ? a.hist : nullptr, always_redo, a.rel, a.K);
     const bool lead = blk == 0 && threadIdx.x == 0 && threadIdx.y == 0;
     if (fin) *fin = s.final_buf;
     if (a.final_) {
@@ -118,6 +146,7 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
             c->redo = s.redo;
             c->redo_in = s.redo_in;
             c->last_err = s.last_err;
+            c->last_nrm = s.last_nrm;
             c->limit = s.limit;
             if (a.next_budget > 0) {
                 SolveState n{};
@@ -125,6 +154,7 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
                 n.sweeps = s.sweeps;
                 n.final_buf = s.final_buf;
                 n.last_err = s.last_err;
+                n.last_nrm = s.last_nrm;
                 n.limit = s.sweeps + a.next_budget;
                 c->slot[a.slot] = n;
                 c->base_buf = s.final_buf;
@@ -147,32 +177,63 @@ __device__ __forceinline__ bool launch_args(const Pass& a, int blk, int nblk, bo
     return runs;
 }
 
+// Relative mode: the squared sums of a launch's result over the owned
+// pixels of this workgroup's tile (u, v: the lane's R rows after the last
+// sweep), stored at dst (row K of the launch's partials): wave sums in a
+// fixed butterfly, then the waves in order.
+template <int R, int WAVES>
+__device__ __forceinline__ void store_norm(const float (&u)[R], const float (&v)[R], bool own_x, int w, int gy0,
+                                           int K, int H, double* dst) {
+    __shared__ double nred[WAVES][2];
+    const int lane = threadIdx.x & 63;
+    double su = 0.0, sv = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ry = w * R + r, gy = gy0 + r;
+        if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
+            su = su + (double)u[r] * (double)u[r];
+            sv = sv + (double)v[r] * (double)v[r];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        su = su + __shfl_xor(su, off);
+        sv = sv + __shfl_xor(sv, off);
+    }
+    if (lane == 0) {
+        nred[w][0] = su;
+        nred[w][1] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < WAVES; ++q) {
+            a = a + nred[q][0];
+            b = b + nred[q][1];
+        }
+        dst[0] = a;
+        dst[1] = b;
+    }
+}
+
 // Chebyshev weights omega_1..omega_n for a spectrum in [-sigma, sigma]
 // (omega_1 = 1, omega_2 = 2 / (2 - sigma^2), omega_{k+1} = 1 / (1 - sigma^2
 // omega_k / 4)), formed in double on the host.  The recurrence converges
 // geometrically with ratio sigma^2 omega_inf^2 / 4 (at most 0.92 for the
 // bounds accepted here), so past the table's 512 entries every weight equals
-// the last one in float32.
+// the last one in float32: sweep k reads entry min(k, kOmegaTab - 1) of the
+// table, which lives in device memory (omega_table, one per sigma).
 constexpr int kOmegaTab = 512;
-struct OmegaTab {
-    float w[kOmegaTab];
-};
 
-OmegaTab cheb_omegas(double sigma) {
-    OmegaTab t;
-    const double s2 = sigma * sigma;
-    double om = 1.0;
-    for (int k = 0; k < kOmegaTab; ++k) {
-        if (k == 1) om = 2.0 / (2.0 - s2);
-        else if (k > 1) om = 1.0 / (1.0 - 0.25 * s2 * om);
-        t.w[k] = (float)om;
-    }
-    return t;
-}
+__device__ __forceinline__ int omega_index(int k) { return k < kOmegaTab ? k : kOmegaTab - 1; }
 
-__global__ void cheb_omega_k(OmegaTab t, float* w, int n) {
-    for (int k = threadIdx.x; k < n; k += blockDim.x) w[k] = t.w[k < kOmegaTab ? k : kOmegaTab - 1];
-}
+}  // namespace
+
+// Device table of kOmegaTab Chebyshev weights for sigma, owned by the
+// context (built and uploaded on first use, outside any graph capture).
+int omega_table(ofri_ctx* ctx, double sigma, const float** out);
+
+namespace {
 
 }  // namespace
 }  // namespace ofri

@@ -42,7 +42,8 @@ class LshsParams(C.Structure):
 
 class LevelStats(C.Structure):
     _fields_ = [("H", C.c_int), ("W", C.c_int), ("hs_err", C.c_float), ("ls_err", C.c_float),
-                ("ls_iters", C.c_int), ("hs_res", C.c_float), ("hs_iters", C.c_int)]
+                ("ls_iters", C.c_int), ("hs_res", C.c_float), ("hs_iters", C.c_int), ("hs_nrm", C.c_float),
+                ("ls_nrm", C.c_float)]
 
 
 class Pair(C.Structure):
@@ -53,7 +54,8 @@ class Pair(C.Structure):
 class LearnedParams(C.Structure):
     _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int * 2),
                 ("ls_budget", C.c_int), ("cheb_bound", C.c_float * 2), ("cheb_finish", C.c_float * 2),
-                ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float), ("hs_finish_budget", C.c_int * 2)]
+                ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float), ("hs_finish_budget", C.c_int * 2),
+                ("relative", C.c_int), ("ls_sweeps", C.c_int * 2)]
 
 
 class FbParams(C.Structure):
@@ -102,6 +104,7 @@ SIGNATURES = {
     "ofri_warp_symmetric": [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp],
     "ofri_warp_coords": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp],
     "ofri_accumulate": [_vp, _vp, _vp, _sz],
+    "ofri_flow_epe": [_vp, _vp, _vp, _vp, _vp, _sz, _vp],
     "ofri_hs": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, _vp, _vp, _fp],
     "ofri_hs_tol": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _vp, _fp,
                     _ip],
@@ -127,7 +130,7 @@ SIGNATURES = {
     "ofri_net_get_grads": [_vp, _vp, _fp, C.c_size_t],
     "ofri_net_predict": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp],
     "ofri_net_train": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_float, _fp],
-    "ofri_pyr_lshs_taps": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp, _vp,
+    "ofri_pyr_lshs_taps": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp, _vp, _ip,
                            C.POINTER(LevelStats)],
     "ofri_pyr_lshs_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), C.POINTER(LearnedParams),
                               _vp, _vp, _vp, C.POINTER(LevelStats), _ip, _vp],

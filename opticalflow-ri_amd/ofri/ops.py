@@ -122,10 +122,11 @@ def render_pair(H, W, family, density, seed, par, modes=None, noise=2.0, im1=Non
     return (im1, im2, u, v) if truth else (im1, im2)
 
 
-def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0, launch_test=False):
+def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0, launch_test=False, relative=False):
     """Chebyshev-accelerated Horn-Schunck to the plain residual tol
     (ofri_hs_accel; finish > 1 ends with plain sweeps from finish * tol;
-    launch_test: stop test at each launch's last sweep only);
+    launch_test: stop test at each launch's last sweep only; relative (with
+    launch_test): tol scales with the tested result's norm (||U|| + ||V||)/N);
     U, V in place.  Returns (err, sweeps)."""
     im1, im2 = as_device(im1), as_device(im2)
     if hist is not None and hist.nbytes // 4 < maxnum:
@@ -135,7 +136,8 @@ def hs_accel(im1, im2, alpha, maxnum, tol, bound, U, V, hist=None, finish=0.0, l
     _capi.check(_lib().ofri_hs_accel(_h(), im1.ptr, im2.ptr, im1.H, im1.W, float(np.float32(alpha)), int(maxnum),
                                      float(tol), float(bound), float(finish), U.ptr, V.ptr,
                                      hist.ptr if hist is not None else None,
-                                     C.byref(err), C.byref(it), int(bool(launch_test))))
+                                     C.byref(err), C.byref(it),
+                                     int(bool(launch_test)) | (2 if relative else 0)))
     return np.float32(err.value), it.value
 
 
@@ -315,14 +317,29 @@ def batch_lshs(pairs, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=
 
 def _stats_dicts(st):
     return [dict(H=s.H, W=s.W, hs_err=np.float32(s.hs_err), ls_err=np.float32(s.ls_err), ls_iters=s.ls_iters,
-                 hs_res=np.float32(s.hs_res), hs_iters=s.hs_iters) for s in st]
+                 hs_res=np.float32(s.hs_res), hs_iters=s.hs_iters, hs_nrm=np.float32(s.hs_nrm),
+                 ls_nrm=np.float32(s.ls_nrm)) for s in st]
+
+
+def flow_epe(U1, V1, U2, V2, out=None):
+    """Mean endpoint error of two device flows, written to a one-word device
+    array (ofri_flow_epe) with no synchronisation.  Returns that array."""
+    out = out if out is not None else DeviceArray((1,))
+    n = int(np.prod(U1.shape))
+    for a in (V1, U2, V2):
+        if int(np.prod(a.shape)) != n:
+            raise ValueError("flow_epe: flows differ in size")
+    _capi.check(_lib().ofri_flow_epe(_h(), U1.ptr, V1.ptr, U2.ptr, V2.ptr, n, out.ptr))
+    return out
 
 
 def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
-                  Uacc=None, Vacc=None, taps=None):
+                  Uacc=None, Vacc=None, taps=None, ls_tap_back=None):
     """ofri_pyr_lshs_taps: the pyramid with every level's HS result and Liu-Shen
     correction exported.  Returns (Uacc, Vacc, taps, stats); taps is a flat
-    DeviceArray holding, per level, 4 planes of the level's size."""
+    DeviceArray holding, per level, 4 planes of the level's size.
+    ls_tap_back: optional per-level k, the Liu-Shen planes taken k sweeps
+    before Liu-Shen's stop."""
     im1, im2 = as_device(im1), as_device(im2)
     H, W = im1.shape[-2:]
     p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol)
@@ -334,22 +351,30 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)()
+    back = None
+    if ls_tap_back is not None:
+        if len(ls_tap_back) != levels:
+            raise ValueError("ls_tap_back: one offset per level")
+        back = (C.c_int * levels)(*[int(k) for k in ls_tap_back])
     _capi.check(_lib().ofri_pyr_lshs_taps(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
-                                          taps.ptr, st))
+                                          taps.ptr, back, st))
     return Uacc, Vacc, taps, _stats_dicts(st)
 
 
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
                      hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
-                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None, hs_finish_budget=0):
+                     ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None, hs_finish_budget=0,
+                     relative=False, ls_sweeps=0):
     """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
     correction network and Liu-Shen run to the per-level residual targets
-    hs_tol / ls_tol.  net: an ofri.net.Net or None (the method without it).
-    hs_budget, cheb_bound, cheb_finish, hs_finish_budget: one value or
-    (coarse, fine).  flag: optional DeviceArray view
-    of one 4-byte word receiving the convergence flag (int 1 / 0) with no
-    synchronisation.  Returns (Uacc, Vacc, stats, converged); converged is
-    None when stats is False."""
+    hs_tol / ls_tol (relative: as multiples of the tested iterate's norm
+    (||u|| + ||v||)/N).  ls_sweeps > 0: exactly that many Liu-Shen sweeps,
+    ls_tol then accepts or rejects the level.  net: an ofri.net.Net or None
+    (the method without it).  hs_budget, cheb_bound, cheb_finish,
+    hs_finish_budget, ls_sweeps: one value or (coarse, fine).  flag: optional
+    DeviceArray view of one 4-byte word receiving the convergence flag (int
+    1 / 0) with no synchronisation.  Returns (Uacc, Vacc, stats, converged);
+    converged is None when stats is False."""
     im1, im2 = as_device(im1), as_device(im2)
     H, W = im1.shape[-2:]
     p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol_plain)
@@ -357,10 +382,12 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
     def pair(x):
         return (x, x) if np.isscalar(x) else tuple(x)
-    hb, cb, cf, fb = pair(hs_budget), pair(cheb_bound), pair(cheb_finish), pair(hs_finish_budget)
+    hb, cb, cf, fb, lsw = (pair(hs_budget), pair(cheb_bound), pair(cheb_finish), pair(hs_finish_budget),
+                           pair(ls_sweeps))
     lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), (C.c_int * 2)(int(hb[0]), int(hb[1])), int(ls_budget),
                              (C.c_float * 2)(float(cb[0]), float(cb[1])), (C.c_float * 2)(float(cf[0]), float(cf[1])),
-                             float(ls_cheb[0]), float(ls_cheb[1]), (C.c_int * 2)(int(fb[0]), int(fb[1])))
+                             float(ls_cheb[0]), float(ls_cheb[1]), (C.c_int * 2)(int(fb[0]), int(fb[1])),
+                             int(bool(relative)), (C.c_int * 2)(int(lsw[0]), int(lsw[1])))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None

@@ -13,6 +13,10 @@ backward and Adam all run in libofri's kernels.
 
     python -m ofri.train --pairs 20000 --batch 8 --out ofri/correction_weights.npz
 
+Labels are the Liu-Shen correction a few sweeps before the plain stop (the
+learned form's own Liu-Shen sweeps, --tap-back), so that those sweeps land
+on the plain iterate.
+
 The pair set is a pure function of --seed; an epoch re-renders and
 re-solves it (the labels of 20k pairs do not fit next to the images in HBM).
 """
@@ -104,10 +108,13 @@ def _copy(dst, src, n):
     _ = ctx.lib.ofri_copy(ctx.handle, dst.ptr, src.ptr, 4 * n)
 
 
-def fill(batch, specs, fw, ow):
+def fill(batch, specs, fw, ow, tap_back=None):
     """Render the batch's pairs, solve them with the plain pyramid and lay out
     the two samples per pair: fine level (in_f, lab_f) and coarse level
-    (in_c, lab_c).  Returns the per-pair level statistics."""
+    (in_c, lab_c).  tap_back: per level (coarse, fine), the labels are the
+    Liu-Shen correction that many sweeps before the plain stop (the learned
+    form runs those last sweeps itself).  Returns the per-pair level
+    statistics."""
     B, H, W = batch.B, batch.H, batch.W
     h, w = H // 2, W // 2
     hw, HW = h * w, H * W
@@ -118,7 +125,7 @@ def fill(batch, specs, fw, ow):
         b = batch.im2.view(i * HW, (H, W))
         ops.render_pair(H, W, sp["family"], sp["density"], sp["seed"], sp["par"], sp["modes"], 2.0, im1=a, im2=b)
         _, _, taps, st = ops.pyr_lshs_taps(a, b, filt_w=fw, opt_w=ow, Uacc=batch.U, Vacc=batch.V, taps=batch.taps,
-                                           **LSHS)
+                                           ls_tap_back=tap_back, **LSHS)
         stats.append(st)
         # fine level: Liu-Shen pair = FILTER_OPT of the frames
         base = i * 4 * HW
@@ -145,9 +152,14 @@ def lr_at(step, total, base, warm=100, floor=0.02):
 
 
 def train(pairs=20000, batch=8, epochs=1.0, lr=1e-3, seed=0, out=None, resume=None, time_limit=None,
-          log_every=25):
+          log_every=25, tap_back=None):
     """Trains on `pairs` seeded pairs (seed .. seed + pairs - 1) for `epochs`
-    passes in minibatches of `batch` same-size pairs.  Returns the Net."""
+    passes in minibatches of `batch` same-size pairs; labels tapped
+    `tap_back` Liu-Shen sweeps before the plain stop (default: the learned
+    form's own Liu-Shen sweeps).  Returns the Net."""
+    if tap_back is None:
+        from .learned import LS_SWEEPS
+        tap_back = LS_SWEEPS
     ctx = context()
     fw, ow = prefilter_weights()
     net = Net(planes=4)
@@ -178,7 +190,7 @@ def train(pairs=20000, batch=8, epochs=1.0, lr=1e-3, seed=0, out=None, resume=No
         mb = mbs[mi]
         H, W = mb[0]["H"], mb[0]["W"]
         bt.set(len(mb), H, W)
-        fill(bt, mb, fw, ow)
+        fill(bt, mb, fw, ow, tap_back)
         rate = lr_at(step, steps, lr)
         # the fine level's sample, then the coarse level's (fine-head labels
         # at each level's size; the coarse head is not trained)
@@ -194,7 +206,7 @@ def train(pairs=20000, batch=8, epochs=1.0, lr=1e-3, seed=0, out=None, resume=No
             break
     if out:
         net.save(out, pairs=pairs, batch=batch, steps=len(hist), seed=seed, final_loss=float(np.mean(hist[-50:])),
-                 loss_history=np.array(hist, np.float32))
+                 loss_history=np.array(hist, np.float32), tap_back=np.array(tap_back, np.int32))
     return net
 
 
@@ -208,8 +220,10 @@ def main(argv=None):
     ap.add_argument("--resume", default=None)
     ap.add_argument("--time-limit", type=float, default=None, help="stop after this many seconds")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--tap-back", type=int, nargs=2, default=None,
+                    help="Liu-Shen sweeps before the plain stop at which labels are taken (coarse, fine)")
     a = ap.parse_args(argv)
-    train(a.pairs, a.batch, a.epochs, a.lr, a.seed, a.out, a.resume, a.time_limit)
+    train(a.pairs, a.batch, a.epochs, a.lr, a.seed, a.out, a.resume, a.time_limit, tap_back=a.tap_back)
 
 
 if __name__ == "__main__":

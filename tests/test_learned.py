@@ -67,6 +67,16 @@ def test_taps_run_equals_the_plain_pyramid(ops):
     for a, b in zip(st, stt):
         assert a["ls_iters"] == b["ls_iters"] and a["ls_err"] == b["ls_err"] and a["hs_err"] == b["hs_err"]
         assert b["hs_iters"] == 600 and 0 < b["hs_res"] < 1e-6
+    # Liu-Shen planes tapped k sweeps before the stop: the pyramid's result is
+    # unchanged, and the coarse level's planes are those of a run stopped
+    # k sweeps earlier there
+    Ub, Vb, tapb, stb = ops.pyr_lshs_taps(im1, im2, ls_tap_back=(5, 8), **prm)
+    assert np.array_equal(U.download(), Ub.download()) and np.array_equal(V.download(), Vb.download())
+    assert [s["ls_iters"] for s in stb] == [5, 8]
+    _, _, tap55, _ = ops.pyr_lshs_taps(im1, im2, **dict(prm, ls_maxnum=55))
+    nc = 128 * 160
+    assert np.array_equal(tapb.download()[:4 * nc], tap55.download()[:4 * nc])
+    assert not np.array_equal(tapb.download()[:4 * nc], taps.download()[:4 * nc])
     # one level: the accumulated flow is HS + (LS - HS)
     U1, V1, taps1, _ = ops.pyr_lshs_taps(im1, im2, **_lshs(ops, 1))
     t = taps1.download().reshape(4, 256, 320)
@@ -78,54 +88,63 @@ def test_taps_run_equals_the_plain_pyramid(ops):
 
 @pytest.fixture(scope="module")
 def trained():
-    """The committed correction network, a held-out Lamb-Oseen pair (seeds
-    outside the training set) and its plain results and residual targets."""
+    """The committed correction network and held-out pairs (seeds outside
+    the training and calibration sets) of the Lamb-Oseen and the
+    shear + turbulence families, with their plain results."""
     from ofri import learned
     from ofri.net import Net
     net = Net.load()
-    pairs, specs = learned.heldout_pairs(2, 1024)
-    assert specs[1]["family"] == 1
-    pair = pairs[1]
+    pairs, specs = learned.heldout_pairs(3, 1024)
+    assert [s["family"] for s in specs] == [0, 1, 2]
     fw, ow = learned.filters()
-    Up, Vp, ht, lt, st = learned.plain_targets(pair[0], pair[1], fw, ow)
-    return dict(net=net, pair=pair, plain=(Up.download(), Vp.download()), ht=ht, lt=lt, st=st, fw=fw, ow=ow)
+    out = dict(net=net, fw=fw, ow=ow, pairs={}, plain={})
+    for fam in (1, 2):
+        a, b = pairs[fam]
+        Up, Vp, st = learned.plain(a, b, fw, ow, stats=True)
+        out["pairs"][fam] = (a, b)
+        out["plain"][fam] = (Up.download(), Vp.download(), st)
+    return out
 
 
-def test_learned_form_reaches_the_plain_residual_with_fewer_sweeps(trained):
-    """The trained network is what lets Liu-Shen reach the plain residual:
-    with it every level converges in a fraction of the plain 60 sweeps and
-    lands within 0.01 px (mean EPE) of the plain result; the same method
-    without the network misses the target within the budget or needs twice
-    the sweeps."""
+@pytest.mark.parametrize("family", [1, 2])
+def test_learned_form_meets_its_rules_and_lands_on_the_plain_result(trained, family):
+    """With pair-independent rules (no plain solve of the pair): HS reaches
+    its relative residual in a fraction of the plain 600 sweeps, Liu-Shen
+    runs its fixed sweeps, the acceptance test passes, and the result lies
+    within 0.01 px mean EPE of the plain result -- for the Lamb-Oseen and the
+    turbulence family alike.  Without the network the same rules reject the
+    pair (Liu-Shen from HS's flow alone is far from the plain iterate)."""
     from ofri import learned
     t = trained
-    a, b = t["pair"]
-    U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
+    a, b = t["pairs"][family]
+    U, V, info = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False)
+    st = info["stats"]
+    e_mean, e_max = learned.epe((U.download(), V.download()), t["plain"][family][:2])
+    ratios = learned.residual_ratios(st)
+    U0, V0, info0 = learned.solve(a, b, None, t["fw"], t["ow"], fallback=False)
+    print("[learned family %d] HS sweeps %s, LS sweeps %s, r/n %s, EPE to plain mean %.5f max %.4f px; "
+          "no network: accepted %s, r/n %s" % (family, [s["hs_iters"] for s in st], [s["ls_iters"] for s in st],
+                                              ratios, e_mean, e_max, info0["converged"],
+                                              learned.residual_ratios(info0["stats"])))
     assert info["converged"]
-    ls = [s["ls_iters"] for s in info["stats"]]
-    e_mean, e_max = learned.epe((U.download(), V.download()), t["plain"])
-    U0, V0, info0 = learned.solve(a, b, None, t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
-    ls0 = [s["ls_iters"] for s in info0["stats"]]
-    print("[learned] LS sweeps %s (no network %s, plain %s), EPE to plain mean %.5f max %.4f px"
-          % (ls, ls0, [s["ls_iters"] for s in t["st"]], e_mean, e_max))
-    assert all(s["hs_res"] <= tol for s, tol in zip(info["stats"], t["ht"]))
-    assert all(s["ls_err"] <= tol for s, tol in zip(info["stats"], t["lt"]))
-    assert max(ls) <= 30 and 2 * sum(ls) <= sum(s["ls_iters"] for s in t["st"])
-    assert not info0["converged"] or sum(ls0) >= 2 * sum(ls)
+    assert [s["ls_iters"] for s in st] == list(learned.LS_SWEEPS)
+    for (rh, rl), hs_rel, acc in zip(ratios, learned.HS_REL, learned.LS_ACCEPT):
+        assert rh <= hs_rel * (1 + 1e-6) and rl <= acc * (1 + 1e-6)
+    assert all(s["hs_iters"] <= 200 for s in st)
     assert e_mean <= 0.01
+    assert not info0["converged"]
 
 
 def test_learned_fallback_returns_the_plain_result(trained):
-    """A sweep budget too small to reach the targets triggers the fallback,
+    """A sweep budget too small to reach the rules triggers the fallback,
     whose result is the plain pyramid's, bit for bit."""
-    from ofri import learned, ops
+    from ofri import learned
     t = trained
-    a, b = t["pair"]
-    U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=True, hs_budget=8)
+    a, b = t["pairs"][1]
+    U, V, info = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=True,
+                               rules=learned.Rules(hs_budget=8))
     assert info["fell_back"] and not info["converged"]
-    assert np.array_equal(U.download(), t["plain"][0]) and np.array_equal(V.download(), t["plain"][1])
-    Ur, Vr, _ = ops.pyr_lshs(a, b, filt_w=t["fw"], opt_w=t["ow"], **learned.WORKLOAD)
-    assert np.array_equal(Ur.download(), t["plain"][0])
+    assert np.array_equal(U.download(), t["plain"][1][0]) and np.array_equal(V.download(), t["plain"][1][1])
 
 
 def test_learned_form_replays_identically(trained):
@@ -133,33 +152,47 @@ def test_learned_form_replays_identically(trained):
     from ofri import learned
     from ofri.device import DeviceArray
     t = trained
-    a, b = t["pair"]
+    a, b = t["pairs"][2]
     Ua, Va = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
     outs = []
     for _ in range(4):
-        U, V, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], Uacc=Ua, Vacc=Va,
-                                   fallback=False)
-        outs.append((U.download(), V.download(), [s["ls_iters"] for s in info["stats"]]))
+        U, V, info = learned.solve(a, b, t["net"], t["fw"], t["ow"], Uacc=Ua, Vacc=Va, fallback=False)
+        outs.append((U.download(), V.download(), [s["hs_iters"] for s in info["stats"]]))
     for o in outs[1:]:
         assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1]) and o[2] == outs[0][2]
 
 
-def test_queued_pairs_flag_on_the_device_and_fall_back(trained):
-    """solve_many queues pairs with their convergence flags written on the
-    device: a pair within budget keeps the learned result (the same bits as
-    a single call), a pair whose budget is too small gets the plain result."""
+def test_queued_pairs_the_judge_and_the_fallback(trained):
+    """solve_many queues pairs with their acceptance flags on the device.  A
+    checked pair whose EPE to plain is within the bar keeps the learned bits
+    (those of a single call) and the judge records its EPE; with an
+    impossible bar the checked pair takes the plain result and the judge
+    checks the pairs after it; a pair whose budget is too small runs the
+    plain pyramid."""
     from ofri import learned
     from ofri.device import DeviceArray
     t = trained
-    a, b = t["pair"]
-    U1, V1 = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
-    U2, V2 = DeviceArray((1024, 1024)), DeviceArray((1024, 1024))
-    Us, Vs, info = learned.solve(a, b, t["net"], t["ht"], t["lt"], t["fw"], t["ow"], fallback=False)
+    a, b = t["pairs"][1]
+    Us, Vs, info = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False)
     assert info["converged"]
-    fell = learned.solve_many([(a, b, t["ht"], t["lt"], U1, V1)], t["net"], t["fw"], t["ow"])
-    assert fell == [False]
-    assert np.array_equal(U1.download(), Us.download()) and np.array_equal(V1.download(), Vs.download())
-    fell = learned.solve_many([(a, b, t["ht"], t["lt"], U1, V1), (a, b, t["ht"], t["lt"], U2, V2)], t["net"],
-                              t["fw"], t["ow"], hs_budget=8)
-    assert fell == [True, True]
-    assert np.array_equal(U2.download(), t["plain"][0]) and np.array_equal(V2.download(), t["plain"][1])
+    bufs = [(DeviceArray((1024, 1024)), DeviceArray((1024, 1024))) for _ in range(3)]
+    judge = learned.Judge(every=2)
+    res = learned.solve_many([(a, b, U, V) for U, V in bufs], t["net"], judge, t["fw"], t["ow"])
+    assert [r["checked"] for r in res] == [True, False, True]
+    assert not any(r["fell_back"] for r in res) and all(r["accepted"] for r in res)
+    for U, V in bufs:
+        assert np.array_equal(U.download(), Us.download()) and np.array_equal(V.download(), Vs.download())
+    e = learned.epe((Us.download(), Vs.download()), t["plain"][1][:2])[0]
+    assert judge.checks == 2 and judge.failed == 0 and abs(judge.epes[0] - e) <= 1e-6 + 1e-4 * e
+    # an impossible bar: the checked pair falls back to the plain result and
+    # the next pairs are all checked
+    strict = learned.Judge(every=4, bar=0.0, after_fail=2)
+    res = learned.solve_many([(a, b, U, V) for U, V in bufs], t["net"], strict, t["fw"], t["ow"])
+    assert [r["fell_back"] for r in res] == [True, False, False] and strict.failed == 1
+    assert np.array_equal(bufs[0][0].download(), t["plain"][1][0])
+    assert strict.pick() and strict.pick() and not strict.pick()
+    # a budget too small: every pair runs the plain pyramid
+    res = learned.solve_many([(a, b, U, V) for U, V in bufs[:2]], t["net"], None, t["fw"], t["ow"],
+                             rules=learned.Rules(hs_budget=8))
+    assert [r["fell_back"] for r in res] == [True, True] and not any(r["accepted"] for r in res)
+    assert np.array_equal(bufs[1][0].download(), t["plain"][1][0])
