@@ -298,6 +298,48 @@ typedef struct ofri_pair {
 int ofri_batch_lshs(ofri_ctx* ctx, const ofri_pair* pairs, int n, const ofri_lshs_params* p,
                     ofri_level_stats* h_stats);
 
+/* ---- generic pyramid with a Farneback or dense-LK main solver ------------ */
+typedef struct ofri_pyr_params {
+    int levels;            /* pyramidalLevels */
+    const float* filt_w;   /* FILTER kernel (3 taps, float32) on the (warped) pair; NULL = none */
+    int filt_n;
+    const float* opt_w;    /* FILTER_OPT kernel (5 taps) on the un-warped pair; NULL = unfiltered */
+    int opt_n;
+    int warping;           /* 1: symmetric bilinear warp by the upsampled flow; 0: no warping */
+    int inter_scaling;     /* pyramidalIntermediateScaling (levels before the last) */
+    int final_scaling;     /* pyramidalScaling (the last level) */
+    float ls_alpha;        /* optional Liu-Shen refinement: h; <= 0 = none */
+    int ls_maxnum;         /* 60 in the reference */
+    double ls_tol;         /* 1e-8 in the reference */
+} ofri_pyr_params;
+
+/* The main solver of ofri_pyr_generic. */
+typedef struct ofri_pyr_main {
+    int kind;                       /* 0: Farneback (one internal level, `fb`); 1: dense LK */
+    const ofri_fb_params* fb;       /* kind 0: flow_scale must be 1 */
+    int lk_half_win, lk_niter;      /* kind 1: halfWindow, Niter (no vorticity enhancement) */
+    unsigned char* d_lk_status;     /* kind 1 (may be NULL): the last level's status map, H x W */
+} ofri_pyr_main;
+
+/* The generic coarse-to-fine driver on the device with a Farneback
+ * (Farneback_PyCL with pyramidalLevels = 1) or dense-LK
+ * (denseLucasKanade_PyCl without vorticity enhancement) main adapter and
+ * an optional Liu-Shen adapter: bicubic downsample, spline upsample with
+ * the level's scaling flag, symmetric bilinear warp or no warping, FILTER
+ * on the (warped) pair, main solver, FILTER_OPT on the un-warped pair,
+ * Liu-Shen seeded with the main solver's flow, accumulation; kLevels = 1.
+ * The caller resolves the adapter defaults (warping, scaling) first.
+ * h_stats (may be NULL): per level, Liu-Shen's last norm and sweeps.
+ * Repeated calls replay a HIP graph.  Replaces genericPyramidalOpticalFlow
+ * (GenericPyramidalOpticalFlow.py:237-415) as driven by
+ * examples/LiuSE_Farneback_Fs0_0_PyrLvls2.py:51-76 and
+ * examples/denseLK_Fs2_0_PyrLvls2.py:49-73, with every adapter call
+ * (Farneback_PyCL.py:461-603, denseLucasKanade_PyCL.py:113-169,
+ * PhysicsBasedOpticalFlowLiuShen.py:37-39) on device buffers. */
+int ofri_pyr_generic(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                     const ofri_pyr_params* p, const ofri_pyr_main* m, float* d_Uacc, float* d_Vacc,
+                     ofri_level_stats* h_stats);
+
 /* ---- Synthetic PIV pairs (training data) --------------------------------- */
 /* Renders a particle-image pair on the device: particles at `density` per
  * pixel (8x8-pixel cells of 16 slots), d ~ N(2.5, 0.5) in [1, 4], I0 ~

@@ -10,8 +10,13 @@ warping, velocity scaling and kLevels = 1 -- the configuration of
 examples/LiuSE_PyHSchunck_Fs3_4_PyrLvls2.py -- the whole pyramid runs as one
 device-resident libofri call (ofri_pyr_lshs): the adapter's alpha list is
 consumed exactly as the reference consumes it and the same numbers come out.
+This package's Farneback (one internal level) and dense-LK (no vorticity
+enhancement) adapters, with the Liu-Shen or no optional adapter, bilinear or
+no warping and kLevels = 1 -- the configurations of
+examples/LiuSE_Farneback_Fs0_0_PyrLvls2.py and examples/denseLK_Fs2_0_PyrLvls2.py
+-- run as one device-resident call too (ofri_pyr_generic).
 Every other combination (any adapter object following the reference's
-adapter protocol, kLevels > 1, no warping, Liu-Shen warping) goes through the
+adapter protocol, kLevels > 1, Liu-Shen warping) goes through the
 level loop below,
 which calls adapter.compute() with numpy arrays as the reference does and
 runs resize / upsample / warp / pre-filter on the GPU.
@@ -139,6 +144,10 @@ def genericPyramidalOpticalFlow(im1, im2, FILTER, mainOFlowAlgoAdapter, pyramida
             (pyramidalLevels == 1 or (pyramidalIntermediateScaling and pyramidalScaling)) and
             len(main.alphas) >= pyramidalLevels and min(min(s) for s in shapes) >= 4):
         return _lshs_device(im1, im2, FILTER, main, pyramidalLevels, FILTER_OPT, optional)
+    dev = _device_main(main, optional)
+    if dev is not None and kLevels == 1 and (biLinear or not warping) and min(min(s) for s in shapes) >= 4:
+        return _generic_device(im1, im2, FILTER, main, dev, pyramidalLevels, FILTER_OPT, optional, warping,
+                               pyramidalIntermediateScaling, pyramidalScaling)
     return _generic_levels(im1, im2, FILTER, main, pyramidalLevels, kLevels, FILTER_OPT, optional, warping,
                            biLinear, pyramidalIntermediateScaling, pyramidalScaling, shapes)
 
@@ -156,6 +165,51 @@ def _lshs_device(im1, im2, FILTER, hs, levels, FILTER_OPT, ls):
         ops.log(hs.getAlgoName() + ' estimated error for image registration: ' + str(st['hs_err']))
         if ls is not None:
             ops.log(ls.getAlgoName() + ' estimated error for image registration: ' + str(st['ls_err']))
+    return Ud.download(), Vd.download()
+
+
+def _device_main(main, optional):
+    """('fb', constants) for this package's Farneback adapter with one internal
+    level, ('lk', (halfWindow, Niter)) for its dense-LK adapter without
+    vorticity enhancement, each with the Liu-Shen (or no) optional adapter;
+    None otherwise (the level loop then calls the adapters)."""
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    if optional is not None and type(optional) is not LiuShenOpticalFlowAlgoAdapter:
+        return None
+    from Farneback_PyCL import Farneback_PyCL
+    from denseLucasKanade_PyCL import denseLucasKanade_PyCl
+    if type(main) is Farneback_PyCL and main.pyramidalLevels == 0:
+        # the reference's compute() checks (Farneback_PyCL.py:462-465)
+        assert main.polyN == 5 or main.polyN == 7
+        assert main.pyrScale < 1
+        return ('fb', main._consts())
+    if type(main) is denseLucasKanade_PyCl and not main.enableVorticityEnhancement:
+        return ('lk', (main.windowHalfWidth, main.Niter))
+    return None
+
+
+def _generic_device(im1, im2, FILTER, main, dev, levels, FILTER_OPT, optional, warping, interScaling,
+                    finalScaling):
+    """The level loop below as one device-resident libofri call
+    (ofri_pyr_generic): every adapter call runs on device buffers."""
+    filt = prepareGaussianKernel(FILTER, 3) if FILTER > 1e-3 else None
+    opt = None
+    if optional is not None and FILTER_OPT is not None and FILTER_OPT > 1e-3:
+        opt = prepareGaussianKernel(FILTER_OPT, 5)
+    kind, arg = dev
+    status = ops.ByteArray(im1.shape) if kind == 'lk' else None
+    Ud, Vd, stats = ops.pyr_generic(im1, im2, levels, filt, opt, warping, interScaling, finalScaling,
+                                    ls_alpha=optional.alpha if optional is not None else 0.0,
+                                    fb_consts=arg if kind == 'fb' else None, lk=arg if kind == 'lk' else None,
+                                    status=status)
+    err = 'Unknown' if kind == 'fb' else True
+    for level, st in enumerate(stats, start=1):
+        ops.log('Level=', level, ' kIter=', 0)
+        ops.log(main.getAlgoName() + ' estimated error for image registration: ' + str(err))
+        if optional is not None:
+            ops.log(optional.getAlgoName() + ' estimated error for image registration: ' + str(st['ls_err']))
+    if status is not None:
+        main.status = status.download()
     return Ud.download(), Vd.download()
 
 

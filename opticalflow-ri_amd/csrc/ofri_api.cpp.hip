@@ -786,6 +786,140 @@ static int run_cached(ofri_ctx* ctx, GraphEntry& g, F&& enqueue, bool stable_onl
     return OFRI_OK;
 }
 
+// The generic level loop (GenericPyramidalOpticalFlow.py:328-414) with a
+// Farneback or dense-LK main solver and an optional Liu-Shen stage, all on
+// device buffers.  Liu-Shen's control words land in "gpyr_stats" (L words).
+static int pyr_generic_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                               const ofri_pyr_params* p, const ofri_pyr_main* m, float* d_Uacc, float* d_Vacc) {
+    const int L = p->levels;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    const size_t NF = (size_t)H * W;
+    // workspace: n1 n2 (resized) w1 w2 (warped) f1 f2 (FILTER) o1 o2 (FILTER_OPT)
+    //            U V (solver) A B / C D (accumulators, ping-pong with the upsampled flow)
+    void* ws;
+    OFRI_TRY(scratch(ctx, "gpyr_ws", 14 * NF * sizeof(float), &ws));
+    float* base = (float*)ws;
+    float *n1 = base, *n2 = base + NF, *w1 = base + 2 * NF, *w2 = base + 3 * NF;
+    float *f1 = base + 4 * NF, *f2 = base + 5 * NF, *o1 = base + 6 * NF, *o2 = base + 7 * NF;
+    float *U = base + 8 * NF, *V = base + 9 * NF;
+    float *Ua = base + 10 * NF, *Va = base + 11 * NF, *Ub = base + 12 * NF, *Vb = base + 13 * NF;
+    void* st;
+    OFRI_TRY(scratch(ctx, "gpyr_stats", (size_t)L * sizeof(SolveCtrl), &st));
+    SolveCtrl* ctrls = (SolveCtrl*)st;
+    float *acc_u = Ua, *acc_v = Va;
+    int ph = 0, pw = 0;
+    for (int l = 0; l < L; ++l) {
+        const int h = lh[l], w = lw[l];
+        const size_t N = (size_t)h * w;
+        const bool last = l == L - 1;
+        const bool scale_here = last ? p->final_scaling != 0 : p->inter_scaling != 0;
+        const float *c1 = d_im1, *c2 = d_im2;
+        if (h != H || w != W) {
+            OFRI_TRY(resize2(ctx, d_im1, n1, d_im2, n2, H, W, h, w, 3));
+            c1 = n1;
+            c2 = n2;
+        }
+        float* out_u = last ? d_Uacc : (acc_u == Ua ? Ub : Ua);
+        float* out_v = last ? d_Vacc : (acc_v == Va ? Vb : Va);
+        const float *s1 = c1, *s2 = c2;
+        if (l > 0) {
+            // updateNextPyramidalLevel (:117-234): the accumulated flow upsampled
+            // (and scaled) into U, V (no warping: the initial flow) or into the
+            // level's accumulator (warping: the pair is warped by it and the
+            // solver starts from zero)
+            float* up_u = p->warping ? out_u : U;
+            float* up_v = p->warping ? out_v : V;
+            if (h != ph || w != pw) {
+                const float sx = scale_here ? (float)w / (float)pw : 1.0f;
+                const float sy = scale_here ? (float)h / (float)ph : 1.0f;
+                OFRI_TRY(spline_upsample2(ctx, acc_u, acc_v, ph, pw, up_u, up_v, h, w, sx, sy));
+            } else {
+                // same size: no resampling, and the size ratio is exactly 1
+                OFRI_HIP(hipMemcpyAsync(up_u, acc_u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+                OFRI_HIP(hipMemcpyAsync(up_v, acc_v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            if (p->warping) {
+                OFRI_TRY(warp_symmetric(ctx, c1, c2, out_u, out_v, h, w, w1, w2));
+                s1 = w1;
+                s2 = w2;
+                OFRI_TRY(fill_zero2(ctx, U, V, N));
+            } else {
+                OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
+            }
+        } else {
+            OFRI_TRY(fill_zero2(ctx, U, V, N));
+            OFRI_TRY(fill_zero2(ctx, out_u, out_v, N));
+        }
+        if (p->filt_w && p->filt_n > 0) {
+            OFRI_TRY(gauss_sep2(ctx, s1, f1, s2, f2, h, w, p->filt_w, p->filt_n));
+            s1 = f1;
+            s2 = f2;
+        }
+        if (m->kind == 0) {
+            OFRI_TRY(farneback(ctx, s1, s2, h, w, U, V, m->fb));
+        } else {
+            OFRI_TRY(lk_dense(ctx, s1, s2, h, w, m->lk_half_win, m->lk_niter, U, V, last ? m->d_lk_status : nullptr,
+                              nullptr));
+        }
+        if (p->ls_alpha > 0.0f) {
+            const float *q1 = c1, *q2 = c2;
+            if (p->opt_w && p->opt_n > 0) {
+                OFRI_TRY(gauss_sep2(ctx, c1, o1, c2, o2, h, w, p->opt_w, p->opt_n));
+                q1 = o1;
+                q2 = o2;
+            }
+            // adapter swap (PhysicsBasedOpticalFlowLiuShen.py:38-39): solver u = V, v = U
+            OFRI_TRY(ls_solve(ctx, q1, q2, h, w, p->ls_alpha, p->ls_maxnum, p->ls_tol, V, U, ctrls + l));
+        }
+        OFRI_TRY(accumulate2(ctx, out_u, U, out_v, V, N));
+        acc_u = out_u;
+        acc_v = out_v;
+        ph = h;
+        pw = w;
+    }
+    return OFRI_OK;
+}
+
+static std::string generic_key(ofri_ctx* ctx, const float* a, const float* b, int H, int W, const ofri_pyr_params* p,
+                               const ofri_pyr_main* m, const float* u, const float* v) {
+    std::string k = "generic";
+    auto put = [&](const void* x, size_t n) { k.append((const char*)x, n); };
+    const void* ptrs[5] = {a, b, u, v, m->d_lk_status};
+    put(ptrs, sizeof ptrs);
+    put(&H, sizeof H);
+    put(&W, sizeof W);
+    put(&p->levels, sizeof p->levels);
+    put(&p->filt_n, sizeof p->filt_n);
+    if (p->filt_w && p->filt_n > 0) put(p->filt_w, sizeof(float) * p->filt_n);
+    put(&p->opt_n, sizeof p->opt_n);
+    if (p->opt_w && p->opt_n > 0) put(p->opt_w, sizeof(float) * p->opt_n);
+    put(&p->warping, sizeof(int) * 3);
+    put(&p->ls_alpha, sizeof p->ls_alpha);
+    put(&p->ls_maxnum, sizeof p->ls_maxnum);
+    put(&p->ls_tol, sizeof p->ls_tol);
+    put(&m->kind, sizeof m->kind);
+    if (m->kind == 0) {
+        const ofri_fb_params* f = m->fb;
+        put(&f->poly_n, sizeof f->poly_n);
+        put(f->g, sizeof(float) * (f->poly_n + 1));
+        put(f->xg, sizeof(float) * (f->poly_n + 1));
+        put(f->xxg, sizeof(float) * (f->poly_n + 1));
+        put(f->ig, sizeof(float) * 4);
+        put(&f->pre_kh, sizeof f->pre_kh);
+        put(f->pre_kernel, sizeof(float) * (f->pre_kh + 1));
+        put(&f->win_kh, sizeof f->win_kh);
+        if (!f->box_filter) put(f->win_kernel, sizeof(float) * (f->win_kh + 1));
+        put(&f->iters, sizeof f->iters);
+        put(&f->box_filter, sizeof f->box_filter);
+    } else {
+        put(&m->lk_half_win, sizeof m->lk_half_win);
+        put(&m->lk_niter, sizeof m->lk_niter);
+    }
+    put(&ctx->tuning, sizeof ctx->tuning);
+    return k;
+}
+
 static std::string batch_key(ofri_ctx* ctx, const ofri_pair* pr, int n, const ofri_lshs_params* p) {
     std::string k = "batch";
     k.append((const char*)pr, sizeof(ofri_pair) * n);
@@ -823,6 +957,49 @@ int ofri_pyr_lshs(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
         OFRI_TRY(run_cached(ctx, g, [&](bool) { return pyr_lshs_enqueue(ctx, d_im1, d_im2, H, W, p, d_Uacc, d_Vacc); }));
     }
     if (h_stats) OFRI_TRY(pyr_stats_download(ctx, L, lh, lw, p, false, h_stats));
+    return OFRI_OK;
+}
+
+int ofri_pyr_generic(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
+                     const ofri_pyr_params* p, const ofri_pyr_main* m, float* d_Uacc, float* d_Vacc,
+                     ofri_level_stats* h_stats) {
+    CHECK_CTX(ctx);
+    if (!p || !m || !d_im1 || !d_im2 || !d_Uacc || !d_Vacc) return fail(OFRI_E_ARG, "ofri_pyr_generic: null pointer");
+    if (p->levels < 1 || p->levels > 12) return fail(OFRI_E_ARG, "ofri_pyr_generic: levels %d", p->levels);
+    if (m->kind == 0) {
+        if (!m->fb || !m->fb->g || !m->fb->xg || !m->fb->xxg || !m->fb->ig || !m->fb->pre_kernel ||
+            (!m->fb->box_filter && !m->fb->win_kernel))
+            return fail(OFRI_E_ARG, "ofri_pyr_generic: Farneback parameters missing");
+        if (m->fb->flow_scale != 1.0f) return fail(OFRI_E_ARG, "ofri_pyr_generic: Farneback flow_scale must be 1");
+    } else if (m->kind != 1) {
+        return fail(OFRI_E_ARG, "ofri_pyr_generic: main solver kind %d", m->kind);
+    }
+    if ((p->filt_w == nullptr) != (p->filt_n <= 0) || (p->opt_w == nullptr) != (p->opt_n <= 0))
+        return fail(OFRI_E_ARG, "ofri_pyr_generic: filter weights and counts disagree");
+    const int L = p->levels;
+    std::vector<int> lh, lw;
+    level_sizes(H, W, L, lh, lw);
+    for (int l = 0; l < L; ++l)
+        if (lh[l] < 4 || lw[l] < 4)
+            return fail(OFRI_E_ARG, "ofri_pyr_generic: level %d size %dx%d below 4x4", l + 1, lh[l], lw[l]);
+    auto enqueue = [&](bool) { return pyr_generic_enqueue(ctx, d_im1, d_im2, H, W, p, m, d_Uacc, d_Vacc); };
+    if (!ctx->graphs_on) OFRI_TRY(enqueue(false));
+    else OFRI_TRY(run_cached(ctx, ctx->graphs[generic_key(ctx, d_im1, d_im2, H, W, p, m, d_Uacc, d_Vacc)], enqueue));
+    if (h_stats) {
+        void* st;
+        OFRI_TRY(scratch(ctx, "gpyr_stats", (size_t)L * sizeof(SolveCtrl), &st));
+        std::vector<SolveCtrl> hc(L);
+        OFRI_HIP(hipMemcpyAsync(hc.data(), st, L * sizeof(SolveCtrl), hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        for (int l = 0; l < L; ++l) {
+            ofri_level_stats& o = h_stats[l];
+            o = ofri_level_stats{};
+            o.H = lh[l];
+            o.W = lw[l];
+            o.ls_err = p->ls_alpha > 0.0f ? hc[l].last_err : 0.0f;
+            o.ls_iters = p->ls_alpha > 0.0f ? hc[l].sweeps : 0;
+        }
+    }
     return OFRI_OK;
 }
 

@@ -75,6 +75,27 @@ class FbParams(C.Structure):
     ]
 
 
+class PyrParams(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int),
+        ("filt_w", C.POINTER(C.c_float)),
+        ("filt_n", C.c_int),
+        ("opt_w", C.POINTER(C.c_float)),
+        ("opt_n", C.c_int),
+        ("warping", C.c_int),
+        ("inter_scaling", C.c_int),
+        ("final_scaling", C.c_int),
+        ("ls_alpha", C.c_float),
+        ("ls_maxnum", C.c_int),
+        ("ls_tol", C.c_double),
+    ]
+
+
+class PyrMain(C.Structure):
+    _fields_ = [("kind", C.c_int), ("fb", C.POINTER(FbParams)), ("lk_half_win", C.c_int), ("lk_niter", C.c_int),
+                ("d_lk_status", C.c_void_p)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("hs_depth", C.c_int), ("ls_depth", C.c_int), ("hs_variant", C.c_int), ("ls_variant", C.c_int)]
 
@@ -136,6 +157,8 @@ SIGNATURES = {
                               _vp, _vp, _vp, C.POINTER(LevelStats), _ip, _vp],
     "ofri_pyr_lshs": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(LshsParams), _vp, _vp,
                       C.POINTER(LevelStats)],
+    "ofri_pyr_generic": [_vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(PyrParams), C.POINTER(PyrMain), _vp, _vp,
+                         C.POINTER(LevelStats)],
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],
     "ofri_get_tuning": [_vp, C.POINTER(Tuning)],
     "ofri_query_tuning": [_vp, C.c_int, C.c_int, C.POINTER(Tuning)],

@@ -201,14 +201,8 @@ def fb_blur(img, half_kernel, out=None):
     return out
 
 
-def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
-    """One Farneback level (Farneback_PyCL.py:515-596); flowx / flowy device
-    arrays updated in place, multiplied by float32(flow_scale) first.
-    consts: dict of float32 arrays g, xg, xxg (polyN+1 taps), ig (4),
-    pre_kernel, win_kernel (half kernels, centre first), and the int iters;
-    an int box_half instead of win_kernel selects the box window
-    (Farneback_PyCL.py:424-439)."""
-    im1, im2 = as_device(im1), as_device(im2)
+def _fb_params(consts, flow_scale=1.0):
+    """FbParams plus the arrays it points into (kept alive by the caller)."""
     box = consts.get("box_half")
     arrs = {k: np.ascontiguousarray(consts[k], dtype=np.float32)
             for k in ("g", "xg", "xxg", "ig", "pre_kernel", "win_kernel") if k in consts}
@@ -224,6 +218,60 @@ def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
         p.box_filter, p.win_kh = 1, int(box)
     p.iters = int(consts["iters"])
     p.flow_scale = float(np.float32(flow_scale))
+    return p, arrs
+
+
+def pyr_generic(im1, im2, levels, filt_w, opt_w, warping, inter_scaling, final_scaling, ls_alpha=0.0,
+                ls_maxnum=60, ls_tol=1e-8, fb_consts=None, lk=None, Uacc=None, Vacc=None, status=None, stats=True):
+    """The generic pyramid on the device (ofri_pyr_generic) with a Farneback
+    (fb_consts: the adapter's constants, one internal level) or dense-LK (lk:
+    (halfWindow, Niter)) main solver and an optional Liu-Shen stage
+    (ls_alpha > 0).  status: optional ByteArray receiving the last level's LK
+    status.  Returns (Uacc, Vacc, stats)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    H, W = im1.shape[-2:]
+    if (fb_consts is None) == (lk is None):
+        raise ValueError("pyr_generic: give exactly one of fb_consts and lk")
+    fw = None if filt_w is None else np.ascontiguousarray(filt_w, dtype=np.float32)
+    ow = None if opt_w is None else np.ascontiguousarray(opt_w, dtype=np.float32)
+    p = _capi.PyrParams()
+    p.levels = int(levels)
+    p.filt_w = _capi.fptr(fw) if fw is not None else None
+    p.filt_n = 0 if fw is None else len(fw)
+    p.opt_w = _capi.fptr(ow) if ow is not None else None
+    p.opt_n = 0 if ow is None else len(ow)
+    p.warping, p.inter_scaling, p.final_scaling = int(bool(warping)), int(bool(inter_scaling)), int(bool(final_scaling))
+    p.ls_alpha = float(ls_alpha) if ls_alpha else 0.0
+    p.ls_maxnum = int(ls_maxnum)
+    p.ls_tol = float(ls_tol)
+    m = _capi.PyrMain()
+    keep = None
+    if fb_consts is not None:
+        fbp, keep = _fb_params(fb_consts)
+        m.kind = 0
+        m.fb = C.pointer(fbp)
+    else:
+        m.kind = 1
+        m.lk_half_win, m.lk_niter = int(lk[0]), int(lk[1])
+        m.d_lk_status = status.ptr if status is not None else None
+    Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
+    Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
+    st = (_capi.LevelStats * levels)() if stats else None
+    _capi.check(_lib().ofri_pyr_generic(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), C.byref(m), Uacc.ptr, Vacc.ptr,
+                                        st))
+    del keep
+    return Uacc, Vacc, (_stats_dicts(st) if stats else [])
+
+
+def farneback(im1, im2, flowx, flowy, consts, flow_scale=1.0):
+    """One Farneback level (Farneback_PyCL.py:515-596); flowx / flowy device
+    arrays updated in place, multiplied by float32(flow_scale) first.
+    consts: dict of float32 arrays g, xg, xxg (polyN+1 taps), ig (4),
+    pre_kernel, win_kernel (half kernels, centre first), and the int iters;
+    an int box_half instead of win_kernel selects the box window
+    (Farneback_PyCL.py:424-439)."""
+    im1, im2 = as_device(im1), as_device(im2)
+    p, arrs = _fb_params(consts, flow_scale)
     _capi.check(_lib().ofri_farneback(_h(), im1.ptr, im2.ptr, im1.H, im1.W, flowx.ptr, flowy.ptr, C.byref(p)))
     return flowx, flowy
 

@@ -152,6 +152,36 @@ def config_pair(cfg, seed=MASTER_SEED, size=None):
     raise ValueError("no single-pair configuration %r" % cfg)
 
 
+def turbulence_modes(H, W, rng, modes=256):
+    """(4, modes) float32 rows kx, ky, phase, amplitude of the turbulence
+    family (as Turbulence draws them) for the device renderer."""
+    kmag = np.exp(rng.uniform(np.log(2 * np.pi / max(H, W) * 2), np.log(2 * np.pi / 8.0), modes))
+    ang = rng.uniform(0, 2 * np.pi, modes)
+    phi = rng.uniform(0, 2 * np.pi, modes)
+    amp = kmag ** (-5.0 / 6.0) * np.sqrt(kmag) / kmag
+    return np.stack([kmag * np.cos(ang), kmag * np.sin(ang), phi, amp]).astype(np.float32)
+
+
+def device_config_pair(cfg, seed=MASTER_SEED, size=None):
+    """The workload configuration's pair rendered on the GPU (ofri_render_pair:
+    the same particle model and flow families, drawn by the device from one
+    seed; rendering a 2048^2 turbulence pair on the host takes half a
+    minute).  Returns (im1, im2, u, v) DeviceArrays.
+      1  1024x1024 Lamb-Oseen (core 80 px, peak 4 px), density 0.04
+      2  2048x2048 shear 2.0 + turbulence (256 modes, max 4 px), density 0.06"""
+    from . import ops
+    rng = np.random.default_rng(np.random.SeedSequence(seed).spawn(8)[cfg])
+    dev_seed = int(rng.integers(0, 2 ** 62))
+    if cfg == 1:
+        n = size or 1024
+        rc = 80.0 * n / 1024.0 if n < 1024 else 80.0
+        return ops.render_pair(n, n, 1, 0.04, dev_seed, [(n - 1) / 2.0, (n - 1) / 2.0, rc, 4.0], truth=True)
+    if cfg == 2:
+        n = size or 2048
+        return ops.render_pair(n, n, 2, 0.06, dev_seed, [2.0, 4.0], turbulence_modes(n, n, rng), truth=True)
+    raise ValueError("no device-rendered configuration %r" % cfg)
+
+
 def _batch_item(args):
     H, W, dens, fam, child, truth = args
     return piv_pair(H, W, dens, fam, np.random.default_rng(child), truth=truth)
