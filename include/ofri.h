@@ -144,6 +144,19 @@ int ofri_hs_accel(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, 
  * helper, physicsBasedOpticalFlowLiuShen); the adapter at :37-39 swaps U, V. */
 int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
             int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
+/* ofri_ls without any device allocation: the coefficient fields, the
+ * ping-pong pair, the stop test's partial sums, the maxima and the control
+ * word live in a fixed 42 MiB device array of the library (one per GPU,
+ * held until the process ends), so the call makes no hipMalloc / hipFree
+ * and allocates nothing else either; images above about 1,048,576 pixels
+ * are rejected (OFRI_E_ARG, the message says how much they need).  Same
+ * arguments, results and sweeps as ofri_ls, bit for bit.  Calls that
+ * overlap on one GPU (several contexts, or threads, of one process) share
+ * that array and race.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid allocation). */
+int ofri_ls_noalloc(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+                    int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters);
 /* Liu-Shen as the learned form runs it: the update norm is tested once per
  * fused launch, at its last sweep, which is the reference update; the
  * sweeps before it in the launch are the same update in float32 with fused

@@ -979,10 +979,11 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
     return select_copy2(ctx, ctrl, s0, s1, dst, nullptr, nullptr, nullptr, n);
 }
 
-int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out) {
+int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out, float* ext_part) {
+    static_assert(2 * kRedBlocks <= kMaxRedPart, "partial maxima");
     const int planes = x2 ? 2 : 1;
-    void* part;
-    OFRI_TRY(scratch(ctx, "max_part", 2 * kRedBlocks * sizeof(float), &part));
+    void* part = ext_part;
+    if (!part) OFRI_TRY(scratch(ctx, "max_part", 2 * kRedBlocks * sizeof(float), &part));
     int nb = (int)((n + kRedThreads - 1) / kRedThreads);
     if (nb > kRedBlocks) nb = kRedBlocks;
     if (nb < 1) nb = 1;

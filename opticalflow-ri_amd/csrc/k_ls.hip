@@ -16,11 +16,21 @@
 // The launch sequence is fixed, so a whole solve can be captured in a HIP
 // graph.
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include <type_traits>
 
 #include "ofri_internal.h"
 #include "solve_check.h"
+
+// The allocation-free entry point's device memory (ofri_ls_noalloc): one
+// array per GPU the library runs on, held until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (ofri::noalloc_scratch): the array's name is a host-side
+// stand-in, not a device address.
+constexpr size_t kNoAllocBytes = (size_t)42 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 #pragma clang fp contract(off)
 
@@ -533,9 +543,9 @@ void ls_default(size_t npx, int* variant, int* depth) {
 // Liu-Shen's coefficient fields for (im1, im2) into F (8 fields of H * W),
 // the images' maxima into mx (2 floats); the control word starts cleared.
 static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h, float* F,
-                      float* mx, SolveCtrl* d_ctrl) {
+                      float* mx, SolveCtrl* d_ctrl, float* max_part = nullptr) {
     const size_t N = (size_t)H * W;
-    OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx));
+    OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx, max_part));
     dim3 g((W + 255) / 256, H);
     ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N, d_ctrl);
     OFRI_HIP(hipGetLastError());
@@ -544,12 +554,14 @@ static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, 
 
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last, float cheb_lo,
-             float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast, bool rel) {
+             float cheb_hi, float* sink_u, float* sink_v, bool sink_add, bool fast, bool rel,
+             const LsWorkspace* ws) {
     if ((sink_u == nullptr) != (sink_v == nullptr)) return fail(OFRI_E_ARG, "ls: one result sink given without the other");
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ls: image %dx%d too small", H, W);
     if (maxnum < 0) return fail(OFRI_E_ARG, "ls: negative sweep budget");
     if (rel && (!d_ctrl || !check_last)) return fail(OFRI_E_ARG, "ls: a relative tolerance needs the per-launch test");
     const bool cheb = cheb_hi > 0.0f;
+    if (ws && (cheb || fast)) return fail(OFRI_E_ARG, "ls: a caller workspace serves plain sweeps only");
     if (cheb && !(cheb_lo > -1.0f && cheb_lo < 0.0f && cheb_hi <= 0.999f))
         return fail(OFRI_E_ARG, "ls: Chebyshev interval [%g, %g] outside (-1, 0) x (0, 0.999]", cheb_lo, cheb_hi);
     size_t N = (size_t)H * W;
@@ -573,16 +585,24 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     dim3 grid((W + outW - 1) / outW, (H + outH - 1) / outH);
     int nblk = grid.x * grid.y;
     void *fbuf, *pbuf, *mbuf;
-    OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float), &fbuf));
     // per-sweep partial sums, two launch-parity slots (row `depth`: the result
     // norms of a relative test)
     const size_t pstride = (size_t)(depth + 1) * nblk * 2;
-    OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
+    if (ws) {
+        if (ws->part_doubles < 2 * pstride)
+            return fail(OFRI_E_ARG, "ls: workspace holds %zu partial sums, %zu needed", ws->part_doubles, 2 * pstride);
+        fbuf = ws->fields;
+        pbuf = ws->part;
+        mbuf = ws->mx;
+    } else {
+        OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float), &fbuf));
+        OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
+        OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
+    }
     float* F = (float*)fbuf;
     float* ub = F + 8 * N;
     float* vb = F + 9 * N;
-    OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
-    OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, h, F, (float*)mbuf, d_ctrl));
+    OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, h, F, (float*)mbuf, d_ctrl, ws ? ws->max_part : nullptr));
 
     LsPass p{};
     p.in[0][0] = u; p.in[0][1] = v; p.in[1][0] = ub; p.in[1][1] = vb;
@@ -743,6 +763,40 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
             f += 10 * N;
         }
     }
+    return OFRI_OK;
+}
+
+size_t ls_part_doubles(ofri_ctx* ctx, int H, int W) {
+    int vi, depth;
+    ls_default((size_t)H * W, &vi, &depth);
+    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
+    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    if (vi >= kNumLsVariants) return 0;
+    const LsVariant& var = kLsVariants[vi];
+    const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
+    if (outW <= 0 || outH <= 0) return 0;
+    const size_t nblk = (size_t)((W + outW - 1) / outW) * ((H + outH - 1) / outH);
+    return 2 * (size_t)(depth + 1) * nblk * 2;
+}
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, void*> g_noalloc_ptr;   // device -> address of that device's copy of ::scratch
+}  // namespace
+
+int noalloc_scratch(int device, void** ptr, size_t* bytes) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    if (it == g_noalloc_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || !p)
+            return fail(OFRI_E_HIP, "no-alloc scratch of device %d: hipGetSymbolAddress failed: %s", device,
+                        hipGetErrorString(e));
+        it = g_noalloc_ptr.emplace(device, p).first;
+    }
+    *ptr = it->second;
+    *bytes = kNoAllocBytes;
     return OFRI_OK;
 }
 

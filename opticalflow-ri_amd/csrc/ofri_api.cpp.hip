@@ -89,6 +89,14 @@ int ofri_create(int device, ofri_ctx** out) {
         delete c;
         return fail(OFRI_E_HIP, "ofri_create: stream/event creation failed");
     }
+    // the allocation-free entry point's scratch on this device (device
+    // current); a failure is kept for that entry point to report
+    if (noalloc_scratch(device, &c->noalloc, &c->noalloc_bytes) != OFRI_OK) {
+        c->noalloc = nullptr;
+        c->noalloc_bytes = 0;
+        c->noalloc_err = ofri_last_error();
+        clear_error();
+    }
     *out = c;
     return OFRI_OK;
 }
@@ -335,6 +343,46 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
     if (h_err || h_iters) {
         SolveCtrl hc;
         OFRI_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
+        OFRI_HIP(hipStreamSynchronize(ctx->stream));
+        if (h_err) *h_err = hc.last_err;
+        if (h_iters) *h_iters = hc.sweeps;
+    }
+    return OFRI_OK;
+}
+
+int ofri_ls_noalloc(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W, float h,
+                    int maxnum, double tol, float* d_u, float* d_v, float* h_err, int* h_iters) {
+    CHECK_CTX(ctx);
+    if (!d_im1 || !d_im2 || !d_u || !d_v) return fail(OFRI_E_ARG, "ofri_ls_noalloc: null pointer");
+    if (H < 2 || W < 2) return fail(OFRI_E_ARG, "ofri_ls_noalloc: image %dx%d too small", H, W);
+    if (!ctx->noalloc)
+        return fail(OFRI_E_STATE, "ofri_ls_noalloc: no device scratch on device %d (%s)", ctx->device,
+                    ctx->noalloc_err.c_str());
+    // layout of the device scratch: control word, maxima, partial maxima,
+    // stop-test partial sums, then the coefficient fields and the ping-pong
+    // pair (10 planes); everything is written before it is read (the setup
+    // kernel clears the control word), so nothing needs clearing
+    const size_t N = (size_t)H * W;
+    const size_t nparts = ls_part_doubles(ctx, H, W);
+    const size_t o_mx = align256(sizeof(SolveCtrl));
+    const size_t o_red = o_mx + 256;
+    const size_t o_part = o_red + align256(kMaxRedPart * sizeof(float));
+    const size_t o_f = o_part + align256(nparts * sizeof(double));
+    const size_t need = o_f + 10 * N * sizeof(float);
+    if (nparts == 0 || need > ctx->noalloc_bytes)
+        return fail(OFRI_E_ARG,
+                    "ofri_ls_noalloc: a %dx%d image needs %zu bytes of device scratch, %zu available "
+                    "(about %zu pixels at most)",
+                    H, W, need, ctx->noalloc_bytes, (ctx->noalloc_bytes - o_f) / (10 * sizeof(float)));
+    char* base = (char*)ctx->noalloc;
+    SolveCtrl* ctrl = (SolveCtrl*)base;
+    LsWorkspace ws{(float*)(base + o_f), (double*)(base + o_part), nparts, (float*)(base + o_mx),
+                   (float*)(base + o_red)};
+    OFRI_TRY(ls_solve(ctx, d_im1, d_im2, H, W, h, maxnum, tol, d_u, d_v, ctrl, false, 0.0f, 0.0f, nullptr, nullptr,
+                      false, false, false, &ws));
+    if (h_err || h_iters) {
+        SolveCtrl hc;
+        OFRI_HIP(hipMemcpyAsync(&hc, ctrl, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
         OFRI_HIP(hipStreamSynchronize(ctx->stream));
         if (h_err) *h_err = hc.last_err;
         if (h_iters) *h_iters = hc.sweeps;

@@ -109,6 +109,11 @@ struct ofri_ctx {
     std::map<std::tuple<int, int, int>, ofri::ResizeTable> resize_tables;
     std::map<std::pair<int, int>, ofri::SplineTable> spline_tables;
     std::map<uint64_t, ofri::DevBuf> omega_tabs;  // Chebyshev weight tables by sigma (bits)
+    // the allocation-free entry point's device scratch on this device
+    // (the address hipGetSymbolAddress gave for it; null if that failed)
+    void* noalloc = nullptr;
+    size_t noalloc_bytes = 0;
+    std::string noalloc_err;
 };
 
 namespace ofri {
@@ -159,8 +164,10 @@ int fill_zero2(ofri_ctx* ctx, float* p1, float* p2, size_t n);
 int fill_zero(ofri_ctx* ctx, float* p, size_t n);
 // d = a - b (float32), n elements; d may alias neither input
 int subtract(ofri_ctx* ctx, float* d, const float* a, const float* b, size_t n);
-// max over n floats -> d_out[0]
-int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out);
+// max over n floats -> d_out[0]; part (optional): kMaxRedPart floats of
+// caller memory for the partial maxima (else a scratch buffer)
+constexpr int kMaxRedPart = 1024;
+int reduce_max2(ofri_ctx* ctx, const float* x1, const float* x2, size_t n, float* d_out, float* part = nullptr);
 int reduce_max(ofri_ctx* ctx, const float* x, size_t n, float* d_out);
 // d_out[0] = (||A-A0|| + ||B-B0||) / n as float32 (differences in float32)
 int diff_norm_err(ofri_ctx* ctx, const float* A, const float* A0, const float* B, const float* B0,
@@ -213,10 +220,24 @@ unsigned lk_window_mask(int hw, int lo, int hi);
 // (added when sink_add, else copied); u, v then hold the start only.
 // rel (needs check_last): tol is relative, as for hs_solve; the control word
 // also publishes the solution norm of the tested launch (last_nrm).
+// ws (optional): caller-provided device memory instead of the context's
+// scratch buffers (plain sweeps only: no Chebyshev, no float32 sweeps).
+struct LsWorkspace {
+    float* fields;        // 10 * H * W floats: 8 coefficient planes, u / v ping-pong
+    double* part;         // the stop test's partial sums, part_doubles of them
+    size_t part_doubles;
+    float* mx;            // the two frames' maxima
+    float* max_part;      // kMaxRedPart floats
+};
 int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h,
              int maxnum, double tol, float* u, float* v, SolveCtrl* d_ctrl, bool check_last = false,
              float cheb_lo = 0.0f, float cheb_hi = 0.0f, float* sink_u = nullptr, float* sink_v = nullptr,
-             bool sink_add = false, bool fast = false, bool rel = false);
+             bool sink_add = false, bool fast = false, bool rel = false, const LsWorkspace* ws = nullptr);
+// Doubles of stop-test partial sums a plain ls_solve of an H x W image uses.
+size_t ls_part_doubles(ofri_ctx* ctx, int H, int W);
+// The library's allocation-free device scratch of this device (the device
+// must be current): its address and size.
+int noalloc_scratch(int device, void** ptr, size_t* bytes);
 
 // ---- batched solves: every image's tiles in the same launches ----
 // One image of a batched solve.  HS: U, V in/out, d_err (may be null) gets

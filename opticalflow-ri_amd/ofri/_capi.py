@@ -132,6 +132,7 @@ SIGNATURES = {
     "ofri_hs_accel": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, C.c_float, C.c_float, _vp,
                       _vp, _vp, _fp, _ip, C.c_int],
     "ofri_ls": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
+    "ofri_ls_noalloc": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_warp_liushen": [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(C.c_double), C.c_int, _vp, _ip],
     "ofri_ls_learned": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_double, _vp, _vp, _fp, _ip],
     "ofri_lk_dense": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)],
