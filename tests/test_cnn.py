@@ -1,7 +1,8 @@
-"""Correction network: the CPU restatement's backward pass against finite
-differences of its own loss, and the device's forward pass, loss and
-gradients against the restatement at the device's bfloat16 rounding points;
-the fused correction step against the forward pass."""
+"""Correction network: the CPU restatement against the same network written
+with torch's operators (forward) and torch's autograd (backward), and
+against finite differences of its own loss; the device's forward pass,
+loss and gradients against the restatement at the device's bfloat16
+rounding points; the fused correction step against the forward pass."""
 import numpy as np
 import pytest
 
@@ -68,6 +69,94 @@ def test_restated_backward_matches_finite_differences():
         lm = cnn.loss_and_head_grads(cnn.forward(pm, inp, bf=False), lc, lf, FW)[0]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-4 * max(1e-3, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def _torch_network(params, inp, planes=4):
+    """The network of the layer table written with torch's own operators
+    (NCHW float64; F.conv2d is cross-correlation with zero padding, the
+    weight [cout][tap][cin] with tap = 3 (dy + 1) + dx + 1 becomes
+    [cout][cin][dy + 1][dx + 1]; space-to-depth channel (2 dy + dx) C + c).
+    Returns the planar heads and the parameter tensors (for autograd)."""
+    import torch
+    import torch.nn.functional as F
+
+    def s2d(a):
+        B, C, H2, W2 = a.shape
+        return a.reshape(B, C, H2 // 2, 2, W2 // 2, 2).permute(0, 3, 5, 1, 2, 4).reshape(B, 4 * C, H2 // 2, W2 // 2)
+
+    def d2s(a):
+        B, C4, h, w = a.shape
+        C = C4 // 4
+        return a.reshape(B, 2, 2, C, h, w).permute(0, 3, 4, 1, 5, 2).reshape(B, C, 2 * h, 2 * w)
+
+    x = torch.tensor(np.asarray(inp, np.float32))
+    # frames scaled in float32, as the network's input stage scales them
+    planes_t = [((x[:, k] - 128.0) * (1.0 / 64.0) if k < 2 else x[:, k]).double() for k in range(x.shape[1])]
+    pieces = {"X0": {0: s2d(torch.stack(planes_t, dim=1))}}
+
+    def read(name, c0, n):
+        ps = pieces[name]
+        cs = sorted(ps)
+        full = torch.cat([ps[c] for c in cs], dim=1)
+        off = cs[0]
+        assert all(cs[i] + ps[cs[i]].shape[1] == cs[i + 1] for i in range(len(cs) - 1))
+        return full[:, c0 - off:c0 - off + n]
+
+    flat = torch.tensor(np.asarray(params, np.float64), requires_grad=True)
+    q = 0
+    for L in cnn.layers(planes):
+        name, src, c0, s2d_in, dst, d0, d2s_out, cin, cout, act = L[:10]
+        w = flat[q:q + 9 * cin * cout].reshape(cout, 3, 3, cin).permute(0, 3, 1, 2)
+        q += 9 * cin * cout
+        b = flat[q:q + cout]
+        q += cout
+        xin = read(src, c0, cin // 4 if s2d_in else cin)
+        if s2d_in:
+            xin = s2d(xin)
+        y = F.conv2d(xin, w, b, padding=1)
+        if act:
+            y = F.leaky_relu(y, cnn.SLOPE)
+        if d2s_out:
+            y = d2s(y)
+        pieces.setdefault(dst, {})[d0] = y
+    hc, hf = pieces["HC"][0], pieces["HF"][0]
+    return hc[:, :4], d2s(hf), flat
+
+
+def test_restatement_matches_torch_forward_and_autograd():
+    """The numpy restatement (float64 mode) against the same network written
+    with torch's conv2d / leaky_relu and differentiated by torch's autograd:
+    a flipped kernel, a transposed tap or channel order, a wrong routing or
+    a wrong gradient term fails here."""
+    import torch
+    inp, lc, lf = _batch(2, 32, 48, 21)
+    p = cnn.init_params(17, scale=1.0, planes=4).astype(np.float64)
+    T = cnn.forward(p, inp, bf=False)
+    rc, rf = cnn.heads(T)
+    tc, tf, flat = _torch_network(p, inp)
+    for r, t in ((rc, tc), (rf, tf)):
+        t = t.detach().numpy()
+        assert r.shape == t.shape
+        assert np.abs(r - t).max() <= 1e-10 * max(1.0, np.abs(t).max())
+    loss, ghc, ghf = cnn.loss_and_head_grads(T, lc, lf, FW)
+    g = cnn.backward(p, T, ghc, ghf, bf=False)
+    # the same weighted Charbonnier loss in torch
+    n = tc.shape[0] * tc.shape[2] * tc.shape[3]
+    eps2 = 1e-6
+    dc = tc - torch.tensor(lc)
+    df = tf - torch.tensor(lf)
+    fw = torch.tensor(FW)
+    tl = ((torch.sqrt(dc * dc + eps2).sum(dim=(0, 2, 3)) * fw[:4]).sum() / n +
+          (torch.sqrt(df * df + eps2).sum(dim=(0, 2, 3)) * fw[4:]).sum() / (4 * n))
+    assert abs(float(tl) - loss) <= 1e-10 * abs(loss)
+    tl.backward()
+    tg = flat.grad.numpy()
+    q = 0
+    for L in cnn.layers(4):
+        k = 9 * L[7] * L[8] + L[8]
+        a, b = g[q:q + k], tg[q:q + k]
+        assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(b) + 1e-14, L[0]
+        q += k
 
 
 # ------------------------------------------------------------------ GPU
