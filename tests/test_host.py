@@ -83,6 +83,30 @@ def test_gaussian_kernel_bitexact_known_answers(golden):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n", [11, 17, 33, 39, 159, 12])
+def test_gaussian_kernel_bitexact_general_case_is_the_sampled_gaussian(n):
+    """The general case against the mathematics it evaluates, in another
+    arithmetic: normalised samples of exp(-k^2 / (2 sigma^2)) at the integer
+    offsets k of the window (the reference's x = 2k runs 1-n .. n-1 in steps
+    of 2 under exp(-0.125 x^2 / sigma^2)), sigma = 0.15 n + 0.35, formed in
+    float64 with numpy: the float64 kernels agree to a few ulps, their
+    float32 roundings (what every caller uses) bit for bit, and the window
+    is symmetric, peaked at its centre and sums to 1."""
+    from GaussianKernelBitExact import getGaussianKernelBitExact
+    s, k = getGaussianKernelBitExact(n, 0.0 if n % 2 == 0 else 2.0)   # sigma >= 0 is ignored
+    sigma = 0.15 * n + 0.35
+    off = np.arange(n) - (n - 1) / 2.0
+    if n % 2 == 0:
+        # even windows: the reference's centre pair both take exp(0)
+        off = np.where(np.abs(off) == 0.5, 0.0, off)
+    g = np.exp(-off ** 2 / (2.0 * sigma * sigma))
+    g = g / g.sum()
+    assert np.abs(k - g).max() <= 8 * np.finfo(np.float64).eps * g.max()
+    assert np.array_equal(k.astype(np.float32), g.astype(np.float32))
+    assert np.array_equal(k, k[::-1]) and k.argmax() == (n - 1) // 2
+    assert abs(float(s) - 1.0) < 1e-14 and abs(k.sum() - 1.0) < 1e-14
+
+
 def test_dropin_api_surface():
     import GenericPyramidalOpticalFlow as G
     import HornSchunck as HSm
