@@ -10,9 +10,9 @@ plain form runs the reference's stopping rules (HS 600 sweeps, Liu-Shen to
 1e-8 or 60 sweeps).  The learned form runs pair-independent rules (HS to a
 relative residual of 2^-22, the correction network, a fixed number of
 Liu-Shen sweeps with a relative-residual acceptance test), falls back to
-the plain form when a rule fails, and lets the plain form judge every 16th
+the plain form when a rule fails, and lets the plain form judge every 32nd
 pair (EPE to plain <= 0.01 px, else the plain result).  One step = one
-batch of 16 pairs, judge and fallbacks included; inputs are resident in HBM
+batch of 32 pairs, judge and fallbacks included; inputs are resident in HBM
 before timing.
 
 Prints ONE JSON line.  `value` is the learned form's Mpix-pairs/s
@@ -510,7 +510,7 @@ def bench_net(args):
 
 def bench_lshs(args):
     """The headline: LS(HS) 2-level at 1024^2, learned form against plain
-    form.  A step is one judged batch of CHECK_EVERY (16) pairs: the learned
+    form.  A step is one judged batch of CHECK_EVERY (32) pairs: the learned
     form with its pair-independent rules, the judge's plain run and EPE check
     on one pair of the batch, and the plain fallback of every pair that
     missed a rule or failed its check -- all inside the timed region.  The
