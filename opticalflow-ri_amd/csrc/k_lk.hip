@@ -16,6 +16,8 @@
 // mask per axis (bit c: window column c, at offset c - hw from the pixel),
 // so the asymmetric (vorticity) windows and the reference's rounding of the
 // window to its tiles are the same code.
+#include <climits>
+
 #include "ofri_internal.h"
 
 #pragma clang fp contract(off)
@@ -27,6 +29,7 @@ constexpr int kTile = 16;
 constexpr int kMaxHalf = 16;
 constexpr int kReg = kTile + 32;                 // window region edge (window columns 0..31)
 constexpr int kRegI = kReg + 2;                  // with the Scharr ring
+constexpr int kStage = 72 * 72;                  // floats of J staged per Gauss-Newton step
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -64,8 +67,8 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
     __syncthreads();
 
     const int j = bx0 + threadIdx.x, i = by0 + threadIdx.y;
-    if (j >= W || i >= H) return;
-    const size_t g = (size_t)i * W + j;
+    const bool valid = j < W && i < H;
+    const size_t g = valid ? (size_t)i * W + j : 0;
     // window of pixel (i, j) starts at region (threadIdx.y, threadIdx.x)
     const int o = threadIdx.y * kReg + threadIdx.x;
     const int oI = (threadIdx.y + 1) * kRegI + threadIdx.x + 1;
@@ -85,52 +88,116 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
     float A11 = (float)a11, A12 = (float)a12, A22 = (float)a22;
     float D = A11 * A22 - A12 * A12;
     unsigned char st = 1;
-    if (D < 1.192092896e-07f) {
-        if (status) status[g] = 0;
-        return;   // flow left as it came in
-    }
+    // a singular tensor stops the pixel at once: flow left as it came in
+    bool active = valid && !(D < 1.192092896e-07f);
+    if (valid && !active) st = 0;
     A11 /= D;
     A12 /= D;
     A22 /= D;
-    float px = (float)j + u[g] - (float)hw, py = (float)i + v[g] - (float)hw;
+    float px = 0.0f, py = 0.0f;
+    if (active) {
+        px = (float)j + u[g] - (float)hw;
+        py = (float)i + v[g] - (float)hw;
+    }
+    // Every Gauss-Newton step samples J over the block's displaced windows:
+    // the bounding box of the active pixels' windows is staged in LDS from
+    // clamp-to-edge reads (indexed by unclamped positions, so every sample
+    // reads what the clamped image read would), unless it exceeds the stage;
+    // then the step reads J from global memory.  The whole block takes the
+    // same number of steps (inactive pixels only keep the barriers).
+    __shared__ float sJ[kStage];
+    __shared__ int bb[4];   // min x0, min y0, max x0, max y0 of the active windows
     for (int k = 0; k < niter; ++k) {
-        if (px < (float)-hw || px >= (float)W || py < (float)-hw || py >= (float)H) {
-            st = 0;
-            break;
+        if (tid == 0) {
+            bb[0] = INT_MAX;
+            bb[1] = INT_MAX;
+            bb[2] = INT_MIN;
+            bb[3] = INT_MIN;
         }
-        const float fx = floorf(px), fy = floorf(py);
+        __syncthreads();   // also: the previous step's reads of sJ are done
+        if (active && (px < (float)-hw || px >= (float)W || py < (float)-hw || py >= (float)H)) {
+            st = 0;
+            active = false;
+        }
+        float fx = 0.0f, fy = 0.0f;
+        int x0 = 0, y0 = 0;
+        if (active) {
+            fx = floorf(px);
+            fy = floorf(py);
+            x0 = (int)fx;
+            y0 = (int)fy;
+            atomicMin(&bb[0], x0);
+            atomicMin(&bb[1], y0);
+            atomicMax(&bb[2], x0);
+            atomicMax(&bb[3], y0);
+        }
+        __syncthreads();
+        const int rx0 = bb[0], ry0 = bb[1];
+        if (rx0 == INT_MAX) break;   // no active pixel left in the block
+        // sample columns x0 .. x0 + wx (bilinear right neighbour included), rows likewise
+        const int rw = bb[2] - rx0 + wx + 1, rh = bb[3] - ry0 + wy + 1;
+        const bool staged = rw <= kStage && rh <= kStage && rw * rh <= kStage;   // no int overflow
+        if (staged) {
+            for (int q = tid; q < rw * rh; q += kTile * kTile) {
+                const int ry = q / rw, rx = q - (q / rw) * rw;
+                const int y = clampi(ry0 + ry, 0, H - 1), x = clampi(rx0 + rx, 0, W - 1);
+                sJ[q] = J[(size_t)y * W + x];
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
         const float a = px - fx, b = py - fy;
-        const int x0 = (int)fx, y0 = (int)fy;
         const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
         const float w01 = (1.0f - a) * b, w11 = a * b;
-        const bool inside = x0 >= 0 && y0 >= 0 && x0 + wx < W && y0 + wy < H;
         double b1 = 0.0, b2 = 0.0;
-        for (int r = 0; r < wy; ++r) {
-            if (!((my >> r) & 1u)) continue;
-            const float* pr = sI + oI + r * kRegI;
-            const float* dxr = sDx + o + r * kReg;
-            const float* dyr = sDy + o + r * kReg;
-            int ya = y0 + r, yb = ya + 1;
-            if (!inside) {
-                ya = clampi(ya, 0, H - 1);
-                yb = clampi(yb, 0, H - 1);
-            }
-            const float* Ja = J + (size_t)ya * W;
-            const float* Jb = J + (size_t)yb * W;
-            for (int c = 0; c < wx; ++c) {
-                if (!((mx >> c) & 1u)) continue;
-                int xa = x0 + c, xb = xa + 1;
-                if (!inside) {
-                    xa = clampi(xa, 0, W - 1);
-                    xb = clampi(xb, 0, W - 1);
+        if (staged) {
+            for (int r = 0; r < wy; ++r) {
+                if (!((my >> r) & 1u)) continue;
+                const float* pr = sI + oI + r * kRegI;
+                const float* dxr = sDx + o + r * kReg;
+                const float* dyr = sDy + o + r * kReg;
+                const float* Ja = sJ + (y0 + r - ry0) * rw + (x0 - rx0);
+                const float* Jb = Ja + rw;
+                for (int c = 0; c < wx; ++c) {
+                    if (!((mx >> c) & 1u)) continue;
+                    float t = w00 * Ja[c];
+                    t = t + w10 * Ja[c + 1];
+                    t = t + w01 * Jb[c];
+                    t = t + w11 * Jb[c + 1];
+                    float diff = t - pr[c];
+                    b1 = b1 + (double)diff * (double)dxr[c];
+                    b2 = b2 + (double)diff * (double)dyr[c];
                 }
-                float t = w00 * Ja[xa];
-                t = t + w10 * Ja[xb];
-                t = t + w01 * Jb[xa];
-                t = t + w11 * Jb[xb];
-                float diff = t - pr[c];
-                b1 = b1 + (double)diff * (double)dxr[c];
-                b2 = b2 + (double)diff * (double)dyr[c];
+            }
+        } else {
+            const bool inside = x0 >= 0 && y0 >= 0 && x0 + wx < W && y0 + wy < H;
+            for (int r = 0; r < wy; ++r) {
+                if (!((my >> r) & 1u)) continue;
+                const float* pr = sI + oI + r * kRegI;
+                const float* dxr = sDx + o + r * kReg;
+                const float* dyr = sDy + o + r * kReg;
+                int ya = y0 + r, yb = ya + 1;
+                if (!inside) {
+                    ya = clampi(ya, 0, H - 1);
+                    yb = clampi(yb, 0, H - 1);
+                }
+                const float* Ja = J + (size_t)ya * W;
+                const float* Jb = J + (size_t)yb * W;
+                for (int c = 0; c < wx; ++c) {
+                    if (!((mx >> c) & 1u)) continue;
+                    int xa = x0 + c, xb = xa + 1;
+                    if (!inside) {
+                        xa = clampi(xa, 0, W - 1);
+                        xb = clampi(xb, 0, W - 1);
+                    }
+                    float t = w00 * Ja[xa];
+                    t = t + w10 * Ja[xb];
+                    t = t + w01 * Jb[xa];
+                    t = t + w11 * Jb[xb];
+                    float diff = t - pr[c];
+                    b1 = b1 + (double)diff * (double)dxr[c];
+                    b2 = b2 + (double)diff * (double)dyr[c];
+                }
             }
         }
         float B1 = (float)b1, B2 = (float)b2;
@@ -138,10 +205,13 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
         float ddy = (A12 * B1 - A11 * B2) * 32.0f;
         px = px + ddx;
         py = py + ddy;
-        if (fabsf(ddx) < 0.01f && fabsf(ddy) < 0.01f) break;
+        if (fabsf(ddx) < 0.01f && fabsf(ddy) < 0.01f) active = false;
     }
-    u[g] = px + (float)hw - (float)j;
-    v[g] = py + (float)hw - (float)i;
+    if (!valid) return;
+    if (!(D < 1.192092896e-07f)) {
+        u[g] = px + (float)hw - (float)j;
+        v[g] = py + (float)hw - (float)i;
+    }
     if (status) status[g] = st;
 }
 

@@ -154,17 +154,22 @@ def ops():
     return _ops
 
 
+# sd: spread of the initial flow.  0.3 keeps every block's displaced windows
+# inside the kernel's LDS stage of J; 12 spreads them beyond it (the step
+# then samples J from global memory) and sends pixels off the image; 3 mixes
+# both kinds of block in one launch.
 @pytest.mark.gpu
-@pytest.mark.parametrize("H,W,hw,asym", [(64, 80, 13, (0, 0, 0, 0)), (131, 97, 13, (0, 0, 0, 0)),
-                                         (40, 50, 5, (0, 0, 0, 0)), (33, 29, 16, (0, 0, 0, 0)),
-                                         (64, 80, 13, (1, 0, 0, 1)), (64, 80, 13, (0, 1, 0, 1)),
-                                         (40, 50, 3, (1, 0, 1, 0)), (37, 45, 16, (0, 1, 1, 1))])
-def test_lk_gpu_matches_oracle(ops, H, W, hw, asym):
+@pytest.mark.parametrize("H,W,hw,asym,sd", [(64, 80, 13, (0, 0, 0, 0), 0.3), (131, 97, 13, (0, 0, 0, 0), 0.3),
+                                            (40, 50, 5, (0, 0, 0, 0), 0.3), (33, 29, 16, (0, 0, 0, 0), 0.3),
+                                            (64, 80, 13, (1, 0, 0, 1), 0.3), (64, 80, 13, (0, 1, 0, 1), 0.3),
+                                            (40, 50, 3, (1, 0, 1, 0), 0.3), (37, 45, 16, (0, 1, 1, 1), 0.3),
+                                            (64, 80, 13, (0, 0, 0, 0), 12.0), (96, 96, 8, (0, 1, 1, 0), 3.0)])
+def test_lk_gpu_matches_oracle(ops, H, W, hw, asym, sd):
     from ofri.device import DeviceArray
     I, J = _pair(H, W, (1.1, 0.5), seed=H + W)
     rng = np.random.default_rng(hw)
-    u0 = (rng.standard_normal((H, W)) * 0.3).astype(np.float32)
-    v0 = (rng.standard_normal((H, W)) * 0.3).astype(np.float32)
+    u0 = (rng.standard_normal((H, W)) * sd).astype(np.float32)
+    v0 = (rng.standard_normal((H, W)) * sd).astype(np.float32)
     u, v = DeviceArray.from_host(u0), DeviceArray.from_host(v0)
     st = ops.ByteArray((H, W))
     ops.lk_dense(I, J, u, v, hw, 5, st, asym)
@@ -173,7 +178,7 @@ def test_lk_gpu_matches_oracle(ops, H, W, hw, asym):
     du, dv, _ = orc.lk_dense(I, J, u0, v0, hw, 5, prec="f64", asym=asym)
     d32 = max(np.abs(gu - ou).max(), np.abs(gv - ov).max())
     d64 = max(np.abs(gu - du).max(), np.abs(gv - dv).max())
-    print("[lk %dx%d hw%d asym %s] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px" % (H, W, hw, asym, d32, d64))
+    print("[lk %dx%d hw%d asym %s sd %g] max|d| vs oracle-f32 %.3g px, vs f64 %.3g px" % (H, W, hw, asym, sd, d32, d64))
     assert d32 <= 1e-4                 # parity bar: against the reference dataflow
     assert d64 <= 1e-2                 # FP32-vs-FP64 error: reported, sanity-bounded only
     assert np.array_equal(gs, os_)
