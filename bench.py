@@ -13,7 +13,8 @@ Liu-Shen sweeps with a relative-residual acceptance test), falls back to
 the plain form when a rule fails, and lets the plain form judge every 32nd
 pair (EPE to plain <= 0.01 px, else the plain result).  One step = one
 batch of 32 pairs, judge and fallbacks included; inputs are resident in HBM
-before timing.
+before timing.  Both forms spread a batch's pairs over the same lanes
+(concurrent HIP streams of the library context, learned.LANES).
 
 Prints ONE JSON line.  `value` is the learned form's Mpix-pairs/s
 (finest-level H*W/1e6 per pair per second); `learned_vs_plain` holds the
@@ -530,34 +531,50 @@ def bench_lshs(args):
     fams = ["lamb_oseen"] + [("poiseuille", "lamb_oseen", "turbulence")[sp["family"]] for sp in specs]
     np_ = len(pairs)
     B = learned.CHECK_EVERY
+    L = learned.LANES
     net = Net.load()
-    outs = [(DeviceArray((n, n)), DeviceArray((n, n))) for _ in range(np_)]
+    # output slots: the t-th occurrence of pair p in a batch writes slot
+    # p + np_ * t, and a slot always runs on lane slot % L, so every slot's
+    # launch sequences stay captured on its lane
+    nslot = np_ * ((B + np_ - 1) // np_)
+    outs = [(DeviceArray((n, n)), DeviceArray((n, n))) for _ in range(nslot)]
     cnt = {"step": 0}
 
-    def batch_jobs(k):
-        return [(pairs[(k + j) % np_][0], pairs[(k + j) % np_][1], outs[(k + j) % np_][0], outs[(k + j) % np_][1])
-                for j in range(B)]
+    def batch(k):
+        seen, jobs, lanes = {}, [], []
+        for j in range(B):
+            p = (k + j) % np_
+            t = seen.get(p, 0)
+            seen[p] = t + 1
+            slot = p + np_ * t
+            jobs.append((pairs[p][0], pairs[p][1], outs[slot][0], outs[slot][1]))
+            lanes.append(slot % L)
+        return jobs, lanes
 
     def plain_step():
-        for a, b, U, V in batch_jobs(cnt["step"]):
-            learned.plain(a, b, fw, ow, Uacc=U, Vacc=V)
+        jobs, lanes = batch(cnt["step"])
+        learned.plain_many(jobs, fw, ow, lane_of=lanes)
         cnt["step"] += 1
 
     runs = {}
 
     def learned_step(nt, judge, key):
-        res = learned.solve_many(batch_jobs(cnt["step"]), nt, judge, fw, ow)
+        jobs, lanes = batch(cnt["step"])
+        res = learned.solve_many(jobs, nt, judge, fw, ow, lane_of=lanes)
         cnt["step"] += 1
         runs.setdefault(key, []).extend(res)
 
-    # graph priming: every pair's learned, plain and judge launch sequences
-    # are captured on their second call
+    # graph priming: every slot's learned and plain launch sequences, and the
+    # judge's plain run of every pair at a batch's first position, are
+    # captured on their second call
     for _ in range(2):
+        jobs, lanes = batch(0)
+        learned.plain_many(jobs, fw, ow, lane_of=lanes)
+        for nt in (net, None):
+            learned.solve_many(jobs, nt, None, fw, ow, lane_of=lanes)
         for k in range(np_):
-            a, b = pairs[k]
-            learned.plain(a, b, fw, ow, Uacc=outs[k][0], Vacc=outs[k][1])
-            for nt in (net, None):
-                learned.solve_many([(a, b, outs[k][0], outs[k][1])], nt, learned.Judge(every=1), fw, ow)
+            jobs, lanes = batch(k)
+            learned.solve_many(jobs[:1], net, learned.Judge(every=1), fw, ow, lane_of=lanes[:1])
     ctx.sync()
 
     def timed(step):
@@ -599,7 +616,8 @@ def bench_lshs(args):
                              "relative-residual acceptance test, plain fallback, EPE-to-plain judge on every "
                              "%dth pair (0.01 px), all timed" % (n, n, rules.ls_sweeps[0], rules.ls_sweeps[1], B),
                  "pairs_per_step": B, "height": n, "width": n, "levels": 2, "eval_pairs": np_,
-                 "hip_graphs": not args.no_graphs, "parallelism": "single GPU"},
+                 "hip_graphs": not args.no_graphs, "lanes": L,
+                 "parallelism": "single GPU, pairs on %d concurrent HIP streams (plain form likewise)" % L},
                 cheb_roofline(ctx, pairs[0][0], pairs[0][1], fw, n))
     out["roofline_plain_hs"] = hs_roofline(ctx, pairs[0][0], pairs[0][1], fw, n)
     out["learned_vs_plain"] = {

@@ -37,6 +37,24 @@ int ofri_version(void);                                   /* 10000*major + 100*m
 int ofri_create(int device, ofri_ctx** out);              /* one HIP stream per context */
 int ofri_destroy(ofri_ctx* ctx);
 int ofri_sync(ofri_ctx* ctx);
+
+/* Lanes (no counterpart in the reference, which runs one pair at a time on
+ * the CPU or one OpenCL queue): child contexts of a context on the same
+ * device, each with a HIP stream (default priority) and scratch, graph and
+ * table caches of its own, so independent pairs queued on different lanes
+ * run concurrently and fill the GPU where one pair's launches do not.  Any
+ * entry point takes a lane in place of a context.  A lane is created on
+ * first request, takes its parent's tuning and graph setting at every
+ * ofri_lane / ofri_fork call, is destroyed with its parent (ofri_destroy on
+ * a lane fails) and has no lanes of its own.  Device buffers are shared by
+ * the whole device: ordering between streams is the caller's, through
+ * ofri_fork (the parent's work queued so far precedes lanes 0..n-1's later
+ * work) and ofri_join (lanes 0..n-1's work so far precedes the parent's
+ * later work). */
+#define OFRI_MAX_LANES 8
+int ofri_lane(ofri_ctx* ctx, int k, ofri_ctx** out);       /* 0 <= k < OFRI_MAX_LANES */
+int ofri_fork(ofri_ctx* ctx, int n);
+int ofri_join(ofri_ctx* ctx, int n);
 int ofri_malloc(ofri_ctx* ctx, size_t bytes, void** d_out);
 int ofri_free(ofri_ctx* ctx, void* d_ptr);
 int ofri_upload(ofri_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);

@@ -786,6 +786,13 @@ struct LayerParams {
 
 }  // namespace
 
+// Offsets of every tensor of a (B, H, W) batch in one activation buffer.
+struct ActLayout {
+    int B = 0, H = 0, W = 0;
+    size_t t_off[T_N] = {};
+    size_t bytes = 0;
+};
+
 struct Net {
     int planes = 2;            // input planes (2: frame pair; 4: pair + flow)
     LayerDef def[kNumLayers];  // the layer table with enc0a's input width
@@ -798,11 +805,11 @@ struct Net {
     u16 *wb = nullptr, *wt = nullptr;
     long long step = 0;
     bool wt_fresh = false;
-    // activations / gradients for the current batch shape
-    int B = 0, H = 0, W = 0;
+    // training activations / gradients for the current batch shape (the
+    // inference entry takes its activations from the calling context)
+    ActLayout lay;
     void* act = nullptr;
     void* gact = nullptr;
-    size_t t_off[T_N] = {};
     size_t act_bytes = 0;
 };
 
@@ -843,41 +850,55 @@ inline dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 // stored channels of a tensor (the input tensor's follow the input planes)
 inline int tchan(const Net* n, int t) { return t == T_X0 ? 4 * n->planes : kTensors[t].C; }
 
-int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
+int check_shape(int B, int H, int W) {
     if (B < 1 || H < 16 || W < 16 || H % 16 || W % 16)
         return fail(OFRI_E_ARG, "net: batch %d of %dx%d (H, W must be multiples of 16)", B, H, W);
-    if (n->act && n->B == B && n->H == H && n->W == W) return OFRI_OK;
+    return OFRI_OK;
+}
+
+ActLayout act_layout(const Net* n, int B, int H, int W) {
+    ActLayout L;
+    L.B = B;
+    L.H = H;
+    L.W = W;
     size_t off = 0;
     for (int t = 0; t < T_N; ++t) {
         const int lh = H >> (kTensors[t].level + 1), lw = W >> (kTensors[t].level + 1);
-        n->t_off[t] = off;
+        L.t_off[t] = off;
         off += align256((size_t)B * lh * lw * tchan(n, t) * (kTensors[t].f32 ? 4 : 2));
     }
+    L.bytes = off;
+    return L;
+}
+
+// The training buffers (act, gact) laid out for a (B, H, W) batch.
+int net_shape(ofri_ctx* ctx, Net* n, int B, int H, int W) {
+    OFRI_TRY(check_shape(B, H, W));
+    if (n->act && n->lay.B == B && n->lay.H == H && n->lay.W == W) return OFRI_OK;
+    const ActLayout L = act_layout(n, B, H, W);
     // the buffers only grow: a smaller batch reuses them with new offsets
-    if (off > n->act_bytes) {
+    if (L.bytes > n->act_bytes) {
         OFRI_HIP(hipStreamSynchronize(ctx->stream));
         if (n->act) OFRI_HIP(hipFree(n->act));
         if (n->gact) OFRI_HIP(hipFree(n->gact));
         n->act = n->gact = nullptr;
         n->act_bytes = 0;
-        OFRI_HIP(hipMalloc(&n->act, off));
-        OFRI_HIP(hipMalloc(&n->gact, off));
-        n->act_bytes = off;
+        OFRI_HIP(hipMalloc(&n->act, L.bytes));
+        OFRI_HIP(hipMalloc(&n->gact, L.bytes));
+        n->act_bytes = L.bytes;
         ctx->ws_gen += 1;   // captured graphs holding the old buffers are stale
     }
-    n->B = B;
-    n->H = H;
-    n->W = W;
+    n->lay = L;
     return OFRI_OK;
 }
 
-View tview(Net* n, void* base, int t, int c0, int mode) {
+View tview(const Net* n, const ActLayout& L, void* base, int t, int c0, int mode) {
     const TensorDef& d = kTensors[t];
-    const int lh = n->H >> (d.level + 1), lw = n->W >> (d.level + 1);
+    const int lh = L.H >> (d.level + 1), lw = L.W >> (d.level + 1);
     const int C = tchan(n, t);
     View v;
     const size_t esz = d.f32 ? 4 : 2;
-    v.p = (char*)base + n->t_off[t] + (size_t)c0 * esz;
+    v.p = (char*)base + L.t_off[t] + (size_t)c0 * esz;
     v.img = (long long)lh * lw * C;
     v.ps = C;
     v.f32 = d.f32;
@@ -909,13 +930,25 @@ int launch_conv(ofri_ctx* ctx, const ConvArgs& a) {
 
 // coarse_head false: the coarse head (layer head_c) is skipped.  add_u,
 // add_v non-null (B = 1): the fine head's fields 0, 1 are added into these
-// full-size planes instead of being stored.
+// full-size planes instead of being stored.  own_act: the activations go to
+// the network's training buffers (kept for the backward pass), else to the
+// context's scratch, so one network can run on several streams at once.
 int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, bool coarse_head = true,
-                float* add_u = nullptr, float* add_v = nullptr) {
-    OFRI_TRY(net_shape(ctx, n, B, H, W));
+                float* add_u = nullptr, float* add_v = nullptr, bool own_act = true) {
+    ActLayout L;
+    void* base;
+    if (own_act) {
+        OFRI_TRY(net_shape(ctx, n, B, H, W));
+        L = n->lay;
+        base = n->act;
+    } else {
+        OFRI_TRY(check_shape(B, H, W));
+        L = act_layout(n, B, H, W);
+        OFRI_TRY(scratch(ctx, "net_act", L.bytes, &base));
+    }
     const long long hw = (long long)(H / 2) * (W / 2);
     const dim3 g((unsigned)((hw + 255) / 256), B);
-    u16* x0 = (u16*)((char*)n->act + n->t_off[T_X0]);
+    u16* x0 = (u16*)((char*)base + L.t_off[T_X0]);
     if (n->planes == 4) net_input_k<4><<<g, 256, 0, ctx->stream>>>(in, H, W, x0);
     else if (n->planes == 2) net_input_k<2><<<g, 256, 0, ctx->stream>>>(in, H, W, x0);
     else return fail(OFRI_E_ARG, "net: %d input planes not supported", n->planes);
@@ -925,12 +958,12 @@ int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, 
         if (!coarse_head && d.dst == T_HC) continue;
         const LayerParams& P = n->L[l];
         ConvArgs a{};
-        a.x = tview(n, n->act, d.src, d.src_c0, d.src_mode);
+        a.x = tview(n, L, base, d.src, d.src_c0, d.src_mode);
         a.gate.p = nullptr;
         a.w = n->wb + P.wb_off;
         a.Kp = P.Kp;
         a.bias = n->par + P.b_off;
-        a.y = tview(n, n->act, d.dst, d.dst_c0, d.dst_mode);
+        a.y = tview(n, L, base, d.dst, d.dst_c0, d.dst_mode);
         a.B = B;
         a.H = H >> (d.level + 1);
         a.W = W >> (d.level + 1);
@@ -951,7 +984,7 @@ int net_forward(ofri_ctx* ctx, Net* n, const InPlanes& in, int B, int H, int W, 
 // T_HF slots: parameter gradients into n->grad (zeroed here), activation
 // gradients through gact.
 int net_backward(ofri_ctx* ctx, Net* n) {
-    const int B = n->B, H = n->H, W = n->W;
+    const int B = n->lay.B, H = n->lay.H, W = n->lay.W;
     zero_f_k<<<grid1((long long)n->n_par), 256, 0, ctx->stream>>>(n->grad, (long long)n->n_par);
     OFRI_HIP(hipGetLastError());
     if (!n->wt_fresh) {
@@ -972,9 +1005,9 @@ int net_backward(ofri_ctx* ctx, Net* n) {
         const LayerParams& P = n->L[l];
         const int lh = H >> (d.level + 1), lw = W >> (d.level + 1);
         DwArgs g{};
-        g.x = tview(n, n->act, d.src, d.src_c0, d.src_mode);
-        g.dy = tview(n, n->gact, d.dst, d.dst_c0, d.dst_mode);
-        g.gate = d.act ? tview(n, n->act, d.dst, d.dst_c0, d.dst_mode) : View{nullptr, 0, 0, 0, 0};
+        g.x = tview(n, n->lay, n->act, d.src, d.src_c0, d.src_mode);
+        g.dy = tview(n, n->lay, n->gact, d.dst, d.dst_c0, d.dst_mode);
+        g.gate = d.act ? tview(n, n->lay, n->act, d.dst, d.dst_c0, d.dst_mode) : View{nullptr, 0, 0, 0, 0};
         g.dw = n->grad + P.w_off;
         g.db = n->grad + P.b_off;
         g.Kp = P.Kp;
@@ -996,7 +1029,7 @@ int net_backward(ofri_ctx* ctx, Net* n) {
         a.w = n->wt + P.wt_off;
         a.Kp = P.Kp_t;
         a.bias = nullptr;
-        a.y = tview(n, n->gact, d.src, d.src_c0, d.src_mode);
+        a.y = tview(n, n->lay, n->gact, d.src, d.src_c0, d.src_mode);
         a.B = B;
         a.H = lh;
         a.W = lw;
@@ -1139,7 +1172,7 @@ int net_predict(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, flo
     OFRI_TRY(net_forward(ctx, n, stacked(n, in, H, W), B, H, W));
     const int h = H / 2, w = W / 2;
     heads_to_planes_k<<<grid1((long long)B * h * w), 256, 0, ctx->stream>>>(
-        (const float*)((char*)n->act + n->t_off[T_HC]), (const float*)((char*)n->act + n->t_off[T_HF]), B, h, w,
+        (const float*)((char*)n->act + n->lay.t_off[T_HC]), (const float*)((char*)n->act + n->lay.t_off[T_HF]), B, h, w,
         coarse, fine);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
@@ -1153,8 +1186,9 @@ int net_correct(ofri_ctx* ctx, Net* n, const float* const* planes, int H, int W,
     for (int i = 0; i < n->planes && i < 4; ++i) in.p[i] = planes[i];
     in.bstride = 0;
     if (!add_u || !add_v) return fail(OFRI_E_ARG, "net_correct: null flow planes");
-    // the head's last kernel adds the correction into the flow
-    return net_forward(ctx, n, in, 1, H, W, false, add_u, add_v);
+    // the head's last kernel adds the correction into the flow; activations
+    // in the context's scratch
+    return net_forward(ctx, n, in, 1, H, W, false, add_u, add_v, false);
 }
 
 // Forward, loss against planar labels, backward; with lr > 0 an Adam step.
@@ -1168,9 +1202,9 @@ int net_train(ofri_ctx* ctx, Net* n, const float* in, int B, int H, int W, const
     OFRI_HIP(hipMemcpyAsync(fwb, h_field_w, 8 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     zero_f_k<<<1, 64, 0, ctx->stream>>>(d_loss, 1);
     loss_k<<<grid1((long long)B * h * w), 256, 0, ctx->stream>>>(
-        (const float*)((char*)n->act + n->t_off[T_HC]), (const float*)((char*)n->act + n->t_off[T_HF]), lab_c,
-        lab_f, B, h, w, (const float*)fwb, (float*)((char*)n->gact + n->t_off[T_HC]),
-        (float*)((char*)n->gact + n->t_off[T_HF]), d_loss);
+        (const float*)((char*)n->act + n->lay.t_off[T_HC]), (const float*)((char*)n->act + n->lay.t_off[T_HF]), lab_c,
+        lab_f, B, h, w, (const float*)fwb, (float*)((char*)n->gact + n->lay.t_off[T_HC]),
+        (float*)((char*)n->gact + n->lay.t_off[T_HF]), d_loss);
     OFRI_HIP(hipGetLastError());
     OFRI_TRY(net_backward(ctx, n));
     if (lr > 0.0f) {

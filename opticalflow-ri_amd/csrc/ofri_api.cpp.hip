@@ -76,17 +76,24 @@ const char* ofri_last_error(void) { return g_err.c_str(); }
 
 int ofri_version(void) { return 100; }
 
-int ofri_create(int device, ofri_ctx** out) {
-    if (!out) return fail(OFRI_E_ARG, "ofri_create: null output");
-    int n = 0;
-    OFRI_HIP(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) return fail(OFRI_E_ARG, "ofri_create: device %d of %d", device, n);
+}  // extern "C"
+
+namespace {
+
+// A context on `device` (made current): stream, timer and ordering events,
+// the allocation-free entry point's scratch address.
+int new_ctx(int device, ofri_ctx** out) {
     OFRI_HIP(hipSetDevice(device));
     ofri_ctx* c = new ofri_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
         delete c;
+        (void)hipGetLastError();
         return fail(OFRI_E_HIP, "ofri_create: stream/event creation failed");
     }
     // the allocation-free entry point's scratch on this device (device
@@ -101,8 +108,7 @@ int ofri_create(int device, ofri_ctx** out) {
     return OFRI_OK;
 }
 
-int ofri_destroy(ofri_ctx* ctx) {
-    if (!ctx) return OFRI_OK;
+void free_ctx(ofri_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->ws)
@@ -124,8 +130,77 @@ int ofri_destroy(ofri_ctx* ctx) {
         if (kv.second.p) (void)hipFree(kv.second.p);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->order_ev);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ofri_create(int device, ofri_ctx** out) {
+    if (!out) return fail(OFRI_E_ARG, "ofri_create: null output");
+    int n = 0;
+    OFRI_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(OFRI_E_ARG, "ofri_create: device %d of %d", device, n);
+    return new_ctx(device, out);
+}
+
+int ofri_destroy(ofri_ctx* ctx) {
+    if (!ctx) return OFRI_OK;
+    if (ctx->parent) return fail(OFRI_E_STATE, "ofri_destroy: a lane is destroyed with its parent context");
+    for (ofri_ctx* l : ctx->lanes) free_ctx(l);
+    ctx->lanes.clear();
+    free_ctx(ctx);
+    return OFRI_OK;
+}
+
+int ofri_lane(ofri_ctx* ctx, int k, ofri_ctx** out) {
+    CHECK_CTX(ctx);
+    if (!out) return fail(OFRI_E_ARG, "ofri_lane: null output");
+    if (ctx->parent) return fail(OFRI_E_STATE, "ofri_lane: a lane has no lanes of its own");
+    if (k < 0 || k >= OFRI_MAX_LANES) return fail(OFRI_E_ARG, "ofri_lane: lane %d outside 0..%d", k, OFRI_MAX_LANES - 1);
+    while ((int)ctx->lanes.size() <= k) {
+        ofri_ctx* l = nullptr;
+        OFRI_TRY(new_ctx(ctx->device, &l));
+        l->parent = ctx;
+        ctx->lanes.push_back(l);
+    }
+    ofri_ctx* l = ctx->lanes[k];
+    l->tuning = ctx->tuning;
+    l->graphs_on = ctx->graphs_on;
+    *out = l;
+    return OFRI_OK;
+}
+
+int ofri_fork(ofri_ctx* ctx, int n) {
+    CHECK_CTX(ctx);
+    if (ctx->parent) return fail(OFRI_E_STATE, "ofri_fork: called on a lane");
+    if (n < 0 || n > (int)ctx->lanes.size())
+        return fail(OFRI_E_ARG, "ofri_fork: %d lanes asked, %zu made", n, ctx->lanes.size());
+    OFRI_HIP(hipSetDevice(ctx->device));
+    OFRI_HIP(hipEventRecord(ctx->order_ev, ctx->stream));
+    for (int k = 0; k < n; ++k) {
+        ofri_ctx* l = ctx->lanes[k];
+        l->tuning = ctx->tuning;
+        l->graphs_on = ctx->graphs_on;
+        OFRI_HIP(hipStreamWaitEvent(l->stream, ctx->order_ev, 0));
+    }
+    return OFRI_OK;
+}
+
+int ofri_join(ofri_ctx* ctx, int n) {
+    CHECK_CTX(ctx);
+    if (ctx->parent) return fail(OFRI_E_STATE, "ofri_join: called on a lane");
+    if (n < 0 || n > (int)ctx->lanes.size())
+        return fail(OFRI_E_ARG, "ofri_join: %d lanes asked, %zu made", n, ctx->lanes.size());
+    OFRI_HIP(hipSetDevice(ctx->device));
+    for (int k = 0; k < n; ++k) {
+        ofri_ctx* l = ctx->lanes[k];
+        OFRI_HIP(hipEventRecord(l->order_ev, l->stream));
+        OFRI_HIP(hipStreamWaitEvent(ctx->stream, l->order_ev, 0));
+    }
     return OFRI_OK;
 }
 

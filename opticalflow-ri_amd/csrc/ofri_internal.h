@@ -114,6 +114,11 @@ struct ofri_ctx {
     void* noalloc = nullptr;
     size_t noalloc_bytes = 0;
     std::string noalloc_err;
+    // lanes: child contexts on the same device with streams of their own,
+    // owned by this context (a lane has a parent and no lanes)
+    ofri_ctx* parent = nullptr;
+    std::vector<ofri_ctx*> lanes;
+    hipEvent_t order_ev = nullptr;   // fork / join ordering point of this stream
 };
 
 namespace ofri {

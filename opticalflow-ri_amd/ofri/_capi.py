@@ -111,6 +111,9 @@ SIGNATURES = {
     "ofri_create": [C.c_int, C.POINTER(_vp)],
     "ofri_destroy": [_vp],
     "ofri_sync": [_vp],
+    "ofri_lane": [_vp, C.c_int, C.POINTER(_vp)],
+    "ofri_fork": [_vp, C.c_int],
+    "ofri_join": [_vp, C.c_int],
     "ofri_malloc": [_vp, _sz, C.POINTER(_vp)],
     "ofri_free": [_vp, _vp],
     "ofri_upload": [_vp, _vp, _vp, _sz],

@@ -32,6 +32,30 @@ class Context:
         if self.handle:
             self.lib.ofri_destroy(self.handle)
             self.handle = None
+        self._lanes = {}
+
+    def lane(self, k):
+        """Lane k of this context (ofri_lane): its own HIP stream on the same
+        device, owned by this context."""
+        lanes = self.__dict__.setdefault("_lanes", {})
+        if k not in lanes:
+            h = C.c_void_p()
+            _capi.check(self.lib.ofri_lane(self.handle, int(k), C.byref(h)))
+            lanes[k] = Lane(self, h, k)
+        else:
+            h = C.c_void_p()   # refreshes the lane's tuning and graph setting
+            _capi.check(self.lib.ofri_lane(self.handle, int(k), C.byref(h)))
+        return lanes[k]
+
+    def fork(self, n):
+        """Work queued on this context so far precedes lanes 0..n-1's later work."""
+        for k in range(n):
+            self.lane(k)
+        _capi.check(self.lib.ofri_fork(self.handle, int(n)))
+
+    def join(self, n):
+        """Lanes 0..n-1's work so far precedes this context's later work."""
+        _capi.check(self.lib.ofri_join(self.handle, int(n)))
 
     def set_tuning(self, hs_depth=0, ls_depth=0, hs_variant=0, ls_variant=0):
         t = _capi.Tuning(int(hs_depth), int(ls_depth), int(hs_variant), int(ls_variant))
@@ -58,6 +82,24 @@ class Context:
         ms = C.c_float()
         _capi.check(self.lib.ofri_timer_stop(self.handle, C.byref(ms)))
         return ms.value
+
+
+class Lane(Context):
+    """A lane of a Context: usable wherever a context is taken; destroyed
+    with its parent (which this object keeps alive)."""
+
+    def __init__(self, parent, handle, k):
+        self.lib = parent.lib
+        self.handle = handle
+        self.device = parent.device
+        self.parent = parent
+        self.index = k
+
+    def close(self):
+        pass
+
+    def lane(self, k):
+        raise ValueError("a lane has no lanes of its own")
 
 
 def context():

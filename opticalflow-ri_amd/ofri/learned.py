@@ -102,9 +102,9 @@ class Rules(object):
 RULES = Rules()
 
 
-def _learned(im1, im2, net, fw, ow, rules, Uacc=None, Vacc=None, stats=True, flag=None):
+def _learned(im1, im2, net, fw, ow, rules, Uacc=None, Vacc=None, stats=True, flag=None, ctx=None):
     return ops.pyr_lshs_learned(im1, im2, net=net, filt_w=fw, opt_w=ow, Uacc=Uacc, Vacc=Vacc, stats=stats,
-                                flag=flag, **rules.kwargs(),
+                                flag=flag, ctx=ctx, **rules.kwargs(),
                                 **{k: WORKLOAD[k] for k in ("levels", "alphas", "niter", "ls_alpha")})
 
 
@@ -195,15 +195,65 @@ def _copy(dst, src):
     ops._capi.check(ctx.lib.ofri_copy(ctx.handle, dst.ptr, src.ptr, src.nbytes))
 
 
-def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None):
+# Lanes (concurrent HIP streams of the context) that many-pair calls spread
+# their pairs over: one pair's launches leave part of the GPU idle (the
+# 512^2 level fills about two thirds of the CUs, the small kernels far
+# less), and a second and third pair on other streams fill it.  Measured on
+# MI355X at 1024^2: 1.21 / 0.92 / 0.83 ms per learned pair on 1 / 2 / 3
+# lanes.  Results do not depend on the lane count.
+LANES = 3
+
+
+def _lanes(n, lanes, lane_of):
+    """(ctx, contexts, assignment): pair k of n is queued on
+    contexts[assignment[k]] -- lanes of the process-wide context ctx (forked
+    from it), or ctx itself when one lane would do and none is named.
+    lane_of: explicit lane per pair (a caller that reuses buffers per lane
+    keeps its graphs that way); else pair k goes to lane k % L, L = lanes
+    (default LANES) capped at n."""
+    ctx = context()
+    if lane_of is not None:
+        lane_of = [int(i) for i in lane_of]
+        if len(lane_of) != n or any(i < 0 or i >= 8 for i in lane_of):
+            raise ValueError("lane_of: one lane index in 0..7 per pair")
+        L = max(lane_of) + 1 if lane_of else 1
+    else:
+        L = max(1, min(int(LANES if lanes is None else lanes), n, 8))
+        if L == 1:
+            return ctx, [ctx], [0] * n
+        lane_of = [k % L for k in range(n)]
+    ctx.fork(L)
+    return ctx, [ctx.lane(i) for i in range(L)], lane_of
+
+
+def _joined(ctx, on):
+    if on[0] is not ctx:
+        ctx.join(len(on))
+
+
+def plain_many(jobs, fw=None, ow=None, lanes=None, lane_of=None):
+    """The plain pyramid for many pairs, spread over lanes like solve_many
+    (no synchronisation; the process-wide context's later work follows it).
+    jobs: list of (im1, im2, Uacc, Vacc)."""
+    if fw is None:
+        fw, ow = filters()
+    ctx, on, lane = _lanes(len(jobs), lanes, lane_of)
+    for k, (a, b, U, V) in enumerate(jobs):
+        ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=U, Vacc=V, stats=False, ctx=on[lane[k]], **WORKLOAD)
+    _joined(ctx, on)
+
+
+def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None, lanes=None, lane_of=None):
     """The learned form for many pairs with one host synchronisation: every
     pair is queued with its acceptance flag written on the device; pairs the
     judge picks also run the plain pyramid and their EPE to it is reduced on
     the device; flags and EPEs are read once; pairs that missed a rule run
     the plain pyramid, checked pairs that failed the EPE bar take the plain
-    result.  jobs: list of (im1, im2, Uacc, Vacc) DeviceArrays (Uacc, Vacc
-    receive the result).  Returns one dict per pair: fell_back, checked, epe
-    (checked pairs) and accepted (the rules met)."""
+    result.  Pair k (with its check) is queued on lane k % L of L = lanes
+    (default LANES) concurrent streams, or on lane lane_of[k].  jobs: list
+    of (im1, im2, Uacc, Vacc) DeviceArrays (Uacc, Vacc receive the result;
+    no two pairs of a call may share them).  Returns one dict per pair:
+    fell_back, checked, epe (checked pairs) and accepted (the rules met)."""
     if fw is None:
         fw, ow = filters()
     rules = rules or RULES
@@ -211,13 +261,16 @@ def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None):
     words = _POOL.flags(n)
     checked = [judge.pick() if judge is not None else False for _ in range(n)]
     refs = {}
+    ctx, on, lane = _lanes(n, lanes, lane_of)
     for k, (a, b, U, V) in enumerate(jobs):
-        _learned(a, b, net, fw, ow, rules, Uacc=U, Vacc=V, stats=False, flag=words.view(k, (1,)))
+        c = on[lane[k]]
+        _learned(a, b, net, fw, ow, rules, Uacc=U, Vacc=V, stats=False, flag=words.view(k, (1,)), ctx=c)
         if checked[k]:
             Up, Vp = _POOL.plain_pair(k, U.shape)
-            plain(a, b, fw, ow, Uacc=Up, Vacc=Vp)
-            ops.flow_epe(U, V, Up, Vp, out=words.view(n + k, (1,)))
+            ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=Up, Vacc=Vp, stats=False, ctx=c, **WORKLOAD)
+            ops.flow_epe(U, V, Up, Vp, out=words.view(n + k, (1,)), ctx=c)
             refs[k] = (Up, Vp)
+    _joined(ctx, on)
     w = words.download().reshape(-1)[:2 * n]
     ok = w[:n].view(np.int32)
     ep = w[n:2 * n]

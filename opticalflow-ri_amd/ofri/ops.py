@@ -25,8 +25,9 @@ def _lib():
     return context().lib
 
 
-def _h():
-    return context().handle
+def _h(ctx=None):
+    """The handle calls go to: ctx (a Context or a lane) or the process-wide context."""
+    return (ctx or context()).handle
 
 
 def gauss_sep(img, weights, out=None):
@@ -369,15 +370,16 @@ def _stats_dicts(st):
                  ls_nrm=np.float32(s.ls_nrm)) for s in st]
 
 
-def flow_epe(U1, V1, U2, V2, out=None):
+def flow_epe(U1, V1, U2, V2, out=None, ctx=None):
     """Mean endpoint error of two device flows, written to a one-word device
-    array (ofri_flow_epe) with no synchronisation.  Returns that array."""
+    array (ofri_flow_epe) with no synchronisation.  Returns that array.
+    ctx: the context or lane queued on (default: the process-wide one)."""
     out = out if out is not None else DeviceArray((1,))
     n = int(np.prod(U1.shape))
     for a in (V1, U2, V2):
         if int(np.prod(a.shape)) != n:
             raise ValueError("flow_epe: flows differ in size")
-    _capi.check(_lib().ofri_flow_epe(_h(), U1.ptr, V1.ptr, U2.ptr, V2.ptr, n, out.ptr))
+    _capi.check(_lib().ofri_flow_epe(_h(ctx), U1.ptr, V1.ptr, U2.ptr, V2.ptr, n, out.ptr))
     return out
 
 
@@ -412,7 +414,7 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
                      hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
                      ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None, hs_finish_budget=0,
-                     relative=False, ls_sweeps=0):
+                     relative=False, ls_sweeps=0, ctx=None):
     """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
     correction network and Liu-Shen run to the per-level residual targets
     hs_tol / ls_tol (relative: as multiples of the tested iterate's norm
@@ -421,8 +423,9 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     (the method without it).  hs_budget, cheb_bound, cheb_finish,
     hs_finish_budget, ls_sweeps: one value or (coarse, fine).  flag: optional
     DeviceArray view of one 4-byte word receiving the convergence flag (int
-    1 / 0) with no synchronisation.  Returns (Uacc, Vacc, stats, converged);
-    converged is None when stats is False."""
+    1 / 0) with no synchronisation.  ctx: the context or lane queued on
+    (default: the process-wide one).  Returns (Uacc, Vacc, stats,
+    converged); converged is None when stats is False."""
     im1, im2 = as_device(im1), as_device(im2)
     H, W = im1.shape[-2:]
     p, keep = _lshs_params(levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum, ls_tol_plain)
@@ -440,7 +443,7 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None
     conv = C.c_int(0)
-    _capi.check(_lib().ofri_pyr_lshs_learned(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), C.byref(lp),
+    _capi.check(_lib().ofri_pyr_lshs_learned(_h(ctx), im1.ptr, im2.ptr, H, W, C.byref(p), C.byref(lp),
                                              net.handle if net is not None else None, Uacc.ptr, Vacc.ptr, st,
                                              C.byref(conv) if stats else None,
                                              flag.ptr if flag is not None else None))
@@ -461,8 +464,9 @@ def level_sizes(H, W, levels):
 
 
 def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum=60, ls_tol=1e-8,
-             Uacc=None, Vacc=None, stats=True):
-    """Whole LS(HS) pyramid on the device.  Returns (Uacc, Vacc, stats list)."""
+             Uacc=None, Vacc=None, stats=True, ctx=None):
+    """Whole LS(HS) pyramid on the device.  ctx: the context or lane queued
+    on (default: the process-wide one).  Returns (Uacc, Vacc, stats list)."""
     im1, im2 = as_device(im1), as_device(im2)
     H, W = im1.shape
     al = np.ascontiguousarray(alphas, dtype=np.float32)
@@ -484,7 +488,7 @@ def pyr_lshs(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_maxnum
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None
-    _capi.check(_lib().ofri_pyr_lshs(_h(), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
+    _capi.check(_lib().ofri_pyr_lshs(_h(ctx), im1.ptr, im2.ptr, H, W, C.byref(p), Uacc.ptr, Vacc.ptr,
                                      st if stats else None))
     out = _stats_dicts(st) if stats else []
     return Uacc, Vacc, out

@@ -196,3 +196,47 @@ def test_queued_pairs_the_judge_and_the_fallback(trained):
                              rules=learned.Rules(hs_budget=8))
     assert [r["fell_back"] for r in res] == [True, True] and not any(r["accepted"] for r in res)
     assert np.array_equal(bufs[1][0].download(), t["plain"][1][0])
+
+
+def test_lanes_give_the_bits_of_one_stream(trained):
+    """Pairs spread over lanes (concurrent streams, each with its own scratch
+    and captured graphs) give the bits of the same pairs queued one after
+    another on the library stream -- on the eager call, the capture and the
+    replays, with the default round-robin and with explicit lanes; the
+    plain form likewise.  A lane cannot be destroyed on its own, has no lanes
+    and the lane index is bounded."""
+    import ctypes as C
+    from ofri import learned, _capi
+    from ofri.device import DeviceArray, context
+    t = trained
+    ref = {}
+    for fam in (1, 2):
+        a, b = t["pairs"][fam]
+        U, V, _ = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False)
+        ref[fam] = (U.download(), V.download())
+    fams = [1, 2, 2, 1, 1, 2]
+    bufs = [(DeviceArray((1024, 1024)), DeviceArray((1024, 1024))) for _ in fams]
+    jobs = [t["pairs"][f] + bufs[k] for k, f in enumerate(fams)]
+    for kw in [dict(lanes=1)] + [dict(lanes=3)] * 3 + [dict(lane_of=[2, 2, 0, 1, 0, 1])] * 3:
+        for U, V in bufs:
+            U.upload(np.full((1024, 1024), np.nan, np.float32))
+            V.upload(np.full((1024, 1024), np.nan, np.float32))
+        res = learned.solve_many(jobs, t["net"], None, t["fw"], t["ow"], **kw)
+        assert all(r["accepted"] and not r["fell_back"] for r in res), kw
+        for k, f in enumerate(fams):
+            assert np.array_equal(bufs[k][0].download(), ref[f][0]), (kw, k)
+            assert np.array_equal(bufs[k][1].download(), ref[f][1]), (kw, k)
+    for rep in range(3):
+        learned.plain_many(jobs[:4], t["fw"], t["ow"], lanes=3)
+        for k, f in enumerate(fams[:4]):
+            assert np.array_equal(bufs[k][0].download(), t["plain"][f][0]), (rep, k)
+            assert np.array_equal(bufs[k][1].download(), t["plain"][f][1]), (rep, k)
+    ctx = context()
+    lib = ctx.lib
+    lane = ctx.lane(0)
+    h = C.c_void_p()
+    assert lib.ofri_destroy(lane.handle) == _capi.OFRI_E_STATE
+    assert lib.ofri_lane(lane.handle, 0, C.byref(h)) == _capi.OFRI_E_STATE
+    assert lib.ofri_lane(ctx.handle, 8, C.byref(h)) == _capi.OFRI_E_ARG
+    assert lib.ofri_fork(ctx.handle, 9) == _capi.OFRI_E_ARG
+    assert lib.ofri_join(lane.handle, 1) == _capi.OFRI_E_STATE
