@@ -707,10 +707,9 @@ int omega_table(ofri_ctx* ctx, double sigma, const float** out) {
         }
         void* p = nullptr;
         if (hipMalloc(&p, sizeof t) != hipSuccess) return fail(OFRI_E_NOMEM, "hs: Chebyshev weight table");
-        hipError_t e = hipMemcpy(p, t, sizeof t, hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
+        if (upload_table(ctx, p, t, sizeof t, "hs: Chebyshev weight table") != OFRI_OK) {
             (void)hipFree(p);
-            return fail(OFRI_E_HIP, "hs: Chebyshev weight upload: %s", hipGetErrorString(e));
+            return OFRI_E_HIP;
         }
         b.p = p;
         b.bytes = sizeof t;

@@ -610,8 +610,8 @@ int get_resize_table(ofri_ctx* ctx, int in, int out, int filter, ResizeTable** t
     r.last = b[2 * (out - 1)] + b[2 * (out - 1) + 1];
     OFRI_HIP(hipMalloc(&r.d_k, k.size() * sizeof(double)));
     OFRI_HIP(hipMalloc(&r.d_bounds, b.size() * sizeof(int)));
-    OFRI_HIP(hipMemcpy(r.d_k, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
-    OFRI_HIP(hipMemcpy(r.d_bounds, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+    OFRI_TRY(upload_table(ctx, r.d_k, k.data(), k.size() * sizeof(double), "resize weight table"));
+    OFRI_TRY(upload_table(ctx, r.d_bounds, b.data(), b.size() * sizeof(int), "resize bounds table"));
     auto res = ctx->resize_tables.emplace(key, r);
     *t = &res.first->second;
     return OFRI_OK;
@@ -682,9 +682,9 @@ int get_spline_table(ofri_ctx* ctx, int n, int N, SplineTable** t) {
     OFRI_HIP(hipMalloc(&s.d_w, w.size() * sizeof(double)));
     OFRI_HIP(hipMalloc(&s.d_wT, wt.size() * sizeof(double)));
     OFRI_HIP(hipMalloc(&s.d_start, st.size() * sizeof(int)));
-    OFRI_HIP(hipMemcpy(s.d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-    OFRI_HIP(hipMemcpy(s.d_wT, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice));
-    OFRI_HIP(hipMemcpy(s.d_start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    OFRI_TRY(upload_table(ctx, s.d_w, w.data(), w.size() * sizeof(double), "spline weight table"));
+    OFRI_TRY(upload_table(ctx, s.d_wT, wt.data(), wt.size() * sizeof(double), "spline weight table"));
+    OFRI_TRY(upload_table(ctx, s.d_start, st.data(), st.size() * sizeof(int), "spline start table"));
     auto res = ctx->spline_tables.emplace(key, s);
     *t = &res.first->second;
     return OFRI_OK;

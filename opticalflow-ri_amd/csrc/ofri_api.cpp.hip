@@ -44,6 +44,15 @@ int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out) {
     return OFRI_OK;
 }
 
+int upload_table(ofri_ctx* ctx, void* dst, const void* src, size_t bytes, const char* what) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    OFRI_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return fail(OFRI_E_STATE, "%s requested while capturing a graph", what);
+    OFRI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    OFRI_HIP(hipStreamSynchronize(ctx->stream));
+    return OFRI_OK;
+}
+
 // Level sizes of the pyramid (GenericPyramidalOpticalFlow.py:291,335-347,414):
 // level l < L is resized to round(dim * 2^-(L-l)) (half-to-even), level L keeps
 // the input size.

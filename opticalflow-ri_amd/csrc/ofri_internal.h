@@ -138,6 +138,11 @@ __device__ __forceinline__ int pf_index(int p, int L, int h) {
 
 // Named scratch buffer of at least `bytes` bytes (grows, never shrinks).
 int scratch(ofri_ctx* ctx, const std::string& name, size_t bytes, void** out);
+// Host table -> device, on the context's stream and finished on return (no
+// copy goes through the device's null stream, which would take a hardware
+// queue of its own); refused while the stream captures a graph.  `what`
+// names the table in the error.
+int upload_table(ofri_ctx* ctx, void* dst, const void* src, size_t bytes, const char* what);
 
 // ---- image kernels (k_image.hip) ----
 // "2" variants: the same operation on two planes in one launch (second plane

@@ -195,40 +195,42 @@ def _copy(dst, src):
     ops._capi.check(ctx.lib.ofri_copy(ctx.handle, dst.ptr, src.ptr, src.nbytes))
 
 
-# Lanes (concurrent HIP streams of the context) that many-pair calls spread
-# their pairs over: one pair's launches leave part of the GPU idle (the
-# 512^2 level fills about two thirds of the CUs, the small kernels far
-# less), and a second and third pair on other streams fill it.  Measured on
-# MI355X at 1024^2: 1.21 / 0.92 / 0.83 ms per learned pair on 1 / 2 / 3
-# lanes.  Results do not depend on the lane count.
-LANES = 3
+# Streams that many-pair calls spread their pairs over: the library
+# context's own and LANES - 1 of its lanes (concurrent HIP streams).  One
+# pair's launches leave part of the GPU idle (the 512^2 level fills about two
+# thirds of the CUs, the small kernels far less); pairs on other streams fill
+# it.  Four streams match the four hardware queues HIP gives a process by
+# default (more streams share queues).  Measured on MI355X at 1024^2:
+# 1.22 / 0.92 / 0.83 / 0.775 ms per learned pair and 4.53 / 3.11 / 3.13 /
+# 2.94 ms per plain pair on 1 / 2 / 3 / 4 streams.  Results do not depend on
+# the stream count.
+LANES = 4
 
 
 def _lanes(n, lanes, lane_of):
     """(ctx, contexts, assignment): pair k of n is queued on
-    contexts[assignment[k]] -- lanes of the process-wide context ctx (forked
-    from it), or ctx itself when one lane would do and none is named.
-    lane_of: explicit lane per pair (a caller that reuses buffers per lane
-    keeps its graphs that way); else pair k goes to lane k % L, L = lanes
-    (default LANES) capped at n."""
+    contexts[assignment[k]]; contexts[0] is the process-wide context ctx,
+    contexts[i] its lane i - 1 (forked from ctx).  lane_of: explicit index
+    per pair (a caller that reuses buffers per index keeps its captured
+    graphs that way); else pair k goes to index k % L, L = lanes (default
+    LANES) capped at n."""
     ctx = context()
     if lane_of is not None:
         lane_of = [int(i) for i in lane_of]
         if len(lane_of) != n or any(i < 0 or i >= 8 for i in lane_of):
-            raise ValueError("lane_of: one lane index in 0..7 per pair")
+            raise ValueError("lane_of: one index in 0..7 per pair")
         L = max(lane_of) + 1 if lane_of else 1
     else:
         L = max(1, min(int(LANES if lanes is None else lanes), n, 8))
-        if L == 1:
-            return ctx, [ctx], [0] * n
         lane_of = [k % L for k in range(n)]
-    ctx.fork(L)
-    return ctx, [ctx.lane(i) for i in range(L)], lane_of
+    if L > 1:
+        ctx.fork(L - 1)
+    return ctx, [ctx] + [ctx.lane(i) for i in range(L - 1)], lane_of
 
 
 def _joined(ctx, on):
-    if on[0] is not ctx:
-        ctx.join(len(on))
+    if len(on) > 1:
+        ctx.join(len(on) - 1)
 
 
 def plain_many(jobs, fw=None, ow=None, lanes=None, lane_of=None):
@@ -249,8 +251,9 @@ def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None, lanes=None, 
     judge picks also run the plain pyramid and their EPE to it is reduced on
     the device; flags and EPEs are read once; pairs that missed a rule run
     the plain pyramid, checked pairs that failed the EPE bar take the plain
-    result.  Pair k (with its check) is queued on lane k % L of L = lanes
-    (default LANES) concurrent streams, or on lane lane_of[k].  jobs: list
+    result.  Pair k (with its check) is queued on stream k % L of L = lanes
+    (default LANES) concurrent streams, or on stream lane_of[k] (0: the
+    library context's, i: its lane i - 1).  jobs: list
     of (im1, im2, Uacc, Vacc) DeviceArrays (Uacc, Vacc receive the result;
     no two pairs of a call may share them).  Returns one dict per pair:
     fell_back, checked, epe (checked pairs) and accepted (the rules met)."""

@@ -217,7 +217,7 @@ def test_lanes_give_the_bits_of_one_stream(trained):
     fams = [1, 2, 2, 1, 1, 2]
     bufs = [(DeviceArray((1024, 1024)), DeviceArray((1024, 1024))) for _ in fams]
     jobs = [t["pairs"][f] + bufs[k] for k, f in enumerate(fams)]
-    for kw in [dict(lanes=1)] + [dict(lanes=3)] * 3 + [dict(lane_of=[2, 2, 0, 1, 0, 1])] * 3:
+    for kw in [dict(lanes=1)] + [dict(lanes=4)] * 3 + [dict(lane_of=[3, 3, 0, 1, 0, 2])] * 3:
         for U, V in bufs:
             U.upload(np.full((1024, 1024), np.nan, np.float32))
             V.upload(np.full((1024, 1024), np.nan, np.float32))
@@ -227,7 +227,7 @@ def test_lanes_give_the_bits_of_one_stream(trained):
             assert np.array_equal(bufs[k][0].download(), ref[f][0]), (kw, k)
             assert np.array_equal(bufs[k][1].download(), ref[f][1]), (kw, k)
     for rep in range(3):
-        learned.plain_many(jobs[:4], t["fw"], t["ow"], lanes=3)
+        learned.plain_many(jobs[:4], t["fw"], t["ow"], lanes=4)
         for k, f in enumerate(fams[:4]):
             assert np.array_equal(bufs[k][0].download(), t["plain"][f][0]), (rep, k)
             assert np.array_equal(bufs[k][1].download(), t["plain"][f][1]), (rep, k)
