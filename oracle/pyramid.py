@@ -21,7 +21,17 @@ generic_pyramid follows genericPyramidalOpticalFlow
     else a copy (:367-377); with an optional adapter, FILTER_OPT > 1e-3: the
     5-tap pre-filter on the un-warped level pair (:379-385);
   - main(im1, im2, U, V) -> (U, V), then optional(opt1, opt2, U, V) -> (U, V),
-    then the accumulator += U, V in float32 (:405-413).  kLevels = 1.
+    then the accumulator += U, V in float32 (:405-413);
+  - biLinear False (the Liu-Shen warping mode, :188-220): the level's first
+    frame is warped in place by the upsampled flow (`ls_warp`) and the second
+    is left alone, so the 5-tap pre-filter and the re-warps below see the
+    warped first frame (:353, :381, :393);
+  - kLevels > 1 (:387-413): iteration k > 0 with warping warps a copy of the
+    level's (possibly already warped) first frame and the second frame by the
+    whole accumulated flow, restarts the solver from zero and pre-filters
+    again when FILTER > 1; without warping the accumulated flow becomes the
+    solver's initial flow and the accumulator restarts at zero, on the same
+    pre-filtered pair.  The optional adapter's pair is formed once per level.
 
 Adapters are callables over numpy arrays (prec selects the oracle instance
 they use); the helpers below build the reference's HS, Liu-Shen, Farneback
@@ -46,9 +56,10 @@ def _weights(sigma, n, prec):
 
 
 def generic_pyramid(im1, im2, FILTER, main, levels=1, FILTER_OPT=None, optional=None, warping=True,
-                    interScaling=True, finalScaling=False, prec="f32", filt_weights=None):
+                    interScaling=True, finalScaling=False, prec="f32", filt_weights=None, biLinear=True, kLevels=1):
     """Returns (Uaccum, Vaccum).  filt_weights(sigma, n) -> weights
-    (default: prepareGaussianKernel's formula in the instance's precision)."""
+    (default: prepareGaussianKernel's formula in the instance's precision).
+    The Liu-Shen warping mode (biLinear False) is restated in float32 only."""
     dt = np.float32 if prec == "f32" else np.float64
     wfun = filt_weights or (lambda s, n: _weights(s, n, prec))
     im1 = np.asarray(im1, dt)
@@ -76,7 +87,11 @@ def generic_pyramid(im1, im2, FILTER, main, levels=1, FILTER_OPT=None, optional=
                 us = (Uacc * sx).astype(dt)
                 vs = (Vacc * sy).astype(dt)
             if warping:
-                w1, w2 = orc.warp_symmetric(n1, n2, us, vs, prec)
+                if biLinear:
+                    w1, w2 = orc.warp_symmetric(n1, n2, us, vs, prec)
+                else:
+                    n1 = orc.ls_warp(n1, us, vs)      # the level's frame itself is moved
+                    w1, w2 = n1, n2
                 U, V = np.zeros((h, w), dt), np.zeros((h, w), dt)
                 Uacc, Vacc = us, vs
             else:
@@ -98,11 +113,27 @@ def generic_pyramid(im1, im2, FILTER, main, levels=1, FILTER_OPT=None, optional=
                 o1, o2 = orc.gauss_sep(n1, k5, prec), orc.gauss_sep(n2, k5, prec)
             else:
                 o1, o2 = n1.copy(), n2
-        U, V = main(f1, f2, U, V, level)
-        if optional is not None:
-            U, V = optional(o1, o2, U, V, level)
-        Uacc = (Uacc + np.asarray(U, dt)).astype(dt)
-        Vacc = (Vacc + np.asarray(V, dt)).astype(dt)
+        for k in range(kLevels):
+            if k > 0:
+                if warping:
+                    if biLinear:
+                        w1, w2 = orc.warp_symmetric(n1, n2, Uacc, Vacc, prec)
+                    else:
+                        w1, w2 = orc.ls_warp(n1, Uacc, Vacc), n2
+                    U, V = np.zeros((h, w), dt), np.zeros((h, w), dt)
+                    if FILTER > 1:
+                        k3 = wfun(FILTER, 3)
+                        f1, f2 = orc.gauss_sep(w1, k3, prec), orc.gauss_sep(w2, k3, prec)
+                    else:
+                        f1, f2 = w1.copy(), w2
+                else:
+                    U, V = Uacc, Vacc
+                    Uacc, Vacc = np.zeros((h, w), dt), np.zeros((h, w), dt)
+            U, V = main(f1, f2, U, V, level)
+            if optional is not None:
+                U, V = optional(o1, o2, U, V, level)
+            Uacc = (Uacc + np.asarray(U, dt)).astype(dt)
+            Vacc = (Vacc + np.asarray(V, dt)).astype(dt)
         prev = (h, w)
         scale *= 2
     return Uacc, Vacc

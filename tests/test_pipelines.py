@@ -75,6 +75,49 @@ def test_composition_one_level_is_the_adapter():
     assert np.allclose(seen[1][1], 1.0, atol=1e-6)      # LK's defaults: no x2 at the last level
 
 
+def test_composition_k_iterations_without_warping_continue_the_solver():
+    """kLevels = 2 without warping hands the accumulated flow back to the
+    solver as its start on the same pre-filtered pair
+    (GenericPyramidalOpticalFlow.py:395-396, 223-227): HS 30 sweeps twice is
+    HS 60 sweeps once, bit for bit (one level)."""
+    from ofri import synth
+    im1, im2, _, _ = synth.config_pair(4, size=64)
+    U2, V2 = opyr.generic_pyramid(im1, im2, 3.4, opyr.hs_adapter([21.0, 21.0], 30), 1, warping=False,
+                                  kLevels=2)
+    U1, V1 = opyr.generic_pyramid(im1, im2, 3.4, opyr.hs_adapter([21.0], 60), 1, warping=False)
+    assert np.array_equal(U1, U2) and np.array_equal(V1, V2)
+
+
+def test_composition_k_iterations_with_warping_rewarp_the_level_frames():
+    """kLevels = 2 with warping: the second iteration warps the level's pair
+    by the whole accumulated flow and restarts the solver from zero
+    (GenericPyramidalOpticalFlow.py:387-394).  A spy solver sees zero flow
+    both times and, in the second, exactly the pre-filtered symmetric warp
+    (bilinear) or Liu-Shen warp of the first frame (biLinear False) by the
+    flow the first iteration returned (up and left: the Liu-Shen mode's
+    targets then wrap instead of leaving the image)."""
+    from ofri import synth
+    im1, im2, _, _ = synth.config_pair(4, size=64)
+    step = np.float32(0.75)
+    for bil in (True, False):
+        seen = []
+
+        def spy(f1, f2, U0, V0, level):
+            seen.append((f1.copy(), f2.copy(), U0.copy(), V0.copy()))
+            return U0 - step, V0 - step
+
+        Ua, Va = opyr.generic_pyramid(im1, im2, 3.4, spy, 1, warping=True, biLinear=bil, kLevels=2)
+        assert len(seen) == 2 and not seen[0][2].any() and not seen[1][2].any() and not seen[1][3].any()
+        acc_u, acc_v = np.full((64, 64), -step), np.full((64, 64), -step)
+        k3 = opyr._weights(3.4, 3, "f32")
+        if bil:
+            w1, w2 = orc.warp_symmetric(im1, im2, acc_u, acc_v)
+        else:
+            w1, w2 = orc.ls_warp(im1, acc_u, acc_v), im2
+        assert np.array_equal(seen[1][0], orc.gauss_sep(w1, k3)) and np.array_equal(seen[1][1], orc.gauss_sep(w2, k3))
+        assert np.array_equal(Ua, acc_u - step) and np.array_equal(Va, acc_v - step)
+
+
 # ------------------------------------------------------------------ GPU
 @pytest.fixture(scope="module")
 def ops():
@@ -229,3 +272,34 @@ def test_lshs_full_size_matches_oracle(ops):
     d32, _, de = _report("LS(HS) 1024^2", U, V, U32, V32, U64, V64, ut, vt, border=64)
     assert d32 <= FLOW_TOL and de <= EPE_TOL
     assert [s["ls_iters"] for s in st] == [int(x) for x in s32[:, 2]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("warping,bil", [(True, True), (True, False), (False, True)])
+def test_k_iterations_and_liu_shen_warping_match_oracle(ops, warping, bil):
+    """genericPyramidalOpticalFlow with kLevels = 2 (re-warp or continue
+    iterations) in each warping mode -- bilinear, the Liu-Shen mode (whose
+    re-warp moves the already warped first frame again) and none -- against
+    the oracle's composition of the same loop: the same bits."""
+    from GenericPyramidalOpticalFlow import genericPyramidalOpticalFlow
+    from HornSchunck import HSOpticalFlowAlgoAdapter
+    from PhysicsBasedOpticalFlowLiuShen import LiuShenOpticalFlowAlgoAdapter
+    from ofri import synth
+
+    class NoDefaults(HSOpticalFlowAlgoAdapter):       # keep the caller's warping mode
+        def hasGenericPyramidalDefaults(self):
+            return False
+
+    # a uniform shift up and left: the Liu-Shen mode's targets wrap rather
+    # than leave the image
+    rng = np.random.default_rng(61)
+    im1, im2, _, _ = synth.piv_pair(96, 128, 0.05, lambda x, y: (-1.3 + 0 * x, -0.6 + 0 * y), rng)
+    alphas = [21.0, 21.0, 45.0, 45.0]
+    U, V = genericPyramidalOpticalFlow(im1.copy(), im2.copy(), 3.4, NoDefaults(list(alphas), 60), 2, 2, 0.48,
+                                       LiuShenOpticalFlowAlgoAdapter(5), warping, bil, True, True)
+    Uo, Vo = opyr.generic_pyramid(im1, im2, 3.4, opyr.hs_adapter(alphas, 60), 2, 0.48, opyr.ls_adapter(5.0),
+                                  warping=warping, interScaling=True, finalScaling=True, biLinear=bil, kLevels=2)
+    d = max(np.abs(U - Uo).max(), np.abs(V - Vo).max())
+    print("[kLevels 2, warping %s, biLinear %s] max|d| vs oracle-f32 %.3g px" % (warping, bil, d))
+    assert np.array_equal(U, Uo) and np.array_equal(V, Vo)
+
