@@ -481,6 +481,20 @@ int ofri_set_tuning(ofri_ctx* ctx, const ofri_tuning* t);
 int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t);
 /* The depths and 1-based tile shapes the sweeps actually use for an H x W image. */
 int ofri_query_tuning(ofri_ctx* ctx, int H, int W, ofri_tuning* out);
+/* Size classes of launch shapes (no counterpart in the reference: the
+ * autotuner's per-size results, which ofri.autotune caches per GPU
+ * architecture).  For a plain Horn-Schunck or Liu-Shen solve of an image, or
+ * a batch by its total pixel count, the class with the largest min_px at or
+ * below the pixel count supplies the depth and tile shape fields it sets
+ * (> 0); ofri_set_tuning's fields (> 0) still override.  The learned form's
+ * accelerated sweeps keep their own shapes.  n = 0 clears; at most 16. */
+typedef struct ofri_tuning_class {
+    long long min_px;
+    ofri_tuning t;
+} ofri_tuning_class;
+int ofri_set_tuning_classes(ofri_ctx* ctx, const ofri_tuning_class* classes, int n);
+/* The device's architecture name (e.g. "gfx950"), NUL-terminated in out[len]. */
+int ofri_arch(ofri_ctx* ctx, char* out, int len);
 
 /* HIP-graph replay of ofri_pyr_lshs (default on): the first call with a given
  * set of pointers, sizes, parameters and tuning runs eagerly, the second

@@ -734,10 +734,14 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         return fail(OFRI_E_ARG, "hs: a relative tolerance needs the per-launch stop test (and no history)");
     size_t N = (size_t)H * W;
     int vi, depth;
-    if (cheb) hs_cheb_default((size_t)H * W, &vi, &depth);
-    else hs_default((size_t)H * W, &vi, &depth);
-    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
-    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    if (cheb) {
+        hs_cheb_default((size_t)H * W, &vi, &depth);
+        if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
+        if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    } else {
+        hs_default((size_t)H * W, &vi, &depth);
+        shape_override(ctx, (size_t)H * W, true, &vi, &depth);
+    }
     if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
     const HsVariant& var = kHsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
@@ -916,8 +920,7 @@ int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter,
     }
     int vi, depth;
     hs_default(total, &vi, &depth);
-    if (ctx->tuning.hs_depth > 0) depth = ctx->tuning.hs_depth;
-    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    shape_override(ctx, total, true, &vi, &depth);
     if (vi >= kNumHsVariants) return fail(OFRI_E_ARG, "hs: tile variant %d not compiled", vi + 1);
     const HsVariant& var = kHsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
@@ -987,8 +990,7 @@ int hs_batch(ofri_ctx* ctx, const BatchImage* im, int n, float alpha, int niter,
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
     int vi, d;
     hs_default(npx, &vi, &d);
-    if (ctx->tuning.hs_depth > 0) d = ctx->tuning.hs_depth;
-    if (ctx->tuning.hs_variant > 0) vi = ctx->tuning.hs_variant - 1;
+    shape_override(ctx, npx, true, &vi, &d);
     *variant1 = vi + 1;
     *depth = d;
 }

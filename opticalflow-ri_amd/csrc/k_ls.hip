@@ -575,8 +575,12 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         vi = 2;      // 4 rows x 8 wavefronts: room for the previous iterate in registers
         depth = 6;
     }
-    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
-    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    if (fast || cheb) {
+        if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
+        if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    } else {
+        shape_override(ctx, (size_t)H * W, false, &vi, &depth);
+    }
     if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
     const LsVariant& var = kLsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
@@ -690,8 +694,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
     }
     int vi, depth;
     ls_default(total, &vi, &depth);
-    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
-    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    shape_override(ctx, total, false, &vi, &depth);
     if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
     const LsVariant& var = kLsVariants[vi];
     if (2 * depth >= var.R * var.waves || 2 * depth >= 64)
@@ -769,8 +772,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
 size_t ls_part_doubles(ofri_ctx* ctx, int H, int W) {
     int vi, depth;
     ls_default((size_t)H * W, &vi, &depth);
-    if (ctx->tuning.ls_depth > 0) depth = ctx->tuning.ls_depth;
-    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    shape_override(ctx, (size_t)H * W, false, &vi, &depth);
     if (vi >= kNumLsVariants) return 0;
     const LsVariant& var = kLsVariants[vi];
     const int outW = 64 - 2 * depth, outH = var.waves * var.R - 2 * depth;
@@ -804,8 +806,7 @@ int noalloc_scratch(int device, void** ptr, size_t* bytes) {
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth) {
     int vi, d;
     ls_default(npx, &vi, &d);
-    if (ctx->tuning.ls_depth > 0) d = ctx->tuning.ls_depth;
-    if (ctx->tuning.ls_variant > 0) vi = ctx->tuning.ls_variant - 1;
+    shape_override(ctx, npx, false, &vi, &d);
     *variant1 = vi + 1;
     *depth = d;
 }

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <algorithm>
+
 #include "ofri_internal.h"
 
 namespace ofri {
@@ -179,6 +181,8 @@ int ofri_lane(ofri_ctx* ctx, int k, ofri_ctx** out) {
     ofri_ctx* l = ctx->lanes[k];
     l->tuning = ctx->tuning;
     l->graphs_on = ctx->graphs_on;
+    l->classes = ctx->classes;
+    l->classes_gen = ctx->classes_gen;
     *out = l;
     return OFRI_OK;
 }
@@ -194,6 +198,8 @@ int ofri_fork(ofri_ctx* ctx, int n) {
         ofri_ctx* l = ctx->lanes[k];
         l->tuning = ctx->tuning;
         l->graphs_on = ctx->graphs_on;
+        l->classes = ctx->classes;
+        l->classes_gen = ctx->classes_gen;
         OFRI_HIP(hipStreamWaitEvent(l->stream, ctx->order_ev, 0));
     }
     return OFRI_OK;
@@ -296,6 +302,35 @@ int ofri_get_tuning(ofri_ctx* ctx, ofri_tuning* t) {
     CHECK_CTX(ctx);
     if (!t) return fail(OFRI_E_ARG, "ofri_get_tuning: null");
     *t = ctx->tuning;
+    return OFRI_OK;
+}
+
+int ofri_set_tuning_classes(ofri_ctx* ctx, const ofri_tuning_class* classes, int n) {
+    CHECK_CTX(ctx);
+    if (n < 0 || n > 16 || (n > 0 && !classes))
+        return fail(OFRI_E_ARG, "ofri_set_tuning_classes: %d classes (at most 16)", n);
+    std::vector<ofri_tuning_class> v(classes, classes + n);
+    for (const auto& c : v) {
+        const ofri_tuning& t = c.t;
+        if (c.min_px < 0 || t.hs_depth < 0 || t.hs_depth > 24 || t.ls_depth < 0 || t.ls_depth > 24 ||
+            t.hs_variant < 0 || t.hs_variant >= 64 || t.ls_variant < 0 || t.ls_variant >= 64)
+            return fail(OFRI_E_ARG, "ofri_set_tuning_classes: class at %lld pixels out of range", c.min_px);
+    }
+    std::sort(v.begin(), v.end(),
+              [](const ofri_tuning_class& a, const ofri_tuning_class& b) { return a.min_px < b.min_px; });
+    ctx->classes = v;
+    ctx->classes_gen += 1;
+    return OFRI_OK;
+}
+
+int ofri_arch(ofri_ctx* ctx, char* out, int len) {
+    CHECK_CTX(ctx);
+    if (!out || len < 1) return fail(OFRI_E_ARG, "ofri_arch: no room");
+    hipDeviceProp_t prop;
+    OFRI_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    int i = 0;
+    for (; i < len - 1 && prop.gcnArchName[i] && prop.gcnArchName[i] != ':'; ++i) out[i] = prop.gcnArchName[i];
+    out[i] = 0;
     return OFRI_OK;
 }
 
@@ -712,6 +747,7 @@ static std::string lshs_key(ofri_ctx* ctx, const float* a, const float* b, int H
     put(&p->ls_maxnum, sizeof p->ls_maxnum);
     put(&p->ls_tol, sizeof p->ls_tol);
     put(&ctx->tuning, sizeof ctx->tuning);
+    put(&ctx->classes_gen, sizeof ctx->classes_gen);
     return k;
 }
 
@@ -1049,6 +1085,7 @@ static std::string generic_key(ofri_ctx* ctx, const float* a, const float* b, in
         put(&m->lk_niter, sizeof m->lk_niter);
     }
     put(&ctx->tuning, sizeof ctx->tuning);
+    put(&ctx->classes_gen, sizeof ctx->classes_gen);
     return k;
 }
 

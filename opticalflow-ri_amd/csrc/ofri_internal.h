@@ -118,6 +118,10 @@ struct ofri_ctx {
     // owned by this context (a lane has a parent and no lanes)
     ofri_ctx* parent = nullptr;
     std::vector<ofri_ctx*> lanes;
+    // launch-shape size classes, ascending min_px (ofri_set_tuning_classes);
+    // the generation enters every graph key
+    std::vector<ofri_tuning_class> classes;
+    uint64_t classes_gen = 0;
     hipEvent_t order_ev = nullptr;   // fork / join ordering point of this stream
 };
 
@@ -134,6 +138,22 @@ struct Taps {
 // reference pads (gaussian_filter.py:63-66): p < 0 -> -1-p, p >= L -> p-h.
 __device__ __forceinline__ int pf_index(int p, int L, int h) {
     return p < 0 ? -1 - p : (p >= L ? p - h : p);
+}
+
+// Launch shape of a plain sweep over npx pixels (an image, or a batch's
+// total): the size class's fields, then the context's tuning, over the
+// built-in default in *vi (0-based variant) and *depth.
+inline void shape_override(const ofri_ctx* ctx, size_t npx, bool hs, int* vi, int* depth) {
+    const ofri_tuning* c = nullptr;
+    for (const auto& k : ctx->classes)
+        if ((long long)npx >= k.min_px) c = &k.t;
+    const ofri_tuning* ts[2] = {c, &ctx->tuning};
+    for (const ofri_tuning* t : ts) {
+        if (!t) continue;
+        const int d = hs ? t->hs_depth : t->ls_depth, v = hs ? t->hs_variant : t->ls_variant;
+        if (d > 0) *depth = d;
+        if (v > 0) *vi = v - 1;
+    }
 }
 
 // Named scratch buffer of at least `bytes` bytes (grows, never shrinks).

@@ -100,6 +100,10 @@ class Tuning(C.Structure):
     _fields_ = [("hs_depth", C.c_int), ("ls_depth", C.c_int), ("hs_variant", C.c_int), ("ls_variant", C.c_int)]
 
 
+class TuningClass(C.Structure):
+    _fields_ = [("min_px", C.c_longlong), ("t", Tuning)]
+
+
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -166,6 +170,8 @@ SIGNATURES = {
     "ofri_set_tuning": [_vp, C.POINTER(Tuning)],
     "ofri_get_tuning": [_vp, C.POINTER(Tuning)],
     "ofri_query_tuning": [_vp, C.c_int, C.c_int, C.POINTER(Tuning)],
+    "ofri_set_tuning_classes": [_vp, C.c_void_p, C.c_int],
+    "ofri_arch": [_vp, C.c_char_p, C.c_int],
     "ofri_set_graphs": [_vp, C.c_int],
 }
 

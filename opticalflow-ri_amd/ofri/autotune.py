@@ -7,7 +7,13 @@ per GPU architecture and size in a JSON file (OFRI_TUNE_CACHE, default
 ~/.cache/ofri/tuning.json).  The library's built-in defaults were chosen
 from these measurements on gfx950.
 
-    python -m ofri.autotune 1024 512        # tune and print
+    python -m ofri.autotune 1024 512        # tune, print and cache
+
+Every new context installs the cached results for its GPU architecture as
+size classes (ofri_set_tuning_classes): a plain solve, or a batch by its
+total pixel count, takes the shape tuned at the nearest tuned size (class
+boundaries at the geometric means of neighbouring tuned sizes).
+OFRI_TUNE_CACHE="" disables the cache.
 """
 import functools
 import json
@@ -88,15 +94,52 @@ def tune(n, depths=(3, 4, 5, 6, 8, 10), verbose=True):
     return out
 
 
-def save(results, arch="gfx950"):
-    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+def save(results, arch=None, path=None):
+    """Merges {n: tune(n)} into the cache under the GPU's architecture."""
+    path = CACHE if path is None else path
+    arch = arch or context().arch()
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     data = {}
-    if os.path.exists(CACHE):
-        with open(CACHE) as f:
+    if os.path.exists(path):
+        with open(path) as f:
             data = json.load(f)
     data.setdefault(arch, {}).update({str(k): v for k, v in results.items()})
-    with open(CACHE, "w") as f:
+    with open(path, "w") as f:
         json.dump(data, f, indent=1)
+
+
+def classes(entries):
+    """Size classes from cached results {n: {'hs': [variant, depth, ms], 'ls':
+    [...]}} (0-based variants; an n x n image): (min_px, hs_variant,
+    hs_depth, ls_variant, ls_depth) rows with 1-based variants, the class of
+    size n starting at the geometric mean of n^2 and the next smaller tuned
+    size's pixel count (at 0 for the smallest)."""
+    sizes = sorted(int(k) for k in entries)
+    rows = []
+    for i, n in enumerate(sizes):
+        r = entries[str(n)] if str(n) in entries else entries[n]
+        lo = 0 if i == 0 else int(round(float(sizes[i - 1]) * n))
+        hs, ls = r.get("hs"), r.get("ls")
+        rows.append((lo, hs[0] + 1 if hs else 0, hs[1] if hs else 0, ls[0] + 1 if ls else 0, ls[1] if ls else 0))
+    return rows
+
+
+def apply(ctx, path=None):
+    """Installs the cache's classes for ctx's architecture (none: nothing).
+    Returns the rows installed."""
+    path = CACHE if path is None else path
+    if not path or not os.path.exists(path):
+        return []
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        entries = data.get(ctx.arch(), {})
+        rows = classes(entries) if entries else []
+    except (OSError, ValueError, KeyError, TypeError, IndexError):
+        return []
+    if rows:
+        ctx.set_tuning_classes(rows)
+    return rows
 
 
 if __name__ == "__main__":

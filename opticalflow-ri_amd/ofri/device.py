@@ -24,6 +24,8 @@ class Context:
         _capi.check(self.lib.ofri_create(int(device), C.byref(h)))
         self.handle = h
         self.device = device
+        from . import autotune
+        autotune.apply(self)       # the autotuner's cached size classes for this GPU, if any
 
     def sync(self):
         _capi.check(self.lib.ofri_sync(self.handle))
@@ -65,6 +67,22 @@ class Context:
         t = _capi.Tuning()
         _capi.check(self.lib.ofri_get_tuning(self.handle, C.byref(t)))
         return t.hs_depth, t.ls_depth, t.hs_variant, t.ls_variant
+
+    def set_tuning_classes(self, classes):
+        """classes: iterable of (min_px, hs_variant, hs_depth, ls_variant,
+        ls_depth) -- 1-based variants, 0 = the built-in default
+        (ofri_set_tuning_classes); empty clears."""
+        cl = list(classes)
+        arr = (_capi.TuningClass * max(1, len(cl)))()
+        for i, (mp, hv, hd, lv, ld) in enumerate(cl):
+            arr[i].min_px = int(mp)
+            arr[i].t = _capi.Tuning(int(hd), int(ld), int(hv), int(lv))
+        _capi.check(self.lib.ofri_set_tuning_classes(self.handle, C.cast(arr, C.c_void_p), len(cl)))
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        _capi.check(self.lib.ofri_arch(self.handle, buf, 64))
+        return buf.value.decode()
 
     def query_tuning(self, H, W):
         """(hs_depth, ls_depth, hs_variant, ls_variant) the sweeps use for H x W."""

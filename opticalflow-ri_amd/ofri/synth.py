@@ -187,6 +187,31 @@ def _batch_item(args):
     return piv_pair(H, W, dens, fam, np.random.default_rng(child), truth=truth)
 
 
+# the device-rendered uneven batch draws its pairs' flow parameters from
+# seeds here on (training 0 .. 19,999; held-out evaluation 1,000,000 on;
+# calibration 3,000,000 on)
+BATCH_SEED = 5000000
+
+
+def device_batch_pairs(n_pairs=64, seed=MASTER_SEED, sizes=None, truth=False):
+    """The uneven batch rendered on the device: H, W drawn from {256, 384,
+    ..., 2048}, flow family, density and flow parameters as the training
+    pairs draw them (ofri.train.pair_spec, seeds from BATCH_SEED).  Returns
+    a list of (im1, im2) or (im1, im2, u, v) DeviceArrays and the specs."""
+    from . import ops
+    from .train import pair_spec
+    rng = np.random.default_rng(np.random.SeedSequence(seed).spawn(8)[4])
+    choices = sizes if sizes is not None else np.arange(256, 2049, 128)
+    out, specs = [], []
+    for k in range(n_pairs):
+        H, W = int(rng.choice(choices)), int(rng.choice(choices))
+        sp = pair_spec(BATCH_SEED + k, H, W)
+        out.append(ops.render_pair(H, W, sp["family"], sp["density"], sp["seed"], sp["par"], sp["modes"],
+                                   truth=truth))
+        specs.append(sp)
+    return out, specs
+
+
 def batch_pairs(n_pairs=64, seed=MASTER_SEED, sizes=None, truth=False, workers=1):
     """Uneven batch: H, W from {256, 384, ..., 2048}, density U[0.02, 0.08],
     flow family uniform over the three families (max 4 px).  Items are

@@ -138,3 +138,77 @@ def test_batch_driver_entry_point(ops):
         Ur, Vr = genericPyramidalOpticalFlow(a, b, 3.4, HSOpticalFlowAlgoAdapter([21, 45], 40), 2, 1, 0.48,
                                              LiuShenOpticalFlowAlgoAdapter(5))
         assert np.array_equal(U, Ur) and np.array_equal(V, Vr)
+
+
+def test_full_uneven_batch_of_64_pairs(ops):
+    """The batched configuration itself: 64 device-rendered pairs of uneven
+    sizes (256..2048 per side, three flow families) in one ofri_batch_lshs
+    call at the reference's sweep counts.  Every flow is finite and close to
+    its rendered truth, a second call gives the same bits, the three smallest
+    pairs equal the f32 oracle's pyramid and three more their single-pair
+    call."""
+    from ofri import synth
+    from ofri.device import DeviceArray
+    pairs, specs = synth.device_batch_pairs(64, truth=True)
+    prm = _params()
+    jobs = [(a, b) for a, b, _, _ in pairs]
+    out, st = ops.batch_lshs(jobs, **prm)
+    out2 = [(DeviceArray(a.shape), DeviceArray(a.shape)) for a, _ in jobs]
+    ops.batch_lshs(jobs, out=out2, stats=False, **prm)
+    epes = []
+    for k, ((U, V), (U2, V2), (_, _, ut, vt)) in enumerate(zip(out, out2, pairs)):
+        u, v = U.download(), V.download()
+        assert np.isfinite(u).all() and np.isfinite(v).all(), k
+        assert np.array_equal(u, U2.download()) and np.array_equal(v, V2.download()), k
+        e = np.sqrt((u - ut.download()) ** 2 + (v - vt.download()) ** 2)[32:-32, 32:-32].mean()
+        epes.append(float(e))
+        assert [s["ls_iters"] for s in st[k]] and all(1 <= s["ls_iters"] <= 60 for s in st[k])
+    print("[batch 64] total %.1f Mpx, sizes %d..%d px, mean EPE to truth per pair: min %.3f median %.3f max %.3f px"
+          % (sum(a.shape[0] * a.shape[1] for a, _ in jobs) / 1e6, min(a.shape[0] * a.shape[1] for a, _ in jobs),
+             max(a.shape[0] * a.shape[1] for a, _ in jobs), min(epes), float(np.median(epes)), max(epes)))
+    assert max(epes) < 1.0
+    order = np.argsort([a.shape[0] * a.shape[1] for a, _ in jobs])
+    for k in order[:3]:
+        a, b = jobs[k][0].download(), jobs[k][1].download()
+        U32, V32, _ = orc.pyr_lshs(a, b, 2, [45.0, 21.0], 600, prm["filt_w"], prm["opt_w"], 5.0)
+        d = max(np.abs(out[k][0].download() - U32).max(), np.abs(out[k][1].download() - V32).max())
+        print("[batch 64] pair %d (%dx%d) max|d| vs oracle-f32 %.3g px" % ((k,) + a.shape + (d,)))
+        assert d <= 1e-5
+    for k in order[[20, 40, 63]]:
+        U, V, _ = ops.pyr_lshs(jobs[k][0], jobs[k][1], **prm)
+        assert np.array_equal(U.download(), out[k][0].download()) and np.array_equal(V.download(),
+                                                                                      out[k][1].download())
+
+
+def test_tuning_classes_pick_the_launch_shape(ops, tmp_path):
+    """Size classes (the autotuner's cached per-size results): a plain solve
+    takes its class's shape, ofri_set_tuning still overrides, a batch takes
+    the class of its total pixel count per level (here both classes, with
+    the single-pair bits), and a cache file for this GPU's architecture is
+    installed as classes with boundaries at the geometric means."""
+    from ofri import autotune
+    from ofri.device import context
+    ctx = context()
+    try:
+        ctx.set_tuning_classes([(0, 2, 3, 2, 2), (100000, 6, 7, 1, 5)])
+        assert ctx.query_tuning(512, 512) == (7, 5, 6, 1)        # (hs_depth, ls_depth, hs_variant, ls_variant)
+        assert ctx.query_tuning(200, 300) == (3, 2, 2, 2)
+        ctx.set_tuning(hs_depth=4)
+        assert ctx.query_tuning(512, 512) == (4, 5, 6, 1)
+        ctx.set_tuning()
+        pairs = _pairs([(256, 384), (130, 200)], seed=12)
+        prm = _params()
+        prm["niter"] = 80
+        out, _ = ops.batch_lshs(pairs, **prm)
+        ref = _singles(ops, pairs, prm)
+        for (U, V), (Ur, Vr, _) in zip(out, ref):
+            assert np.array_equal(U.download(), Ur) and np.array_equal(V.download(), Vr)
+        cache = tmp_path / "tuning.json"
+        autotune.save({512: {"hs": [4, 5, 0.1], "ls": [0, 5, 0.2]}, 1024: {"hs": [10, 10, 0.3], "ls": [2, 4, 0.4]}},
+                      path=str(cache))
+        rows = autotune.apply(ctx, path=str(cache))
+        assert rows == [(0, 5, 5, 1, 5), (524288, 11, 10, 3, 4)]
+        assert ctx.query_tuning(512, 512) == (5, 5, 5, 1) and ctx.query_tuning(1024, 1024) == (10, 4, 11, 3)
+    finally:
+        ctx.set_tuning_classes([])
+        ctx.set_tuning()

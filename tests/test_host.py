@@ -153,3 +153,15 @@ def test_synthetic_generator_is_deterministic():
     im1, im2, u, v = a
     assert im1.dtype == np.float32 and im1.shape == (64, 64)
     assert np.abs(u).max() <= 3.5 + 1e-6 and np.all(v == 0)
+
+
+def test_autotune_size_classes_from_cached_results():
+    """The autotuner's cached per-size results become size classes with
+    1-based variants, each class starting at the geometric mean of its
+    size's pixel count and the next smaller tuned size's (the smallest at 0)."""
+    from ofri import autotune
+    rows = autotune.classes({"1024": {"hs": [10, 10, 0.3], "ls": [2, 4, 0.4]}, "256": {"hs": [1, 3, 0.01]},
+                             "512": {"hs": [4, 5, 0.1], "ls": [0, 5, 0.2]}})
+    assert rows == [(0, 2, 3, 0, 0), (131072, 5, 5, 1, 5), (524288, 11, 10, 3, 4)]
+    assert autotune.apply(object(), path="") == []
+
