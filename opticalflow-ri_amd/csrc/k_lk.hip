@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
     __shared__ float sDx[kReg * kReg];
     __shared__ float sDy[kReg * kReg];
     const int wx = 32 - __builtin_clz(mx), wy = 32 - __builtin_clz(my);   // extent: last set bit + 1
+    const unsigned full_x = wx >= 32 ? 0xFFFFFFFFu : (1u << wx) - 1u;        // every column counted
     const int regx = kTile + wx - 1, regy = kTile + wy - 1;                // region actually used
     const int regIx = regx + 2, regIy = regy + 2;
     const int bx0 = blockIdx.x * kTile, by0 = blockIdx.y * kTile;
@@ -158,15 +159,41 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                 const float* dyr = sDy + o + r * kReg;
                 const float* Ja = sJ + (y0 + r - ry0) * rw + (x0 - rx0);
                 const float* Jb = Ja + rw;
-                for (int c = 0; c < wx; ++c) {
-                    if (!((mx >> c) & 1u)) continue;
-                    float t = w00 * Ja[c];
-                    t = t + w10 * Ja[c + 1];
-                    t = t + w01 * Jb[c];
-                    t = t + w11 * Jb[c + 1];
-                    float diff = t - pr[c];
-                    b1 = b1 + (double)diff * (double)dxr[c];
-                    b2 = b2 + (double)diff * (double)dyr[c];
+                // the bilinear right neighbours are carried to the next
+                // column: two J reads per column instead of four.  A window
+                // row with every column counted (symmetric windows) runs
+                // without the per-column mask test, so the loads of the
+                // unrolled columns can be issued ahead of their use.
+                float ja0 = Ja[0], jb0 = Jb[0];
+                if (mx == full_x) {
+#pragma unroll 4
+                    for (int c = 0; c < wx; ++c) {
+                        const float ja1 = Ja[c + 1], jb1 = Jb[c + 1];
+                        float t = w00 * ja0;
+                        t = t + w10 * ja1;
+                        t = t + w01 * jb0;
+                        t = t + w11 * jb1;
+                        float diff = t - pr[c];
+                        b1 = b1 + (double)diff * (double)dxr[c];
+                        b2 = b2 + (double)diff * (double)dyr[c];
+                        ja0 = ja1;
+                        jb0 = jb1;
+                    }
+                } else {
+                    for (int c = 0; c < wx; ++c) {
+                        const float ja1 = Ja[c + 1], jb1 = Jb[c + 1];
+                        if ((mx >> c) & 1u) {
+                            float t = w00 * ja0;
+                            t = t + w10 * ja1;
+                            t = t + w01 * jb0;
+                            t = t + w11 * jb1;
+                            float diff = t - pr[c];
+                            b1 = b1 + (double)diff * (double)dxr[c];
+                            b2 = b2 + (double)diff * (double)dyr[c];
+                        }
+                        ja0 = ja1;
+                        jb0 = jb1;
+                    }
                 }
             }
         } else {
