@@ -78,12 +78,22 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
         if (!((my >> r) & 1u)) continue;
         const float* dxr = sDx + o + r * kReg;
         const float* dyr = sDy + o + r * kReg;
-        for (int c = 0; c < wx; ++c) {
-            if (!((mx >> c) & 1u)) continue;
-            double dx = (double)dxr[c], dy = (double)dyr[c];
-            a11 = a11 + dx * dx;
-            a12 = a12 + dx * dy;
-            a22 = a22 + dy * dy;
+        if (mx == full_x) {   // every column counted: no mask test, loads issued ahead
+#pragma unroll 4
+            for (int c = 0; c < wx; ++c) {
+                double dx = (double)dxr[c], dy = (double)dyr[c];
+                a11 = a11 + dx * dx;
+                a12 = a12 + dx * dy;
+                a22 = a22 + dy * dy;
+            }
+        } else {
+            for (int c = 0; c < wx; ++c) {
+                if (!((mx >> c) & 1u)) continue;
+                double dx = (double)dxr[c], dy = (double)dyr[c];
+                a11 = a11 + dx * dx;
+                a12 = a12 + dx * dy;
+                a22 = a22 + dy * dy;
+            }
         }
     }
     float A11 = (float)a11, A12 = (float)a12, A22 = (float)a22;
