@@ -240,9 +240,11 @@ def plain_many(jobs, fw=None, ow=None, lanes=None, lane_of=None):
     if fw is None:
         fw, ow = filters()
     ctx, on, lane = _lanes(len(jobs), lanes, lane_of)
-    for k, (a, b, U, V) in enumerate(jobs):
-        ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=U, Vacc=V, stats=False, ctx=on[lane[k]], **WORKLOAD)
-    _joined(ctx, on)
+    try:
+        for k, (a, b, U, V) in enumerate(jobs):
+            ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=U, Vacc=V, stats=False, ctx=on[lane[k]], **WORKLOAD)
+    finally:
+        _joined(ctx, on)     # also after an error: later work on ctx follows what was queued
 
 
 def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None, lanes=None, lane_of=None):
@@ -265,15 +267,17 @@ def solve_many(jobs, net, judge=None, fw=None, ow=None, rules=None, lanes=None, 
     checked = [judge.pick() if judge is not None else False for _ in range(n)]
     refs = {}
     ctx, on, lane = _lanes(n, lanes, lane_of)
-    for k, (a, b, U, V) in enumerate(jobs):
-        c = on[lane[k]]
-        _learned(a, b, net, fw, ow, rules, Uacc=U, Vacc=V, stats=False, flag=words.view(k, (1,)), ctx=c)
-        if checked[k]:
-            Up, Vp = _POOL.plain_pair(k, U.shape)
-            ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=Up, Vacc=Vp, stats=False, ctx=c, **WORKLOAD)
-            ops.flow_epe(U, V, Up, Vp, out=words.view(n + k, (1,)), ctx=c)
-            refs[k] = (Up, Vp)
-    _joined(ctx, on)
+    try:
+        for k, (a, b, U, V) in enumerate(jobs):
+            c = on[lane[k]]
+            _learned(a, b, net, fw, ow, rules, Uacc=U, Vacc=V, stats=False, flag=words.view(k, (1,)), ctx=c)
+            if checked[k]:
+                Up, Vp = _POOL.plain_pair(k, U.shape)
+                ops.pyr_lshs(a, b, filt_w=fw, opt_w=ow, Uacc=Up, Vacc=Vp, stats=False, ctx=c, **WORKLOAD)
+                ops.flow_epe(U, V, Up, Vp, out=words.view(n + k, (1,)), ctx=c)
+                refs[k] = (Up, Vp)
+    finally:
+        _joined(ctx, on)     # also after an error: later work on ctx follows what was queued
     w = words.download().reshape(-1)[:2 * n]
     ok = w[:n].view(np.int32)
     ep = w[n:2 * n]
