@@ -462,6 +462,8 @@ typedef struct ofri_learned_params {
     int hs_finish_budget[2];  /* HS plain finishing sweeps per level at most (0: hs_budget / 2), within hs_budget */
     int relative;             /* 1: hs_tol / ls_tol are relative to the tested iterate's norm */
     int ls_sweeps[2];         /* per level: > 0 = exactly this many Liu-Shen sweeps, ls_tol accepts */
+    int hs_finish_first[2];   /* per level: > 0 = the first plain finishing launch runs at most this many
+                               * sweeps (0: the fused depth); used where cheb_bound > 0 and cheb_finish > 1 */
 } ofri_learned_params;
 int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
                           const ofri_lshs_params* p, const ofri_learned_params* lp, ofri_net* net,

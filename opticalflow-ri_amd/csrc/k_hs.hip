@@ -45,11 +45,25 @@ __device__ __forceinline__ float from_right(float x) {
 // Derivatives: HS(im2=I0, im1=I1) -> computeDerivatives(I1, I0)
 // (HornSchunck.py:36,72,83,106-126), 2x2 stencils with mirror padding.
 // What the first workgroup of the derivative launch also sets up for a
-// solve with a stop test: its control word cleared (ctrl non-null) and n
-// Chebyshev weights written.
+// solve with a stop test: its control word cleared (ctrl non-null).
 struct SolveStart {
     SolveCtrl* ctrl;
 };
+
+__device__ __forceinline__ void clear_ctrl(const SolveStart& st) {
+    SolveCtrl* c = st.ctrl;
+    c->done = 0;
+    c->sweeps = 0;
+    c->final_buf = 0;
+    c->redo = 0;
+    c->redo_in = 0;
+    c->last_err = 0.0f;
+    c->limit = 0;
+    c->base_buf = 0;
+    c->last_nrm = 0.0f;
+    c->slot[0] = SolveState{};
+    c->slot[1] = SolveState{};
+}
 
 __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restrict__ I1, int H,
                             int W, float* __restrict__ fx, float* __restrict__ fy,
@@ -57,20 +71,7 @@ __global__ void hs_derivs_k(const float* __restrict__ I0, const float* __restric
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (blockIdx.x == 0 && blockIdx.y == 0) {
-        if (st.ctrl && threadIdx.x == 0) {
-            SolveCtrl* c = st.ctrl;
-            c->done = 0;
-            c->sweeps = 0;
-            c->final_buf = 0;
-            c->redo = 0;
-            c->redo_in = 0;
-            c->last_err = 0.0f;
-            c->limit = 0;
-            c->base_buf = 0;
-            c->last_nrm = 0.0f;
-            c->slot[0] = SolveState{};
-            c->slot[1] = SolveState{};
-        }
+        if (st.ctrl && threadIdx.x == 0) clear_ctrl(st);
     }
     if (j >= W) return;
     int i1 = (i + 1 < H) ? i + 1 : H - 2;
@@ -113,20 +114,7 @@ __global__ __launch_bounds__(512) void hs_prefilter_derivs_k(const float* __rest
     const int h = t.n / 2;
     const int x0 = blockIdx.x * kDTW, y0 = blockIdx.y * kDTH;
     if (blockIdx.x == 0 && blockIdx.y == 0) {
-        if (st.ctrl && tid == 0) {
-            SolveCtrl* c = st.ctrl;
-            c->done = 0;
-            c->sweeps = 0;
-            c->final_buf = 0;
-            c->redo = 0;
-            c->redo_in = 0;
-            c->last_err = 0.0f;
-            c->limit = 0;
-            c->base_buf = 0;
-            c->last_nrm = 0.0f;
-            c->slot[0] = SolveState{};
-            c->slot[1] = SolveState{};
-        }
+        if (st.ctrl && tid == 0) clear_ctrl(st);
     }
     const int R = FR + 2 * h, Cw = FC + 2 * h;
     {
@@ -721,7 +709,7 @@ int omega_table(ofri_ctx* ctx, double sigma, const float** out) {
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol, SolveCtrl* d_ctrl,
              float* d_hist, float cheb_b, float cheb_switch, bool check_last, int fin_budget,
-             bool zero_start, const float* pre_w, int pre_n, bool rel) {
+             bool zero_start, const float* pre_w, int pre_n, bool rel, int fin_first) {
     if (pre_w && pre_n > 0 && (pre_n > kMaxTaps || (pre_n & 1) == 0 || H < pre_n || W < pre_n))
         return fail(OFRI_E_ARG, "hs: pre-filter of %d taps on a %dx%d image", pre_n, H, W);
     if (H < 2 || W < 2) return fail(OFRI_E_ARG, "hs: image %dx%d too small", H, W);
@@ -732,6 +720,9 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         return fail(OFRI_E_ARG, "hs: Chebyshev mode needs a stop test and a bound in (0, 0.999], got %g", cheb_b);
     if (rel && (!d_ctrl || !check_last || d_hist))
         return fail(OFRI_E_ARG, "hs: a relative tolerance needs the per-launch stop test (and no history)");
+    if (fin_first < 0 || (fin_first > 0 && (!cheb || !(cheb_switch > 1.0f))))
+        return fail(OFRI_E_ARG, "hs: a first finishing launch of %d sweeps needs Chebyshev sweeps with a "
+                                "finishing phase", fin_first);
     size_t N = (size_t)H * W;
     int vi, depth;
     if (cheb) {
@@ -831,13 +822,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     // count reaches the budget), then the final launch (test of the last
     // launch, re-run of a launch stopped inside, result published).  Launch
     // parities start at slot0; next_budget > 0 makes the final launch arm a
-    // following phase instead of copying the result back into U, V.
-    // Returns the number of regular launches in *nl.
-    auto phase = [&](void (*kern)(HsPass), double t, int sweeps, int slot0, int next_budget, int* nl) -> int {
+    // following phase instead of copying the result back into U, V.  first
+    // > 0: the first launch runs at most that many sweeps.  Returns the
+    // number of regular launches in *nl.
+    auto phase = [&](void (*kern)(HsPass), double t, int sweeps, int slot0, int next_budget, int first,
+                     int* nl) -> int {
         p.tol = t;
         int done = 0, src = 0, j = 0;
         while (done < sweeps) {
             int ns = sweeps - done < depth ? sweeps - done : depth;
+            if (j == 0 && first > 0 && ns > first) ns = first;
             p.nsweeps = ns;
             p.src = src;
             p.sweep0 = done;
@@ -873,9 +867,9 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     p.rel_src = 0;
     p.zero_start = zero_start ? 1 : 0;
     if (!cheb) {
-        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol, niter, 0, 0, nullptr));
+        OFRI_TRY(phase(d_ctrl ? var.normed : var.kernel, tol, niter, 0, 0, 0, nullptr));
     } else if (!(cheb_switch > 1.0f)) {
-        OFRI_TRY(phase(var.cheb, tol, niter, 0, 0, nullptr));
+        OFRI_TRY(phase(var.cheb, tol, niter, 0, 0, 0, nullptr));
     } else {
         // Chebyshev sweeps stall at float32 rounding level, where plain sweeps
         // still settle: accelerate to cheb_switch * tol in at most niter - fin
@@ -883,14 +877,16 @@ int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         // plain sweeps to tol, on one sweep count and history (niter sweeps
         // in all).  The finishing phase starts from the accelerated phase's
         // result buffer (its launches take buffers relative to it) and the
-        // opposite parity.
+        // opposite parity.  fin_first > 0: the finishing phase's first
+        // launch runs at most that many sweeps (the plain sweeps after the
+        // hand-over often need only a few).
         int fin = fin_budget > 0 ? fin_budget : (niter / 2 > 0 ? niter / 2 : 1);
         if (fin > niter) fin = niter;
         int nl = 0;
-        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter - fin, 0, niter > 0 ? fin : 0, &nl));
+        OFRI_TRY(phase(var.cheb, tol * (double)cheb_switch, niter - fin, 0, niter > 0 ? fin : 0, 0, &nl));
         p.rel_src = 1;
         p.zero_start = 0;
-        OFRI_TRY(phase(var.normed, tol, fin, nl + 1, 0, nullptr));
+        OFRI_TRY(phase(var.normed, tol, fin, nl + 1, 0, fin_first, nullptr));
         p.rel_src = 0;
     }
     if (d_err) OFRI_TRY(diff_norm_err(ctx, U, U0, V, V0, N, d_err));

@@ -655,9 +655,13 @@ static int pyr_lshs_enqueue(ofri_ctx* ctx, const float* d_im1, const float* d_im
         if (lrn) {
             const ofri_learned_params* lp = lrn->lp;
             // (HS's own error value, the change from the initial flow, is not formed here)
+            // (the first finishing launch's length applies where Chebyshev
+            // sweeps hand over to a finishing phase)
+            const bool phases = lp->cheb_bound[l] > 0.0f && lp->cheb_finish[l] > 1.0f;
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], lp->hs_budget[l], U, V, nullptr, lp->hs_tol[l],
                               hs_ctrls + l, nullptr, lp->cheb_bound[l], lp->cheb_finish[l], true,
-                              lp->hs_finish_budget[l], true, hpw, hpn, lp->relative != 0));
+                              lp->hs_finish_budget[l], true, hpw, hpn, lp->relative != 0,
+                              phases ? lp->hs_finish_first[l] : 0));
         } else if (record_res) {
             OFRI_TRY(hs_solve(ctx, s1, s2, h, w, p->alphas[l], p->niter, U, V, hs_errs + l, -1.0, hs_ctrls + l,
                               nullptr, 0.0f, 0.0f, false, 0, false, hpw, hpn));
@@ -1368,7 +1372,8 @@ int ofri_pyr_lshs_learned(ofri_ctx* ctx, const float* d_im1, const float* d_im2,
     if (!p->alphas || !lp->hs_tol || !lp->ls_tol) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: missing arrays");
     if (H < 8 || W < 8) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: %dx%d too small", H, W);
     for (int l = 0; l < 2; ++l)
-        if (lp->hs_budget[l] < 1 || lp->hs_finish_budget[l] < 0 || !(lp->cheb_finish[l] >= 0.0f))
+        if (lp->hs_budget[l] < 1 || lp->hs_finish_budget[l] < 0 || !(lp->cheb_finish[l] >= 0.0f) ||
+            lp->hs_finish_first[l] < 0)
             return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: HS budget / finish at level %d", l);
     if (lp->ls_budget < 0) return fail(OFRI_E_ARG, "ofri_pyr_lshs_learned: Liu-Shen budget");
     for (int l = 0; l < 2; ++l)

@@ -222,11 +222,13 @@ int select_copy(ofri_ctx* ctx, const SolveCtrl* ctrl, const float* s0, const flo
 // launch, on its last sweep (no re-run of the stopping launch).
 // rel (needs check_last): tol is relative -- a launch's tested norm is
 // compared with tol * (||U||_2 + ||V||_2) / (H*W) of that launch's result.
+// With Chebyshev sweeps and a finishing phase: fin_first > 0 caps the
+// finishing phase's first launch at that many sweeps.
 int hs_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float alpha,
              int niter, float* U, float* V, float* d_err, double tol = -1.0,
              SolveCtrl* d_ctrl = nullptr, float* d_hist = nullptr, float cheb_b = 0.0f,
              float cheb_switch = 0.0f, bool check_last = false, int fin_budget = 0, bool zero_start = false,
-             const float* pre_w = nullptr, int pre_n = 0, bool rel = false);
+             const float* pre_w = nullptr, int pre_n = 0, bool rel = false, int fin_first = 0);
 
 void hs_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);
 void ls_effective(ofri_ctx* ctx, size_t npx, int* variant1, int* depth);

@@ -55,7 +55,7 @@ class LearnedParams(C.Structure):
     _fields_ = [("hs_tol", C.POINTER(C.c_float)), ("ls_tol", C.POINTER(C.c_float)), ("hs_budget", C.c_int * 2),
                 ("ls_budget", C.c_int), ("cheb_bound", C.c_float * 2), ("cheb_finish", C.c_float * 2),
                 ("ls_cheb_lo", C.c_float), ("ls_cheb_hi", C.c_float), ("hs_finish_budget", C.c_int * 2),
-                ("relative", C.c_int), ("ls_sweeps", C.c_int * 2)]
+                ("relative", C.c_int), ("ls_sweeps", C.c_int * 2), ("hs_finish_first", C.c_int * 2)]
 
 
 class FbParams(C.Structure):

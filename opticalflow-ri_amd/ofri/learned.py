@@ -60,6 +60,12 @@ LS_ACCEPT = (0.0045, 0.0085)
 # budgets stay close to what the calibration pairs use.
 HS_BUDGET = (180, 160)
 HS_FINISH_BUDGET = (48, 40)
+# The first plain finishing launch's sweeps per level (the later ones run
+# the fused depth, 12 / 10).  On the calibration pairs the finishing phase
+# mostly ended within 5 / 2 sweeps: the mean HS sweeps went from 86.5 /
+# 91.3 (first launch of the fused depth) to 79.8 / 84.3, with the same two
+# pairs of 24 missing the rules.
+HS_FINISH_FIRST = (5, 2)
 
 # The judge: every 32nd pair is checked against the plain pyramid (its plain
 # run costs about three learned pairs, so the check adds a tenth to the
@@ -83,7 +89,8 @@ class Rules(object):
     module constants)."""
 
     def __init__(self, hs_rel=HS_REL, ls_sweeps=LS_SWEEPS, ls_accept=LS_ACCEPT, hs_budget=HS_BUDGET,
-                 hs_finish_budget=HS_FINISH_BUDGET, cheb_bound=CHEB_BOUND, cheb_finish=CHEB_FINISH):
+                 hs_finish_budget=HS_FINISH_BUDGET, cheb_bound=CHEB_BOUND, cheb_finish=CHEB_FINISH,
+                 hs_finish_first=HS_FINISH_FIRST):
         self.hs_rel = _levels(hs_rel)
         self.ls_sweeps = _levels(ls_sweeps)
         self.ls_accept = _levels(ls_accept)
@@ -92,11 +99,12 @@ class Rules(object):
                                                                           self.hs_budget))
         self.cheb_bound = _levels(cheb_bound)
         self.cheb_finish = _levels(cheb_finish)
+        self.hs_finish_first = _levels(hs_finish_first)
 
     def kwargs(self):
         return dict(hs_tol=self.hs_rel, ls_tol=self.ls_accept, relative=True, ls_sweeps=self.ls_sweeps,
                     hs_budget=self.hs_budget, hs_finish_budget=self.hs_finish_budget, cheb_bound=self.cheb_bound,
-                    cheb_finish=self.cheb_finish, ls_cheb=LS_CHEB)
+                    cheb_finish=self.cheb_finish, ls_cheb=LS_CHEB, hs_finish_first=self.hs_finish_first)
 
 
 RULES = Rules()

@@ -414,14 +414,15 @@ def pyr_lshs_taps(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, ls_m
 def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, net, hs_tol, ls_tol,
                      hs_budget=600, ls_budget=60, cheb_bound=0.99, cheb_finish=4.0, ls_cheb=(0.0, 0.0), ls_maxnum=60,
                      ls_tol_plain=1e-8, Uacc=None, Vacc=None, stats=True, flag=None, hs_finish_budget=0,
-                     relative=False, ls_sweeps=0, ctx=None):
+                     relative=False, ls_sweeps=0, hs_finish_first=0, ctx=None):
     """The learned form (ofri_pyr_lshs_learned): accelerated HS, the
     correction network and Liu-Shen run to the per-level residual targets
     hs_tol / ls_tol (relative: as multiples of the tested iterate's norm
     (||u|| + ||v||)/N).  ls_sweeps > 0: exactly that many Liu-Shen sweeps,
     ls_tol then accepts or rejects the level.  net: an ofri.net.Net or None
     (the method without it).  hs_budget, cheb_bound, cheb_finish,
-    hs_finish_budget, ls_sweeps: one value or (coarse, fine).  flag: optional
+    hs_finish_budget, hs_finish_first, ls_sweeps: one value or (coarse,
+    fine).  flag: optional
     DeviceArray view of one 4-byte word receiving the convergence flag (int
     1 / 0) with no synchronisation.  ctx: the context or lane queued on
     (default: the process-wide one).  Returns (Uacc, Vacc, stats,
@@ -433,12 +434,13 @@ def pyr_lshs_learned(im1, im2, levels, alphas, niter, filt_w, opt_w, ls_alpha, n
     lt = np.ascontiguousarray(ls_tol, dtype=np.float32)
     def pair(x):
         return (x, x) if np.isscalar(x) else tuple(x)
-    hb, cb, cf, fb, lsw = (pair(hs_budget), pair(cheb_bound), pair(cheb_finish), pair(hs_finish_budget),
-                           pair(ls_sweeps))
+    hb, cb, cf, fb, lsw, ff = (pair(hs_budget), pair(cheb_bound), pair(cheb_finish), pair(hs_finish_budget),
+                               pair(ls_sweeps), pair(hs_finish_first))
     lp = _capi.LearnedParams(_capi.fptr(ht), _capi.fptr(lt), (C.c_int * 2)(int(hb[0]), int(hb[1])), int(ls_budget),
                              (C.c_float * 2)(float(cb[0]), float(cb[1])), (C.c_float * 2)(float(cf[0]), float(cf[1])),
                              float(ls_cheb[0]), float(ls_cheb[1]), (C.c_int * 2)(int(fb[0]), int(fb[1])),
-                             int(bool(relative)), (C.c_int * 2)(int(lsw[0]), int(lsw[1])))
+                             int(bool(relative)), (C.c_int * 2)(int(lsw[0]), int(lsw[1])),
+                             (C.c_int * 2)(int(ff[0]), int(ff[1])))
     Uacc = Uacc if Uacc is not None else DeviceArray((H, W))
     Vacc = Vacc if Vacc is not None else DeviceArray((H, W))
     st = (_capi.LevelStats * levels)() if stats else None

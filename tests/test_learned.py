@@ -147,6 +147,30 @@ def test_learned_fallback_returns_the_plain_result(trained):
     assert np.array_equal(U.download(), t["plain"][1][0]) and np.array_equal(V.download(), t["plain"][1][1])
 
 
+def test_short_first_finishing_launch_moves_only_the_finishing_sweeps(trained):
+    """The first plain finishing launch of f sweeps (HS_FINISH_FIRST) leaves
+    the accelerated phase as it was: at each level the HS sweep count
+    differs from that of the run whose first finishing launch has the fused
+    depth d (12 at 512^2, 10 at 1024^2: the residual-mode defaults) by f - d
+    (the short launch met the target) or by f (one more launch of d did).
+    Both runs meet the rules and land within the EPE bar of each other."""
+    from ofri import learned
+    t = trained
+    a, b = t["pairs"][2]
+    f = learned.HS_FINISH_FIRST
+    assert all(0 < x for x in f)
+    U, V, info = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False)
+    U0, V0, info0 = learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False,
+                                  rules=learned.Rules(hs_finish_first=(0, 0)))
+    hs = [s["hs_iters"] for s in info["stats"]]
+    hs0 = [s["hs_iters"] for s in info0["stats"]]
+    print("[first finishing launch %s] HS sweeps %s, with launches of the fused depth %s" % (f, hs, hs0))
+    assert info["converged"] and info0["converged"]
+    for l, d in enumerate((12, 10)):
+        assert hs[l] - hs0[l] in (f[l] - d, f[l])
+    assert learned.epe((U.download(), V.download()), (U0.download(), V0.download()))[0] <= learned.EPE_BAR
+
+
 def test_learned_form_replays_identically(trained):
     """Eager run, graph capture and replays of the learned form give the same bits."""
     from ofri import learned
