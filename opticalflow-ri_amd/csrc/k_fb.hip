@@ -73,6 +73,83 @@ __global__ __launch_bounds__(256) void fb_blur_h(const float* __restrict__ tmp, 
     dst[(size_t)p * H * W + (size_t)y * W + x] = res;
 }
 
+// The same two passes for a half size KH fixed at compile time (the
+// Farnebäck window and pre-filter sizes), each output formed with the same
+// multiply and FMA sequence as above.  Vertical: a thread takes kTY
+// consecutive rows of one column and holds the kTY + 2 KH source values of
+// its window in registers, each read once (reflect-101 index per source row:
+// rows above the image only arise as y - j, rows below only as y + j).
+// Horizontal: a thread forms 4 consecutive outputs of a 1024-wide segment
+// from 16-byte LDS reads of its 4 + 2 KH window.
+constexpr int kTY = 16, kHX = 1024;
+
+template <int KH>
+__global__ __launch_bounds__(256) void fb_blur_v_reg(const float* __restrict__ src, float* __restrict__ tmp, int H,
+                                                     int W, HalfKernel k) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y0 = blockIdx.y * kTY, p = blockIdx.z;
+    if (x >= W) return;
+    const float* s = src + (size_t)p * H * W + x;
+    float w[kTY + 2 * KH];
+    if (y0 >= KH && y0 + kTY + KH <= H) {
+#pragma unroll
+        for (int m = 0; m < kTY + 2 * KH; ++m) w[m] = s[(size_t)(y0 - KH + m) * W];
+    } else {
+#pragma unroll
+        for (int m = 0; m < kTY + 2 * KH; ++m) {
+            const int r = y0 - KH + m;
+            w[m] = s[(size_t)(r < 0 ? r101_low(r, H - 1) : r101_high(r, H - 1)) * W];
+        }
+    }
+    float* d = tmp + (size_t)p * H * W + x;
+#pragma unroll
+    for (int i = 0; i < kTY; ++i) {
+        if (y0 + i >= H) break;
+        float v = w[i + KH] * k.g[0];
+#pragma unroll
+        for (int j = 1; j <= KH; ++j) v = fmaf(w[i + KH - j] + w[i + KH + j], k.g[j], v);
+        d[(size_t)(y0 + i) * W] = v;
+    }
+}
+
+template <int KH>
+__global__ __launch_bounds__(256) void fb_blur_h_reg(const float* __restrict__ tmp, float* __restrict__ dst, int H,
+                                                     int W, HalfKernel k) {
+    constexpr int WN = (4 + 2 * KH + 3) / 4;   // float4 reads per thread
+    __shared__ __attribute__((aligned(16))) float row[kHX + 4 * WN];
+    const int y = blockIdx.y, p = blockIdx.z;
+    const int x0 = blockIdx.x * kHX;
+    const float* t = tmp + (size_t)p * H * W + (size_t)y * W;
+    const int n = kHX + 2 * KH < W - x0 + 2 * KH ? kHX + 2 * KH : W - x0 + 2 * KH;
+    for (int i = threadIdx.x; i < n; i += 256) row[i] = t[r101_col(x0 + i - KH, W - 1)];
+    __syncthreads();
+    const int xl = 4 * threadIdx.x;
+    if (x0 + xl >= W) return;
+    float r[4 * WN];
+#pragma unroll
+    for (int q = 0; q < WN; ++q) {
+        const float4 f = *(const float4*)(row + xl + 4 * q);
+        r[4 * q] = f.x;
+        r[4 * q + 1] = f.y;
+        r[4 * q + 2] = f.z;
+        r[4 * q + 3] = f.w;
+    }
+    float o[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        float res = r[e + KH] * k.g[0];
+#pragma unroll
+        for (int i = 1; i <= KH; ++i) res = fmaf(r[e + KH - i] + r[e + KH + i], k.g[i], res);
+        o[e] = res;
+    }
+    float* d = dst + (size_t)p * H * W + (size_t)y * W + x0 + xl;
+    if (x0 + xl + 4 <= W && ((uintptr_t)d & 15) == 0) {
+        *(float4*)d = float4{o[0], o[1], o[2], o[3]};
+    } else {
+        for (int e = 0; e < 4 && x0 + xl + e < W; ++e) d[e] = o[e];
+    }
+}
+
 // Box window (updateFlowBoxFilter): replicate borders, running sums
 // v + (a + b) in float32 outward from the centre, vertical then horizontal,
 // times 1 / (2 kh + 1)^2.  The vertical sums of clamped columns equal those of
@@ -248,11 +325,29 @@ __global__ void fb_scale_k(float* __restrict__ fx, float* __restrict__ fy, size_
     }
 }
 
+template <int KH>
+void blur_reg(ofri_ctx* ctx, const float* src, float* tmp, float* dst, int H, int W, int planes,
+              const HalfKernel& k) {
+    fb_blur_v_reg<KH><<<dim3((W + 255) / 256, (H + kTY - 1) / kTY, planes), 256, 0, ctx->stream>>>(src, tmp, H, W,
+                                                                                                  k);
+    fb_blur_h_reg<KH><<<dim3((W + kHX - 1) / kHX, H, planes), 256, 0, ctx->stream>>>(tmp, dst, H, W, k);
+}
+
 int blur(ofri_ctx* ctx, const float* src, float* dst, int H, int W, int planes, const HalfKernel& k) {
     void* tmp;
     OFRI_TRY(scratch(ctx, "fb_blur_tmp", (size_t)planes * H * W * sizeof(float), &tmp));
-    fb_blur_v<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>(src, (float*)tmp, H, W, k);
-    fb_blur_h<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>((const float*)tmp, dst, H, W, k);
+    // the pre-filter (3 taps) and the Farnebäck windows of 21 and 33 taps
+    // have register-window passes; other sizes take the general ones
+    if (k.kh == 1) {
+        blur_reg<1>(ctx, src, (float*)tmp, dst, H, W, planes, k);
+    } else if (k.kh == 10) {
+        blur_reg<10>(ctx, src, (float*)tmp, dst, H, W, planes, k);
+    } else if (k.kh == 16) {
+        blur_reg<16>(ctx, src, (float*)tmp, dst, H, W, planes, k);
+    } else {
+        fb_blur_v<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>(src, (float*)tmp, H, W, k);
+        fb_blur_h<<<dim3((W + 255) / 256, H, planes), 256, 0, ctx->stream>>>((const float*)tmp, dst, H, W, k);
+    }
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }

@@ -200,9 +200,13 @@ def test_farneback_dropin_adapter(ops):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("H,W,n,sigma", [(192, 224, 3, 0.0), (192, 224, 39, 7.5), (97, 70, 19, 3.5),
-                                         (40, 36, 159, 31.5)])
+                                         (40, 36, 159, 31.5), (515, 1030, 33, 4.95), (20, 300, 33, 4.95),
+                                         (97, 70, 21, 3.15), (13, 1027, 3, 0.0)])
 def test_fb_level_blur_matches_oracle(ops, H, W, n, sigma):
-    # the last case has a half kernel (79) wider than the image: reflect-101 indexes wrap
+    # (40, 36, 159): a half kernel (79) wider than the image, reflect-101
+    # indexes wrap; 3, 21 and 33 taps take the register-window passes (1030
+    # and 1027 columns: a partial 1024-wide segment, rows not a multiple of
+    # 4 floats; 20 rows against a 33-tap window: wrapped rows)
     rng = np.random.default_rng(n)
     img = (rng.random((H, W)) * 255).astype(np.float32)
     k = level_kernel(n, sigma)
