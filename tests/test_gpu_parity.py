@@ -739,8 +739,10 @@ def test_generic_pyramid_liu_shen_warping_runs_the_reference_loop(ops):
 
 
 def test_full_size_pipeline_properties(ops):
-    """1024^2 LS(HS) 2-level: deterministic across runs, finite, and close to
-    the synthetic ground truth away from the borders."""
+    """1024^2 LS(HS) 2-level, the headline configuration at its own size:
+    deterministic across runs, finite, close to the synthetic ground truth
+    away from the borders, and within FLOW_TOL of the float32 oracle's
+    pyramid (same Liu-Shen sweep counts per level)."""
     from ofri import synth
     fw, ow = _lshs_params()
     im1, im2, ut, vt = synth.config_pair(4)
@@ -752,6 +754,11 @@ def test_full_size_pipeline_properties(ops):
     epe = _epe(a, b, ut, vt)[64:-64, 64:-64].mean()
     print("  1024^2 EPE to truth %.4f px, stats %s" % (epe, st))
     assert epe < 1.0
+    U32, V32, s32 = orc.pyr_lshs(im1, im2, 2, [45.0, 21.0], 600, fw, ow, 5.0)
+    d32 = max(np.abs(a - U32).max(), np.abs(b - V32).max())
+    print("  1024^2 max|d| vs oracle-f32 %.3g px" % d32)
+    assert d32 <= FLOW_TOL
+    assert [s["ls_iters"] for s in st] == [int(s32[l, 2]) for l in range(2)]
 
 
 def test_pyramid_graph_replays_match_eager(ops):
