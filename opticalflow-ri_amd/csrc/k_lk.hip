@@ -73,6 +73,10 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
     // window of pixel (i, j) starts at region (threadIdx.y, threadIdx.x)
     const int o = threadIdx.y * kReg + threadIdx.x;
     const int oI = (threadIdx.y + 1) * kRegI + threadIdx.x + 1;
+    // Window sums in double as the reference forms them, sum = sum + x * y
+    // with x, y float32 values: their product is exact in double (24 + 24
+    // significand bits of 53, exponents in range), so one rounded FMA gives
+    // the same bits as the rounded product-then-add.
     double a11 = 0.0, a12 = 0.0, a22 = 0.0;
     for (int r = 0; r < wy; ++r) {
         if (!((my >> r) & 1u)) continue;
@@ -82,17 +86,17 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
 #pragma unroll 4
             for (int c = 0; c < wx; ++c) {
                 double dx = (double)dxr[c], dy = (double)dyr[c];
-                a11 = a11 + dx * dx;
-                a12 = a12 + dx * dy;
-                a22 = a22 + dy * dy;
+                a11 = fma(dx, dx, a11);
+                a12 = fma(dx, dy, a12);
+                a22 = fma(dy, dy, a22);
             }
         } else {
             for (int c = 0; c < wx; ++c) {
                 if (!((mx >> c) & 1u)) continue;
                 double dx = (double)dxr[c], dy = (double)dyr[c];
-                a11 = a11 + dx * dx;
-                a12 = a12 + dx * dy;
-                a22 = a22 + dy * dy;
+                a11 = fma(dx, dx, a11);
+                a12 = fma(dx, dy, a12);
+                a22 = fma(dy, dy, a22);
             }
         }
     }
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
         const float a = px - fx, b = py - fy;
         const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
         const float w01 = (1.0f - a) * b, w11 = a * b;
-        double b1 = 0.0, b2 = 0.0;
+        double b1 = 0.0, b2 = 0.0;   // (exact products: one FMA per term, as the window sums)
         if (staged) {
             for (int r = 0; r < wy; ++r) {
                 if (!((my >> r) & 1u)) continue;
@@ -184,8 +188,8 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                         t = t + w01 * jb0;
                         t = t + w11 * jb1;
                         float diff = t - pr[c];
-                        b1 = b1 + (double)diff * (double)dxr[c];
-                        b2 = b2 + (double)diff * (double)dyr[c];
+                        b1 = fma((double)diff, (double)dxr[c], b1);
+                        b2 = fma((double)diff, (double)dyr[c], b2);
                         ja0 = ja1;
                         jb0 = jb1;
                     }
@@ -198,8 +202,8 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                             t = t + w01 * jb0;
                             t = t + w11 * jb1;
                             float diff = t - pr[c];
-                            b1 = b1 + (double)diff * (double)dxr[c];
-                            b2 = b2 + (double)diff * (double)dyr[c];
+                            b1 = fma((double)diff, (double)dxr[c], b1);
+                            b2 = fma((double)diff, (double)dyr[c], b2);
                         }
                         ja0 = ja1;
                         jb0 = jb1;
@@ -232,8 +236,8 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                     t = t + w01 * Jb[xa];
                     t = t + w11 * Jb[xb];
                     float diff = t - pr[c];
-                    b1 = b1 + (double)diff * (double)dxr[c];
-                    b2 = b2 + (double)diff * (double)dyr[c];
+                    b1 = fma((double)diff, (double)dxr[c], b1);
+                    b2 = fma((double)diff, (double)dyr[c], b2);
                 }
             }
         }
