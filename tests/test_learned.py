@@ -169,6 +169,8 @@ def test_short_first_finishing_launch_moves_only_the_finishing_sweeps(trained):
     for l, d in enumerate((12, 10)):
         assert hs[l] - hs0[l] in (f[l] - d, f[l])
     assert learned.epe((U.download(), V.download()), (U0.download(), V0.download()))[0] <= learned.EPE_BAR
+    with pytest.raises(Exception, match="HS budget / finish"):
+        learned.solve(a, b, t["net"], t["fw"], t["ow"], fallback=False, rules=learned.Rules(hs_finish_first=(-1, 2)))
 
 
 def test_learned_form_replays_identically(trained):
