@@ -1,12 +1,15 @@
 // Liu-Shen physics-based refinement (reference:
 // PhysicsBasedOpticalFlowLiuShen.py:47-158).
 //
-// Setup: one per-pixel kernel forms the eight per-pixel coefficient fields
-// IIx, IIy, II, Ixt, Iyt, B11, B12, B22 (the images normalised by their own
-// maxima, generate_invmatrix's 2x2 inverse).  Sweeps: temporally blocked like
-// Horn-Schunck (64-column extended tiles, R rows per lane in registers with
-// the eight coefficients, DPP for horizontal neighbours, LDS for the edge rows
-// of each wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries
+// Setup: one per-pixel kernel forms the normalised first frame I and the
+// temporal difference It (the images divided by their own maxima) and
+// generate_invmatrix's 2x2 inverse B11, B12, B22.  Sweeps: temporally
+// blocked like Horn-Schunck (64-column extended tiles, R rows per lane in
+// registers with the eight per-pixel coefficients IIx, IIy, II, Ixt, Iyt,
+// B11, B12, B22 -- the first five formed from I and It in each launch's load
+// phase, as the setup of the reference forms them --, DPP for horizontal
+// neighbours, LDS for the edge rows of each wavefront), with 'nearest'
+// (clamp) and 'constant' (zero) boundaries
 // as explicit selects in the workgroups whose tile touches the image border
 // (interior tiles skip them).  Each launch also reduces, per fused sweep, the
 // squared update of the pixels it owns; the last workgroup to finish turns
@@ -45,7 +48,8 @@ __device__ __forceinline__ float from_right(float x) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128,138 and :47-73).
+// Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128,138 and :47-73):
+// planes I, It, B11, B12, B22 of F (kLsPlanes planes of N).
 __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restrict__ im2, int H,
                            int W, const float* __restrict__ mx, float h, float* __restrict__ F,
                            size_t N, SolveCtrl* ctrl) {
@@ -75,10 +79,6 @@ __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restric
         return i2 - I(a, b);
     };
     float x = I(i, j);
-    float dr = (float)(-0.5 * (double)I(im, j) + 0.5 * (double)I(ip, j));
-    float dc = (float)(-0.5 * (double)I(i, jm) + 0.5 * (double)I(i, jp));
-    float dtr = (float)(-0.5 * (double)It(im, j) + 0.5 * (double)It(ip, j));
-    float dtc = (float)(-0.5 * (double)It(i, jm) + 0.5 * (double)It(i, jp));
     int cnt = (i > 0) + (i < H - 1);
     int cnt_c = (j > 0) + (j < W - 1);
     float cm = (float)((cnt + 1) * (cnt_c + 1) - 1);
@@ -93,20 +93,17 @@ __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restric
     float A12 = x * mm;
     float det = A11 * A22 - A12 * A12;
     size_t q = (size_t)i * W + j;
-    F[q] = x * dr;              // IIx
-    F[N + q] = x * dc;          // IIy
-    F[2 * N + q] = x * x;       // II
-    F[3 * N + q] = x * dtr;     // Ixt
-    F[4 * N + q] = x * dtc;     // Iyt
-    F[5 * N + q] = A22 / det;   // B11
-    F[6 * N + q] = (-A12) / det;  // B12
-    F[7 * N + q] = A11 / det;   // B22
+    F[q] = x;
+    F[N + q] = It(i, j);
+    F[2 * N + q] = A22 / det;     // B11
+    F[3 * N + q] = (-A12) / det;  // B12
+    F[4 * N + q] = A11 / det;     // B22
 }
 
 struct LsPass {
     const float* in[2][2];  // [ping-pong][u, v]
     float* out[2][2];
-    const float* F;         // 8 planes of N
+    const float* F;         // kLsPlanes planes of N: I, It, B11, B12, B22
     size_t N;
     int H, W;
     int K;
@@ -399,6 +396,15 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
     float up[R], vp[R];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
+    // I and It on the lane's rows and one row either side (clamped rows, as
+    // the setup's neighbour indices are)
+    float Ir[R + 2], Tr[R + 2];
+#pragma unroll
+    for (int m = 0; m < R + 2; ++m) {
+        const size_t q = (size_t)clampi(gy0 - 1 + m, 0, H - 1) * W + cx;
+        Ir[m] = a.F[q];
+        Tr[m] = a.F[N + q];
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int cy = clampi(gy0 + r, 0, H - 1);
@@ -409,14 +415,26 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
             up[r] = a.pin[src][0][q];
             vp[r] = a.pin[src][1][q];
         }
-        IIX[r] = a.F[q];
-        IIY[r] = a.F[N + q];
-        II[r] = a.F[2 * N + q];
-        IXT[r] = a.F[3 * N + q];
-        IYT[r] = a.F[4 * N + q];
-        B11[r] = a.F[5 * N + q];
-        B12[r] = a.F[6 * N + q];
-        B22[r] = a.F[7 * N + q];
+        B11[r] = a.F[2 * N + q];
+        B12[r] = a.F[3 * N + q];
+        B22[r] = a.F[4 * N + q];
+        // IIx, IIy, II, Ixt, Iyt (PhysicsBasedOpticalFlowLiuShen.py:124-128)
+        // with the setup's operations: the row neighbours are the loaded
+        // rows, the column neighbours the adjacent lanes' values (a lane's
+        // column is clamped into the image, so at the border its neighbour
+        // lane holds the clamped column).  Exact wherever the sweeps use them
+        // (the tile's outermost lanes, whose neighbours are missing, lie in
+        // the halo that is never written).
+        const float x = Ir[r + 1], t = Tr[r + 1];
+        const float dr = (float)(-0.5 * (double)Ir[r] + 0.5 * (double)Ir[r + 2]);
+        const float dc = (float)(-0.5 * (double)from_left(x) + 0.5 * (double)from_right(x));
+        const float dtr = (float)(-0.5 * (double)Tr[r] + 0.5 * (double)Tr[r + 2]);
+        const float dtc = (float)(-0.5 * (double)from_left(t) + 0.5 * (double)from_right(t));
+        IIX[r] = x * dr;
+        IIY[r] = x * dc;
+        II[r] = x * x;
+        IXT[r] = x * dtr;
+        IYT[r] = x * dtc;
     }
     if (edge_tile)
         ls_sweeps<R, WAVES, true, CHEB, FAST>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY,
@@ -599,13 +617,13 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
         pbuf = ws->part;
         mbuf = ws->mx;
     } else {
-        OFRI_TRY(scratch(ctx, "ls_fields", (cheb ? 14 : 10) * N * sizeof(float), &fbuf));
+        OFRI_TRY(scratch(ctx, "ls_fields", (kLsFieldPlanes + (cheb ? 4 : 0)) * N * sizeof(float), &fbuf));
         OFRI_TRY(scratch(ctx, "ls_part", 2 * pstride * sizeof(double), &pbuf));
         OFRI_TRY(scratch(ctx, "ls_max", 2 * sizeof(float), &mbuf));
     }
     float* F = (float*)fbuf;
-    float* ub = F + 8 * N;
-    float* vb = F + 9 * N;
+    float* ub = F + kLsPlanes * N;
+    float* vb = F + (kLsPlanes + 1) * N;
     OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, h, F, (float*)mbuf, d_ctrl, ws ? ws->max_part : nullptr));
 
     LsPass p{};
@@ -624,7 +642,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     if (cheb) {
         // previous-iterate buffers (the first step's previous iterate is the
         // start) and the weights for [cheb_lo, cheb_hi]
-        float* P0 = F + 10 * N;
+        float* P0 = F + kLsFieldPlanes * N;
         OFRI_HIP(hipMemcpyAsync(P0, u, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         OFRI_HIP(hipMemcpyAsync(P0 + N, v, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
         const double lo = cheb_lo, hi = cheb_hi;
@@ -706,7 +724,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         nparts += 2 * (size_t)depth * tiles * 2;
     }
     void *fbuf, *pbuf, *mbuf;
-    OFRI_TRY(scratch(ctx, "lsb_fields", 10 * total * sizeof(float), &fbuf));
+    OFRI_TRY(scratch(ctx, "lsb_fields", kLsFieldPlanes * total * sizeof(float), &fbuf));
     OFRI_TRY(scratch(ctx, "lsb_part", nparts * sizeof(double), &pbuf));
     OFRI_TRY(scratch(ctx, "lsb_max", 2 * (size_t)n * sizeof(float), &mbuf));
     std::vector<LsPass> tab(n);
@@ -718,8 +736,8 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         const int H = im[i].H, W = im[i].W;
         const size_t N = (size_t)H * W;
         float* F = f;
-        float *ub = F + 8 * N, *vb = F + 9 * N;
-        f += 10 * N;
+        float *ub = F + kLsPlanes * N, *vb = F + (kLsPlanes + 1) * N;
+        f += kLsFieldPlanes * N;
         float* mx = (float*)mbuf + 2 * i;
         OFRI_TRY(reduce_max2(ctx, im[i].im1, im[i].im2, N, mx));
         dim3 g((W + 255) / 256, H);
@@ -762,8 +780,9 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         f = (float*)fbuf;
         for (int i = 0; i < n; ++i) {
             const size_t N = (size_t)im[i].H * im[i].W;
-            OFRI_TRY(select_copy2(ctx, d_ctrls + i, im[i].U, f + 8 * N, im[i].U, im[i].V, f + 9 * N, im[i].V, N));
-            f += 10 * N;
+            OFRI_TRY(select_copy2(ctx, d_ctrls + i, im[i].U, f + kLsPlanes * N, im[i].U, im[i].V,
+                                  f + (kLsPlanes + 1) * N, im[i].V, N));
+            f += kLsFieldPlanes * N;
         }
     }
     return OFRI_OK;

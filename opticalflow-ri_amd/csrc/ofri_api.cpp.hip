@@ -478,8 +478,8 @@ int ofri_ls_noalloc(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H
         return fail(OFRI_E_STATE, "ofri_ls_noalloc: no device scratch on device %d (%s)", ctx->device,
                     ctx->noalloc_err.c_str());
     // layout of the device scratch: control word, maxima, partial maxima,
-    // stop-test partial sums, then the coefficient fields and the ping-pong
-    // pair (10 planes); everything is written before it is read (the setup
+    // stop-test partial sums, then the setup's planes and the ping-pong pair
+    // (kLsFieldPlanes planes); everything is written before it is read (the setup
     // kernel clears the control word), so nothing needs clearing
     const size_t N = (size_t)H * W;
     const size_t nparts = ls_part_doubles(ctx, H, W);
@@ -487,12 +487,12 @@ int ofri_ls_noalloc(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H
     const size_t o_red = o_mx + 256;
     const size_t o_part = o_red + align256(kMaxRedPart * sizeof(float));
     const size_t o_f = o_part + align256(nparts * sizeof(double));
-    const size_t need = o_f + 10 * N * sizeof(float);
+    const size_t need = o_f + kLsFieldPlanes * N * sizeof(float);
     if (nparts == 0 || need > ctx->noalloc_bytes)
         return fail(OFRI_E_ARG,
                     "ofri_ls_noalloc: a %dx%d image needs %zu bytes of device scratch, %zu available "
                     "(about %zu pixels at most)",
-                    H, W, need, ctx->noalloc_bytes, (ctx->noalloc_bytes - o_f) / (10 * sizeof(float)));
+                    H, W, need, ctx->noalloc_bytes, (ctx->noalloc_bytes - o_f) / (kLsFieldPlanes * sizeof(float)));
     char* base = (char*)ctx->noalloc;
     SolveCtrl* ctrl = (SolveCtrl*)base;
     LsWorkspace ws{(float*)(base + o_f), (double*)(base + o_part), nparts, (float*)(base + o_mx),
