@@ -166,7 +166,7 @@ int ofri_ls(ofri_ctx* ctx, const float* d_im1, const float* d_im2, int H, int W,
  * ping-pong pair, the stop test's partial sums, the maxima and the control
  * word live in a fixed 42 MiB device array of the library (one per GPU,
  * held until the process ends), so the call makes no hipMalloc / hipFree
- * and allocates nothing else either; images above about 1,560,000 pixels
+ * and allocates nothing else either; images above about 2,700,000 pixels
  * are rejected (OFRI_E_ARG, the message says how much they need).  Same
  * arguments, results and sweeps as ofri_ls, bit for bit.  Calls that
  * overlap on one GPU (several contexts, or threads, of one process) share

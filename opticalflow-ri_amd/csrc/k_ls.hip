@@ -2,14 +2,13 @@
 // PhysicsBasedOpticalFlowLiuShen.py:47-158).
 //
 // Setup: one per-pixel kernel forms the normalised first frame I and the
-// temporal difference It (the images divided by their own maxima) and
-// generate_invmatrix's 2x2 inverse B11, B12, B22.  Sweeps: temporally
-// blocked like Horn-Schunck (64-column extended tiles, R rows per lane in
-// registers with the eight per-pixel coefficients IIx, IIy, II, Ixt, Iyt,
-// B11, B12, B22 -- the first five formed from I and It in each launch's load
-// phase, as the setup of the reference forms them --, DPP for horizontal
-// neighbours, LDS for the edge rows of each wavefront), with 'nearest'
-// (clamp) and 'constant' (zero) boundaries
+// temporal difference It (the images divided by their own maxima).  Sweeps:
+// temporally blocked like Horn-Schunck (64-column extended tiles, R rows per
+// lane in registers with the eight per-pixel coefficients IIx, IIy, II, Ixt,
+// Iyt and generate_invmatrix's 2x2 inverse B11, B12, B22 -- formed from I and
+// It in each launch's load phase, as the setup of the reference forms them
+// --, DPP for horizontal neighbours, LDS for the edge rows of each
+// wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries
 // as explicit selects in the workgroups whose tile touches the image border
 // (interior tiles skip them).  Each launch also reduces, per fused sweep, the
 // squared update of the pixels it owns; the last workgroup to finish turns
@@ -48,11 +47,11 @@ __device__ __forceinline__ float from_right(float x) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128,138 and :47-73):
-// planes I, It, B11, B12, B22 of F (kLsPlanes planes of N).
+// Setup (PhysicsBasedOpticalFlowLiuShen.py:96-97,124-128): planes I and It
+// of F (kLsPlanes planes of N).
 __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restrict__ im2, int H,
-                           int W, const float* __restrict__ mx, float h, float* __restrict__ F,
-                           size_t N, SolveCtrl* ctrl) {
+                           int W, const float* __restrict__ mx, float* __restrict__ F, size_t N,
+                           SolveCtrl* ctrl) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (ctrl && i == 0 && j == 0) {
@@ -71,39 +70,20 @@ __global__ void ls_setup_k(const float* __restrict__ im1, const float* __restric
     }
     if (j >= W) return;
     const float m1 = mx[0], m2 = mx[1];
-    int im = clampi(i - 1, 0, H - 1), ip = clampi(i + 1, 0, H - 1);
-    int jm = clampi(j - 1, 0, W - 1), jp = clampi(j + 1, 0, W - 1);
     auto I = [&](int a, int b) -> float { return im1[(size_t)a * W + b] / m1; };
     auto It = [&](int a, int b) -> float {
         float i2 = im2[(size_t)a * W + b] / m2;
         return i2 - I(a, b);
     };
-    float x = I(i, j);
-    int cnt = (i > 0) + (i < H - 1);
-    int cnt_c = (j > 0) + (j < W - 1);
-    float cm = (float)((cnt + 1) * (cnt_c + 1) - 1);
-    float d2r = (float)(1.0 * (double)I(im, j) - 2.0 * (double)x + 1.0 * (double)I(ip, j));
-    float d2c = (float)(1.0 * (double)I(i, jm) - 2.0 * (double)x + 1.0 * (double)I(i, jp));
-    float mm = (float)(0.25 * (double)I(im, jm) - 0.25 * (double)I(im, jp) -
-                       0.25 * (double)I(ip, jm) + 0.25 * (double)I(ip, jp));
-    float two_x = 2.0f * x;
-    float hc = h * cm;
-    float A11 = x * (d2r - two_x) - hc;
-    float A22 = x * (d2c - two_x) - hc;
-    float A12 = x * mm;
-    float det = A11 * A22 - A12 * A12;
-    size_t q = (size_t)i * W + j;
-    F[q] = x;
+    const size_t q = (size_t)i * W + j;
+    F[q] = I(i, j);
     F[N + q] = It(i, j);
-    F[2 * N + q] = A22 / det;     // B11
-    F[3 * N + q] = (-A12) / det;  // B12
-    F[4 * N + q] = A11 / det;     // B22
 }
 
 struct LsPass {
     const float* in[2][2];  // [ping-pong][u, v]
     float* out[2][2];
-    const float* F;         // kLsPlanes planes of N: I, It, B11, B12, B22
+    const float* F;         // kLsPlanes planes of N: I, It
     size_t N;
     int H, W;
     int K;
@@ -415,19 +395,17 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
             up[r] = a.pin[src][0][q];
             vp[r] = a.pin[src][1][q];
         }
-        B11[r] = a.F[2 * N + q];
-        B12[r] = a.F[3 * N + q];
-        B22[r] = a.F[4 * N + q];
-        // IIx, IIy, II, Ixt, Iyt (PhysicsBasedOpticalFlowLiuShen.py:124-128)
-        // with the setup's operations: the row neighbours are the loaded
-        // rows, the column neighbours the adjacent lanes' values (a lane's
-        // column is clamped into the image, so at the border its neighbour
-        // lane holds the clamped column).  Exact wherever the sweeps use them
-        // (the tile's outermost lanes, whose neighbours are missing, lie in
-        // the halo that is never written).
+        // the coefficients (PhysicsBasedOpticalFlowLiuShen.py:124-128,138 and
+        // generate_invmatrix, :47-73) with the setup's operations: the row
+        // neighbours are the loaded rows, the column neighbours the adjacent
+        // lanes' values (a lane's column is clamped into the image, so at the
+        // border its neighbour lane holds the clamped column).  Exact
+        // wherever the sweeps use them (the tile's outermost lanes, whose
+        // neighbours are missing, lie in the halo that is never written).
         const float x = Ir[r + 1], t = Tr[r + 1];
+        const float xl = from_left(x), xr = from_right(x);
         const float dr = (float)(-0.5 * (double)Ir[r] + 0.5 * (double)Ir[r + 2]);
-        const float dc = (float)(-0.5 * (double)from_left(x) + 0.5 * (double)from_right(x));
+        const float dc = (float)(-0.5 * (double)xl + 0.5 * (double)xr);
         const float dtr = (float)(-0.5 * (double)Tr[r] + 0.5 * (double)Tr[r + 2]);
         const float dtc = (float)(-0.5 * (double)from_left(t) + 0.5 * (double)from_right(t));
         IIX[r] = x * dr;
@@ -435,6 +413,22 @@ __device__ __forceinline__ void ls_tile(const LsPass& a, int nsw, int src, int b
         II[r] = x * x;
         IXT[r] = x * dtr;
         IYT[r] = x * dtc;
+        // generate_invmatrix: the 2x2 system of the pixel and its inverse
+        const int cnt = (cy > 0) + (cy < H - 1), cnt_c = (cx > 0) + (cx < W - 1);
+        const float cm = (float)((cnt + 1) * (cnt_c + 1) - 1);
+        const float d2r = (float)(1.0 * (double)Ir[r] - 2.0 * (double)x + 1.0 * (double)Ir[r + 2]);
+        const float d2c = (float)(1.0 * (double)xl - 2.0 * (double)x + 1.0 * (double)xr);
+        const float mm = (float)(0.25 * (double)from_left(Ir[r]) - 0.25 * (double)from_right(Ir[r]) -
+                                 0.25 * (double)from_left(Ir[r + 2]) + 0.25 * (double)from_right(Ir[r + 2]));
+        const float two_x = 2.0f * x;
+        const float hc = a.h * cm;
+        const float A11 = x * (d2r - two_x) - hc;
+        const float A22 = x * (d2c - two_x) - hc;
+        const float A12 = x * mm;
+        const float det = A11 * A22 - A12 * A12;
+        B11[r] = A22 / det;
+        B12[r] = (-A12) / det;
+        B22[r] = A11 / det;
     }
     if (edge_tile)
         ls_sweeps<R, WAVES, true, CHEB, FAST>(a, nsw, base, blk, nblk, lane, w, gx, gy0, u, v, up, vp, IIX, IIY,
@@ -558,14 +552,14 @@ void ls_default(size_t npx, int* variant, int* depth) {
 
 }  // namespace
 
-// Liu-Shen's coefficient fields for (im1, im2) into F (8 fields of H * W),
+// Liu-Shen's setup planes for (im1, im2) into F (kLsPlanes planes of H * W),
 // the images' maxima into mx (2 floats); the control word starts cleared.
-static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float h, float* F,
+static int ls_prepare(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, float* F,
                       float* mx, SolveCtrl* d_ctrl, float* max_part = nullptr) {
     const size_t N = (size_t)H * W;
     OFRI_TRY(reduce_max2(ctx, im1, im2, N, mx, max_part));
     dim3 g((W + 255) / 256, H);
-    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, h, F, N, d_ctrl);
+    ls_setup_k<<<g, 256, 0, ctx->stream>>>(im1, im2, H, W, mx, F, N, d_ctrl);
     OFRI_HIP(hipGetLastError());
     return OFRI_OK;
 }
@@ -624,7 +618,7 @@ int ls_solve(ofri_ctx* ctx, const float* im1, const float* im2, int H, int W, fl
     float* F = (float*)fbuf;
     float* ub = F + kLsPlanes * N;
     float* vb = F + (kLsPlanes + 1) * N;
-    OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, h, F, (float*)mbuf, d_ctrl, ws ? ws->max_part : nullptr));
+    OFRI_TRY(ls_prepare(ctx, im1, im2, H, W, F, (float*)mbuf, d_ctrl, ws ? ws->max_part : nullptr));
 
     LsPass p{};
     p.in[0][0] = u; p.in[0][1] = v; p.in[1][0] = ub; p.in[1][1] = vb;
@@ -741,7 +735,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         float* mx = (float*)mbuf + 2 * i;
         OFRI_TRY(reduce_max2(ctx, im[i].im1, im[i].im2, N, mx));
         dim3 g((W + 255) / 256, H);
-        ls_setup_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, mx, h, F, N, d_ctrls + i);
+        ls_setup_k<<<g, 256, 0, ctx->stream>>>(im[i].im1, im[i].im2, H, W, mx, F, N, d_ctrls + i);
         OFRI_HIP(hipGetLastError());
         LsPass& p = tab[i];
         p = LsPass{};

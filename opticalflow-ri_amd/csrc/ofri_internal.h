@@ -254,9 +254,9 @@ unsigned lk_window_mask(int hw, int lo, int hi);
 // also publishes the solution norm of the tested launch (last_nrm).
 // ws (optional): caller-provided device memory instead of the context's
 // scratch buffers (plain sweeps only: no Chebyshev, no float32 sweeps).
-// Planes of a Liu-Shen solve's field buffer: the setup's I, It, B11, B12,
-// B22, then the u / v ping-pong pair.
-constexpr int kLsPlanes = 5;
+// Planes of a Liu-Shen solve's field buffer: the setup's I and It, then the
+// u / v ping-pong pair.
+constexpr int kLsPlanes = 2;
 constexpr int kLsFieldPlanes = kLsPlanes + 2;
 struct LsWorkspace {
     float* fields;        // kLsFieldPlanes * H * W floats: the setup's planes, u / v ping-pong

@@ -48,14 +48,14 @@ def test_noalloc_ls_matches_ofri_ls_on_two_calls():
 def test_noalloc_ls_rejects_images_above_its_maximum():
     from ofri import _capi
     from ofri.device import context
-    H = W = 1300
+    H = W = 1700
     a = np.zeros((H, W), np.float32)
     st, _, _, _, _ = _call("ofri_ls_noalloc", a, a, 5.0, 5, 1e-8, a, a)
     assert st == _capi.OFRI_E_ARG
     msg = context().lib.ofri_last_error().decode()
-    print("[ls_noalloc 1300x1300] %s" % msg)
-    assert "device scratch" in msg and "1300x1300" in msg
-    # 1200 x 1200 (1.44 Mpx) still fits the array
-    b = np.zeros((1200, 1200), np.float32)
+    print("[ls_noalloc 1700x1700] %s" % msg)
+    assert "device scratch" in msg and "1700x1700" in msg
+    # 1600 x 1600 (2.56 Mpx) still fits the array
+    b = np.zeros((1600, 1600), np.float32)
     st, _, _, _, _ = _call("ofri_ls_noalloc", b, b, 5.0, 5, 1e-8, b, b)
     assert st == _capi.OFRI_OK
