@@ -597,13 +597,18 @@ template <int R, int WAVES, bool NORM, bool CHEB>
 __global__ __launch_bounds__(64 * WAVES) void hs_pass_k(HsPass a) {
     int nsw = a.nsweeps, src = a.src, base = 0, fin = 0;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
+    int bx, by;
+    xcd_tile(blk, nblk, gridDim.x, bx, by);
     if (NORM && !launch_args(a, blk, nblk, CHEB && !a.check_last, nsw, src, base, &fin)) {
         // a final launch with nothing to re-run: it delivers the result
-        if (a.final_ && a.copy_back && fin == 1) hs_copy_owned<R, WAVES>(a, blockIdx.x, blockIdx.y);
+        if (a.final_ && a.copy_back && fin == 1) hs_copy_owned<R, WAVES>(a, bx, by);
         return;
     }
-    if constexpr (CHEB) hs_cheb_tile<R, WAVES>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
-    else hs_tile<R, WAVES, NORM>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
+    // partial sums are kept per tile (row-major), so their reduction order
+    // does not depend on which workgroup ran which tile
+    const int tile = by * gridDim.x + bx;
+    if constexpr (CHEB) hs_cheb_tile<R, WAVES>(a, nsw, src, base, bx, by, tile, nblk);
+    else hs_tile<R, WAVES, NORM>(a, nsw, src, base, bx, by, tile, nblk);
 }
 
 
@@ -616,7 +621,8 @@ template <int R, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void hs_batch_k(const HsPass* __restrict__ tab,
                                                          const int* __restrict__ blk0, int n,
                                                          int nsweeps, int src) {
-    const int b = blockIdx.x;
+    // the tile order that keeps neighbouring tiles on one XCD
+    const int b = xcd_run(blockIdx.x, gridDim.x);
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;

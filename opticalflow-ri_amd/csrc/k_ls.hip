@@ -488,13 +488,17 @@ template <int R, int WAVES, bool CHEB, bool FAST = false>
 __global__ __launch_bounds__(64 * WAVES) void ls_pass_k(LsPass a) {
     int nsw, src, base, fin = 0;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x, nblk = gridDim.x * gridDim.y;
+    int bx, by;
+    xcd_tile(blk, nblk, gridDim.x, bx, by);
     if (!launch_args(a, blk, nblk, false, nsw, src, base, &fin)) {
         // a final launch with nothing to re-run: it delivers the result
         if (a.final_ && a.dst[0] && (a.dst[0] != a.in[fin][0] || a.dst_add))
-            ls_deliver<R, WAVES>(a, fin, blockIdx.x, blockIdx.y);
+            ls_deliver<R, WAVES>(a, fin, bx, by);
         return;
     }
-    ls_tile<R, WAVES, CHEB, FAST>(a, nsw, src, base, blockIdx.x, blockIdx.y, blk, nblk);
+    // partial sums are kept per tile (row-major), so their reduction order
+    // does not depend on which workgroup ran which tile
+    ls_tile<R, WAVES, CHEB, FAST>(a, nsw, src, base, bx, by, by * gridDim.x + bx, nblk);
 }
 
 // Many images in one launch (workgroup b is tile b - blk0[p] of image p,
@@ -505,7 +509,8 @@ template <int R, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ls_batch_k(const LsPass* __restrict__ tab,
                                                          const int* __restrict__ blk0, int n,
                                                          int nsweeps, int src0, int slot, int final_) {
-    const int b = blockIdx.x;
+    // the tile order that keeps neighbouring tiles on one XCD
+    const int b = xcd_run(blockIdx.x, gridDim.x);
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
