@@ -232,11 +232,15 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
     const int g = lane >> 4, r16 = lane & 15;
     const int H = a.H, W = a.W;
     const int tiles_x = (W + TS - 1) / TS, tiles_y = (H + TS - 1) / TS;
-    int tile = blockIdx.x;
+    // XCD-aware order: a workgroup's run of (tile, channel group) pairs is
+    // contiguous with the channel group fastest, so the groups that stage the
+    // same patch, and neighbouring tiles, share one XCD's L2
+    const int lin = xcd_run(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    int tile = lin / gridDim.y;
     const int b = tile / (tiles_x * tiles_y);
     tile -= b * tiles_x * tiles_y;
     const int y0 = (tile / tiles_x) * TS, x0 = (tile % tiles_x) * TS;
-    const int cobase = blockIdx.y * (NT * 16);
+    const int cobase = (lin - (lin / gridDim.y) * gridDim.y) * (NT * 16);
 
     f4 acc[MT][NT];
 #pragma unroll

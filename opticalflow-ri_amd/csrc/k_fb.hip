@@ -86,8 +86,13 @@ constexpr int kTY = 16, kHX = 1024;
 template <int KH>
 __global__ __launch_bounds__(256) void fb_blur_v_reg(const float* __restrict__ src, float* __restrict__ tmp, int H,
                                                      int W, HalfKernel k) {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y0 = blockIdx.y * kTY, p = blockIdx.z;
+    // XCD-aware order: vertically adjacent row blocks, which read 2 KH of
+    // the same rows, run on one XCD (row block fastest within a run)
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int lin = xcd_run((blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x, gx * gy * gridDim.z);
+    const int by = lin % gy, bx = (lin / gy) % gx, p = lin / (gy * gx);
+    const int x = bx * 256 + threadIdx.x;
+    const int y0 = by * kTY;
     if (x >= W) return;
     const float* s = src + (size_t)p * H * W + x;
     float w[kTY + 2 * KH];

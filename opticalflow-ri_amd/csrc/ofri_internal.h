@@ -32,6 +32,26 @@ void clear_error();
         if (s_ != OFRI_OK) return s_; \
     } while (0)
 
+// Tile of workgroup blk among nblk, for a tile grid tiles_x wide.  The
+// dispatcher deals workgroups to the chip's 8 XCDs round-robin by linear
+// index (XCD blk % 8), and each XCD has its own L2.  Neighbouring tiles read
+// overlapping halos (and share the 128-byte lines their unaligned rows
+// straddle), so XCD j takes a contiguous row-major run of tiles: run j holds
+// the nblk / 8 (+1 for the first nblk % 8 runs) tiles after those of runs
+// 0..j-1, and its workgroups take them in dispatch order.  A bijection of
+// [0, nblk); results do not depend on it.
+__device__ __forceinline__ int xcd_run(int blk, int nblk) {
+    const int q = nblk >> 3, r = nblk & 7, j = blk & 7;
+    return j * q + (j < r ? j : r) + (blk >> 3);
+}
+
+__device__ __forceinline__ void xcd_tile(int blk, int nblk, int tiles_x, int& bx, int& by) {
+    const int t = xcd_run(blk, nblk);
+    by = t / tiles_x;
+    bx = t - by * tiles_x;
+}
+
+
 // Device-side control word of a solver run with a convergence test.
 // Stop-test state of a Jacobi solve as one launch hands it to the next: the
 // decision of every test evaluated so far, plus the launch whose per-sweep

@@ -227,25 +227,6 @@ constexpr int kOmegaTab = 512;
 
 __device__ __forceinline__ int omega_index(int k) { return k < kOmegaTab ? k : kOmegaTab - 1; }
 
-// Tile of workgroup blk among nblk, for a tile grid tiles_x wide.  The
-// dispatcher deals workgroups to the chip's 8 XCDs round-robin by linear
-// index (XCD blk % 8), and each XCD has its own L2.  Neighbouring tiles read
-// overlapping halos (and share the 128-byte lines their unaligned rows
-// straddle), so XCD j takes a contiguous row-major run of tiles: run j holds
-// the nblk / 8 (+1 for the first nblk % 8 runs) tiles after those of runs
-// 0..j-1, and its workgroups take them in dispatch order.  A bijection of
-// [0, nblk); results do not depend on it.
-__device__ __forceinline__ int xcd_run(int blk, int nblk) {
-    const int q = nblk >> 3, r = nblk & 7, j = blk & 7;
-    return j * q + (j < r ? j : r) + (blk >> 3);
-}
-
-__device__ __forceinline__ void xcd_tile(int blk, int nblk, int tiles_x, int& bx, int& by) {
-    const int t = xcd_run(blk, nblk);
-    by = t / tiles_x;
-    bx = t - by * tiles_x;
-}
-
 }  // namespace
 
 // Device table of kOmegaTab Chebyshev weights for sigma, owned by the
