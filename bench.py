@@ -57,11 +57,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="lshs", choices=["lshs", "plain", "batch", "fbls", "lk", "net"],
+    ap.add_argument("--workload", default="lshs", choices=["lshs", "plain", "batch", "fbls", "lk", "net", "hs"],
                     help="lshs: 1024^2 LS(HS) 2-level learned vs plain (default); plain: the plain form alone; "
                          "batch: 64 uneven pairs LS(HS); "
                          "fbls: 1024^2 Farneback 2-level + Liu-Shen; lk: 2048^2 dense LK 2-level; "
-                         "net: the correction network's forward pass over a 4-plane 1024^2 input")
+                         "net: the correction network's forward pass over a 4-plane 1024^2 input; "
+                         "hs: the HS sweep kernels alone (plain and Chebyshev, library stream)")
     ap.add_argument("--pairs", type=int, default=64, help="pairs in the batch workload")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel eagerly instead of replaying HIP graphs (for tools that trace "
@@ -435,7 +436,8 @@ def hs_roofline(ctx, d1, d2, fw, n):
             "kernel": "hs_pass_k (temporally blocked HS sweep, tile variant %d, %d sweeps/launch)" % (hs_variant, depth),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-            "launch_ms": round(hs_ms / launches, 5), "algorithmic_bytes_per_launch": round(alg_bytes / launches)}
+            "launch_ms": round(hs_ms / launches, 5), "algorithmic_bytes_per_launch": round(alg_bytes / launches),
+            "sweeps_per_launch": depth}
 
 
 def cheb_roofline(ctx, d1, d2, fw, n):
@@ -520,6 +522,35 @@ def bench_net(args):
     return _line("TFLOP/s, correction network forward %dx%d" % (n, n), value, "TFLOP/s", args, wall, gpu_ms,
                  {"workload": "correction network forward, 4 input planes %dx%d, 16 layers (both heads)" % (n, n),
                   "height": n, "width": n, "flops_per_pass": fl, "parallelism": "single GPU"}, roof, dtype="bf16")
+
+
+def bench_hs(args):
+    """The HS sweep kernels alone on the workload pair's pre-filtered n x n
+    frames, on the library stream: a plain 600-sweep solve (alpha 21) and the
+    accelerated (Chebyshev) solve's full launches, each timed with HIP
+    events.  A short target for counter collection; value = plain sweeps'
+    Mpix-sweeps/s."""
+    from gaussian_filter import prepareGaussianKernel
+    from ofri import synth
+    from ofri.device import DeviceArray, context
+    ctx = context()
+    n = args.size
+    im1, im2, _, _ = synth.config_pair(4, size=n)
+    fw = prepareGaussianKernel(3.4, 3)
+    d1, d2 = DeviceArray.from_host(im1), DeviceArray.from_host(im2)
+    t0 = time.perf_counter()
+    plain = hs_roofline(ctx, d1, d2, fw, n)
+    cheb = cheb_roofline(ctx, d1, d2, fw, n)
+    wall = time.perf_counter() - t0
+    _calibration_copy(ctx)
+    value = n * n / 1e6 * plain["sweeps_per_launch"] / (plain["launch_ms"] * 1e-3)
+    out = _line("Mpix-sweeps/s, plain HS sweep %dx%d" % (n, n), value, "Mpix-sweeps/s", args, wall,
+                wall * 1e3, {"workload": "HS sweep kernels alone, %dx%d, alpha 21: plain 600-sweep solve and "
+                             "the Chebyshev solve's full launches" % (n, n), "height": n, "width": n,
+                             "parallelism": "single GPU, library stream"}, plain)
+    out["steps"], out["warmup"] = 1, 0
+    out["roofline_cheb"] = cheb
+    return out
 
 
 def bench_lshs(args):
@@ -669,7 +700,7 @@ def main():
         context().set_graphs(False)
     if args.workload != "lshs":
         fn = {"plain": bench_plain, "batch": bench_batch, "fbls": bench_fbls, "lk": bench_lk,
-              "net": bench_net}[args.workload]
+              "net": bench_net, "hs": bench_hs}[args.workload]
         print(json.dumps(fn(args)))
         return
     print(json.dumps(bench_lshs(args)))
