@@ -423,14 +423,18 @@ __device__ __forceinline__ float lane_right0(float x) {
 // both multiply-adds fused; the previous iterate x_{k-1} rides along in
 // registers.  These iterates are not the plain method's, and the phase only
 // has to bring the residual |T(x_k) - x_k| to a multiple of the target before
-// plain sweeps finish the solve, so T is formed in float32 throughout: the
-// 3x3 average as vertical pair sums p, then corners p_l + p_r and edges
-// p + (x_l + x_r) weighted with one FMA (every addition still pairs
+// plain sweeps finish the solve, so T is formed in float32 throughout, scaled
+// by 1/w (w = float32(1/12), the corner weight; the edge weight is 2w): the
+// average's kernel is the separable [1,2,1] x [1,2,1] less its centre, so with
+// the vertical pair sum p = x_up + x_down the column sum c = fma(2, x, p) and
+// S = fma(2, p, c_l + c_r) = 12 x average (the centre never enters: no
+// cancellation); with ft' = RN(ft / w), der' = fma(fx, S_u, fma(fy, S_v, ft'))
+// times the held float32 reciprocal of den, t = fma(-fx, der', S_u) = T / w,
+// and the update is T(x) - x = fma(w, t, -x).  Every addition pairs
 // mirror-image operands, so the mirror-extended tile stays exact under the
-// sweeps), num = fx ub + fy vb + ft and T = ub - fx num / den with FMAs and
-// the held float32 reciprocal of den.  The residual tested is that of this
-// T.  The re-run of a stopping launch makes its last sweep a plain one: the
-// result is T(x_k) for the first iterate x_k that met the tolerance.
+// sweeps.  The residual tested is that of this T.  The re-run of a stopping
+// launch makes its last sweep a plain one, x = RN(w t): the result is T(x_k)
+// for the first iterate x_k that met the tolerance.
 template <int R, int WAVES>
 __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, int base, int bx, int by,
                                              int blk, int nblk) {
@@ -443,6 +447,7 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
     const int cx = mirror_idx(gx, W);
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
+    constexpr float wc = (float)(1.0 / 12.0);
     float U[R], V[R], UP[R], VP[R], FX[R], FY[R], FT[R], RCP[R];
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
@@ -459,7 +464,7 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
         VP[r] = base == 0 ? V[r] : a.pin[src][1][q];
         FX[r] = a.fx[q];
         FY[r] = a.fy[q];
-        FT[r] = a.ft[q];
+        FT[r] = a.ft[q] / wc;   // correctly rounded quotient
         RCP[r] = 1.0f / fmaf(FY[r], FY[r], fmaf(FX[r], FX[r], a.a2));
     }
     // the launch's Chebyshev weights, one per lane, broadcast per sweep
@@ -479,7 +484,7 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
         part[((size_t)sw * nblk + blk) * 2] = tu;
         part[((size_t)sw * nblk + blk) * 2 + 1] = tv;
     };
-    const float wc = (float)(1.0 / 12.0), we = (float)(1.0 / 6.0), g = a.gamma;
+    const float g = a.gamma;
     // one sweep; NS: reduce its residual, PL: a plain sweep (x_{k+1} = T(x_k))
     auto sweep = [&](int s, auto ns_c, auto pl_c) {
         constexpr bool NS = decltype(ns_c)::value;
@@ -496,30 +501,34 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
         const float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : U[R - 1];
         const float vdn = w < WAVES - 1 ? edge[b][w + 1][0][1][lane] : V[R - 1];
         const float om = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(om_lane), s));
-        double su = 0.0, sv = 0.0;
+        double qu = 0.0, qv = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float pu = up_u + (r + 1 < R ? U[r + 1] : udn);
             const float pv = up_v + (r + 1 < R ? V[r + 1] : vdn);
-            const float ub = fmaf(wc, lane_left0(pu) + lane_right0(pu), we * (pu + (lane_left0(U[r]) + lane_right0(U[r]))));
-            const float vb = fmaf(wc, lane_left0(pv) + lane_right0(pv), we * (pv + (lane_left0(V[r]) + lane_right0(V[r]))));
+            const float cu = fmaf(2.0f, U[r], pu);
+            const float cv = fmaf(2.0f, V[r], pv);
+            const float su = fmaf(2.0f, pu, lane_left0(cu) + lane_right0(cu));
+            const float sv = fmaf(2.0f, pv, lane_left0(cv) + lane_right0(cv));
             const float x = FX[r], y = FY[r];
-            const float der = fmaf(x, ub, fmaf(y, vb, FT[r])) * RCP[r];
-            const float du = fmaf(-x, der, ub) - U[r];
-            const float dv = fmaf(-y, der, vb) - V[r];
+            const float der = fmaf(x, su, fmaf(y, sv, FT[r])) * RCP[r];
+            const float tu = fmaf(-x, der, su);
+            const float tv = fmaf(-y, der, sv);
+            const float du = fmaf(wc, tu, -U[r]);
+            const float dv = fmaf(wc, tv, -V[r]);
             if constexpr (NS) {
                 const int ry = w * R + r, gy = gy0 + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
-                    su = su + (double)du * (double)du;
-                    sv = sv + (double)dv * (double)dv;
+                    qu = qu + (double)du * (double)du;
+                    qv = qv + (double)dv * (double)dv;
                 }
             }
             up_u = U[r];
             up_v = V[r];
             float nu, nv;
             if constexpr (PL) {
-                nu = fmaf(-x, der, ub);
-                nv = fmaf(-y, der, vb);
+                nu = wc * tu;
+                nv = wc * tv;
             } else {
                 nu = fmaf(om, fmaf(g, du, U[r]) - UP[r], UP[r]);
                 nv = fmaf(om, fmaf(g, dv, V[r]) - VP[r], VP[r]);
@@ -532,12 +541,12 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
         if constexpr (NS) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
-                su = su + __shfl_xor(su, off);
-                sv = sv + __shfl_xor(sv, off);
+                qu = qu + __shfl_xor(qu, off);
+                qv = qv + __shfl_xor(qv, off);
             }
             if (lane == 0) {
-                red[b][w][0] = su;
-                red[b][w][1] = sv;
+                red[b][w][0] = qu;
+                red[b][w][1] = qv;
             }
         }
     };

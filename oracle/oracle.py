@@ -120,28 +120,45 @@ def fma32(a, b, c):
             + np.asarray(c, np.longdouble)).astype(np.float32)
 
 
-def hs_sweep_accel(fx, fy, ft, alpha, U, V):
+ACCEL_W = np.float32(1.0 / 12.0)   # the corner weight; the edge weight is 2 ACCEL_W exactly
+
+
+def hs_sweep_accel_scaled(fx, fy, ft, alpha, U, V):
     """The Horn-Schunck sweep of the accelerated (Chebyshev) phase, all
-    float32: the same operator as hs_sweep (HornSchunck.py:51-66, 3x3 average
-    with scipy 'mirror' borders, then ub - fx (fx ub + fy vb + ft) / den) with
-    the device's operation order: vertical pair sums p, ub = fma(1/12, p_l +
-    p_r, (1/6) (p + (x_l + x_r))), num = fma(fx, ub, fma(fy, vb, ft)),
-    der = num * RN(1 / fma(fy, fy, fma(fx, fx, alpha^2))), T = fma(-fx, der,
-    ub).  Returns (Tu, Tv)."""
+    float32, scaled by 1/w (w = float32(1/12)): the same operator as hs_sweep
+    (HornSchunck.py:51-66, the 3x3 average with scipy 'mirror' borders, then
+    ub - fx (fx ub + fy vb + ft) / den) with the device's operation order.
+    The average's kernel [[1,2,1],[2,0,2],[1,2,1]] w is the separable
+    [1,2,1] x [1,2,1] kernel less its centre 4, so S = 12 x average is
+    formed from the vertical pair sum p = x_up + x_down as c = fma(2, x, p)
+    (the column's [1,2,1] sum), S = fma(2, p, c_left + c_right) (no
+    cancellation: the centre never enters).  With ub = w S and the time
+    derivative taken as ft' = RN(ft / w):
+    der' = fma(fx, S_u, fma(fy, S_v, ft')) * RN(1 / fma(fy, fy, fma(fx, fx, alpha^2))),
+    t = fma(-fx, der', S_u) = T / w (up to rounding).  Returns (t_u, t_v)."""
     f32 = np.float32
     fx, fy, ft, U, V = (np.asarray(x, f32) for x in (fx, fy, ft, U, V))
-    wc, we = f32(1.0 / 12.0), f32(1.0 / 6.0)
     a2 = f32(alpha) * f32(alpha)
+    two = f32(2.0)
 
-    def avg(X):
+    def S(X):
         P = np.pad(X, 1, mode="reflect")
-        pv = P[:-2, :] + P[2:, :]
-        mid = P[1:-1, :-2] + P[1:-1, 2:]
-        return fma32(wc, pv[:, :-2] + pv[:, 2:], we * (pv[:, 1:-1] + mid))
-    ub, vb = avg(U), avg(V)
+        p = P[:-2, :] + P[2:, :]
+        c = fma32(two, P[1:-1, :], p)
+        return fma32(two, p[:, 1:-1], c[:, :-2] + c[:, 2:])
+    su, sv = S(U), S(V)
+    fts = ft / ACCEL_W
     rcp = f32(1.0) / fma32(fy, fy, fma32(fx, fx, a2))
-    der = fma32(fx, ub, fma32(fy, vb, ft)) * rcp
-    return fma32(-fx, der, ub), fma32(-fy, der, vb)
+    der = fma32(fx, su, fma32(fy, sv, fts)) * rcp
+    return fma32(-fx, der, su), fma32(-fy, der, sv)
+
+
+def hs_sweep_accel(fx, fy, ft, alpha, U, V):
+    """T(x) of the accelerated phase's sweep, RN(w t) with (t_u, t_v) from
+    hs_sweep_accel_scaled: the value a plain step of that phase stores (its
+    update is fma(w, t, -x)).  Returns (Tu, Tv)."""
+    tu, tv = hs_sweep_accel_scaled(fx, fy, ft, alpha, U, V)
+    return ACCEL_W * tu, ACCEL_W * tv
 
 
 def ls_sweep_fast(fields, h, u, v):

@@ -343,6 +343,15 @@ def test_hs_tol_stops_inside_a_fused_launch(ops):
 _fma = orc.fma32
 
 
+def _accel_step(fx, fy, ft, alpha, U, V):
+    """One accelerated-phase sweep from (U, V) as the device forms it:
+    t = T / w from oracle.hs_sweep_accel_scaled, the plain step's value
+    T = RN(w t) and the update fma(w, t, -x).  Returns (Tu, Tv, du, dv)."""
+    tu, tv = orc.hs_sweep_accel_scaled(fx, fy, ft, alpha, U, V)
+    w = orc.ACCEL_W
+    return w * tu, w * tv, _fma(w, tu, -U), _fma(w, tv, -V)
+
+
 def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     """Chebyshev semi-iteration over the accelerated phase's float32 sweep T
     (oracle.hs_sweep_accel), with the device's float32 operation order: x_{k+1} = x_{k-1} + w_{k+1} ((x_k +
@@ -367,8 +376,7 @@ def _cheb_restated(im1, im2, alpha, bound, tol, maxnum, finish=0.0):
     hist = []
     n_acc = maxnum - max(1, maxnum // 2) if finish > 1 else maxnum
     for k in range(n_acc):
-        Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
-        du, dv = Tu - U, Tv - V
+        Tu, Tv, du, dv = _accel_step(fx, fy, ft, alpha, U, V)
         r = np.float32((np.sqrt((du.astype(np.float64) ** 2).sum()) + np.sqrt((dv.astype(np.float64) ** 2).sum()))
                        / U.size)
         hist.append(r)
@@ -418,8 +426,7 @@ def _cheb_restated_launches(im1, im2, alpha, bound, tol, maxnum, finish, depth):
     count, r = 0, None
     n_acc = maxnum - max(1, maxnum // 2) if finish > 1 else maxnum
     for k in range(n_acc):
-        Tu, Tv = orc.hs_sweep_accel(fx, fy, ft, alpha, U, V)
-        du, dv = Tu - U, Tv - V
+        Tu, Tv, du, dv = _accel_step(fx, fy, ft, alpha, U, V)
         tested = k % depth == depth - 1 or k == n_acc - 1
         if tested:
             r = res(du, dv)
