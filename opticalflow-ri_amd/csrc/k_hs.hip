@@ -275,19 +275,32 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     const bool own_x = lane >= K && lane < 64 - K && gx < W;
 
     float U[R], V[R], FX[R], FY[R], FT[R], DEN[R], RCP[R];
+    src = __builtin_amdgcn_readfirstlane(src);
+    base = __builtin_amdgcn_readfirstlane(base);
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
-    // a zero start: the launch holding the first sweep (or its re-run) reads
-    // no flow
+    // a zero start: the launch holding the first sweep (or its re-run) uses
+    // no flow (the input buffers are read anyway, and dropped)
     const bool zs = a.zero_start && (NORM ? base : a.sweep0) == 0;
+    // every load of the tile is issued before any value is used, so the
+    // rows' memory latencies overlap instead of adding up
+    size_t qs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) qs[r] = (size_t)mirror_idx(gy0 + r, H) * W + cx;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
-        U[r] = zs ? 0.0f : uin[q];
-        V[r] = zs ? 0.0f : vin[q];
-        FX[r] = a.fx[q];
-        FY[r] = a.fy[q];
-        FT[r] = a.ft[q];
+        U[r] = uin[qs[r]];
+        V[r] = vin[qs[r]];
+        FX[r] = a.fx[qs[r]];
+        FY[r] = a.fy[qs[r]];
+        FT[r] = a.ft[qs[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (zs) {
+            U[r] = 0.0f;
+            V[r] = 0.0f;
+        }
         float d = a.a2 + FX[r] * FX[r];
         DEN[r] = d + FY[r] * FY[r];
         RCP[r] = 1.0f / DEN[r];   // correctly rounded reciprocal
@@ -449,22 +462,42 @@ __device__ __forceinline__ void hs_cheb_tile(const HsPass& a, int nsw, int src, 
 
     constexpr float wc = (float)(1.0 / 12.0);
     float U[R], V[R], UP[R], VP[R], FX[R], FY[R], FT[R], RCP[R];
+    src = __builtin_amdgcn_readfirstlane(src);
+    base = __builtin_amdgcn_readfirstlane(base);
     const float* uin = a.in[src][0];
     const float* vin = a.in[src][1];
-    // a zero start: the launch holding the first sweep (or its re-run) reads
-    // no flow
+    const float* upin = a.pin[src][0];
+    const float* vpin = a.pin[src][1];
+    // a zero start: the launch holding the first sweep (or its re-run) uses
+    // no flow; before the first sweep x_{-1} = x_0 (the buffers are read
+    // anyway, and dropped)
     const bool zs = a.zero_start && base == 0;
+    // every load of the tile is issued before any value is used, so the
+    // rows' memory latencies overlap instead of adding up
+    size_t qs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) qs[r] = (size_t)mirror_idx(gy0 + r, H) * W + cx;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        size_t q = (size_t)mirror_idx(gy0 + r, H) * W + cx;
-        U[r] = zs ? 0.0f : uin[q];
-        V[r] = zs ? 0.0f : vin[q];
-        // x_{-1} = x_0 before the first sweep
-        UP[r] = base == 0 ? U[r] : a.pin[src][0][q];
-        VP[r] = base == 0 ? V[r] : a.pin[src][1][q];
-        FX[r] = a.fx[q];
-        FY[r] = a.fy[q];
-        FT[r] = a.ft[q] / wc;   // correctly rounded quotient
+        U[r] = uin[qs[r]];
+        V[r] = vin[qs[r]];
+        UP[r] = upin[qs[r]];
+        VP[r] = vpin[qs[r]];
+        FX[r] = a.fx[qs[r]];
+        FY[r] = a.fy[qs[r]];
+        FT[r] = a.ft[qs[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (zs) {
+            U[r] = 0.0f;
+            V[r] = 0.0f;
+        }
+        if (base == 0) {
+            UP[r] = U[r];
+            VP[r] = V[r];
+        }
+        FT[r] = FT[r] / wc;   // correctly rounded quotient
         RCP[r] = 1.0f / fmaf(FY[r], FY[r], fmaf(FX[r], FX[r], a.a2));
     }
     // the launch's Chebyshev weights, one per lane, broadcast per sweep
