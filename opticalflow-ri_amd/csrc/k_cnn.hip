@@ -406,6 +406,20 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
             const int sub = co >> 2, dy = sub >> 1, dx = sub & 1;
             float* pl = (co & 1) ? a.add_v : a.add_u;
             const int Wd = W << 1;
+            // every flow value this lane updates is read before any is
+            // written (distinct pixels), so the reads' latencies overlap
+            float old[MT][4];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int p = (wv * MT + mt) * 16 + 4 * g + i;
+                    const int y = y0 + p / TS, x = x0 + p % TS;
+                    old[mt][i] = 0.0f;
+                    if (y < H && x < W)
+                        old[mt][i] = pl[(long long)b * 4 * H * W + ((y << 1) + dy) * Wd + ((x << 1) + dx)];
+                }
+            }
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -415,8 +429,7 @@ __global__ __launch_bounds__(256) void conv3_lds_k(ConvArgs a) {
                     if (y >= H || x >= W) continue;
                     float v = acc[mt][nt][i] + bv;
                     if (a.act) v = v > 0.0f ? v : v * kSlope;
-                    float* d = pl + (long long)b * 4 * H * W + ((y << 1) + dy) * Wd + ((x << 1) + dx);
-                    *d = *d + v;
+                    pl[(long long)b * 4 * H * W + ((y << 1) + dy) * Wd + ((x << 1) + dx)] = old[mt][i] + v;
                 }
             }
         }

@@ -624,15 +624,25 @@ __device__ __forceinline__ void hs_copy_owned(const HsPass& a, int bx, int by) {
     const int gx = bx * outW - K + lane;
     const int gy0 = by * outH - K + w * R;
     if (!(lane >= K && lane < 64 - K && gx < a.W)) return;
+    // every value is read before any is written (rows outside the owned band
+    // read row 0 and are dropped), so the reads' latencies overlap
+    size_t q[R];
+    bool own[R];
+    float u[R], v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ry = w * R + r, gy = gy0 + r;
-        if (ry >= K && ry < WAVES * R - K && gy < a.H) {
-            const size_t q = (size_t)gy * a.W + gx;
-            a.out[0][0][q] = a.in[1][0][q];
-            a.out[0][1][q] = a.in[1][1][q];
-        }
+        own[r] = ry >= K && ry < WAVES * R - K && gy < a.H;
+        q[r] = own[r] ? (size_t)gy * a.W + gx : (size_t)gx;
+        u[r] = a.in[1][0][q[r]];
+        v[r] = a.in[1][1][q[r]];
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (own[r]) {
+            a.out[0][0][q[r]] = u[r];
+            a.out[0][1][q[r]] = v[r];
+        }
 }
 
 template <int R, int WAVES, bool NORM, bool CHEB>

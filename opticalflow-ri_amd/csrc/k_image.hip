@@ -107,7 +107,21 @@ __global__ void resize_h(Planes2 p, int W, int W2, int Ht, int first, const doub
     const double* kk = k + (size_t)xx * ks;
     const float* row = src + (size_t)(y + first) * W + xmin;
     double ss = 0.0;
-    for (int x = 0; x < xn; ++x) ss = ss + (double)row[x] * kk[x];
+    // 8 taps per step, loaded before use (taps past xn read tap xn - 1 and
+    // are not added)
+    for (int x0 = 0; x0 < xn; x0 += 8) {
+        float v[8];
+        double kv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int x = min(x0 + u, xn - 1);
+            v[u] = row[x];
+            kv[u] = kk[x];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (x0 + u < xn) ss = ss + (double)v[u] * kv[u];
+    }
     tmp[(size_t)y * W2 + xx] = (float)ss;
 }
 
@@ -121,7 +135,21 @@ __global__ void resize_v(Planes2 p, int W2, int H2, int first, const double* __r
     int ymin = b[2 * yy] - first, yn = b[2 * yy + 1];
     const double* kk = k + (size_t)yy * ks;
     double ss = 0.0;
-    for (int y = 0; y < yn; ++y) ss = ss + (double)tmp[(size_t)(y + ymin) * W2 + xx] * kk[y];
+    // 8 taps per step, loaded before use (taps past yn read tap yn - 1 and
+    // are not added)
+    for (int y0 = 0; y0 < yn; y0 += 8) {
+        float v[8];
+        double kv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int y = min(y0 + u, yn - 1);
+            v[u] = tmp[(size_t)(y + ymin) * W2 + xx];
+            kv[u] = kk[y];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (y0 + u < yn) ss = ss + (double)v[u] * kv[u];
+    }
     dst[(size_t)yy * W2 + xx] = (float)ss;
 }
 
@@ -250,12 +278,15 @@ __global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, i
     const double* tmp = p.tmp[z] + x;
     float* dst = p.dst[z] + x;
     const float sc = p.scale[z];
+    // even rows: the x-pass rows themselves (all read before any store); the
+    // last coarse row is read up front for the clamped last output row
+    double ev[kSplineRowsY];
+#pragma unroll
+    for (int q = 0; q < kSplineRowsY; ++q) ev[q] = tmp[(size_t)min(k0 + q, Hc - 1) * W];
+    const double last = tmp[(size_t)(Hc - 1) * W];
 #pragma unroll
     for (int q = 0; q < kSplineRowsY; ++q)
-        if (q < nk) {
-            float o = (float)tmp[(size_t)(k0 + q) * W];
-            dst[(size_t)(2 * (k0 + q)) * W] = o * sc;
-        }
+        if (q < nk) dst[(size_t)(2 * (k0 + q)) * W] = (float)ev[q] * sc;
     double s[kSplineRowsY];
     int sj[kSplineRowsY];
     int j0 = 0x3fffffff, j1 = 0;
@@ -270,13 +301,22 @@ __global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, i
             j1 = max(j1, sj[q] + width);
         }
     }
-#pragma unroll 8
-    for (int j = j0; j < j1; ++j) {
-        const double v = tmp[(size_t)j * W];
+    // 8 coarse rows per step, all loaded before any is used (rows past j1
+    // read row j1 - 1 again); a row outside a band adds 0.0 * v, which leaves
+    // the sum's bits unchanged (it is never -0.0: it starts at +0.0)
+    for (int j = j0; j < j1; j += 8) {
+        double v[8];
 #pragma unroll
-        for (int q = 0; q < kSplineRowsY; ++q) {
-            const int t = j - sj[q];
-            if (t >= 0 && t < width) s[q] = s[q] + ws[q][t] * v;
+        for (int u = 0; u < 8; ++u) v[u] = tmp[(size_t)min(j + u, j1 - 1) * W];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+#pragma unroll
+            for (int q = 0; q < kSplineRowsY; ++q) {
+                const int t = j + u - sj[q];
+                const bool in = t >= 0 && t < width;
+                const double wq = ws[q][in ? t : 0];
+                s[q] = s[q] + (in ? wq : 0.0) * v[u];
+            }
         }
     }
 #pragma unroll
@@ -284,7 +324,7 @@ __global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, i
         const int y = 2 * (k0 + q) + 1;
         if (q < nk) {
             // the last output row is clamped onto the last coarse row
-            float o = y < H - 1 ? (float)s[q] : (float)tmp[(size_t)(Hc - 1) * W];
+            float o = y < H - 1 ? (float)s[q] : (float)last;
             dst[(size_t)y * W] = o * sc;
         }
     }
@@ -313,13 +353,22 @@ __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, co
         s[i] = 0.0;
         sj[i] = i < ny ? st[y0 + i] : 0x3fffffff;
     }
-#pragma unroll 8
-    for (int j = j0; j < j1; ++j) {
-        const double v = tmp[(size_t)j * W];
+    // 8 coarse rows per step, all loaded before any is used (rows past j1
+    // read row j1 - 1 again); a row outside a band adds 0.0 * v, which leaves
+    // the sum's bits unchanged (it is never -0.0: it starts at +0.0)
+    for (int j = j0; j < j1; j += 8) {
+        double v[8];
 #pragma unroll
-        for (int i = 0; i < kSplineRowsY; ++i) {
-            const int t = j - sj[i];
-            if (t >= 0 && t < width) s[i] = s[i] + ws[i][t] * v;
+        for (int u = 0; u < 8; ++u) v[u] = tmp[(size_t)min(j + u, j1 - 1) * W];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+#pragma unroll
+            for (int i = 0; i < kSplineRowsY; ++i) {
+                const int t = j + u - sj[i];
+                const bool in = t >= 0 && t < width;
+                const double wq = ws[i][in ? t : 0];
+                s[i] = s[i] + (in ? wq : 0.0) * v[u];
+            }
         }
     }
     float* dst = p.dst[z] + x;

@@ -472,16 +472,43 @@ __device__ __forceinline__ void ls_deliver(const LsPass& a, int from, int bx, in
     if (!(lane >= K && lane < 64 - K && gx < a.W)) return;
     const float* su = a.in[from][0];
     const float* sv = a.in[from][1];
+    // every value is read before any is written (rows outside the owned
+    // band read row 0 and are dropped), so the reads' latencies overlap
+    size_t q[R];
+    bool own[R];
+    float u[R], v[R], du[R], dv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ry = w * R + r, gy = gy0 + r;
-        if (ry >= K && ry < WAVES * R - K && gy < a.H) {
-            const size_t q = (size_t)gy * a.W + gx;
-            const float u = su[q], v = sv[q];
-            a.dst[0][q] = a.dst_add ? a.dst[0][q] + u : u;
-            a.dst[1][q] = a.dst_add ? a.dst[1][q] + v : v;
+        own[r] = ry >= K && ry < WAVES * R - K && gy < a.H;
+        q[r] = own[r] ? (size_t)gy * a.W + gx : (size_t)gx;
+    }
+    if (a.dst_add) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            u[r] = su[q[r]];
+            v[r] = sv[q[r]];
+            du[r] = a.dst[0][q[r]];
+            dv[r] = a.dst[1][q[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            u[r] = du[r] + u[r];
+            v[r] = dv[r] + v[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            u[r] = su[q[r]];
+            v[r] = sv[q[r]];
         }
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (own[r]) {
+            a.dst[0][q[r]] = u[r];
+            a.dst[1][q[r]] = v[r];
+        }
 }
 
 template <int R, int WAVES, bool CHEB, bool FAST = false>

@@ -7,6 +7,28 @@
 namespace ofri {
 namespace {
 
+// Lane-strided sums of n (x, y) pairs: a += p[2i], b += p[2i + 1] for
+// i = lane, lane + 64, ... in that order.  Four strides are loaded before
+// any is added, so their latencies overlap (a stride past n re-reads pair
+// n - 1 and is not added).
+__device__ __forceinline__ void lane_sums(const double* p, int n, int lane, double& a, double& b) {
+    for (int i0 = lane; i0 < n; i0 += 256) {
+        double x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + 64 * u, n - 1);
+            x[u] = p[(size_t)i * 2];
+            y[u] = p[(size_t)i * 2 + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + 64 * u < n) {
+                a = a + x[u];
+                b = b + y[u];
+            }
+    }
+}
+
 // Stop test at the start of a launch.  Every workgroup of launch j reads the
 // state launch j-1 published (slot `prev`) and, when that launch is pending,
 // reduces its per-sweep partial sums (part_prev: [nsw][nblk][2] doubles, the
@@ -45,10 +67,7 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
         // (lane-strided, then a fixed butterfly)
         for (int k = s.pend_s0 + wq; k < s.pend_nsw; k += nw) {
             double a = 0.0, b = 0.0;
-            for (int i = lane; i < nblk; i += 64) {
-                a = a + part_prev[((size_t)k * nblk + i) * 2];
-                b = b + part_prev[((size_t)k * nblk + i) * 2 + 1];
-            }
+            lane_sums(part_prev + (size_t)k * nblk * 2, nblk, lane, a, b);
             for (int off = 32; off > 0; off >>= 1) {
                 a = a + __shfl_xor(a, off);
                 b = b + __shfl_xor(b, off);
@@ -62,10 +81,7 @@ __device__ SolveState take_test(const SolveCtrl* ctrl, int prev, const double* p
             // the tested launch's solution norm partials (row K of the slot)
             const double* pn = part_prev + (size_t)K * nblk * 2;
             double a = 0.0, b = 0.0;
-            for (int i = lane; i < nblk; i += 64) {
-                a = a + pn[(size_t)i * 2];
-                b = b + pn[(size_t)i * 2 + 1];
-            }
+            lane_sums(pn, nblk, lane, a, b);
             for (int off = 32; off > 0; off >>= 1) {
                 a = a + __shfl_xor(a, off);
                 b = b + __shfl_xor(b, off);
