@@ -9,7 +9,9 @@
 // and every pixel's window is read from there.  The structure tensor A and
 // the per-iteration right-hand side b are summed in double; J is sampled
 // bilinearly (the OpenCL LINEAR sampler at pixel centres, clamp-to-edge)
-// with the pixel's fractional weights formed once per iteration.
+// with the pixel's fractional weights formed once per iteration, the four
+// weighted samples accumulated with fused multiply-adds (the sampler's own
+// precision is the hardware's; the oracle forms the same sums).
 //
 // The window is the reference's 32x32 grid of 4 tiles of 8 per axis with
 // 0/1 weights (pyrlkDenseLargeW.cl:319-371): it is given here as one bit
@@ -184,9 +186,9 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                     for (int c = 0; c < wx; ++c) {
                         const float ja1 = Ja[c + 1], jb1 = Jb[c + 1];
                         float t = w00 * ja0;
-                        t = t + w10 * ja1;
-                        t = t + w01 * jb0;
-                        t = t + w11 * jb1;
+                        t = fmaf(w10, ja1, t);
+                        t = fmaf(w01, jb0, t);
+                        t = fmaf(w11, jb1, t);
                         float diff = t - pr[c];
                         b1 = fma((double)diff, (double)dxr[c], b1);
                         b2 = fma((double)diff, (double)dyr[c], b2);
@@ -198,9 +200,9 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                         const float ja1 = Ja[c + 1], jb1 = Jb[c + 1];
                         if ((mx >> c) & 1u) {
                             float t = w00 * ja0;
-                            t = t + w10 * ja1;
-                            t = t + w01 * jb0;
-                            t = t + w11 * jb1;
+                            t = fmaf(w10, ja1, t);
+                            t = fmaf(w01, jb0, t);
+                            t = fmaf(w11, jb1, t);
                             float diff = t - pr[c];
                             b1 = fma((double)diff, (double)dxr[c], b1);
                             b2 = fma((double)diff, (double)dyr[c], b2);
@@ -232,9 +234,9 @@ __global__ __launch_bounds__(kTile * kTile) void lk_dense_k(const float* __restr
                         xb = clampi(xb, 0, W - 1);
                     }
                     float t = w00 * Ja[xa];
-                    t = t + w10 * Ja[xb];
-                    t = t + w01 * Jb[xa];
-                    t = t + w11 * Jb[xb];
+                    t = fmaf(w10, Ja[xb], t);
+                    t = fmaf(w01, Jb[xa], t);
+                    t = fmaf(w11, Jb[xb], t);
                     float diff = t - pr[c];
                     b1 = fma((double)diff, (double)dxr[c], b1);
                     b2 = fma((double)diff, (double)dyr[c], b2);

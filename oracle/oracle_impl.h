@@ -870,10 +870,12 @@ void NAME(orc_lk_dense)(const REAL* I, const REAL* J, int H, int W, int hw, int 
                             if (!wxm[c]) continue;
                             size_t q = (size_t)r * win + c;
                             int xa = NAME(clp)(x0 + c, W), xb = NAME(clp)(x0 + c + 1, W);
+                            /* the four weighted samples accumulated with fused
+                             * multiply-adds, as the kernel does */
                             REAL t = w00 * J[IDX(ya, xa)];
-                            t = t + w10 * J[IDX(ya, xb)];
-                            t = t + w01 * J[IDX(yb, xa)];
-                            t = t + w11 * J[IDX(yb, xb)];
+                            t = NAME(fma_)(w10, J[IDX(ya, xb)], t);
+                            t = NAME(fma_)(w01, J[IDX(yb, xa)], t);
+                            t = NAME(fma_)(w11, J[IDX(yb, xb)], t);
                             REAL diff = t - P[q];
                             b1 += (double)diff * Dx[q];
                             b2 += (double)diff * Dy[q];
