@@ -34,12 +34,16 @@
 namespace ofri {
 namespace {
 
-// value of the lane to the left (lane-1) / right (lane+1) of this lane
-__device__ __forceinline__ float from_left(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x138, 0xF, 0xF, false));
+// Value of the lane to the left (lane-1) / right (lane+1) of this lane; the
+// vacated edge lane reads 0 (bound_ctrl), so the shift needs no copy of the
+// old value and the compiler can fold it into a consuming add as a DPP
+// operand.  Only a tile's outermost lanes see the 0, and they lie in the
+// halo that is never written.
+__device__ __forceinline__ float lane_left0(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
 }
-__device__ __forceinline__ float from_right(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x130, 0xF, 0xF, false));
+__device__ __forceinline__ float lane_right0(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 
 // Derivatives: HS(im2=I0, im1=I1) -> computeDerivatives(I1, I0)
@@ -228,7 +232,7 @@ struct Trip {
 };
 
 __device__ __forceinline__ Trip trip(float x) {
-    return Trip{(double)from_left(x), (double)x, (double)from_right(x)};
+    return Trip{(double)lane_left0(x), (double)x, (double)lane_right0(x)};
 }
 
 // Average of the 8 neighbours with weights wc (corners) and we (edges),
@@ -249,12 +253,18 @@ __device__ __forceinline__ float hs_avg(const Trip& up, const Trip& mid, const T
 // num / den correctly rounded, from rcp = RN(1/den): q = RN(num*rcp) is within
 // an ulp of the quotient and one FMA residual correction rounds it exactly
 // (Markstein's theorem for round-to-nearest binary arithmetic).  Tiny
-// numerators, where the residual could underflow, take the IEEE division.
+// numerators, where the residual could underflow, take the IEEE division;
+// the test for them is one branch for the whole wavefront, taken when any
+// of its lanes has one.
 __device__ __forceinline__ float quotient(float num, float den, float rcp) {
-    if (__builtin_expect(fabsf(num) < 0x1p-100f, 0)) return num / den;
     float q = num * rcp;
     float res = fmaf(-den, q, num);
-    return fmaf(res, rcp, q);
+    q = fmaf(res, rcp, q);
+    const bool tiny = fabsf(num) < 0x1p-100f;
+    if (__builtin_expect(__any(tiny), 0)) {
+        if (tiny) q = num / den;
+    }
+    return q;
 }
 
 // The extended tile is loaded as the mirror extension of the image (every
@@ -419,15 +429,6 @@ __device__ __forceinline__ void hs_tile(const HsPass& a, int nsw, int src, int b
     }
 }
 
-// Lane shifts whose vacated edge lane reads 0 (bound_ctrl), so the compiler
-// can fold them into the consuming add as a DPP operand.  Only the tile's
-// outermost lanes see the 0, and they lie in the halo that is never written.
-__device__ __forceinline__ float lane_left0(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float lane_right0(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
-}
 
 // The accelerated phase's tile: Chebyshev semi-iteration over the sweep T.
 // With the damped step T_g(x) = x + gamma (T(x) - x), whose spectrum lies in
