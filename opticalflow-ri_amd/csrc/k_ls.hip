@@ -39,11 +39,15 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 namespace ofri {
 namespace {
 
+// Value of the lane to the left (lane-1) / right (lane+1); the vacated edge
+// lane reads 0 (bound_ctrl: no copy of the old value).  Only a tile's
+// outermost lanes see the 0, and they lie in the halo that is never written
+// (the halo is at least one column wide).
 __device__ __forceinline__ float from_left(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
