@@ -708,15 +708,17 @@ const HsVariant kHsVariants[] = {
 };
 #undef HS_SHAPE
 constexpr int kNumHsVariants = sizeof(kHsVariants) / sizeof(kHsVariants[0]);
-// Defaults measured by the autotuner on gfx950 (ms per sweep over 120
-// sweeps): about 1M pixels and up favour 8-row lanes in 16-wavefront tiles
-// fused 10 sweeps deep (fewer launches outweigh the wider halo); smaller
-// images 4-row lanes in 8-wavefront tiles, 5 deep (more workgroups to spread
-// over the 256 CUs).
-// Residual-mode defaults (measured on gfx950, Chebyshev and plain sweeps
-// with norms): 8 rows x 16 wavefronts fused 10 deep from about 1M pixels,
-// 4 rows x 16 wavefronts fused 10 deep below (fewer launches than the plain
-// sweep's shape, whose per-launch test would cost more).
+// Defaults measured on gfx950 (600-sweep solves over every compiled shape
+// and depth 4..12, after the XCD tile order and the batched tile loads):
+// about 1M pixels and up favour 6-row lanes in 16-wavefront tiles fused 5
+// sweeps deep (1.85 ms at 1024^2; 8 x 16 fused 10 deep, the earlier
+// default, 1.93 ms: the narrower halo now outweighs the extra launches);
+// smaller images 4-row lanes in 8-wavefront tiles, 5 deep (0.91 ms at 512^2).
+// Chebyshev defaults (measured on gfx950 over every compiled shape and depth
+// 6..14): 8 rows x 16 wavefronts fused 10 deep from about 1M pixels (2.93 us
+// per sweep at 1024^2), 4 rows x 16 wavefronts fused 12 deep below (1.66 us
+// per sweep at 512^2; 14 deep runs 1.55 us per sweep but tests the stop rule
+// only every 14 sweeps); the finishing plain sweeps of a solve keep its shape.
 void hs_cheb_default(size_t npx, int* variant, int* depth) {
     *variant = npx >= (size_t)700000 ? 11 : 9;
     *depth = npx >= (size_t)700000 ? 10 : 12;
@@ -724,8 +726,8 @@ void hs_cheb_default(size_t npx, int* variant, int* depth) {
 
 void hs_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
-        *variant = 11;
-        *depth = 10;
+        *variant = 8;
+        *depth = 5;
     } else {
         *variant = 5;
         *depth = 5;

@@ -575,13 +575,16 @@ const LsVariant kLsVariants[] = {
 };
 #undef LS_SHAPE
 constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
-// Defaults measured by the autotuner on gfx950.
+// Defaults measured on gfx950 (60-sweep solves over every compiled shape and
+// depth 3..8): 12 rows x 4 wavefronts 4 deep from about 0.7 Mpx (0.68 ms at
+// 1024^2), 4 rows x 8 wavefronts 5 deep below (0.31 ms at 512^2, 11 % less
+// than 8 rows x 4 wavefronts).
 void ls_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
         *variant = 3;
         *depth = 4;
     } else {
-        *variant = 1;
+        *variant = 2;
         *depth = 5;
     }
 }
