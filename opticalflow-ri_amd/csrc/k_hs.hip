@@ -192,7 +192,7 @@ struct HsPass {
     int final_;              // 1: the phase's final launch (test, re-run, publish)
     int next_budget;         // final launch: > 0 arms a following phase of this many sweeps
     int rel_src;             // 1: buffer indices relative to ctrl->base_buf
-    int zero_start;          // 1: the solve starts from U = V = 0 (the first sweep reads no input)
+    int zero_start;          // 1: the solve starts from U = V = 0 (the first sweep uses no input)
     double tol;
     int maxnum;
     size_t N;

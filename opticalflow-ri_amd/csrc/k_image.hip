@@ -302,8 +302,8 @@ __global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, i
         }
     }
     // 8 coarse rows per step, all loaded before any is used (rows past j1
-    // read row j1 - 1 again); a row outside a band adds 0.0 * v, which leaves
-    // the sum's bits unchanged (it is never -0.0: it starts at +0.0)
+    // read row j1 - 1 again); a row outside an output's band leaves its sum
+    // as it was (a select, not a branch)
     for (int j = j0; j < j1; j += 8) {
         double v[8];
 #pragma unroll
@@ -314,8 +314,8 @@ __global__ __launch_bounds__(256) void spline_y2(SplinePlanes p, int W, int H, i
             for (int q = 0; q < kSplineRowsY; ++q) {
                 const int t = j + u - sj[q];
                 const bool in = t >= 0 && t < width;
-                const double wq = ws[q][in ? t : 0];
-                s[q] = s[q] + (in ? wq : 0.0) * v[u];
+                const double sn = s[q] + ws[q][in ? t : 0] * v[u];
+                s[q] = in ? sn : s[q];
             }
         }
     }
@@ -354,8 +354,8 @@ __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, co
         sj[i] = i < ny ? st[y0 + i] : 0x3fffffff;
     }
     // 8 coarse rows per step, all loaded before any is used (rows past j1
-    // read row j1 - 1 again); a row outside a band adds 0.0 * v, which leaves
-    // the sum's bits unchanged (it is never -0.0: it starts at +0.0)
+    // read row j1 - 1 again); a row outside an output's band leaves its sum
+    // as it was (a select, not a branch)
     for (int j = j0; j < j1; j += 8) {
         double v[8];
 #pragma unroll
@@ -366,8 +366,8 @@ __global__ __launch_bounds__(256) void spline_y(SplinePlanes p, int W, int H, co
             for (int i = 0; i < kSplineRowsY; ++i) {
                 const int t = j + u - sj[i];
                 const bool in = t >= 0 && t < width;
-                const double wq = ws[i][in ? t : 0];
-                s[i] = s[i] + (in ? wq : 0.0) * v[u];
+                const double sn = s[i] + ws[i][in ? t : 0] * v[u];
+                s[i] = in ? sn : s[i];
             }
         }
     }

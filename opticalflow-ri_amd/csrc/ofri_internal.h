@@ -39,7 +39,7 @@ void clear_error();
 // straddle), so XCD j takes a contiguous row-major run of tiles: run j holds
 // the nblk / 8 (+1 for the first nblk % 8 runs) tiles after those of runs
 // 0..j-1, and its workgroups take them in dispatch order.  A bijection of
-// [0, nblk); results do not depend on it.
+// [0, nblk) whatever the XCD count; results do not depend on it.
 __device__ __forceinline__ int xcd_run(int blk, int nblk) {
     const int q = nblk >> 3, r = nblk & 7, j = blk & 7;
     return j * q + (j < r ? j : r) + (blk >> 3);
@@ -50,7 +50,6 @@ __device__ __forceinline__ void xcd_tile(int blk, int nblk, int tiles_x, int& bx
     by = t / tiles_x;
     bx = t - by * tiles_x;
 }
-
 
 // Device-side control word of a solver run with a convergence test.
 // Stop-test state of a Jacobi solve as one launch hands it to the next: the
