@@ -194,6 +194,24 @@ def test_device_forward_matches_restatement(net, B, H, W):
 
 
 @pytest.mark.gpu
+def test_device_forward_batch_equals_single_images(net):
+    """24 images of 256 x 256 in one pass run the full-resolution layers on
+    16 x 16 tiles with 32-channel groups; one image alone runs them on 8 x 8
+    tiles.  Every image's heads equal, bit for bit, those of the image passed
+    alone: an output is the same sequence of MFMAs whichever tile shape and
+    workgroup compute it."""
+    B, H, W = 24, 256, 256
+    inp, _, _ = _batch(B, H, W, 17)
+    net.set_params(cnn.init_params(3, planes=4))
+    coarse, fine = net.predict(inp)
+    gc, gf = coarse.download(), fine.download()
+    assert np.isfinite(gc).all() and np.isfinite(gf).all()
+    for i in (0, 5, 11, 23):
+        c1, f1 = net.predict(inp[i:i + 1])
+        assert np.array_equal(c1.download()[0], gc[i]) and np.array_equal(f1.download()[0], gf[i]), i
+
+
+@pytest.mark.gpu
 def test_device_gradients_match_restatement(net):
     """Per layer, the device gradient's distance from the bf16 restatement is
     measured against the bf16 rounding noise itself (the distance between the
