@@ -8,9 +8,10 @@
 // Iyt and generate_invmatrix's 2x2 inverse B11, B12, B22 -- formed from I and
 // It in each launch's load phase, as the setup of the reference forms them
 // --, DPP for horizontal neighbours, LDS for the edge rows of each
-// wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries
-// as explicit selects in the workgroups whose tile touches the image border
-// (interior tiles skip them).  Each launch also reduces, per fused sweep, the
+// wavefront), with 'nearest' (clamp) and 'constant' (zero) boundaries in
+// the workgroups whose tile touches the image border (interior tiles skip
+// them): the column tests once per row entering a lane's register window,
+// the image's first and last rows in a branch of their own.  Each launch also reduces, per fused sweep, the
 // squared update of the pixels it owns; the last workgroup to finish turns
 // the per-sweep sums into the reference's norm (||du||_2 + ||dv||_2)/(r*c),
 // applies `while err > tol and k < maxnum` on the device, and marks a re-run
@@ -126,58 +127,82 @@ struct Raw {
 
 __device__ __forceinline__ Raw raw(float x) { return Raw{from_left(x), x, from_right(x)}; }
 
-// One LS update at a pixel from the 3x3 neighbourhoods of u and v (rows
-// pu/cu/nu etc.).  EDGE adds the 'nearest' / 'constant' boundary handling;
-// tiles that do not touch the image border skip it.
+// One row of u or v around a pixel as the update reads it: with the
+// 'nearest' boundary (clamped columns: the derivative and corner terms) and
+// with the 'constant' one (zero outside the image: the 8-neighbour sum).  In a
+// tile away from the image border both are the lane neighbours as they are;
+// in a border tile the lane's column tests are applied once, when the row
+// enters the register window.  Rows above and below the image are replaced
+// where a pixel row reads them (border_rows).
+struct Row {
+    Raw n, z;
+    // the values the double-precision sums read, converted once per row
+    double nl, nr, zl, zc, zr;
+};
+
+template <bool EDGE>
+__device__ __forceinline__ Row make_row(float x, bool el, bool er) {
+    const Raw q = raw(x);
+    Row w;
+    if constexpr (!EDGE) {
+        w.n = q;
+        w.z = q;
+    } else {
+        w.n = Raw{el ? x : q.l, x, er ? x : q.r};
+        w.z = Raw{el ? 0.0f : q.l, x, er ? 0.0f : q.r};
+    }
+    w.nl = (double)w.n.l;
+    w.nr = (double)w.n.r;
+    w.zl = (double)w.z.l;
+    w.zc = (double)x;
+    w.zr = (double)w.z.r;
+    return w;
+}
+
+// The rows above and below pixel row gy as its update reads them: at the
+// image's first (last) row the row above (below) is the row itself with the
+// 'nearest' boundary and zero with the 'constant' one.
+__device__ __forceinline__ Row border_row(const Row& c) {
+    Row w;
+    w.n = c.n;
+    w.z = Raw{0.0f, 0.0f, 0.0f};
+    w.nl = c.nl;
+    w.nr = c.nr;
+    w.zl = 0.0;
+    w.zc = 0.0;
+    w.zr = 0.0;
+    return w;
+}
+
+// One LS update at a pixel from the rows above (p), at (c) and below (n) of
+// u and v.
 //   Dr, Dc, Fr, Fc: two-term sums scaled by a power of two are formed in
 //   float32, which rounds exactly as the double sum rounded to float does;
 //   M and the 8-neighbour sum H0 are summed in double as the reference does.
-template <bool EDGE>
-__device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Raw& nu, const Raw& pv,
-                                          const Raw& cv, const Raw& nv, bool top, bool bot, bool el,
-                                          bool er, float h, float iix, float iiy, float ii2, float ixt,
-                                          float iyt, float b11, float b12, float b22, float& un,
+__device__ __forceinline__ void ls_update(const Row& pu, const Row& cu, const Row& nu, const Row& pv,
+                                          const Row& cv, const Row& nv, float h, float iix, float iiy, float ii2,
+                                          float ixt, float iyt, float b11, float b12, float b22, float& un,
                                           float& vn) {
-    // clamped rows / columns ('nearest')
-    const Raw& uu = (EDGE && top) ? cu : pu;
-    const Raw& ud = (EDGE && bot) ? cu : nu;
-    const Raw& vu = (EDGE && top) ? cv : pv;
-    const Raw& vd = (EDGE && bot) ? cv : nv;
-    float uu_l = (EDGE && el) ? uu.c : uu.l, uu_r = (EDGE && er) ? uu.c : uu.r;
-    float ud_l = (EDGE && el) ? ud.c : ud.l, ud_r = (EDGE && er) ? ud.c : ud.r;
-    float vu_l = (EDGE && el) ? vu.c : vu.l, vu_r = (EDGE && er) ? vu.c : vu.r;
-    float vd_l = (EDGE && el) ? vd.c : vd.l, vd_r = (EDGE && er) ? vd.c : vd.r;
-    float cu_l = (EDGE && el) ? cu.c : cu.l, cu_r = (EDGE && er) ? cu.c : cu.r;
-    float cv_l = (EDGE && el) ? cv.c : cv.l, cv_r = (EDGE && er) ? cv.c : cv.r;
+    const Raw &uu = pu.n, &ud = nu.n, &vu = pv.n, &vd = nv.n;
     float dru = 0.5f * (ud.c - uu.c);
-    float dcu = 0.5f * (cu_r - cu_l);
+    float dcu = 0.5f * (cu.n.r - cu.n.l);
     float drv = 0.5f * (vd.c - vu.c);
-    float dcv = 0.5f * (cv_r - cv_l);
+    float dcv = 0.5f * (cv.n.r - cv.n.l);
     float fru = uu.c + ud.c;
-    float fcv = cv_l + cv_r;
+    float fcv = cv.n.l + cv.n.r;
     // the reference's double sum of quarter-weighted corners, with the exact
     // power-of-two scale taken out of the sum (it commutes with rounding)
-    float mu = (float)((((double)uu_l - (double)uu_r) - (double)ud_l) + (double)ud_r) * 0.25f;
-    float mv = (float)((((double)vu_l - (double)vu_r) - (double)vd_l) + (double)vd_r) * 0.25f;
-    // 8-neighbour sums with zero outside the image ('constant')
-    float zl_pu = (EDGE && (el || top)) ? 0.0f : pu.l, zc_pu = (EDGE && top) ? 0.0f : pu.c;
-    float zr_pu = (EDGE && (er || top)) ? 0.0f : pu.r;
-    float zl_nu = (EDGE && (el || bot)) ? 0.0f : nu.l, zc_nu = (EDGE && bot) ? 0.0f : nu.c;
-    float zr_nu = (EDGE && (er || bot)) ? 0.0f : nu.r;
-    float zl_pv = (EDGE && (el || top)) ? 0.0f : pv.l, zc_pv = (EDGE && top) ? 0.0f : pv.c;
-    float zr_pv = (EDGE && (er || top)) ? 0.0f : pv.r;
-    float zl_nv = (EDGE && (el || bot)) ? 0.0f : nv.l, zc_nv = (EDGE && bot) ? 0.0f : nv.c;
-    float zr_nv = (EDGE && (er || bot)) ? 0.0f : nv.r;
-    float zl_cu = (EDGE && el) ? 0.0f : cu.l, zr_cu = (EDGE && er) ? 0.0f : cu.r;
-    float zl_cv = (EDGE && el) ? 0.0f : cv.l, zr_cv = (EDGE && er) ? 0.0f : cv.r;
+    float mu = (float)(((pu.nl - pu.nr) - nu.nl) + nu.nr) * 0.25f;
+    float mv = (float)(((pv.nl - pv.nr) - nv.nl) + nv.nr) * 0.25f;
+    // 8-neighbour sums with zero outside the image
     double hu = 0.0;
-    hu = hu + (double)zl_pu; hu = hu + (double)zc_pu; hu = hu + (double)zr_pu;
-    hu = hu + (double)zl_cu; hu = hu + (double)zr_cu;
-    hu = hu + (double)zl_nu; hu = hu + (double)zc_nu; hu = hu + (double)zr_nu;
+    hu = hu + pu.zl; hu = hu + pu.zc; hu = hu + pu.zr;
+    hu = hu + cu.zl; hu = hu + cu.zr;
+    hu = hu + nu.zl; hu = hu + nu.zc; hu = hu + nu.zr;
     double hv = 0.0;
-    hv = hv + (double)zl_pv; hv = hv + (double)zc_pv; hv = hv + (double)zr_pv;
-    hv = hv + (double)zl_cv; hv = hv + (double)zr_cv;
-    hv = hv + (double)zl_nv; hv = hv + (double)zc_nv; hv = hv + (double)zr_nv;
+    hv = hv + pv.zl; hv = hv + pv.zc; hv = hv + pv.zr;
+    hv = hv + cv.zl; hv = hv + cv.zr;
+    hv = hv + nv.zl; hv = hv + nv.zc; hv = hv + nv.zr;
     float h0u = (float)hu, h0v = (float)hv;
     float bu = (2.0f * iix) * dru;
     bu = bu + iix * dcv;
@@ -202,41 +227,19 @@ __device__ __forceinline__ void ls_update(const Raw& pu, const Raw& cu, const Ra
 // The same update in float32 with fused multiply-adds, for the sweeps the
 // learned form runs between its tests: full differences with the halves
 // folded into the coefficients (0.5 iix, 0.5 iiy), the corner term as a
-// difference of row differences, the 8-neighbour sum in pairs.  Same
-// boundary handling as ls_update.
-template <bool EDGE>
-__device__ __forceinline__ void ls_update_fast(const Raw& pu, const Raw& cu, const Raw& nu, const Raw& pv,
-                                               const Raw& cv, const Raw& nv, bool top, bool bot, bool el,
-                                               bool er, float h, float iix, float iiy, float ii2, float ixt,
-                                               float iyt, float b11, float b12, float b22, float& un,
-                                               float& vn) {
-    const Raw& uu = (EDGE && top) ? cu : pu;
-    const Raw& ud = (EDGE && bot) ? cu : nu;
-    const Raw& vu = (EDGE && top) ? cv : pv;
-    const Raw& vd = (EDGE && bot) ? cv : nv;
-    const float uu_l = (EDGE && el) ? uu.c : uu.l, uu_r = (EDGE && er) ? uu.c : uu.r;
-    const float ud_l = (EDGE && el) ? ud.c : ud.l, ud_r = (EDGE && er) ? ud.c : ud.r;
-    const float vu_l = (EDGE && el) ? vu.c : vu.l, vu_r = (EDGE && er) ? vu.c : vu.r;
-    const float vd_l = (EDGE && el) ? vd.c : vd.l, vd_r = (EDGE && er) ? vd.c : vd.r;
-    const float cu_l = (EDGE && el) ? cu.c : cu.l, cu_r = (EDGE && er) ? cu.c : cu.r;
-    const float cv_l = (EDGE && el) ? cv.c : cv.l, cv_r = (EDGE && er) ? cv.c : cv.r;
-    const float dru = ud.c - uu.c, dcu = cu_r - cu_l;   // twice Dr, Dc
-    const float drv = vd.c - vu.c, dcv = cv_r - cv_l;
-    const float fru = uu.c + ud.c, fcv = cv_l + cv_r;
-    const float mu = 0.25f * ((uu_l - uu_r) - (ud_l - ud_r));
-    const float mv = 0.25f * ((vu_l - vu_r) - (vd_l - vd_r));
-    const float zl_pu = (EDGE && (el || top)) ? 0.0f : pu.l, zc_pu = (EDGE && top) ? 0.0f : pu.c;
-    const float zr_pu = (EDGE && (er || top)) ? 0.0f : pu.r;
-    const float zl_nu = (EDGE && (el || bot)) ? 0.0f : nu.l, zc_nu = (EDGE && bot) ? 0.0f : nu.c;
-    const float zr_nu = (EDGE && (er || bot)) ? 0.0f : nu.r;
-    const float zl_pv = (EDGE && (el || top)) ? 0.0f : pv.l, zc_pv = (EDGE && top) ? 0.0f : pv.c;
-    const float zr_pv = (EDGE && (er || top)) ? 0.0f : pv.r;
-    const float zl_nv = (EDGE && (el || bot)) ? 0.0f : nv.l, zc_nv = (EDGE && bot) ? 0.0f : nv.c;
-    const float zr_nv = (EDGE && (er || bot)) ? 0.0f : nv.r;
-    const float zl_cu = (EDGE && el) ? 0.0f : cu.l, zr_cu = (EDGE && er) ? 0.0f : cu.r;
-    const float zl_cv = (EDGE && el) ? 0.0f : cv.l, zr_cv = (EDGE && er) ? 0.0f : cv.r;
-    const float h0u = ((zl_pu + zr_pu) + (zl_nu + zr_nu)) + ((zc_pu + zc_nu) + (zl_cu + zr_cu));
-    const float h0v = ((zl_pv + zr_pv) + (zl_nv + zr_nv)) + ((zc_pv + zc_nv) + (zl_cv + zr_cv));
+// difference of row differences, the 8-neighbour sum in pairs.
+__device__ __forceinline__ void ls_update_fast(const Row& pu, const Row& cu, const Row& nu, const Row& pv,
+                                               const Row& cv, const Row& nv, float h, float iix, float iiy,
+                                               float ii2, float ixt, float iyt, float b11, float b12, float b22,
+                                               float& un, float& vn) {
+    const Raw &uu = pu.n, &ud = nu.n, &vu = pv.n, &vd = nv.n;
+    const float dru = ud.c - uu.c, dcu = cu.n.r - cu.n.l;   // twice Dr, Dc
+    const float drv = vd.c - vu.c, dcv = cv.n.r - cv.n.l;
+    const float fru = uu.c + ud.c, fcv = cv.n.l + cv.n.r;
+    const float mu = 0.25f * ((uu.l - uu.r) - (ud.l - ud.r));
+    const float mv = 0.25f * ((vu.l - vu.r) - (vd.l - vd.r));
+    const float h0u = ((pu.z.l + pu.z.r) + (nu.z.l + nu.z.r)) + ((pu.z.c + nu.z.c) + (cu.z.l + cu.z.r));
+    const float h0v = ((pv.z.l + pv.z.r) + (nv.z.l + nv.z.r)) + ((pv.z.c + nv.z.c) + (cv.z.l + cv.z.r));
     const float hix = 0.5f * iix, hiy = 0.5f * iiy;
     float bu = fmaf(iix, dru, ixt);
     bu = fmaf(hix, dcv, bu);
@@ -299,22 +302,38 @@ __device__ __forceinline__ void ls_sweeps(const LsPass& a, int nsw, int base, in
         float vup = w > 0 ? edge[b][w - 1][1][1][lane] : v[0];
         float udn = w < WAVES - 1 ? edge[b][w + 1][0][0][lane] : u[R - 1];
         float vdn = w < WAVES - 1 ? edge[b][w + 1][0][1][lane] : v[R - 1];
-        Raw pu = raw(uup), pv = raw(vup);
-        Raw cu = raw(u[0]), cv = raw(v[0]);
+        Row pu = make_row<EDGE>(uup, el, er), pv = make_row<EDGE>(vup, el, er);
+        Row cu = make_row<EDGE>(u[0], el, er), cv = make_row<EDGE>(v[0], el, er);
         double su = 0.0, sv = 0.0;
         const float om = CHEB ? a.omega[omega_index(base + s)] : 1.0f;
+        // the wavefront's first row, opaque to the compiler: the border tests
+        // on it are formed per sweep in scalar registers rather than hoisted
+        // out of the sweep loop and held in vector lanes
+        int gys = __builtin_amdgcn_readfirstlane(gy0);
+        if (EDGE) asm volatile("" : "+s"(gys));
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            Raw nu = raw(r + 1 < R ? u[r + 1] : udn);
-            Raw nv = raw(r + 1 < R ? v[r + 1] : vdn);
-            const int gy = gy0 + r;
+            Row nu = make_row<EDGE>(r + 1 < R ? u[r + 1] : udn, el, er);
+            Row nv = make_row<EDGE>(r + 1 < R ? v[r + 1] : vdn, el, er);
+            const int gy = gys + r;
             float un, vn;
-            if constexpr (F)
-                ls_update_fast<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r],
-                                     II[r], IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
-            else
-                ls_update<EDGE>(pu, cu, nu, pv, cv, nv, gy == 0, gy == H - 1, el, er, h, IIX[r], IIY[r], II[r],
-                                IXT[r], IYT[r], B11[r], B12[r], B22[r], un, vn);
+            auto update = [&](const Row& pu_, const Row& nu_, const Row& pv_, const Row& nv_) {
+                if constexpr (F)
+                    ls_update_fast(pu_, cu, nu_, pv_, cv, nv_, h, IIX[r], IIY[r], II[r], IXT[r], IYT[r], B11[r],
+                                   B12[r], B22[r], un, vn);
+                else
+                    ls_update(pu_, cu, nu_, pv_, cv, nv_, h, IIX[r], IIY[r], II[r], IXT[r], IYT[r], B11[r], B12[r],
+                              B22[r], un, vn);
+            };
+            // gy is the same for the whole wavefront: the image's first and
+            // last rows take their own branch
+            if (EDGE && (gy == 0 || gy == H - 1)) {
+                const bool top = gy == 0, bot = gy == H - 1;
+                update(top ? border_row(cu) : pu, bot ? border_row(cu) : nu, top ? border_row(cv) : pv,
+                       bot ? border_row(cv) : nv);
+            } else {
+                update(pu, nu, pv, nv);
+            }
             if (norm) {
                 const int ry = w * R + r;
                 if (own_x && ry >= K && ry < WAVES * R - K && gy < H) {
