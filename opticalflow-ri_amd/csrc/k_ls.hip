@@ -595,8 +595,8 @@ const LsVariant kLsVariants[] = {
 #undef LS_SHAPE
 constexpr int kNumLsVariants = sizeof(kLsVariants) / sizeof(kLsVariants[0]);
 // Defaults measured on gfx950 (60-sweep solves over every compiled shape and
-// depth 3..8): 12 rows x 4 wavefronts 4 deep from about 0.7 Mpx (0.68 ms at
-// 1024^2), 4 rows x 8 wavefronts 5 deep below (0.31 ms at 512^2, 11 % less
+// depth 3..8): 12 rows x 4 wavefronts 4 deep from about 0.7 Mpx (0.53 ms at
+// 1024^2), 4 rows x 8 wavefronts 5 deep below (0.27 ms at 512^2, 12 % less
 // than 8 rows x 4 wavefronts).
 void ls_default(size_t npx, int* variant, int* depth) {
     if (npx >= (size_t)700000) {
@@ -604,6 +604,18 @@ void ls_default(size_t npx, int* variant, int* depth) {
         *depth = 4;
     } else {
         *variant = 2;
+        *depth = 5;
+    }
+}
+
+// The batched launch from about 0.7 Mpx in all: 8 rows x 8 wavefronts 5 deep
+// (measured on gfx950 over 64 uneven LS(HS) pairs of 256..2048-pixel sides:
+// 5 % less per batch than the single-image shape, whose batched kernel holds
+// one workgroup per CU; depths 4..6 within 0.5 %).
+void ls_batch_default(size_t npx, int* variant, int* depth) {
+    ls_default(npx, variant, depth);
+    if (npx >= (size_t)700000) {
+        *variant = 0;
         *depth = 5;
     }
 }
@@ -763,7 +775,7 @@ int ls_batch(ofri_ctx* ctx, const BatchImage* im, int n, float h, int maxnum, do
         total += (size_t)im[i].H * im[i].W;
     }
     int vi, depth;
-    ls_default(total, &vi, &depth);
+    ls_batch_default(total, &vi, &depth);
     shape_override(ctx, total, false, &vi, &depth);
     if (vi >= kNumLsVariants) return fail(OFRI_E_ARG, "ls: tile variant %d not compiled", vi + 1);
     const LsVariant& var = kLsVariants[vi];
