@@ -677,6 +677,11 @@ def bench_lshs(args):
         "epe_to_plain_by_family": fam_epe,
         "rules": {"hs_rel": rules.hs_rel, "ls_sweeps": rules.ls_sweeps, "ls_accept": rules.ls_accept,
                   "hs_budget": rules.hs_budget},
+        "arithmetic": "learned HS phase: Chebyshev sweeps in float32 (fused multiply-adds) to the relative "
+                      "residual hs_rel, then the reference's update (float32 fields, float64 stencil sums); "
+                      "where the plain 600 sweeps end below hs_rel the learned HS stops at a looser residual. "
+                      "Liu-Shen: the reference's update on every tested sweep, float32 FMA sweeps between; "
+                      "network: bfloat16 operands, float32 accumulation",
         "pairs": rows,
     }
     out["roofline_net"] = net_roofline(ctx, net, n)
