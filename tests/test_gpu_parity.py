@@ -185,12 +185,14 @@ def test_hs_tile_variants_identical(ops, variant):
         assert np.array_equal(U.download(), U32) and np.array_equal(V.download(), V32), (variant, depth)
 
 
-@pytest.mark.parametrize("H,W", [(170, 230), (5, 7), (2, 66)])
+@pytest.mark.parametrize("H,W", [(170, 230), (5, 7), (2, 66), (730, 1000)])
 @pytest.mark.parametrize("variant", range(7))
 def test_ls_tile_variants_identical(ops, variant, H, W):
     # (5, 7) and (2, 66): every pixel on the image border, most of each tile
     # outside the image (the coefficients formed in each launch's load phase
-    # take the clamped neighbours there)
+    # take the clamped neighbours there); (730, 1000): border tiles cut at the
+    # right and bottom edges around many interior tiles, the image's first
+    # and last rows at different rows of a wavefront across the shapes
     from ofri.device import DeviceArray, context
     im1, im2 = _piv(H, W, seed=14)
     u32, v32, e32, it32 = orc.ls(im1, im2, 5.0, np.zeros((H, W)), np.zeros((H, W)), maxnum=17, tol=0.0)
